@@ -1,0 +1,181 @@
+// ORACLE — test infrastructure only (see oracle_solver.cpp header).
+// Plain C entry points so tests/ can drive the restatement through ctypes.
+
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+
+#include "oracle_geometry.hpp"
+
+using namespace lsxo;
+
+namespace {
+thread_local std::string g_err;
+
+struct SolverH {
+  Csr A;
+  int N, M, K;
+  int kind;  // 0 FGMRES, 1 BiCGStab
+  int restart, max_it;
+  double eps;
+  int precond;  // 0 none, 1 multigrid cascade
+  Multigrid mg;
+  Identity id;
+};
+
+struct GeomH {
+  Geometry g;
+  Csr A;
+};
+
+Csr make_csr(long n, long ncols, const int* rowptr, const int* col, const double* val) {
+  Csr A;
+  A.rows = n;
+  A.cols = ncols;
+  A.rowptr.assign(rowptr, rowptr + n + 1);
+  const long nnz = rowptr[n];
+  A.col.assign(col, col + nnz);
+  A.val.assign(val, val + nnz);
+  return A;
+}
+}  // namespace
+
+extern "C" {
+
+const char* lsxo_last_error() { return g_err.c_str(); }
+
+// ---- geometry ------------------------------------------------------------------------
+void* lsxo_geom_create(int N, int M, int K, const double* dx, const double* dy, const double* dz, double x0,
+                       double y0, double z0) {
+  try {
+    auto* h = new GeomH();
+    h->g.init_grid(std::vector<double>(dx, dx + N), std::vector<double>(dy, dy + M), std::vector<double>(dz, dz + K), x0,
+                   y0, z0);
+    body_free_vof(h->g);
+    h->g.build_masses();
+    return h;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+void lsxo_geom_destroy(void* p) { delete static_cast<GeomH*>(p); }
+
+// Overwrite types/theta/vof (used by tests to build synthetic cut geometries), then
+// recompute masses.
+int lsxo_geom_set_fields(void* p, const int* type, const double* thx, const double* thy, const double* thz,
+                         const double* vof) {
+  auto* h = static_cast<GeomH*>(p);
+  const long n = h->g.ncells();
+  if (type) std::memcpy(h->g.type.data(), type, n * sizeof(int));
+  if (thx) std::memcpy(h->g.thx.data(), thx, n * sizeof(double));
+  if (thy) std::memcpy(h->g.thy.data(), thy, n * sizeof(double));
+  if (thz) std::memcpy(h->g.thz.data(), thz, n * sizeof(double));
+  if (vof) std::memcpy(h->g.vof.data(), vof, n * sizeof(double));
+  h->g.build_masses();
+  return 0;
+}
+
+int lsxo_geom_get_fields(void* p, int* type, double* thx, double* thy, double* thz, double* vof) {
+  auto* h = static_cast<GeomH*>(p);
+  const long n = h->g.ncells();
+  if (type) std::memcpy(type, h->g.type.data(), n * sizeof(int));
+  if (thx) std::memcpy(thx, h->g.thx.data(), n * sizeof(double));
+  if (thy) std::memcpy(thy, h->g.thy.data(), n * sizeof(double));
+  if (thz) std::memcpy(thz, h->g.thz.data(), n * sizeof(double));
+  if (vof) std::memcpy(vof, h->g.vof.data(), n * sizeof(double));
+  return 0;
+}
+
+long lsxo_geom_build_laplace(void* p, int pin) {
+  auto* h = static_cast<GeomH*>(p);
+  try {
+    h->A = build_laplace(h->g, pin != 0);
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  return (long)h->A.rowptr[h->A.rows];
+}
+
+int lsxo_geom_get_csr(void* p, int* rowptr, int* col, double* val) {
+  auto* h = static_cast<GeomH*>(p);
+  std::memcpy(rowptr, h->A.rowptr.data(), (h->A.rows + 1) * sizeof(int));
+  std::memcpy(col, h->A.col.data(), h->A.col.size() * sizeof(int));
+  std::memcpy(val, h->A.val.data(), h->A.val.size() * sizeof(double));
+  return 0;
+}
+
+// ---- solver ----------------------------------------------------------------------------
+void* lsxo_solver_create(int N, int M, int K, const int* rowptr, const int* col, const double* val, int kind,
+                         int restart, int max_it, double eps, int precond, int n_post, double omega) {
+  try {
+    auto h = std::make_unique<SolverH>();
+    const long n = (long)N * M * K;
+    h->A = make_csr(n, n, rowptr, col, val);
+    h->N = N;
+    h->M = M;
+    h->K = K;
+    h->kind = kind;
+    h->restart = restart;
+    h->max_it = max_it;
+    h->eps = eps;
+    h->precond = precond;
+    h->id.n = n;
+    if (precond == 1) h->mg.set(h->A, N, M, K, n_post, omega);
+    return h.release();
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+void lsxo_solver_destroy(void* p) { delete static_cast<SolverH*>(p); }
+
+void lsxo_solver_set_eps(void* p, double eps) { static_cast<SolverH*>(p)->eps = eps; }
+
+int lsxo_solver_solve(void* p, const double* b, double* x, double* norm_r) {
+  auto* h = static_cast<SolverH*>(p);
+  Preconditioner& pr = h->precond == 1 ? (Preconditioner&)h->mg : (Preconditioner&)h->id;
+  if (h->kind == 0) return fgmres_solve(h->A, pr, b, x, h->eps, h->max_it, h->restart, norm_r);
+  return bicgstab_solve(h->A, pr, b, x, h->eps, h->max_it, norm_r);
+}
+
+void lsxo_mg_apply(void* p, const double* b, double* x) { static_cast<SolverH*>(p)->mg.apply(b, x); }
+
+int lsxo_mg_levels(void* p) { return (int)static_cast<SolverH*>(p)->mg.levels.size(); }
+
+long lsxo_mg_level_info(void* p, int l, int* dims) {
+  auto& L = static_cast<SolverH*>(p)->mg.levels[l];
+  dims[0] = L.N;
+  dims[1] = L.M;
+  dims[2] = L.K;
+  return (long)L.A.rowptr[L.A.rows];
+}
+
+int lsxo_mg_level_csr(void* p, int l, int* rowptr, int* col, double* val) {
+  auto& A = static_cast<SolverH*>(p)->mg.levels[l].A;
+  std::memcpy(rowptr, A.rowptr.data(), (A.rows + 1) * sizeof(int));
+  std::memcpy(col, A.col.data(), A.col.size() * sizeof(int));
+  std::memcpy(val, A.val.data(), A.val.size() * sizeof(double));
+  return 0;
+}
+
+// ADLJ factors of level l, direction dir (0 x, 1 y, 2 z): c (n-1), d (n), e (n-1)
+int lsxo_mg_level_adlj(void* p, int l, int dir, double* c, double* d, double* e) {
+  auto& L = static_cast<SolverH*>(p)->mg.levels[l];
+  std::memcpy(c, L.adlj.c[dir].data(), L.adlj.c[dir].size() * sizeof(double));
+  std::memcpy(d, L.adlj.dg[dir].data(), L.adlj.dg[dir].size() * sizeof(double));
+  std::memcpy(e, L.adlj.e[dir].data(), L.adlj.e[dir].size() * sizeof(double));
+  return 0;
+}
+
+// Free functions on a caller CSR
+void lsxo_spmv(long n, const int* rowptr, const int* col, const double* val, const double* x, double* y) {
+  Csr A = make_csr(n, n, rowptr, col, val);
+  spmv(A, x, y);
+}
+
+}  // extern "C"

@@ -1,0 +1,43 @@
+// ORACLE — test infrastructure only (see oracle_solver.cpp header).
+#pragma once
+
+#include <vector>
+
+#include "oracle_internal.hpp"
+
+namespace lsxo {
+
+// Cell type codes of the reference, src/Base/LS_STAG_Mesh/cut_cells.h:26-168.  Interior cut
+// cells carry an 8-digit decimal code of their fluid corners (BSW BSE BNE BNW TSW TSE TNE TNW,
+// most significant first); domain border cells carry negative codes.
+enum CellType : int {
+  SolidCell = 0,
+  FluidCell = 11111111,
+  T_Border = -2, B_Border = -3, N_Border = -4, S_Border = -5, E_Border = -6, W_Border = -7,
+  TN_Edge = -8, TS_Edge = -9, TE_Edge = -10, TW_Edge = -11, BN_Edge = -12, BS_Edge = -13,
+  BE_Edge = -14, BW_Edge = -15, NE_Edge = -16, NW_Edge = -17, SE_Edge = -18, SW_Edge = -19,
+  TNE_Corner = -20, TNW_Corner = -21, TSE_Corner = -22, TSW_Corner = -23,
+  BNE_Corner = -24, BNW_Corner = -25, BSE_Corner = -26, BSW_Corner = -27
+};
+
+// Cut-cell geometry of the base (pressure) grid.  Index num = x + N*y + N*M*z.
+// thx/thy/thz: fluid fraction of the east/north/top face of each cell (reference theta).
+struct Geometry {
+  int N = 0, M = 0, K = 0;
+  double x0 = 0, y0 = 0, z0 = 0;
+  std::vector<double> dx, dy, dz;  // per-axis cell widths (tensor grid)
+  std::vector<int> type;
+  std::vector<double> thx, thy, thz, vof;
+  std::vector<double> inv_mx, inv_my, inv_mz;  // inverse staggered masses on x/y/z faces
+  long ncells() const { return (long)N * M * K; }
+  long idx(int x, int y, int z) const { return x + (long)N * (y + (long)M * z); }
+  void init_grid(const std::vector<double>& dxs, const std::vector<double>& dys, const std::vector<double>& dzs,
+                 double a, double b, double c);
+  void build_masses();
+};
+
+int boundary_type(int x, int y, int z, int N, int M, int K);
+Csr build_laplace(const Geometry& g, bool pin_outlet);
+void body_free_vof(Geometry& g);
+
+}  // namespace lsxo
