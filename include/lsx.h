@@ -1,0 +1,120 @@
+/*
+ * lsx.h — C ABI of the MI355X pressure-projection solver (cut-cell Poisson operator,
+ * geometric multigrid, FGMRES / BiCGStab) for LS-STAG.
+ *
+ * The entry points replace the reference's abstract solver interface
+ * IterativeSolverInterface (src/Base/Solvers/iterative_solver_interface.h:31-73), whose
+ * concrete classes MGFGMRES (src/Base/Solvers/FGMRES/mg_fgmres.h:33-52) and MGBiCGStab
+ * (src/Base/Solvers/BiCGStab/mg_bicgstab.h:33-52) the Poisson equation selects at compile
+ * time (src/DiscreteEquations/poisson_discrete_equation.h:19-61).  All arguments are plain
+ * pointers and sizes; matrices use the reference CSR layout (sparse_matrix.h:220-228:
+ * Number = row pointers int32[n+1], Portrait = column indices int32[nnz], Cell = values
+ * double[nnz]) with rows numbered x fastest: num = x + N*y + N*M*z.
+ *
+ * Every function returns LSX_OK (0) or a negative status; lsx_last_error() gives the
+ * message of the last failure on the calling thread.  A handle is used by one host thread
+ * at a time; distinct handles are independent (no global mutable state per handle).
+ * Work is ordered on the handle's HIP stream.
+ */
+#ifndef LSX_H_
+#define LSX_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSX_OK 0
+#define LSX_ERR_ARG (-1)      /* invalid argument (sizes, pointers, unsupported layout) */
+#define LSX_ERR_HIP (-2)      /* HIP runtime failure */
+#define LSX_ERR_STATE (-3)    /* call out of order (e.g. solve before set) */
+#define LSX_ERR_NOTCONV (-4)  /* reserved: the solvers report non-convergence via it/norm_r */
+
+#define LSX_KRYLOV_FGMRES 0   /* FGMRESInterface, fgmres_interface.h:34-177 */
+#define LSX_KRYLOV_BICGSTAB 1 /* BiCGStabInterface, bicgstab_interface.h:37-140 */
+
+#define LSX_MG_CASCADE 0 /* reference cascade: restrict RHS, coarsest LU, prolong + ADLJ
+                            post-smoothing (multigrid_solver.h:48-75) */
+#define LSX_MG_VCYCLE 1  /* V-cycle with red-black Gauss-Seidel (mixed-precision mode) */
+
+#define LSX_PREC_FP64 0  /* everything in FP64 */
+#define LSX_PREC_MIXED 1 /* FP32 multigrid, FP16 coarse face coefficients, FP64 Krylov */
+
+#define LSX_MEM_HOST 0
+#define LSX_MEM_DEVICE 1
+
+typedef struct lsx_solver lsx_solver;
+
+typedef struct lsx_options {
+  int krylov;        /* LSX_KRYLOV_*                                          */
+  int restart;       /* FGMRES restart length (FGMRES_RESTART_NUM)            */
+  int max_it;        /* iteration cap (MAX_ITER)                              */
+  double eps;        /* absolute residual 2-norm tolerance (eps)              */
+  int mg;            /* LSX_MG_*                                              */
+  int n_post;        /* post-smoothing sweeps per level (n_post)             */
+  int n_pre;         /* pre-smoothing sweeps (V-cycle only; the cascade has none) */
+  double omega;      /* ADLJ damping (DEFAULT_OMEGA_3D = 0.805)               */
+  int precision;     /* LSX_PREC_*                                            */
+  int levels;        /* 0 = automatic level count; >0 = cap (V-cycle)        */
+  int z_head_fix;    /* 0 = reproduce the reference z-line head read (default) */
+} lsx_options;
+
+/* Fill defaults: FGMRES(13), max_it 200, eps 1e-5, cascade MG, n_post 3, omega 0.805, FP64
+ * (base_task_description.h:41-46, ls_stag_real_constants.h:47, 62). */
+void lsx_options_default(lsx_options* opt);
+
+/* Create / destroy a solver handle.  stream: a hipStream_t, or NULL for a stream owned by
+ * the handle.  Replaces construction of MGFGMRES / MGBiCGStab. */
+int lsx_create(lsx_solver** out, const lsx_options* opt, void* stream);
+int lsx_destroy(lsx_solver* s);
+
+/* Set the matrix and build the preconditioner: replaces IterativeSolverInterface::set
+ * (iterative_solver_interface.h:46-55) with its MultiGridSolver::set
+ * (multigrid_solver.h:76-101).  The matrix must have the reference pressure-operator
+ * pattern: each row's columns are a subset of {num-N*M, num-N, num-1, num, num+1, num+N,
+ * num+N*M}.  mem says whether the three arrays are host or device pointers.  N, M, K must
+ * halve to 2 along the smallest dimension with every level even (even-even-even transfers). */
+int lsx_set_csr(lsx_solver* s, int N, int M, int K, const int* rowptr, const int* col, const double* val, long nnz,
+                int mem);
+
+/* New values with the same pattern, then rebuild the hierarchy: replaces
+ * rebuildPreconditioner (mg_fgmres.h:35-39, multigrid_solver.h:102-106). */
+int lsx_update_values(lsx_solver* s, const double* val, int mem);
+int lsx_rebuild(lsx_solver* s);
+
+/* Solve A x = b in place (x holds the initial guess, the reference passes p = 0):
+ * replaces IterativeSolverInterface::solve (fgmres_interface.h:78-142,
+ * bicgstab_interface.h:76-111).  *it and *norm_r mirror the public members it / norm_r. */
+int lsx_solve(lsx_solver* s, const double* b, double* x, int mem, int* it, double* norm_r);
+
+/* Tolerance and counters (public members eps / it / norm_r of the reference interface). */
+int lsx_set_eps(lsx_solver* s, double eps);
+int lsx_get_stats(const lsx_solver* s, int* it, double* norm_r, int* levels);
+
+/* One preconditioner application z = M^{-1} v: replaces MGFGMRES::preconditioner
+ * (mg_fgmres.h:46-50) / MultiGridSolver::solve; exposed for parity testing. */
+int lsx_precondition(lsx_solver* s, const double* v, double* z, int mem);
+
+/* y = A_level x on multigrid level `level` (0 = fine): replaces
+ * RealValuedMeshFunction::setProduct (real_valued_mesh_function.h:81-88, 262-272). */
+int lsx_apply_operator(lsx_solver* s, int level, const double* x, double* y, int mem);
+
+/* Dimensions of level `level`; returns its cell count or a negative status. */
+long lsx_level_dims(const lsx_solver* s, int level, int* N, int* M, int* K);
+
+/* Copy level `level`'s operator as dense stencil coefficients to host memory:
+ * npts = 7 ([b,s,w,p,e,n,t]) on the fine level, 27 on coarse levels (index
+ * (dx+1)+3(dy+1)+9(dz+1)); layout coef[q*n + num].  Replaces reading CoarseGrid::AH
+ * (coarse_grid.h:103-104).  Returns npts or a negative status. */
+int lsx_level_stencil(const lsx_solver* s, int level, double* coef_host, long capacity);
+
+/* Synchronise the handle's stream. */
+int lsx_synchronize(lsx_solver* s);
+
+/* Thread-local message of the last failure ("" if none). */
+const char* lsx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSX_H_ */

@@ -1,0 +1,138 @@
+// extern "C" entry points of include/lsx.h.  Exceptions never cross the boundary: each call
+// returns a status and records the message for lsx_last_error().
+
+#include <exception>
+#include <new>
+#include <string>
+
+#include "../../include/lsx.h"
+#include "lsx_solver.hpp"
+
+struct lsx_solver {
+  lsx::Solver impl;
+  lsx_solver(const lsx_options& o, hipStream_t s) : impl(o, s) {}
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    g_last_error.clear();
+    f();
+    return LSX_OK;
+  } catch (const lsx::Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return LSX_ERR_HIP;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return LSX_ERR_ARG;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+void lsx_options_default(lsx_options* o) {
+  if (!o) return;
+  o->krylov = LSX_KRYLOV_FGMRES;
+  o->restart = 13;
+  o->max_it = 200;
+  o->eps = 1e-5;
+  o->mg = LSX_MG_CASCADE;
+  o->n_post = 3;
+  o->n_pre = 0;
+  o->omega = 0.805;
+  o->precision = LSX_PREC_FP64;
+  o->levels = 0;
+  o->z_head_fix = 0;
+}
+
+int lsx_create(lsx_solver** out, const lsx_options* opt, void* stream) {
+  if (!out || !opt) {
+    g_last_error = "null argument";
+    return LSX_ERR_ARG;
+  }
+  *out = nullptr;
+  return guarded([&] { *out = new lsx_solver(*opt, (hipStream_t)stream); });
+}
+
+int lsx_destroy(lsx_solver* s) {
+  return guarded([&] { delete s; });
+}
+
+int lsx_set_csr(lsx_solver* s, int N, int M, int K, const int* rowptr, const int* col, const double* val, long nnz,
+                int mem) {
+  if (!s) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.set_csr(N, M, K, rowptr, col, val, nnz, mem); });
+}
+
+int lsx_update_values(lsx_solver* s, const double* val, int mem) {
+  if (!s) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.update_values(val, mem); });
+}
+
+int lsx_rebuild(lsx_solver* s) {
+  if (!s) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.rebuild(); });
+}
+
+int lsx_solve(lsx_solver* s, const double* b, double* x, int mem, int* it, double* norm_r) {
+  if (!s) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.solve(b, x, mem, it, norm_r); });
+}
+
+int lsx_set_eps(lsx_solver* s, double eps) {
+  if (!s) return LSX_ERR_ARG;
+  if (!(eps >= 0.0)) {
+    g_last_error = "eps must be >= 0";
+    return LSX_ERR_ARG;
+  }
+  s->impl.opt.eps = eps;
+  return LSX_OK;
+}
+
+int lsx_get_stats(const lsx_solver* s, int* it, double* norm_r, int* levels) {
+  if (!s) return LSX_ERR_ARG;
+  if (it) *it = s->impl.it;
+  if (norm_r) *norm_r = s->impl.norm_r;
+  if (levels) *levels = s->impl.num_levels();
+  return LSX_OK;
+}
+
+int lsx_precondition(lsx_solver* s, const double* v, double* z, int mem) {
+  if (!s) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.precondition(v, z, mem); });
+}
+
+int lsx_apply_operator(lsx_solver* s, int level, const double* x, double* y, int mem) {
+  if (!s) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.apply_operator(level, x, y, mem); });
+}
+
+long lsx_level_dims(const lsx_solver* s, int level, int* N, int* M, int* K) {
+  if (!s) return LSX_ERR_ARG;
+  long r = 0;
+  const int st = guarded([&] { r = s->impl.level_dims(level, N, M, K); });
+  return st == LSX_OK ? r : st;
+}
+
+int lsx_level_stencil(const lsx_solver* s, int level, double* coef_host, long capacity) {
+  if (!s) return LSX_ERR_ARG;
+  int r = 0;
+  const int st = guarded([&] { r = s->impl.level_stencil(level, coef_host, capacity); });
+  return st == LSX_OK ? r : st;
+}
+
+int lsx_synchronize(lsx_solver* s) {
+  if (!s) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.synchronize(); });
+}
+
+const char* lsx_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

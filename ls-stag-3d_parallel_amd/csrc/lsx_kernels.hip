@@ -1,0 +1,764 @@
+// FP64 kernels of the pressure solve (reference-semantics path).
+//
+// Built with -ffp-contract=off: every product and sum is rounded separately, in the order the
+// reference writes it, so the GPU path and a scalar CPU evaluation of the same formulas agree
+// to the last bit wherever the summation order is the same (stencil rows, transfers, line
+// solves, Galerkin products).  Reductions are two-stage and deterministic (fixed partition,
+// fixed combination order), but their order differs from a sequential sum.
+//
+// Index convention (reference ls_stag_uniform_mesh.h:174-185): num = x + N*y + N*M*z.
+
+#include "lsx_kernels.hpp"
+
+#include <cmath>
+
+namespace lsx {
+
+namespace {
+
+constexpr double PLUS_TINY = 1e-16;        // ls_stag_real_constants.h:27
+constexpr double EQ_DOUBLE = 1e-5;         // :28
+constexpr double EQ_TINY2_DOUBLE = 1e-5;   // :30
+constexpr double EQ_TINY_DOUBLE = 1e-12;   // :31
+constexpr double EQ_TINY3_DOUBLE = 1e-13;  // :32
+
+inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
+
+// ---- deterministic block reduction ------------------------------------------------------------
+template <int BS>
+__device__ double block_sum(double v) {
+  __shared__ double lds[BS / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < BS / 64; ++w) s += lds[w];
+  __syncthreads();
+  return s;  // valid in thread 0
+}
+
+template <int BS>
+__device__ double block_max(double v) {
+  __shared__ double lds[BS / 64];
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < BS / 64; ++w) s = fmax(s, lds[w]);
+  __syncthreads();
+  return s;
+}
+
+// contiguous chunk of [0, n) owned by block b of nb
+__device__ inline void chunk_of(long n, int b, int nb, long* lo, long* hi) {
+  const long per = (n + nb - 1) / nb;
+  *lo = per * b;
+  long h = *lo + per;
+  *hi = h < n ? h : n;
+}
+
+// ---- stencil geometry helpers -------------------------------------------------------------
+__device__ inline void decode(int i, int N, int M, int* x, int* y, int* z) {
+  const int NM = N * M;
+  *z = i / NM;
+  const int r = i - *z * NM;
+  *y = r / N;
+  *x = r - *y * N;
+}
+
+__device__ inline int slot_sub(int npts, int dir) {  // coupling to the previous cell of a line
+  if (npts == 7) return dir == 0 ? 2 : (dir == 1 ? 1 : 0);
+  return dir == 0 ? 12 : (dir == 1 ? 10 : 4);
+}
+__device__ inline int slot_super(int npts, int dir) {  // coupling to the next cell of a line
+  if (npts == 7) return dir == 0 ? 4 : (dir == 1 ? 5 : 6);
+  return dir == 0 ? 14 : (dir == 1 ? 16 : 22);
+}
+__device__ inline int slot_diag(int npts) { return npts == 7 ? 3 : 13; }
+
+// Row i of the operator applied to x, accumulated in ascending column order (the CSR order
+// of the reference matrices: [b,s,w,p,e,n,t] on the fine level, sorted columns on Galerkin
+// levels).  mode: 0 -> acc starts at 0 and adds; 1 -> acc starts at b and subtracts.
+template <int NPTS>
+__device__ inline double row_apply(const double* __restrict__ coef, long n, int i, int x, int y, int z, int N, int M,
+                                   int K, const double* __restrict__ xv, double acc, bool subtract) {
+  const int NM = N * M;
+  if (NPTS == 7) {
+    const int off[7] = {-NM, -N, -1, 0, 1, N, NM};
+    const bool ok[7] = {z > 0, y > 0, x > 0, true, x < N - 1, y < M - 1, z < K - 1};
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+      if (ok[q]) {
+        const double t = coef[q * n + i] * xv[i + off[q]];
+        acc = subtract ? acc - t : acc + t;
+      }
+  } else {
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+      for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int zz = z + dz, yy = y + dy, xx = x + dx;
+          if (zz < 0 || zz >= K || yy < 0 || yy >= M || xx < 0 || xx >= N) continue;
+          const int q = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+          const double t = coef[q * n + i] * xv[i + dx + N * dy + NM * dz];
+          acc = subtract ? acc - t : acc + t;
+        }
+  }
+  return acc;
+}
+
+// ---- kernels: setup ----------------------------------------------------------------------------
+__global__ void k_csr_to_stencil7(const int* __restrict__ rowptr, const int* __restrict__ col,
+                                  const double* __restrict__ val, int N, int M, int K, double* __restrict__ coef,
+                                  int* err) {
+  const long n = (long)N * M * K;
+  const int NM = N * M;
+  for (long li = blockIdx.x * (long)blockDim.x + threadIdx.x; li < n; li += (long)gridDim.x * blockDim.x) {
+    const int i = (int)li;
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    double a[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) {
+      const int off = col[j] - i;
+      int q = -1;
+      if (off == -NM && z > 0) q = 0;
+      else if (off == -N && y > 0) q = 1;
+      else if (off == -1 && x > 0) q = 2;
+      else if (off == 0) q = 3;
+      else if (off == 1 && x < N - 1) q = 4;
+      else if (off == N && y < M - 1) q = 5;
+      else if (off == NM && z < K - 1) q = 6;
+      if (q < 0) {
+        atomicOr(err, 1);
+      } else {
+        a[q] = val[j];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 7; ++q) coef[q * n + i] = a[q];
+  }
+}
+
+// Galerkin product A_c = R (A_f P): coarse_grid.h:103-104 with product_sparse_matrix.h:72-104.
+// Terms reach each entry in the reference order: V = R A_f accumulates over the 8 children in
+// R's column order, then A_c accumulates V's fine columns in ascending order.
+__device__ inline void axis_w(int f, int nf, int* c, double* w, int* cnt) {
+  const int p = f >> 1;
+  if (f == 0 || f == nf - 1) {
+    c[0] = p; w[0] = 1.0; *cnt = 1;
+  } else if (f & 1) {
+    c[0] = p; w[0] = 0.75; c[1] = p + 1; w[1] = 0.25; *cnt = 2;
+  } else {
+    c[0] = p - 1; w[0] = 0.25; c[1] = p; w[1] = 0.75; *cnt = 2;
+  }
+}
+
+template <int NPTS>
+__global__ void k_galerkin(const double* __restrict__ cf, int Nf, int Mf, int Kf, double* __restrict__ cc, int Nc,
+                           int Mc, int Kc) {
+  const long nc = (long)Nc * Mc * Kc, nf = (long)Nf * Mf * Kf;
+  const int NMf = Nf * Mf;
+  for (long li = blockIdx.x * (long)blockDim.x + threadIdx.x; li < nc; li += (long)gridDim.x * blockDim.x) {
+    const int I = (int)li;
+    int X, Y, Z;
+    decode(I, Nc, Mc, &X, &Y, &Z);
+    double V[64];
+#pragma unroll
+    for (int q = 0; q < 64; ++q) V[q] = 0.0;
+    for (int ch = 0; ch < 8; ++ch) {
+      const int hx = 2 * X + (ch & 1), hy = 2 * Y + ((ch >> 1) & 1), hz = 2 * Z + (ch >> 2);
+      const int h = hx + Nf * hy + NMf * hz;
+      if (NPTS == 7) {
+        const int dxs[7] = {0, 0, -1, 0, 1, 0, 0}, dys[7] = {0, -1, 0, 0, 0, 1, 0}, dzs[7] = {-1, 0, 0, 0, 0, 0, 1};
+        for (int q = 0; q < 7; ++q) {
+          const int fx = hx + dxs[q], fy = hy + dys[q], fz = hz + dzs[q];
+          if (fx < 0 || fx >= Nf || fy < 0 || fy >= Mf || fz < 0 || fz >= Kf) continue;
+          const int b = (fx - 2 * X + 1) + 4 * (fy - 2 * Y + 1) + 16 * (fz - 2 * Z + 1);
+          V[b] = V[b] + 0.125 * cf[q * nf + h];
+        }
+      } else {
+        for (int q = 0; q < 27; ++q) {
+          const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
+          const int fx = hx + dx, fy = hy + dy, fz = hz + dz;
+          if (fx < 0 || fx >= Nf || fy < 0 || fy >= Mf || fz < 0 || fz >= Kf) continue;
+          const int b = (fx - 2 * X + 1) + 4 * (fy - 2 * Y + 1) + 16 * (fz - 2 * Z + 1);
+          V[b] = V[b] + 0.125 * cf[q * nf + h];
+        }
+      }
+    }
+    double A[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) A[q] = 0.0;
+    for (int b = 0; b < 64; ++b) {
+      const int fx = 2 * X - 1 + (b & 3), fy = 2 * Y - 1 + ((b >> 2) & 3), fz = 2 * Z - 1 + (b >> 4);
+      if (fx < 0 || fx >= Nf || fy < 0 || fy >= Mf || fz < 0 || fz >= Kf) continue;
+      int cx[2], cy[2], cz[2], nx, ny, nz;
+      double wx[2], wy[2], wz[2];
+      axis_w(fx, Nf, cx, wx, &nx);
+      axis_w(fy, Mf, cy, wy, &ny);
+      axis_w(fz, Kf, cz, wz, &nz);
+      const double v = V[b];
+      for (int c3 = 0; c3 < nz; ++c3)
+        for (int c2 = 0; c2 < ny; ++c2)
+          for (int c1 = 0; c1 < nx; ++c1) {
+            const double w = wx[c1] * wy[c2] * wz[c3];
+            const int q = (cx[c1] - X + 1) + 3 * (cy[c2] - Y + 1) + 9 * (cz[c3] - Z + 1);
+            A[q] = A[q] + v * w;
+          }
+    }
+#pragma unroll
+    for (int q = 0; q < 27; ++q) cc[q * nc + I] = A[q];
+  }
+}
+
+template <int BS>
+__global__ void k_diag_absmax_partial(const double* __restrict__ diag, long n, double* partial) {
+  long lo, hi;
+  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
+  double m = 0.0;
+  for (long i = lo + threadIdx.x; i < hi; i += BS) m = fmax(m, fabs(diag[i]));
+  m = block_max<BS>(m);
+  if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
+template <int BS>
+__global__ void k_max_finalize(const double* partial, int nb, double* out) {
+  double m = 0.0;
+  for (int i = threadIdx.x; i < nb; i += BS) m = fmax(m, partial[i]);
+  m = block_max<BS>(m);
+  if (threadIdx.x == 0) out[0] = m;
+}
+
+// Line decomposition of direction dir: tridiagonal_matrix.h:45-67 (triLU) applied to the
+// lines of smoother_adlj.h:40-86.  One thread per line; the zero-pivot threshold is global
+// (max |diag| over the level), pivots below it are set to 0 which decouples the row.
+__device__ inline void line_geometry(int t, int dir, int N, int M, int K, int* start, int* stride, int* len) {
+  if (dir == 0) {  // lines (y, z)
+    *start = t * N;
+    *stride = 1;
+    *len = N;
+  } else if (dir == 1) {  // lines (x, z)
+    const int x = t % N, z = t / N;
+    *start = x + N * M * z;
+    *stride = N;
+    *len = M;
+  } else {  // lines (x, y)
+    *start = t;
+    *stride = N * M;
+    *len = K;
+  }
+}
+
+__global__ void k_line_factor(const double* __restrict__ coef, int npts, int N, int M, int K, int dir,
+                              const double* absmax, double* __restrict__ dinv, double* __restrict__ eu) {
+  const long n = (long)N * M * K;
+  const int nlines = dir == 0 ? M * K : (dir == 1 ? N * K : N * M);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlines) return;
+  double tiny = EQ_TINY2_DOUBLE * absmax[0];
+  if (tiny < EQ_TINY_DOUBLE) tiny = EQ_TINY_DOUBLE;
+  int start, stride, len;
+  line_geometry(t, dir, N, M, K, &start, &stride, &len);
+  const double* dg = coef + slot_diag(npts) * n;
+  const double* sb = coef + slot_sub(npts, dir) * n;
+  const double* sp = coef + slot_super(npts, dir) * n;
+  int i = start;
+  double aux = dg[i];
+  double dprev = (fabs(aux) < tiny) ? 0.0 : (1.0 / aux);
+  dinv[i] = dprev;
+  for (int k = 1; k < len; ++k) {
+    const int ip = i;
+    i += stride;
+    const double e = sp[ip] * dprev;
+    eu[ip] = e;
+    aux = dg[i] - sb[i] * e;
+    dprev = (fabs(aux) < tiny) ? 0.0 : (1.0 / aux);
+    dinv[i] = dprev;
+  }
+  eu[i] = 0.0;
+}
+
+// Dense LU of the coarsest level: lu_decomposition_sparse_matrix.h:60-215 (initScalePerm,
+// calcLU).  A single thread: the coarsest level has 8-16 unknowns.
+__device__ inline double dense_at(const double* coef, int n, int N, int M, int K, int i, int j) {
+  int xi, yi, zi, xj, yj, zj;
+  decode(i, N, M, &xi, &yi, &zi);
+  decode(j, N, M, &xj, &yj, &zj);
+  const int dx = xj - xi, dy = yj - yi, dz = zj - zi;
+  if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) return 0.0;
+  return coef[((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) * (long)n + i];
+}
+
+__global__ void k_lu_factor(const double* __restrict__ coef, int N, int M, int K, double* scale, int* perm, double* L,
+                            double* U, double* colv) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int n = N * M * K;
+  for (int i = 0; i < n; ++i) {
+    const double diag = fabs(dense_at(coef, n, N, M, K, i, i));
+    scale[i] = (diag > EQ_DOUBLE) ? (1.0 / sqrt(diag)) : 1.0;
+    perm[i] = i;
+  }
+  for (int q = 0; q < n * n; ++q) {
+    L[q] = 0.0;
+    U[q] = 0.0;
+  }
+  double maxAbs = 0.0, maxVal = 0.0;
+  int pivot = 0;
+  double si = scale[0];
+  for (int i = 0; i < n; ++i) {
+    const double a0 = dense_at(coef, n, N, M, K, i, 0);
+    const double absVal = fabs(a0 * scale[i] * si);
+    if (absVal > maxVal) {
+      maxVal = absVal;
+      pivot = i;
+    }
+    for (int j = 0; j < n; ++j) maxAbs = fmax(maxAbs, fabs(dense_at(coef, n, N, M, K, i, j)));
+  }
+  const double tiny = EQ_TINY3_DOUBLE * maxAbs;
+  if (pivot != 0) {
+    const int tmp = perm[0];
+    perm[0] = perm[pivot];
+    perm[pivot] = tmp;
+  }
+  int permi = perm[0];
+  si = scale[permi];
+  for (int j = 0; j < n; ++j) U[j] = dense_at(coef, n, N, M, K, permi, j) * si * scale[j];
+  double U00 = U[0];
+  if (fabs(U00) < tiny) U00 = (U00 < 0) ? -tiny : tiny;
+  U[0] = U00;
+  const double invU00 = scale[0] / U00;
+  for (int k = 1; k < n; ++k) {
+    const int pk = perm[k];
+    L[k * n] = dense_at(coef, n, N, M, K, pk, 0) * scale[pk] * invU00;
+  }
+  L[0] = 1.0;
+  for (int i = 1; i < n; ++i) {
+    maxVal = 0.0;
+    pivot = i;
+    si = scale[i];
+    for (int k = i; k < n; ++k) {
+      double sum = 0.0;
+      for (int t = 0; t < i; ++t) sum += L[k * n + t] * U[t * n + i];
+      const int pk = perm[k];
+      const double val = dense_at(coef, n, N, M, K, pk, i) * scale[pk] * si - sum;
+      colv[k] = val;
+      if (fabs(val) > maxVal) {
+        maxVal = fabs(val);
+        pivot = k;
+      }
+    }
+    if (pivot != i) {
+      int tmp = perm[i];
+      perm[i] = perm[pivot];
+      perm[pivot] = tmp;
+      for (int t = 0; t < i; ++t) {
+        const double d = L[i * n + t];
+        L[i * n + t] = L[pivot * n + t];
+        L[pivot * n + t] = d;
+      }
+      const double v = colv[i];
+      colv[i] = colv[pivot];
+      colv[pivot] = v;
+    }
+    permi = perm[i];
+    si = scale[permi];
+    for (int j = i; j < n; ++j) {
+      double sum = 0.0;
+      for (int t = 0; t < i; ++t) sum += L[i * n + t] * U[t * n + j];
+      U[i * n + j] = dense_at(coef, n, N, M, K, permi, j) * si * scale[j] - sum;
+    }
+    double Uii = U[i * n + i];
+    if (fabs(Uii) < tiny) Uii = (Uii < 0) ? -tiny : tiny;
+    U[i * n + i] = Uii;
+    const double invUii = 1.0 / Uii;
+    for (int k = i + 1; k < n; ++k) L[k * n + i] = colv[k] * invUii;
+    L[i * n + i] = 1.0;
+  }
+}
+
+// ---- kernels: solve phase ---------------------------------------------------------------------
+template <int NPTS, int MODE, int BS>
+__global__ void k_stencil(const double* __restrict__ coef, int N, int M, int K, const double* __restrict__ x,
+                          const double* __restrict__ b, double* __restrict__ y, double* __restrict__ partial) {
+  const long n = (long)N * M * K;
+  long lo, hi;
+  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
+  double ss = 0.0;
+  for (long li = lo + threadIdx.x; li < hi; li += BS) {
+    const int i = (int)li;
+    int xx, yy, zz;
+    decode(i, N, M, &xx, &yy, &zz);
+    if (MODE == ST_PROD) {
+      y[i] = row_apply<NPTS>(coef, n, i, xx, yy, zz, N, M, K, x, 0.0, false);
+    } else if (MODE == ST_RESID) {
+      const double r = row_apply<NPTS>(coef, n, i, xx, yy, zz, N, M, K, x, b[i], true);
+      y[i] = r;
+      ss += r * r;
+    } else {
+      const double q = row_apply<NPTS>(coef, n, i, xx, yy, zz, N, M, K, x, 0.0, false);
+      y[i] = y[i] - q;
+    }
+  }
+  if (MODE == ST_RESID) {
+    ss = block_sum<BS>(ss);
+    if (threadIdx.x == 0) partial[blockIdx.x] = ss;
+  }
+}
+
+// r_c = R r_f: each coarse cell sums 1/8 of its 8 children (R row order a,b,c,d,aa,bb,cc,dd)
+__global__ void k_restrict(const double* __restrict__ rf, int Nf, int Mf, double* __restrict__ rc, int Nc, int Mc,
+                           int Kc) {
+  const long nc = (long)Nc * Mc * Kc;
+  const int NMf = Nf * Mf;
+  for (long li = blockIdx.x * (long)blockDim.x + threadIdx.x; li < nc; li += (long)gridDim.x * blockDim.x) {
+    int X, Y, Z;
+    decode((int)li, Nc, Mc, &X, &Y, &Z);
+    const int a = 2 * X + Nf * 2 * Y + NMf * 2 * Z;
+    double acc = 0.0;
+    acc = acc + 0.125 * rf[a];
+    acc = acc + 0.125 * rf[a + 1];
+    acc = acc + 0.125 * rf[a + Nf];
+    acc = acc + 0.125 * rf[a + Nf + 1];
+    acc = acc + 0.125 * rf[a + NMf];
+    acc = acc + 0.125 * rf[a + NMf + 1];
+    acc = acc + 0.125 * rf[a + NMf + Nf];
+    acc = acc + 0.125 * rf[a + NMf + Nf + 1];
+    rc[li] = acc;
+  }
+}
+
+// ch_f = P ch_c (trilinear), c_f = ch_f
+__global__ void k_prolong(const double* __restrict__ chc, int Nc, int Mc, double* __restrict__ chf,
+                          double* __restrict__ cf, int Nf, int Mf, int Kf) {
+  const long nf = (long)Nf * Mf * Kf;
+  const int NMc = Nc * Mc;
+  for (long li = blockIdx.x * (long)blockDim.x + threadIdx.x; li < nf; li += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode((int)li, Nf, Mf, &x, &y, &z);
+    int cx[2], cy[2], cz[2], nx, ny, nz;
+    double wx[2], wy[2], wz[2];
+    axis_w(x, Nf, cx, wx, &nx);
+    axis_w(y, Mf, cy, wy, &ny);
+    axis_w(z, Kf, cz, wz, &nz);
+    double acc = 0.0;
+    for (int c3 = 0; c3 < nz; ++c3)
+      for (int c2 = 0; c2 < ny; ++c2)
+        for (int c1 = 0; c1 < nx; ++c1) {
+          const double w = wx[c1] * wy[c2] * wz[c3];
+          acc = acc + w * chc[cx[c1] + Nc * cy[c2] + NMc * cz[c3]];
+        }
+    chf[li] = acc;
+    cf[li] = acc;
+  }
+}
+
+// c = omega * T_dir^{-1} r ; ch += c (smoother_adlj_interface.h:73-82 with the TDMA kernels
+// smoother_adlj_x.h:38-48, smoother_adlj_y.h:36-50).  One thread per line.  For z lines the
+// reference reads the first right-hand-side element at index y*N*K + x (its z call passes
+// N*K where the y kernel expects N*M); z_head_fix = 0 reproduces that read.
+__global__ void k_line_solve(const double* __restrict__ coef, int npts, int N, int M, int K, int dir,
+                             const double* __restrict__ dinv, const double* __restrict__ eu,
+                             const double* __restrict__ r, double* __restrict__ c, double* __restrict__ ch,
+                             double omega, int z_head_fix) {
+  const long n = (long)N * M * K;
+  const int nlines = dir == 0 ? M * K : (dir == 1 ? N * K : N * M);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlines) return;
+  int start, stride, len;
+  line_geometry(t, dir, N, M, K, &start, &stride, &len);
+  const double* sb = coef + slot_sub(npts, dir) * n;
+  int head = start;
+  if (dir == 2 && !z_head_fix) {
+    const int x = t % N, y = t / N;
+    head = y * N * K + x;
+  }
+  int i = start;
+  double yprev = r[head] * dinv[i];
+  c[i] = yprev;
+  for (int k = 1; k < len; ++k) {
+    i += stride;
+    yprev = (r[i] - sb[i] * yprev) * dinv[i];
+    c[i] = yprev;
+  }
+  // backward: x_k = y_k - e_k x_{k+1}; write omega*x and accumulate into ch
+  double xnext = yprev;
+  double cv = xnext * omega;
+  c[i] = cv;
+  ch[i] = ch[i] + cv;
+  for (int k = len - 2; k >= 0; --k) {
+    i -= stride;
+    const double xv = c[i] - eu[i] * xnext;
+    xnext = xv;
+    cv = xv * omega;
+    c[i] = cv;
+    ch[i] = ch[i] + cv;
+  }
+}
+
+// lu_decomposition_sparse_matrix.h:216-259 + decomposition_sparse_matrix_interface.h:43-71
+__global__ void k_lu_solve(const double* scale, const int* perm, const double* L, const double* U, int n,
+                           const double* __restrict__ b, double* __restrict__ x) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double bs[64], yv[64];
+  for (int i = 0; i < n; ++i) bs[i] = scale[perm[i]] * b[perm[i]];
+  for (int i = 0; i < n; ++i) {
+    double aux = bs[i];
+    for (int t = 0; t < i; ++t) aux -= L[i * n + t] * yv[t];
+    yv[i] = aux;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double aux = yv[i];
+    for (int j = i + 1; j < n; ++j) aux -= U[i * n + j] * bs[j];
+    bs[i] = aux / U[i * n + i];
+  }
+  for (int i = 0; i < n; ++i) x[i] = scale[i] * bs[i];
+}
+
+// ---- kernels: BLAS-1 ---------------------------------------------------------------------------
+__global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
+}
+
+__global__ void k_scale_copy(const double* __restrict__ w, const double* __restrict__ normw, double* __restrict__ v,
+                             long n) {
+  const double f = 1.0 / normw[0];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) v[i] = f * w[i];
+}
+
+template <int BS>
+__global__ void k_dot_partial(const double* __restrict__ a, const double* __restrict__ b, long n, double* partial) {
+  long lo, hi;
+  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
+  double s = 0.0;
+  for (long i = lo + threadIdx.x; i < hi; i += BS) s += a[i] * b[i];
+  s = block_sum<BS>(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+template <int BS>
+__global__ void k_finalize(const double* partial, int nb, double* out, int sqrt_flag) {
+  double s = 0.0;
+  // each thread sums a contiguous run so the combination order is fixed
+  const int per = (nb + BS - 1) / BS;
+  const int lo = threadIdx.x * per;
+  for (int i = lo; i < lo + per && i < nb; ++i) s += partial[i];
+  s = block_sum<BS>(s);
+  if (threadIdx.x == 0) out[0] = sqrt_flag ? sqrt(s) : s;
+}
+
+template <int BS>
+__global__ void k_axpy_dot(double* __restrict__ w, const double* __restrict__ v, const double* __restrict__ h,
+                           const double* __restrict__ vnext, long n, double* partial) {
+  long lo, hi;
+  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
+  const double hv = h[0];
+  double s = 0.0;
+  for (long i = lo + threadIdx.x; i < hi; i += BS) {
+    const double aux = v[i] * hv;
+    const double wn = w[i] - aux;
+    w[i] = wn;
+    s += vnext ? wn * vnext[i] : wn * wn;
+  }
+  s = block_sum<BS>(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+struct YCoef {
+  double y[64];
+};
+
+__global__ void k_sol_update(double* __restrict__ x, const double* __restrict__ Z, long ldz, YCoef yk, int count,
+                             long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double aux = x[i];
+    for (int k = count - 1; k >= 0; --k) aux = aux + Z[k * ldz + i] * yk.y[k];
+    x[i] = aux;
+  }
+}
+
+__global__ void k_bicg_s(double* __restrict__ nu, const double* __restrict__ r, double* __restrict__ sv, double alpha,
+                         long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double nv = nu[i] * alpha;
+    nu[i] = nv;
+    sv[i] = r[i] - nv;
+  }
+}
+
+__global__ void k_bicg_sol(const double* __restrict__ sv, double* __restrict__ y, double* __restrict__ z,
+                           double* __restrict__ x, double* __restrict__ t, double* __restrict__ r, double alpha,
+                           double w, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double yi = y[i] * alpha;
+    y[i] = yi;
+    const double zi = z[i] * w;
+    z[i] = zi;
+    const double cs = yi + zi;
+    x[i] = x[i] + cs;
+    const double ti = t[i] * w;
+    t[i] = ti;
+    r[i] = sv[i] - ti;
+  }
+}
+
+template <int BS>
+__global__ void k_bicg_p(double* __restrict__ p, const double* __restrict__ nu, const double* __restrict__ r,
+                         double coef, double beta, long n, double* partial) {
+  long lo, hi;
+  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
+  double s = 0.0;
+  for (long i = lo + threadIdx.x; i < hi; i += BS) {
+    const double cv = (p[i] - nu[i] * coef) * beta;
+    const double re = r[i];
+    p[i] = re + cv;
+    s += re * re;
+  }
+  s = block_sum<BS>(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void k_sub(const double* __restrict__ b, const double* __restrict__ z, double* __restrict__ r, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) r[i] = b[i] - z[i];
+}
+
+int grid_1d(long n, int bs = 256) {
+  long g = (n + bs - 1) / bs;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+}  // namespace
+
+// ================================================================================================
+// launchers
+// ================================================================================================
+int reduction_blocks(long n) {
+  long b = (n + RED_THREADS * 4 - 1) / (RED_THREADS * 4);
+  if (b > 1024) b = 1024;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+void csr_to_stencil7(const int* rowptr, const int* col, const double* val, Dims d, double* coef7, int* err_flag,
+                     hipStream_t s) {
+  k_csr_to_stencil7<<<grid_1d(d.n()), 256, 0, s>>>(rowptr, col, val, d.N, d.M, d.K, coef7, err_flag);
+}
+
+void galerkin(const double* coef_f, int npts_f, Dims df, double* coef27_c, Dims dc, hipStream_t s) {
+  if (npts_f == 7)
+    k_galerkin<7><<<grid_1d(dc.n(), 64), 64, 0, s>>>(coef_f, df.N, df.M, df.K, coef27_c, dc.N, dc.M, dc.K);
+  else
+    k_galerkin<27><<<grid_1d(dc.n(), 64), 64, 0, s>>>(coef_f, df.N, df.M, df.K, coef27_c, dc.N, dc.M, dc.K);
+}
+
+void diag_absmax(const double* coef, int npts, Dims d, double* partial, double* out, hipStream_t s) {
+  const int nb = reduction_blocks(d.n());
+  const double* dg = coef + (long)(npts == 7 ? 3 : 13) * d.n();
+  k_diag_absmax_partial<RED_THREADS><<<nb, RED_THREADS, 0, s>>>(dg, d.n(), partial);
+  k_max_finalize<RED_THREADS><<<1, RED_THREADS, 0, s>>>(partial, nb, out);
+}
+
+void line_factor(const double* coef, int npts, Dims d, int dir, const double* absmax, double* dinv, double* eu,
+                 hipStream_t s) {
+  const int nlines = dir == 0 ? d.M * d.K : (dir == 1 ? d.N * d.K : d.N * d.M);
+  k_line_factor<<<div_up(nlines, 64), 64, 0, s>>>(coef, npts, d.N, d.M, d.K, dir, absmax, dinv, eu);
+}
+
+void lu_factor(const double* coef27, Dims d, double* scale, int* perm, double* L, double* U, double* col_scratch,
+               hipStream_t s) {
+  k_lu_factor<<<1, 1, 0, s>>>(coef27, d.N, d.M, d.K, scale, perm, L, U, col_scratch);
+}
+
+void stencil_apply(const double* coef, int npts, Dims d, const double* x, const double* b, double* y, int mode,
+                   double* partial, hipStream_t s) {
+  const int nb = reduction_blocks(d.n());
+#define LSX_ST(NP, MD) k_stencil<NP, MD, RED_THREADS><<<nb, RED_THREADS, 0, s>>>(coef, d.N, d.M, d.K, x, b, y, partial)
+  if (npts == 7) {
+    if (mode == ST_PROD) LSX_ST(7, ST_PROD);
+    else if (mode == ST_RESID) LSX_ST(7, ST_RESID);
+    else LSX_ST(7, ST_SUBR);
+  } else {
+    if (mode == ST_PROD) LSX_ST(27, ST_PROD);
+    else if (mode == ST_RESID) LSX_ST(27, ST_RESID);
+    else LSX_ST(27, ST_SUBR);
+  }
+#undef LSX_ST
+}
+
+void restrict_full(const double* rf, Dims df, double* rc, Dims dc, hipStream_t s) {
+  k_restrict<<<grid_1d(dc.n()), 256, 0, s>>>(rf, df.N, df.M, rc, dc.N, dc.M, dc.K);
+}
+
+void prolong_copy(const double* chc, Dims dc, double* chf, double* cf, Dims df, hipStream_t s) {
+  k_prolong<<<grid_1d(df.n()), 256, 0, s>>>(chc, dc.N, dc.M, chf, cf, df.N, df.M, df.K);
+}
+
+void line_solve(const double* coef, int npts, Dims d, int dir, const double* dinv, const double* eu, const double* r,
+                double* c, double* ch, double omega, int z_head_fix, hipStream_t s) {
+  const int nlines = dir == 0 ? d.M * d.K : (dir == 1 ? d.N * d.K : d.N * d.M);
+  k_line_solve<<<div_up(nlines, 64), 64, 0, s>>>(coef, npts, d.N, d.M, d.K, dir, dinv, eu, r, c, ch, omega,
+                                                 z_head_fix);
+}
+
+void lu_solve(const double* scale, const int* perm, const double* L, const double* U, int n, const double* b,
+              double* x, hipStream_t s) {
+  k_lu_solve<<<1, 1, 0, s>>>(scale, perm, L, U, n, b, x);
+}
+
+void copy(const double* a, double* b, long n, hipStream_t s) { k_copy<<<grid_1d(n), 256, 0, s>>>(a, b, n); }
+
+void scale_copy(const double* w, const double* normw, double* v, long n, hipStream_t s) {
+  k_scale_copy<<<grid_1d(n), 256, 0, s>>>(w, normw, v, n);
+}
+
+void dot_partial(const double* a, const double* b, long n, double* partial, hipStream_t s) {
+  k_dot_partial<RED_THREADS><<<reduction_blocks(n), RED_THREADS, 0, s>>>(a, b, n, partial);
+}
+
+void finalize(const double* partial, int nb, double* out, int sqrt_flag, hipStream_t s) {
+  k_finalize<RED_THREADS><<<1, RED_THREADS, 0, s>>>(partial, nb, out, sqrt_flag);
+}
+
+void axpy_dot(double* w, const double* v, const double* h, const double* vnext, long n, double* partial,
+              hipStream_t s) {
+  k_axpy_dot<RED_THREADS><<<reduction_blocks(n), RED_THREADS, 0, s>>>(w, v, h, vnext, n, partial);
+}
+
+void sol_update(double* x, const double* Z, long ldz, const double* yk, int count, long n, hipStream_t s) {
+  // chunks of 64 coefficients, highest k first (the reference order k = last..0)
+  int hi = count;
+  while (hi > 0) {
+    const int lo = hi > 64 ? hi - 64 : 0;
+    YCoef c;
+    for (int k = lo; k < hi; ++k) c.y[k - lo] = yk[k];
+    k_sol_update<<<grid_1d(n), 256, 0, s>>>(x, Z + (long)lo * ldz, ldz, c, hi - lo, n);
+    hi = lo;
+  }
+}
+
+void bicg_s_update(double* nu, const double* r, double* sv, double alpha, long n, hipStream_t s) {
+  k_bicg_s<<<grid_1d(n), 256, 0, s>>>(nu, r, sv, alpha, n);
+}
+
+void bicg_sol_update(const double* sv, double* y, double* z, double* x, double* t, double* r, double alpha, double w,
+                     long n, hipStream_t s) {
+  k_bicg_sol<<<grid_1d(n), 256, 0, s>>>(sv, y, z, x, t, r, alpha, w, n);
+}
+
+void bicg_p_update(double* p, const double* nu, const double* r, double coef, double beta, long n, double* partial,
+                   hipStream_t s) {
+  k_bicg_p<RED_THREADS><<<reduction_blocks(n), RED_THREADS, 0, s>>>(p, nu, r, coef, beta, n, partial);
+}
+
+void sub_vec(const double* b, const double* z, double* r, long n, hipStream_t s) {
+  k_sub<<<grid_1d(n), 256, 0, s>>>(b, z, r, n);
+}
+
+}  // namespace lsx

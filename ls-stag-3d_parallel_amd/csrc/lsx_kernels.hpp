@@ -1,0 +1,70 @@
+// Host-side launchers of the HIP kernels (lsx_kernels.hip).  All launches are asynchronous
+// on the given stream; sizes are checked by the callers in lsx_solver.cpp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lsx {
+
+struct Dims {
+  int N, M, K;
+  long n() const { return (long)N * M * K; }
+};
+
+// Operator storage: npts SoA coefficient planes of n values each.
+//  npts == 7 : [b, s, w, p, e, n, t]                       (fine level, reference row order)
+//  npts == 27: index (dx+1) + 3(dy+1) + 9(dz+1)            (Galerkin coarse levels)
+enum StencilMode { ST_PROD = 0, ST_RESID = 1, ST_SUBR = 2 };
+
+// Deterministic reductions: partial sums per block into `partial`, then finalize sums them in
+// a fixed order into out[out_index] (device memory).
+constexpr int RED_THREADS = 256;
+int reduction_blocks(long n);
+
+// ---- setup --------------------------------------------------------------------------------
+// CSR (device) -> 7 stencil planes; err_flag set to 1 if a column is not a 7-point neighbour.
+void csr_to_stencil7(const int* rowptr, const int* col, const double* val, Dims d, double* coef7, int* err_flag,
+                     hipStream_t s);
+// Galerkin coarse operator A_c = R A_f P (even-even-even transfers), 27 planes on the coarse grid.
+void galerkin(const double* coef_f, int npts_f, Dims df, double* coef27_c, Dims dc, hipStream_t s);
+// max |diag| of an operator into out[0]
+void diag_absmax(const double* coef, int npts, Dims d, double* partial, double* out, hipStream_t s);
+// Thomas LU factors of the line systems of direction dir (0 x, 1 y, 2 z): dinv, eu (scaled upper)
+void line_factor(const double* coef, int npts, Dims d, int dir, const double* absmax, double* dinv, double* eu,
+                 hipStream_t s);
+// Dense LU of the coarsest operator (single thread): scale, perm, L, U (n x n row-major)
+void lu_factor(const double* coef27, Dims d, double* scale, int* perm, double* L, double* U, double* col_scratch,
+               hipStream_t s);
+
+// ---- solve phase ------------------------------------------------------------------------------
+// y = A x (ST_PROD), y = b - A x with partial sums of y^2 (ST_RESID), y -= A x (ST_SUBR)
+void stencil_apply(const double* coef, int npts, Dims d, const double* x, const double* b, double* y, int mode,
+                   double* partial, hipStream_t s);
+void restrict_full(const double* rf, Dims df, double* rc, Dims dc, hipStream_t s);
+void prolong_copy(const double* chc, Dims dc, double* chf, double* cf, Dims df, hipStream_t s);
+void line_solve(const double* coef, int npts, Dims d, int dir, const double* dinv, const double* eu, const double* r,
+                double* c, double* ch, double omega, int z_head_fix, hipStream_t s);
+void lu_solve(const double* scale, const int* perm, const double* L, const double* U, int n, const double* b,
+              double* x, hipStream_t s);
+
+// ---- BLAS-1 (FP64) ------------------------------------------------------------------------
+void copy(const double* a, double* b, long n, hipStream_t s);
+void scale_copy(const double* w, const double* normw, double* v, long n, hipStream_t s);  // v = (1/normw) w
+void dot_partial(const double* a, const double* b, long n, double* partial, hipStream_t s);
+void finalize(const double* partial, int nb, double* out, int sqrt_flag, hipStream_t s);
+// w -= h*v ; then partial dot(w, vnext) (vnext may be null => partial sum of w^2)
+void axpy_dot(double* w, const double* v, const double* h, const double* vnext, long n, double* partial,
+              hipStream_t s);
+// x += sum_k y_k z_k in the order k = last..0 (coefficients from host array, up to 64)
+void sol_update(double* x, const double* Z, long ldz, const double* yk, int count, long n, hipStream_t s);
+// BiCGStab fused updates (bicgstab_interface.h:39-56)
+void bicg_s_update(double* nu, const double* r, double* sv, double alpha, long n, hipStream_t s);
+void bicg_sol_update(const double* sv, double* y, double* z, double* x, double* t, double* r, double alpha, double w,
+                     long n, hipStream_t s);
+void bicg_p_update(double* p, const double* nu, const double* r, double coef, double beta, long n, double* partial,
+                   hipStream_t s);
+void sub_vec(const double* b, const double* z, double* r, long n, hipStream_t s);  // r = b - z
+
+}  // namespace lsx

@@ -1,0 +1,444 @@
+// Host orchestration of the pressure solve: hierarchy setup, multigrid cascade, FGMRES and
+// BiCGStab loops.  Device work is queued on the handle's stream; the host only reads the
+// scalars the Krylov recurrences need (one small copy per Arnoldi step for FGMRES).
+
+#include "lsx_solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace lsx {
+
+namespace {
+// scalar slots in `scalars`
+constexpr int SC_NORM = 0;    // residual / ||w|| results
+constexpr int SC_TINY = 1;    // max |diag| of a level during setup
+constexpr int SC_TMP = 2;     // scratch dot results
+constexpr int SC_H = 8;       // Hessenberg column entries H[hNum]
+}  // namespace
+
+Solver::Solver(const lsx_options& o, hipStream_t s) : opt(o) {
+  if (opt.restart < 1 || opt.restart > 256) throw Error(LSX_ERR_ARG, "restart must be in [1, 256]");
+  if (opt.max_it < 1) throw Error(LSX_ERR_ARG, "max_it must be >= 1");
+  if (opt.n_post < 0) throw Error(LSX_ERR_ARG, "n_post must be >= 0");
+  if (opt.krylov != LSX_KRYLOV_FGMRES && opt.krylov != LSX_KRYLOV_BICGSTAB) throw Error(LSX_ERR_ARG, "unknown krylov");
+  if (opt.mg != LSX_MG_CASCADE) throw Error(LSX_ERR_ARG, "only the cascade multigrid is available in this build");
+  if (opt.precision != LSX_PREC_FP64) throw Error(LSX_ERR_ARG, "only FP64 precision is available in this build");
+  if (s) {
+    stream = s;
+  } else {
+    LSX_HIP(hipStreamCreate(&stream));  // default priority
+    own_stream = true;
+  }
+  err.alloc(1);
+  scalars.alloc(SC_H + 256 * 257 / 2 + 8);
+  partial.alloc(2048);
+}
+
+Solver::~Solver() {
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+void Solver::synchronize() { LSX_HIP(hipStreamSynchronize(stream)); }
+
+// -----------------------------------------------------------------------------------------------
+// setup
+// -----------------------------------------------------------------------------------------------
+static int level_count(int N, int M, int K) {
+  // coarse_grids_hierarchy.h:73-81: halve the smallest dimension while it exceeds 2
+  double aux = (double)std::min(std::min(N, M), K);
+  int num = 0;
+  while (aux > 2.0) {
+    aux *= 0.5;
+    ++num;
+  }
+  return num;
+}
+
+void Solver::set_csr(int N, int M, int K, const int* rp, const int* cl, const double* vl, long nnz, int mem) {
+  if (N < 3 || M < 3 || K < 3) throw Error(LSX_ERR_ARG, "grid dimensions must be >= 3");
+  if ((long)N * M * K >= (1L << 31)) throw Error(LSX_ERR_ARG, "grid too large for 32-bit row indices");
+  if (!rp || !cl || !vl || nnz <= 0) throw Error(LSX_ERR_ARG, "null CSR array or empty matrix");
+  if (mem != LSX_MEM_HOST && mem != LSX_MEM_DEVICE) throw Error(LSX_ERR_ARG, "mem must be LSX_MEM_HOST or LSX_MEM_DEVICE");
+  const int nmg = level_count(N, M, K);
+  {
+    int n = N, m = M, k = K;
+    for (int l = 0; l < nmg; ++l) {
+      if ((n & 1) || (m & 1) || (k & 1))
+        throw Error(LSX_ERR_ARG, "every multigrid level must have even dimensions (even-even-even transfers)");
+      n /= 2;
+      m /= 2;
+      k /= 2;
+    }
+    if ((long)n * m * k > 64) throw Error(LSX_ERR_ARG, "coarsest level exceeds 64 unknowns");
+  }
+  d0 = Dims{N, M, K};
+  const long n = d0.n();
+  if (mem == LSX_MEM_HOST) {
+    // rows must be well formed before anything touches the device
+    if (rp[0] != 0 || rp[n] != nnz) throw Error(LSX_ERR_ARG, "row pointers inconsistent with nnz");
+    for (long i = 0; i < n; ++i)
+      if (rp[i + 1] < rp[i] || rp[i + 1] - rp[i] > 7) throw Error(LSX_ERR_ARG, "row pointers not a 7-point pattern");
+    for (long j = 0; j < nnz; ++j)
+      if (cl[j] < 0 || cl[j] >= n) throw Error(LSX_ERR_ARG, "column index out of range");
+  }
+  nnz0 = nnz;
+  rowptr.alloc(n + 1);
+  col.alloc(nnz);
+  val.alloc(nnz);
+  const hipMemcpyKind kind = mem == LSX_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  LSX_HIP(hipMemcpyAsync(rowptr.p, rp, (n + 1) * sizeof(int), kind, stream));
+  LSX_HIP(hipMemcpyAsync(col.p, cl, nnz * sizeof(int), kind, stream));
+  LSX_HIP(hipMemcpyAsync(val.p, vl, nnz * sizeof(double), kind, stream));
+  if (mem == LSX_MEM_DEVICE) {
+    // validate the device CSR on the host before any kernel indexes with it
+    std::vector<int> hrp(n + 1), hcl(nnz);
+    LSX_HIP(hipMemcpyAsync(hrp.data(), rp, (n + 1) * sizeof(int), hipMemcpyDeviceToHost, stream));
+    LSX_HIP(hipMemcpyAsync(hcl.data(), cl, nnz * sizeof(int), hipMemcpyDeviceToHost, stream));
+    LSX_HIP(hipStreamSynchronize(stream));
+    if (hrp[0] != 0 || hrp[n] != nnz) throw Error(LSX_ERR_ARG, "row pointers inconsistent with nnz");
+    for (long i = 0; i < n; ++i)
+      if (hrp[i + 1] < hrp[i] || hrp[i + 1] - hrp[i] > 7) throw Error(LSX_ERR_ARG, "row pointers not a 7-point pattern");
+    for (long j = 0; j < nnz; ++j)
+      if (hcl[j] < 0 || hcl[j] >= n) throw Error(LSX_ERR_ARG, "column index out of range");
+  }
+  levels.clear();
+  levels.resize(nmg + 1);
+  {
+    int nn = N, mm = M, kk = K;
+    for (int l = 0; l <= nmg; ++l) {
+      levels[l].d = Dims{nn, mm, kk};
+      levels[l].npts = l == 0 ? 7 : 27;
+      nn /= 2;
+      mm /= 2;
+      kk /= 2;
+    }
+  }
+  for (size_t l = 0; l < levels.size(); ++l) {
+    MGLevel& L = levels[l];
+    const long nl = L.d.n();
+    L.coef.alloc((size_t)L.npts * nl);
+    L.r.alloc(nl);
+    if (l > 0) L.ch.alloc(nl);
+    if (l + 1 < levels.size())
+      for (int dir = 0; dir < 3; ++dir) {
+        L.dinv[dir].alloc(nl);
+        L.eu[dir].alloc(nl);
+      }
+  }
+  cwork.alloc(n);
+  const long nc = levels.back().d.n();
+  lu_scale.alloc(nc);
+  lu_perm.alloc(nc);
+  lu_L.alloc(nc * nc);
+  lu_U.alloc(nc * nc);
+  lu_col.alloc(nc);
+  build_hierarchy();
+  alloc_krylov();
+  ready = true;
+}
+
+void Solver::update_values(const double* vl, int mem) {
+  if (!ready) throw Error(LSX_ERR_STATE, "update_values before set_csr");
+  const hipMemcpyKind kind = mem == LSX_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  LSX_HIP(hipMemcpyAsync(val.p, vl, nnz0 * sizeof(double), kind, stream));
+}
+
+void Solver::rebuild() {
+  if (!ready) throw Error(LSX_ERR_STATE, "rebuild before set_csr");
+  build_hierarchy();
+}
+
+void Solver::build_hierarchy() {
+  MGLevel& F = levels[0];
+  LSX_HIP(hipMemsetAsync(err.p, 0, sizeof(int), stream));
+  csr_to_stencil7(rowptr.p, col.p, val.p, F.d, F.coef.p, err.p, stream);
+  int herr = 0;
+  LSX_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+  LSX_HIP(hipStreamSynchronize(stream));
+  if (herr) throw Error(LSX_ERR_ARG, "matrix has a column outside the 7-point neighbourhood of its row");
+  for (size_t l = 0; l + 1 < levels.size(); ++l) {
+    MGLevel& f = levels[l];
+    MGLevel& c = levels[l + 1];
+    galerkin(f.coef.p, f.npts, f.d, c.coef.p, c.d, stream);
+    diag_absmax(f.coef.p, f.npts, f.d, partial.p, scalars.p + SC_TINY, stream);
+    for (int dir = 0; dir < 3; ++dir)
+      line_factor(f.coef.p, f.npts, f.d, dir, scalars.p + SC_TINY, f.dinv[dir].p, f.eu[dir].p, stream);
+  }
+  MGLevel& C = levels.back();
+  lu_factor(C.coef.p, C.d, lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, lu_col.p, stream);
+  LSX_HIP(hipGetLastError());
+  LSX_HIP(hipStreamSynchronize(stream));
+}
+
+void Solver::alloc_krylov() {
+  const long n = d0.n();
+  if (opt.krylov == LSX_KRYLOV_FGMRES) {
+    V.alloc((size_t)opt.restart * n);
+    Z.alloc((size_t)opt.restart * n);
+    w.alloc(n);
+    H.assign(opt.restart * (opt.restart + 1) / 2, 0.0);
+    cs.assign(opt.restart, 0.0);
+    sn.assign(opt.restart, 0.0);
+    y.assign(opt.restart, 0.0);
+  } else {
+    bnu.alloc(n);
+    bt.alloc(n);
+    by.alloc(n);
+    bz.alloc(n);
+    br.alloc(n);
+    bp.alloc(n);
+    bs.alloc(n);
+    brr.alloc(n);
+  }
+}
+
+// -----------------------------------------------------------------------------------------------
+// multigrid cascade — multigrid_solver.h:48-75
+// -----------------------------------------------------------------------------------------------
+void Solver::mg_apply(const double* v, double* z) {
+  const int L = (int)levels.size() - 1;
+  copy(v, levels[0].r.p, d0.n(), stream);
+  for (int k = 0; k < L; ++k) restrict_full(levels[k].r.p, levels[k].d, levels[k + 1].r.p, levels[k + 1].d, stream);
+  lu_solve(lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, (int)levels[L].d.n(), levels[L].r.p, levels[L].ch.p, stream);
+  for (int k = L - 1; k >= 0; --k) {
+    MGLevel& lv = levels[k];
+    double* ch = k == 0 ? z : lv.ch.p;
+    prolong_copy(levels[k + 1].ch.p, levels[k + 1].d, ch, cwork.p, lv.d, stream);
+    for (int s = 0; s < opt.n_post; ++s)
+      for (int dir = 2; dir >= 0; --dir) {  // z, y, x
+        stencil_apply(lv.coef.p, lv.npts, lv.d, cwork.p, nullptr, lv.r.p, ST_SUBR, nullptr, stream);
+        line_solve(lv.coef.p, lv.npts, lv.d, dir, lv.dinv[dir].p, lv.eu[dir].p, lv.r.p, cwork.p, ch, opt.omega,
+                   opt.z_head_fix, stream);
+      }
+  }
+}
+
+double Solver::reduce_to_host(const double* dev_scalar) {
+  double h = 0.0;
+  LSX_HIP(hipMemcpyAsync(&h, dev_scalar, sizeof(double), hipMemcpyDeviceToHost, stream));
+  LSX_HIP(hipStreamSynchronize(stream));
+  return h;
+}
+
+double Solver::dot(const double* a, const double* b, long n, double* out_dev) {
+  dot_partial(a, b, n, partial.p, stream);
+  finalize(partial.p, reduction_blocks(n), out_dev, 0, stream);
+  return reduce_to_host(out_dev);
+}
+
+// -----------------------------------------------------------------------------------------------
+// FGMRES(m) — fgmres_interface.h:78-142, solUpdate :153-168
+// -----------------------------------------------------------------------------------------------
+int Solver::fgmres(const double* b, double* x) {
+  const long n = d0.n();
+  const int m = opt.restart;
+  const int m_size = m * (m + 1) / 2;
+  const int nb = reduction_blocks(n);
+  double* dnorm = scalars.p + SC_NORM;
+  double* dH = scalars.p + SC_H;
+  const double eps = opt.eps;
+  const MGLevel& F = levels[0];
+
+  auto sol_upd = [&](int last, int lastH) {
+    std::vector<double> ycoef(last + 1);
+    int hNum = lastH;
+    for (int k = last; k > -1; --k, --hNum) {
+      const double curY = y[k] / H[hNum];
+      for (int i = k - 1; i > -1; --i) {
+        --hNum;
+        y[i] -= curY * H[hNum];
+      }
+      ycoef[k] = curY;
+    }
+    sol_update(x, Z.p, n, ycoef.data(), last + 1, n, stream);
+  };
+
+  stencil_apply(F.coef.p, 7, d0, x, b, w.p, ST_RESID, partial.p, stream);
+  finalize(partial.p, nb, dnorm, 1, stream);
+  norm_r = reduce_to_host(dnorm);
+  if (!(norm_r > eps)) return 0;
+  int iters = 1;
+  double normW = norm_r;
+  double curH = norm_r;
+  std::vector<double> hcol(m + 2);
+  for (;;) {
+    // normW lives on the device for the scaling kernel
+    LSX_HIP(hipMemcpyAsync(dnorm, &normW, sizeof(double), hipMemcpyHostToDevice, stream));
+    for (int j = 0, hColStart = 0, hNum = 0; j < m; ++j, hColStart = hNum, ++iters) {
+      double* vj = V.p + (size_t)j * n;
+      double* zj = Z.p + (size_t)j * n;
+      scale_copy(w.p, dnorm, vj, n, stream);
+      mg_apply(vj, zj);
+      stencil_apply(F.coef.p, 7, d0, zj, nullptr, w.p, ST_PROD, nullptr, stream);
+      // modified Gram-Schmidt: h_i = w . v_i ; w -= h_i v_i  (the dot of the next basis vector
+      // is fused into the update pass; the last pass yields ||w||^2)
+      dot_partial(w.p, V.p, n, partial.p, stream);
+      finalize(partial.p, nb, dH + hColStart, 0, stream);
+      for (int i = 0; i <= j; ++i) {
+        const double* vi = V.p + (size_t)i * n;
+        const double* vnext = i < j ? V.p + (size_t)(i + 1) * n : nullptr;
+        axpy_dot(w.p, vi, dH + hColStart + i, vnext, n, partial.p, stream);
+        if (vnext)
+          finalize(partial.p, nb, dH + hColStart + i + 1, 0, stream);
+        else
+          finalize(partial.p, nb, dnorm, 1, stream);
+      }
+      hNum = hColStart + j + 1;
+      LSX_HIP(hipMemcpyAsync(hcol.data(), dH + hColStart, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, stream));
+      LSX_HIP(hipMemcpyAsync(&normW, dnorm, sizeof(double), hipMemcpyDeviceToHost, stream));
+      LSX_HIP(hipStreamSynchronize(stream));
+      for (int i = 0; i <= j; ++i) H[hColStart + i] = hcol[i];
+      curH = H[hColStart];
+      for (int i = 0, cur = hColStart, nxt = cur + 1; i < j; ++i, cur = nxt, ++nxt) {
+        const double cc = cs[i], ss = sn[i], nextH = H[nxt];
+        H[cur] = cc * curH - ss * nextH;
+        curH = ss * curH + cc * nextH;
+      }
+      double curC = curH * curH + normW * normW;
+      const double curA = std::sqrt(curC);
+      curC = 1.0 / curA;
+      const double curS = -normW * curC;
+      curC *= curH;
+      H[hNum - 1] = curA;
+      y[j] = curC * norm_r;
+      norm_r *= curS;
+      curH = std::fabs(norm_r);
+      if ((curH > eps) && (iters < opt.max_it)) {
+        cs[j] = curC;
+        sn[j] = curS;
+      } else {
+        norm_r = curH;
+        sol_upd(j, hNum - 1);
+        return iters;
+      }
+    }
+    sol_upd(m - 1, m_size - 1);
+    // w = b - A x starts the next cycle; its norm is the carried estimate |gamma|
+    stencil_apply(F.coef.p, 7, d0, x, b, w.p, ST_RESID, partial.p, stream);
+    normW = curH;
+  }
+}
+
+// -----------------------------------------------------------------------------------------------
+// BiCGStab — bicgstab_interface.h:76-111
+// -----------------------------------------------------------------------------------------------
+int Solver::bicgstab(const double* b, double* x) {
+  const long n = d0.n();
+  const int nb = reduction_blocks(n);
+  double* tmp = scalars.p + SC_TMP;
+  double* dnorm = scalars.p + SC_NORM;
+  const MGLevel& F = levels[0];
+  int iters = 1;
+  stencil_apply(F.coef.p, 7, d0, x, nullptr, bz.p, ST_PROD, nullptr, stream);
+  sub_vec(b, bz.p, br.p, n, stream);
+  norm_r = std::sqrt(dot(br.p, br.p, n, tmp));
+  if (norm_r > opt.eps) {
+    copy(br.p, brr.p, n, stream);
+    copy(br.p, bp.p, n, stream);
+    do {
+      mg_apply(bp.p, by.p);
+      stencil_apply(F.coef.p, 7, d0, by.p, nullptr, bnu.p, ST_PROD, nullptr, stream);
+      double vs = dot(brr.p, br.p, n, tmp);
+      double coef = dot(brr.p, bnu.p, n, tmp);
+      const double alpha = vs / coef;
+      bicg_s_update(bnu.p, br.p, bs.p, alpha, n, stream);
+      mg_apply(bs.p, bz.p);
+      stencil_apply(F.coef.p, 7, d0, bz.p, nullptr, bt.p, ST_PROD, nullptr, stream);
+      double wv = dot(bt.p, bs.p, n, tmp);
+      coef = dot(bt.p, bt.p, n, tmp);
+      wv /= coef;
+      bicg_sol_update(bs.p, by.p, bz.p, x, bt.p, br.p, alpha, wv, n, stream);
+      coef = wv / alpha;
+      vs *= coef;
+      double beta = dot(brr.p, br.p, n, tmp);
+      beta /= vs;
+      bicg_p_update(bp.p, bnu.p, br.p, coef, beta, n, partial.p, stream);
+      finalize(partial.p, nb, dnorm, 1, stream);
+      norm_r = reduce_to_host(dnorm);
+      ++iters;
+    } while ((norm_r > opt.eps) && (iters < opt.max_it));
+  }
+  return iters;
+}
+
+// -----------------------------------------------------------------------------------------------
+// public operations
+// -----------------------------------------------------------------------------------------------
+void Solver::solve(const double* b, double* x, int mem, int* it_out, double* nr_out) {
+  if (!ready) throw Error(LSX_ERR_STATE, "solve before set_csr");
+  if (!b || !x) throw Error(LSX_ERR_ARG, "null vector");
+  const long n = d0.n();
+  const double* db = b;
+  double* dx = x;
+  if (mem == LSX_MEM_HOST) {
+    hb.alloc(n);
+    hx.alloc(n);
+    LSX_HIP(hipMemcpyAsync(hb.p, b, n * sizeof(double), hipMemcpyHostToDevice, stream));
+    LSX_HIP(hipMemcpyAsync(hx.p, x, n * sizeof(double), hipMemcpyHostToDevice, stream));
+    db = hb.p;
+    dx = hx.p;
+  }
+  it = opt.krylov == LSX_KRYLOV_FGMRES ? fgmres(db, dx) : bicgstab(db, dx);
+  if (mem == LSX_MEM_HOST) LSX_HIP(hipMemcpyAsync(x, dx, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+  LSX_HIP(hipGetLastError());
+  LSX_HIP(hipStreamSynchronize(stream));
+  if (it_out) *it_out = it;
+  if (nr_out) *nr_out = norm_r;
+}
+
+void Solver::precondition(const double* v, double* z, int mem) {
+  if (!ready) throw Error(LSX_ERR_STATE, "precondition before set_csr");
+  const long n = d0.n();
+  if (mem == LSX_MEM_HOST) {
+    hb.alloc(n);
+    hx.alloc(n);
+    LSX_HIP(hipMemcpyAsync(hb.p, v, n * sizeof(double), hipMemcpyHostToDevice, stream));
+    mg_apply(hb.p, hx.p);
+    LSX_HIP(hipMemcpyAsync(z, hx.p, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+  } else {
+    mg_apply(v, z);
+  }
+  LSX_HIP(hipGetLastError());
+  LSX_HIP(hipStreamSynchronize(stream));
+}
+
+void Solver::apply_operator(int level, const double* x, double* yv, int mem) {
+  if (!ready) throw Error(LSX_ERR_STATE, "apply_operator before set_csr");
+  if (level < 0 || level >= (int)levels.size()) throw Error(LSX_ERR_ARG, "level out of range");
+  const MGLevel& L = levels[level];
+  const long n = L.d.n();
+  if (mem == LSX_MEM_HOST) {
+    hb.alloc(d0.n());
+    hx.alloc(d0.n());
+    LSX_HIP(hipMemcpyAsync(hb.p, x, n * sizeof(double), hipMemcpyHostToDevice, stream));
+    stencil_apply(L.coef.p, L.npts, L.d, hb.p, nullptr, hx.p, ST_PROD, nullptr, stream);
+    LSX_HIP(hipMemcpyAsync(yv, hx.p, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+  } else {
+    stencil_apply(L.coef.p, L.npts, L.d, x, nullptr, yv, ST_PROD, nullptr, stream);
+  }
+  LSX_HIP(hipGetLastError());
+  LSX_HIP(hipStreamSynchronize(stream));
+}
+
+long Solver::level_dims(int level, int* N, int* M, int* K) const {
+  if (level < 0 || level >= (int)levels.size()) throw Error(LSX_ERR_ARG, "level out of range");
+  const Dims& d = levels[level].d;
+  if (N) *N = d.N;
+  if (M) *M = d.M;
+  if (K) *K = d.K;
+  return d.n();
+}
+
+int Solver::level_stencil(int level, double* host, long capacity) const {
+  if (level < 0 || level >= (int)levels.size()) throw Error(LSX_ERR_ARG, "level out of range");
+  const MGLevel& L = levels[level];
+  const long need = (long)L.npts * L.d.n();
+  if (capacity < need) throw Error(LSX_ERR_ARG, "capacity too small for the level stencil");
+  LSX_HIP(hipMemcpyAsync(host, L.coef.p, need * sizeof(double), hipMemcpyDeviceToHost, stream));
+  LSX_HIP(hipStreamSynchronize(stream));
+  return L.npts;
+}
+
+}  // namespace lsx

@@ -1,0 +1,120 @@
+// Host-side solver: owns the device multigrid hierarchy and Krylov workspace of one handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/lsx.h"
+#include "lsx_kernels.hpp"
+
+namespace lsx {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define LSX_HIP(expr)                                                                                   \
+  do {                                                                                                  \
+    hipError_t e_ = (expr);                                                                             \
+    if (e_ != hipSuccess)                                                                               \
+      throw ::lsx::Error(LSX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_) + " at " +      \
+                                          __FILE__ + ":" + std::to_string(__LINE__));                   \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count) LSX_HIP(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct MGLevel {
+  Dims d{0, 0, 0};
+  int npts = 7;
+  DevBuf<double> coef;          // npts planes
+  DevBuf<double> dinv[3], eu[3];  // line LU factors per direction
+  DevBuf<double> r, ch;         // level right-hand side / correction
+};
+
+class Solver {
+ public:
+  Solver(const lsx_options& opt, hipStream_t stream);
+  ~Solver();
+
+  void set_csr(int N, int M, int K, const int* rowptr, const int* col, const double* val, long nnz, int mem);
+  void update_values(const double* val, int mem);
+  void rebuild();
+  void solve(const double* b, double* x, int mem, int* it, double* norm_r);
+  void precondition(const double* v, double* z, int mem);
+  void apply_operator(int level, const double* x, double* y, int mem);
+  long level_dims(int level, int* N, int* M, int* K) const;
+  int level_stencil(int level, double* host, long capacity) const;
+  void synchronize();
+
+  lsx_options opt;
+  int it = 0;
+  double norm_r = 0.0;
+  int num_levels() const { return (int)levels.size(); }
+
+ private:
+  void build_hierarchy();
+  void alloc_krylov();
+  void mg_apply(const double* v_dev, double* z_dev);
+  int fgmres(const double* b, double* x);
+  int bicgstab(const double* b, double* x);
+  double reduce_to_host(const double* dev_scalar);
+  double dot(const double* a, const double* b, long n, double* out_dev);
+
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  Dims d0{0, 0, 0};
+  long nnz0 = 0;
+  DevBuf<int> rowptr, col;
+  DevBuf<double> val;
+  DevBuf<int> err;
+  std::vector<MGLevel> levels;  // levels.back() is solved by the dense LU
+  DevBuf<double> lu_scale, lu_L, lu_U, lu_col;
+  DevBuf<int> lu_perm;
+  DevBuf<double> cwork;   // correction increment c (fine size, reused on every level)
+  DevBuf<double> partial, scalars;
+  // Krylov workspace
+  DevBuf<double> V, Z, w;                             // FGMRES
+  DevBuf<double> bnu, bt, by, bz, br, bp, bs, brr;   // BiCGStab
+  DevBuf<double> hb, hx;                              // staging for host inputs
+  std::vector<double> H, cs, sn, y;
+  bool ready = false;
+};
+
+}  // namespace lsx

@@ -1,0 +1,208 @@
+"""Python mirror of the reference solver interface over the liblsx C ABI.
+
+``MGFGMRES`` / ``MGBiCGStab`` follow the reference's IterativeSolverInterface
+(src/Base/Solvers/iterative_solver_interface.h:31-73): ``set(A, rhs, sol, task)`` stores the
+matrix and builds the multigrid preconditioner, ``solve()`` updates ``sol`` in place and
+returns the iteration count, ``rebuildPreconditioner()`` rebuilds the hierarchy after a
+geometry change, and the public attributes ``eps``, ``norm_r`` and ``it`` mirror the
+reference members.  The numerical work runs in liblsx.so (HIP kernels for gfx950); there is no
+CPU fallback: loading fails loudly if the library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblsx.so")
+
+KRYLOV_FGMRES, KRYLOV_BICGSTAB = 0, 1
+MG_CASCADE, MG_VCYCLE = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+MEM_HOST, MEM_DEVICE = 0, 1
+
+
+class LsxError(RuntimeError):
+    pass
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("krylov", C.c_int), ("restart", C.c_int), ("max_it", C.c_int), ("eps", C.c_double),
+        ("mg", C.c_int), ("n_post", C.c_int), ("n_pre", C.c_int), ("omega", C.c_double),
+        ("precision", C.c_int), ("levels", C.c_int), ("z_head_fix", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load liblsx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LsxError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.lsx_last_error.restype = C.c_char_p
+    L.lsx_options_default.argtypes = [C.POINTER(Options)]
+    L.lsx_options_default.restype = None
+    L.lsx_create.argtypes = [C.POINTER(vp), C.POINTER(Options), vp]
+    L.lsx_destroy.argtypes = [vp]
+    L.lsx_set_csr.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_long, C.c_int]
+    L.lsx_update_values.argtypes = [vp, vp, C.c_int]
+    L.lsx_rebuild.argtypes = [vp]
+    L.lsx_solve.argtypes = [vp, vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.lsx_set_eps.argtypes = [vp, C.c_double]
+    L.lsx_get_stats.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.lsx_precondition.argtypes = [vp, vp, vp, C.c_int]
+    L.lsx_apply_operator.argtypes = [vp, C.c_int, vp, vp, C.c_int]
+    L.lsx_level_dims.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.lsx_level_dims.restype = C.c_long
+    L.lsx_level_stencil.argtypes = [vp, C.c_int, vp, C.c_long]
+    L.lsx_synchronize.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def _check(status):
+    if status != 0:
+        raise LsxError(f"liblsx status {status}: {lib().lsx_last_error().decode()}")
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+class Task:
+    """The solver fields of the reference BaseTaskDescription (base_task_description.h:41-46)."""
+
+    def __init__(self, eps=1e-5, MAX_ITER=200, FGMRES_RESTART_NUM=13, n_post=3, omega=0.805):
+        self.eps = eps
+        self.MAX_ITER = MAX_ITER
+        self.FGMRES_RESTART_NUM = FGMRES_RESTART_NUM
+        self.n_post = n_post
+        self.omega = omega
+
+
+class IterativeSolver:
+    """Base of MGFGMRES / MGBiCGStab (reference IterativeSolverInterface)."""
+
+    krylov = KRYLOV_FGMRES
+
+    def __init__(self, precision=PREC_FP64, mg=MG_CASCADE, z_head_fix=False, stream=None):
+        self.precision = precision
+        self.mg = mg
+        self.z_head_fix = z_head_fix
+        self.stream = stream
+        self.h = None
+        self.eps = 0.0
+        self.norm_r = 0.0
+        self.it = 1
+        self._rhs = None
+        self._sol = None
+
+    # -- reference interface --------------------------------------------------------------
+    def set(self, A, rhs, sol, task):
+        """A: object with dims (N, M, K) and CSR arrays rowptr/col/val (reference SparseMatrix)."""
+        self.close()
+        opt = Options()
+        lib().lsx_options_default(C.byref(opt))
+        opt.krylov = self.krylov
+        opt.restart = int(task.FGMRES_RESTART_NUM)
+        opt.max_it = int(task.MAX_ITER)
+        opt.eps = float(task.eps)
+        opt.n_post = int(task.n_post)
+        opt.omega = float(task.omega)
+        opt.precision = self.precision
+        opt.mg = self.mg
+        opt.z_head_fix = 1 if self.z_head_fix else 0
+        h = C.c_void_p()
+        _check(lib().lsx_create(C.byref(h), C.byref(opt), self.stream))
+        self.h = h
+        self.eps = float(task.eps)
+        N, M, K = A.dims
+        self._A = A
+        rp = np.ascontiguousarray(A.rowptr, dtype=np.int32)
+        cl = np.ascontiguousarray(A.col, dtype=np.int32)
+        vl = np.ascontiguousarray(A.val, dtype=np.float64)
+        _check(lib().lsx_set_csr(self.h, N, M, K, _ptr(rp), _ptr(cl), _ptr(vl), len(vl), MEM_HOST))
+        self._rhs = rhs
+        self._sol = sol
+
+    def solve(self):
+        b = np.ascontiguousarray(self._rhs, dtype=np.float64)
+        if not (isinstance(self._sol, np.ndarray) and self._sol.dtype == np.float64 and self._sol.flags.c_contiguous):
+            raise LsxError("sol must be a contiguous float64 array (updated in place)")
+        _check(lib().lsx_set_eps(self.h, self.eps))
+        it = C.c_int(0)
+        nr = C.c_double(0.0)
+        _check(lib().lsx_solve(self.h, _ptr(b), _ptr(self._sol), MEM_HOST, C.byref(it), C.byref(nr)))
+        self.it = it.value
+        self.norm_r = nr.value
+        return self.it
+
+    def rebuildPreconditioner(self):
+        _check(lib().lsx_rebuild(self.h))
+
+    # -- extras used by tests and benchmarks -----------------------------------------------
+    def update_values(self, val):
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        _check(lib().lsx_update_values(self.h, _ptr(v), MEM_HOST))
+
+    def precondition(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        z = np.zeros_like(v)
+        _check(lib().lsx_precondition(self.h, _ptr(v), _ptr(z), MEM_HOST))
+        return z
+
+    def apply_operator(self, x, level=0):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        N, M, K = self.level_dims(level)
+        y = np.zeros(N * M * K)
+        _check(lib().lsx_apply_operator(self.h, level, _ptr(x), _ptr(y), MEM_HOST))
+        return y
+
+    def num_levels(self):
+        it, nr, lv = C.c_int(), C.c_double(), C.c_int()
+        _check(lib().lsx_get_stats(self.h, C.byref(it), C.byref(nr), C.byref(lv)))
+        return lv.value
+
+    def level_dims(self, level):
+        N, M, K = C.c_int(), C.c_int(), C.c_int()
+        r = lib().lsx_level_dims(self.h, level, C.byref(N), C.byref(M), C.byref(K))
+        if r < 0:
+            _check(int(r))
+        return N.value, M.value, K.value
+
+    def level_stencil(self, level):
+        N, M, K = self.level_dims(level)
+        n = N * M * K
+        buf = np.zeros(27 * n)
+        npts = lib().lsx_level_stencil(self.h, level, _ptr(buf), len(buf))
+        if npts < 0:
+            _check(npts)
+        return buf[: npts * n].reshape(npts, n)
+
+    def close(self):
+        if self.h:
+            lib().lsx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MGFGMRES(IterativeSolver):
+    """Reference MGFGMRES (src/Base/Solvers/FGMRES/mg_fgmres.h:33-52)."""
+    krylov = KRYLOV_FGMRES
+
+
+class MGBiCGStab(IterativeSolver):
+    """Reference MGBiCGStab (src/Base/Solvers/BiCGStab/mg_bicgstab.h:33-52)."""
+    krylov = KRYLOV_BICGSTAB

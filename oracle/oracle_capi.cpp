@@ -1,6 +1,8 @@
 // ORACLE — test infrastructure only (see oracle_solver.cpp header).
 // Plain C entry points so tests/ can drive the restatement through ctypes.
 
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -88,6 +90,49 @@ int lsxo_geom_get_fields(void* p, int* type, double* thx, double* thy, double* t
   if (vof) std::memcpy(vof, h->g.vof.data(), n * sizeof(double));
   return 0;
 }
+
+// Cut-cell geometry of a sphere (phi = r - |x - c|, flow_sphere_task_description.h:57-74)
+// or an axis-aligned ellipsoid (phi = 1 - |(x - c)/a|, scaled by min(a) so it is a distance
+// near the surface) inside the box x in [hs,hf), y in [vs,vf).
+int lsxo_geom_build_body(void* p, int kind, double cx, double cy, double cz, double a, double b, double c, int hs,
+                         int hf, int vs, int vf) {
+  auto* h = static_cast<GeomH*>(p);
+  try {
+    Box box{hs, hf, vs, vf};
+    if (kind == 0) {
+      build_cut_geometry(h->g, box, [=](double x, double y, double z) {
+        const double qx = x - cx, qy = y - cy, qz = z - cz;
+        return a - std::sqrt(qx * qx + qy * qy + qz * qz);
+      });
+    } else {
+      const double s = std::min(a, std::min(b, c));
+      build_cut_geometry(h->g, box, [=](double x, double y, double z) {
+        const double qx = (x - cx) / a, qy = (y - cy) / b, qz = (z - cz) / c;
+        return s * (1.0 - std::sqrt(qx * qx + qy * qy + qz * qz));
+      });
+    }
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  return 0;
+}
+
+int lsxo_geom_get_level_set(void* p, double* phi, double* ax, double* ay, double* az, int* sx, int* sy, int* sz) {
+  auto* h = static_cast<GeomH*>(p);
+  const long n = h->g.ncells();
+  if (h->g.phi.size() != (size_t)n) return -1;
+  std::memcpy(phi, h->g.phi.data(), n * sizeof(double));
+  std::memcpy(ax, h->g.alpha[0].data(), n * sizeof(double));
+  std::memcpy(ay, h->g.alpha[1].data(), n * sizeof(double));
+  std::memcpy(az, h->g.alpha[2].data(), n * sizeof(double));
+  std::memcpy(sx, h->g.side[0].data(), n * sizeof(int));
+  std::memcpy(sy, h->g.side[1].data(), n * sizeof(int));
+  std::memcpy(sz, h->g.side[2].data(), n * sizeof(int));
+  return 0;
+}
+
+int lsxo_vof_table_size() { return vof_table_size(); }
 
 long lsxo_geom_build_laplace(void* p, int pin) {
   auto* h = static_cast<GeomH*>(p);

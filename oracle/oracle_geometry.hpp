@@ -1,6 +1,7 @@
 // ORACLE — test infrastructure only (see oracle_solver.cpp header).
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "oracle_internal.hpp"
@@ -29,6 +30,9 @@ struct Geometry {
   std::vector<int> type;
   std::vector<double> thx, thy, thz, vof;
   std::vector<double> inv_mx, inv_my, inv_mz;  // inverse staggered masses on x/y/z faces
+  std::vector<double> phi;                     // level set at each cell's upper corner (> 0 in the body)
+  std::vector<double> alpha[3];                // fluid fraction of the x/y/z edges ending at that corner
+  std::vector<int> side[3];                    // type of the east/north/top face (corner code)
   long ncells() const { return (long)N * M * K; }
   long idx(int x, int y, int z) const { return x + (long)N * (y + (long)M * z); }
   void init_grid(const std::vector<double>& dxs, const std::vector<double>& dys, const std::vector<double>& dzs,
@@ -36,7 +40,20 @@ struct Geometry {
   void build_masses();
 };
 
+// Region where cut cells may appear: x in [hs,hf), y in [vs,vf), all z planes.
+struct Box {
+  int hs, hf, vs, vf;
+};
+
 int boundary_type(int x, int y, int z, int N, int M, int K);
+void corner_level_set(Geometry& g, const Box& box, const std::function<double(double, double, double)>& phi_fn);
+void build_alpha(Geometry& g, const Box& box);
+void build_types(Geometry& g, const Box& box);
+void build_theta(Geometry& g, const Box& box);
+void build_vof(Geometry& g);
+void build_cut_geometry(Geometry& g, const Box& box, const std::function<double(double, double, double)>& phi_fn);
+int vof_table_size();
+std::vector<double> cell_centres(const std::vector<double>& widths, double origin);
 Csr build_laplace(const Geometry& g, bool pin_outlet);
 void body_free_vof(Geometry& g);
 

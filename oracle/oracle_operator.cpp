@@ -179,8 +179,8 @@ Csr build_laplace(const Geometry& g, bool pin_outlet) {
 }
 
 void body_free_vof(Geometry& g) {
-  // ls_stag_uniform_mesh.h:1838-1853: fluid and border cells keep the full volume
-  // (B/T borders scale by theta_z, which is 1 without a body).
+  // ls_stag_uniform_mesh.h:1838-1853, 2713: fluid and border cells keep the full volume
+  // (B/T borders scale by theta_z, which is 1 without a body), plus PLUS_TINY.
   for (int z = 0; z < g.K; ++z)
     for (int y = 0; y < g.M; ++y)
       for (int x = 0; x < g.N; ++x) {
@@ -188,7 +188,7 @@ void body_free_vof(Geometry& g) {
         double vol = g.dx[x] * g.dy[y] * g.dz[z];
         if (g.type[i] == B_Border || g.type[i] == T_Border) vol *= g.thz[i];
         if (g.type[i] == SolidCell) vol = 0.0;
-        g.vof[i] = vol;
+        g.vof[i] = vol + PLUS_TINY;  // ls_stag_uniform_mesh.h:2713
       }
 }
 

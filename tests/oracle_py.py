@@ -48,6 +48,9 @@ def lib():
     L.lsxo_mg_level_csr.argtypes = [C.c_void_p, C.c_int, PI, PI, PD]
     L.lsxo_mg_level_adlj.argtypes = [C.c_void_p, C.c_int, C.c_int, PD, PD, PD]
     L.lsxo_spmv.argtypes = [C.c_long, PI, PI, PD, PD, PD]
+    L.lsxo_geom_build_body.argtypes = [C.c_void_p, C.c_int] + [C.c_double] * 6 + [C.c_int] * 4
+    L.lsxo_geom_get_level_set.argtypes = [C.c_void_p, PD, PD, PD, PD, PI, PI, PI]
+    L.lsxo_vof_table_size.restype = C.c_int
     _lib = L
     return L
 
@@ -114,6 +117,25 @@ class Geometry:
 
         lib().lsxo_geom_set_fields(self.h, ptr(type, np.int32), ptr(thx, np.float64), ptr(thy, np.float64),
                                    ptr(thz, np.float64), ptr(vof, np.float64))
+
+    def build_body(self, kind, center, axes, box=None):
+        """Cut-cell geometry of a sphere (kind 'sphere', axes = (r,)) or an axis-aligned
+        ellipsoid (kind 'ellipsoid', axes = semi-axes) inside box (hs, hf, vs, vf)."""
+        N, M, K = self.dims
+        hs, hf, vs, vf = box or (1, N - 1, 1, M - 1)
+        a = tuple(axes) + (axes[0],) * (3 - len(axes))
+        st = lib().lsxo_geom_build_body(self.h, 0 if kind == "sphere" else 1, *center, *a, hs, hf, vs, vf)
+        if st != 0:
+            raise RuntimeError(lib().lsxo_last_error().decode())
+        return self
+
+    def level_set(self):
+        n = self.n
+        phi, ax, ay, az = (np.zeros(n) for _ in range(4))
+        sx, sy, sz = (np.zeros(n, np.int32) for _ in range(3))
+        if lib().lsxo_geom_get_level_set(self.h, phi, ax, ay, az, sx, sy, sz) != 0:
+            raise RuntimeError("no level set built")
+        return dict(phi=phi, alpha=(ax, ay, az), side=(sx, sy, sz))
 
     def laplace(self, pin=True):
         nnz = lib().lsxo_geom_build_laplace(self.h, 1 if pin else 0)
