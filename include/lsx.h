@@ -107,8 +107,65 @@ long lsx_level_dims(const lsx_solver* s, int level, int* N, int* M, int* K);
  * (coarse_grid.h:103-104).  Returns npts or a negative status. */
 int lsx_level_stencil(const lsx_solver* s, int level, double* coef_host, long capacity);
 
+/* Copy the diagonal of the fine-level operator (N*M*K values) to host memory.  Rows whose
+ * diagonal is 0 are cut cells with every face closed; a consistent right-hand side is 0 there. */
+int lsx_operator_diagonal(const lsx_solver* s, double* diag_host);
+
 /* Synchronise the handle's stream. */
 int lsx_synchronize(lsx_solver* s);
+
+/* Kernel timing (no reference counterpart; the reference's profiler times host sections,
+ * profiler.h:53-274).  While enabled, the fine-level launches of each kernel class below are
+ * bracketed by HIP events on the handle's stream.  lsx_kernel_stats reports the launch count,
+ * the summed device time and the kernel's algorithmic HBM bytes per launch (compulsory
+ * traffic: each array read or written once). */
+#define LSX_K_RESIDUAL 0   /* r -= A c inside the smoother                 */
+#define LSX_K_LINE_X 1     /* x-line solve, c = w T^-1 r, ch += c           */
+#define LSX_K_LINE_Y 2
+#define LSX_K_LINE_Z 3
+#define LSX_K_OPERATOR 4   /* w = A z in the Krylov loop                    */
+#define LSX_K_ORTHO 5      /* fused w -= h v_i ; h' = w . v_{i+1}           */
+#define LSX_K_SMOOTH 6     /* fused smoother sweep (V-cycle)                */
+#define LSX_K_COUNT 8
+int lsx_kernel_timing(lsx_solver* s, int enable);
+int lsx_kernel_stats(lsx_solver* s, int kernel, long* launches, double* total_ms, double* bytes_per_launch);
+
+/* ---- cut-cell geometry of the pressure grid (built on the GPU) ------------------------------
+ * Replaces the reference's mesh-function build for a static body: LSSTAGUniformMesh nodes and
+ * widths (ls_stag_uniform_mesh.h:174-185), the level set (flow_sphere_task_description.h:57-74),
+ * edge fractions, face/cell types and face fractions (FlowTaskDescription::calcAlpha /
+ * calcTypeOfCells / calcTheta, flow_task_description.h:102-241), cut volumes and staggered
+ * masses (LSSTAGUniformMesh::buildMeshFunctions, ls_stag_uniform_mesh.h:4047-4062, 1838-2826). */
+typedef struct lsx_geometry lsx_geometry;
+
+#define LSX_BODY_SPHERE 0    /* phi = r - |x - c|                                  */
+#define LSX_BODY_ELLIPSOID 1 /* phi = min(a) (1 - |(x - c) / a|), axis-aligned axes a */
+
+typedef struct lsx_body {
+  int kind;
+  double center[3];
+  double axes[3]; /* sphere: axes[0] = radius */
+} lsx_body;
+
+/* Tensor grid with per-cell widths dx[N], dy[M], dz[K] (host arrays) and lower corner
+ * (x0, y0, z0).  A uniform axis places centres at x0 + (k + 1/2) h as the reference does. */
+int lsx_geometry_create(lsx_geometry** out, int N, int M, int K, const double* dx, const double* dy,
+                        const double* dz, double x0, double y0, double z0, void* stream);
+int lsx_geometry_destroy(lsx_geometry* g);
+/* Build every field for `body`; cut cells may appear for x in [hs,hf), y in [vs,vf) (all z),
+ * the reference's geometry box (1 <= hs < hf <= N-1, 1 <= vs < vf <= M-1). */
+int lsx_geometry_build(lsx_geometry* g, const lsx_body* body, int hs, int hf, int vs, int vf);
+/* Copy fields to host arrays of N*M*K entries (any pointer may be NULL): cell type codes,
+ * east/north/top face fractions, cut volumes, upper-corner level set. */
+int lsx_geometry_fields(const lsx_geometry* g, int* type, double* thx, double* thy, double* thz, double* vof,
+                        double* phi);
+
+/* Set the solver's matrix to the cut-cell pressure operator of `g` (built on the device, no
+ * CSR round trip) and build the preconditioner; pin != 0 turns rows (N-1,0,0) and
+ * (N-1,M-1,0) into identity rows.  Replaces LaplaceDiscreteOperator::build
+ * (laplace_discrete_operator.h:752-768, calcLap :107-686, fixPoint :809-821) followed by
+ * IterativeSolverInterface::set. */
+int lsx_set_geometry(lsx_solver* s, const lsx_geometry* g, int pin);
 
 /* Thread-local message of the last failure ("" if none). */
 const char* lsx_last_error(void);

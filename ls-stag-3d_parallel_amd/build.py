@@ -29,6 +29,8 @@ COMMON = [
 SOURCES = {
     "lsx_kernels.hip": ["-ffp-contract=off"],
     "lsx_solver.cpp": ["-ffp-contract=off"],
+    "lsx_geometry.hip": ["-ffp-contract=off"],
+    "lsx_geometry_host.cpp": ["-ffp-contract=off"],
     "lsx_capi.cpp": [],
 }
 

@@ -6,11 +6,19 @@
 #include <string>
 
 #include "../../include/lsx.h"
+#include "lsx_geometry_host.hpp"
 #include "lsx_solver.hpp"
 
 struct lsx_solver {
   lsx::Solver impl;
   lsx_solver(const lsx_options& o, hipStream_t s) : impl(o, s) {}
+};
+
+struct lsx_geometry {
+  lsx::Geometry impl;
+  lsx_geometry(int N, int M, int K, const double* dx, const double* dy, const double* dz, double x0, double y0,
+               double z0, hipStream_t s)
+      : impl(N, M, K, dx, dy, dz, x0, y0, z0, s) {}
 };
 
 namespace {
@@ -128,11 +136,64 @@ int lsx_level_stencil(const lsx_solver* s, int level, double* coef_host, long ca
   return st == LSX_OK ? r : st;
 }
 
+int lsx_operator_diagonal(const lsx_solver* s, double* diag_host) {
+  if (!s || !diag_host) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.operator_diagonal(diag_host); });
+}
+
 int lsx_synchronize(lsx_solver* s) {
   if (!s) return LSX_ERR_ARG;
   return guarded([&] { s->impl.synchronize(); });
 }
 
 const char* lsx_last_error(void) { return g_last_error.c_str(); }
+
+int lsx_geometry_create(lsx_geometry** out, int N, int M, int K, const double* dx, const double* dy,
+                        const double* dz, double x0, double y0, double z0, void* stream) {
+  if (!out) {
+    g_last_error = "null argument";
+    return LSX_ERR_ARG;
+  }
+  *out = nullptr;
+  return guarded([&] { *out = new lsx_geometry(N, M, K, dx, dy, dz, x0, y0, z0, (hipStream_t)stream); });
+}
+
+int lsx_geometry_destroy(lsx_geometry* g) {
+  return guarded([&] { delete g; });
+}
+
+int lsx_geometry_build(lsx_geometry* g, const lsx_body* body, int hs, int hf, int vs, int vf) {
+  if (!g || !body) return LSX_ERR_ARG;
+  return guarded([&] {
+    lsx::geo::Body b;
+    b.kind = body->kind;
+    for (int q = 0; q < 3; ++q) {
+      b.c[q] = body->center[q];
+      b.a[q] = body->axes[q];
+    }
+    g->impl.build(b, lsx::geo::Box{hs, hf, vs, vf});
+  });
+}
+
+int lsx_geometry_fields(const lsx_geometry* g, int* type, double* thx, double* thy, double* thz, double* vof,
+                        double* phi) {
+  if (!g) return LSX_ERR_ARG;
+  return guarded([&] { g->impl.fields(type, thx, thy, thz, vof, phi); });
+}
+
+int lsx_kernel_timing(lsx_solver* s, int enable) {
+  if (!s) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.set_timing(enable != 0); });
+}
+
+int lsx_kernel_stats(lsx_solver* s, int kernel, long* launches, double* total_ms, double* bytes_per_launch) {
+  if (!s) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.kernel_stats(kernel, launches, total_ms, bytes_per_launch); });
+}
+
+int lsx_set_geometry(lsx_solver* s, const lsx_geometry* g, int pin) {
+  if (!s || !g) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.set_geometry(g->impl, pin); });
+}
 
 }  // extern "C"

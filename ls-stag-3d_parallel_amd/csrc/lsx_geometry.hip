@@ -1,0 +1,594 @@
+// Cut-cell geometry of the pressure grid and the pressure operator's coefficients, on the GPU.
+//
+// One thread per node/cell for every stage except the level-set oscillation fix, whose outcome
+// depends on the order nodes are visited (reference ls_stag_uniform_mesh.h:2906-2916 inside
+// the serial sweep flow_task_description.h:102-137).  A node can only change there if its
+// level set is positive and one of its upper neighbours is negative, so a flag pass marks
+// those few nodes and a single workgroup replays the checks for them in index order; every
+// other node keeps its value and all later stages read final values only.
+//
+// Built with -ffp-contract=off: every formula is evaluated with the operation order of the
+// reference expressions it restates.
+
+#include <hip/hip_runtime.h>
+
+#include "lsx_geometry.hpp"
+
+namespace lsx {
+namespace geo {
+
+namespace {
+
+constexpr double PLUS_TINY = 1e-16;
+constexpr double EQ_DOUBLE = 1e-5;
+constexpr double THETA_EQ_ZERO = 0.05;
+constexpr double EQ_TINY_DOUBLE = 1e-12;
+constexpr double ONE_THIRD = 1.0 / 3.0;
+constexpr int FLUID_SIDE = 1111;
+
+__device__ inline void decode(long i, int N, int M, int* x, int* y, int* z) {
+  const long NM = (long)N * M;
+  *z = (int)(i / NM);
+  const long r = i - (long)*z * NM;
+  *y = (int)(r / N);
+  *x = (int)(r - (long)*y * N);
+}
+
+__device__ inline bool in_box(int x, int y, const Box& b) { return x >= b.hs && x < b.hf && y >= b.vs && y < b.vf; }
+
+// ---- level set -------------------------------------------------------------------------------
+__global__ void k_level_set(const double* __restrict__ cx, const double* __restrict__ cy, const double* __restrict__ cz,
+                            int N, int M, int K, Box box, Body body, double* __restrict__ phi) {
+  const long n = (long)N * M * K;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    double v = 0.0;
+    if (in_box(x, y, box)) {
+      if (body.kind == BODY_SPHERE) {
+        const double qx = cx[x] - body.c[0], qy = cy[y] - body.c[1], qz = cz[z] - body.c[2];
+        v = body.a[0] - sqrt(qx * qx + qy * qy + qz * qz);
+      } else {
+        const double s = fmin(body.a[0], fmin(body.a[1], body.a[2]));
+        const double qx = (cx[x] - body.c[0]) / body.a[0], qy = (cy[y] - body.c[1]) / body.a[1],
+                     qz = (cz[z] - body.c[2]) / body.a[2];
+        v = s * (1.0 - sqrt(qx * qx + qy * qy + qz * qz));
+      }
+    }
+    phi[i] = v;
+  }
+}
+
+__global__ void k_osc_flags(const double* __restrict__ phi, int N, int M, int K, Box box, unsigned char* __restrict__ flag) {
+  const long n = (long)N * M * K, NM = (long)N * M;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    unsigned char f = 0;
+    if (in_box(x, y, box) && phi[i] > EQ_DOUBLE) {
+      const bool mid = z > 0 && z < K - 1;
+      f = (phi[i + 1] < 0) || (phi[i + N] < 0) || (mid && phi[i + NM] < 0);
+    }
+    flag[i] = f;
+  }
+}
+
+__device__ inline void check_osc(double* phi, long i, double& f, double lower, double upper) {
+  if ((f > EQ_DOUBLE) && (lower < 0) && (upper < 0)) {
+    f = 0.5 * (lower + upper);
+    phi[i] = f;
+  }
+}
+
+// One workgroup walks the flags in index order; thread 0 replays the checks of flagged nodes.
+__global__ void k_osc_replay(double* __restrict__ phi, const unsigned char* __restrict__ flag, int N, int M, int K) {
+  const long n = (long)N * M * K, NM = (long)N * M;
+  __shared__ int any;
+  for (long base = 0; base < n; base += blockDim.x) {
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    const long i = base + threadIdx.x;
+    if (i < n && flag[i]) atomicOr(&any, 1);
+    __syncthreads();
+    if (any && threadIdx.x == 0) {
+      const long end = base + blockDim.x < n ? base + blockDim.x : n;
+      for (long j = base; j < end; ++j) {
+        if (!flag[j]) continue;
+        const int z = (int)(j / NM);
+        double f = phi[j];
+        check_osc(phi, j, f, phi[j - 1], phi[j + 1]);
+        check_osc(phi, j, f, phi[j - N], phi[j + N]);
+        if (z > 0 && z < K - 1) check_osc(phi, j, f, phi[j - NM], phi[j + NM]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__device__ inline double edge_fraction(double f, double fo) {
+  return (f * fo < 0) ? ((f > 0) ? (fo / (fo - f)) : (f / (f - fo))) : ((f > -fo) ? 0.0 : 1.0);
+}
+
+__global__ void k_alpha(const double* __restrict__ phi, int N, int M, int K, Box box, double* __restrict__ ax,
+                        double* __restrict__ ay, double* __restrict__ az) {
+  const long n = (long)N * M * K, NM = (long)N * M;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    double a0 = 1.0, a1 = 1.0, a2 = 1.0;
+    if (in_box(x, y, box)) {
+      const double f = phi[i];
+      a0 = edge_fraction(f, phi[i - 1]);
+      a1 = edge_fraction(f, phi[i - N]);
+      a2 = (z == 0 || z == K - 1) ? ((f > 0.0) ? 0.0 : 1.0) : edge_fraction(f, phi[i - NM]);
+    }
+    ax[i] = a0;
+    ay[i] = a1;
+    az[i] = a2;
+  }
+}
+
+// ---- types -----------------------------------------------------------------------------------
+__device__ inline int type_of_side(double phi, double phiW, double phiSW, double phiS) {
+  int number = (phiS < EQ_DOUBLE) ? 1 : 0;
+  if (phiSW < EQ_DOUBLE) number += 10;
+  if (phiW < EQ_DOUBLE) number += 100;
+  if (phi < EQ_DOUBLE) number += 1000;
+  return number;
+}
+
+__device__ inline int border_type(int x, int y, int z, int N, int M, int K) {
+  const bool w = x == 0, e = x == N - 1, s = y == 0, nn = y == M - 1, b = z == 0, t = z == K - 1;
+  if (b) {
+    if (s) return w ? BSW_Corner : (e ? BSE_Corner : BS_Edge);
+    if (nn) return w ? BNW_Corner : (e ? BNE_Corner : BN_Edge);
+    return w ? BW_Edge : (e ? BE_Edge : B_Border);
+  }
+  if (t) {
+    if (s) return w ? TSW_Corner : (e ? TSE_Corner : TS_Edge);
+    if (nn) return w ? TNW_Corner : (e ? TNE_Corner : TN_Edge);
+    return w ? TW_Edge : (e ? TE_Edge : T_Border);
+  }
+  if (s) return w ? SW_Edge : (e ? SE_Edge : S_Border);
+  if (nn) return w ? NW_Edge : (e ? NE_Edge : N_Border);
+  if (w) return W_Border;
+  if (e) return E_Border;
+  return FluidCell;
+}
+
+__global__ void k_types(const double* __restrict__ p, int N, int M, int K, Box box, int* __restrict__ type,
+                        int* __restrict__ sx, int* __restrict__ sy, int* __restrict__ sz) {
+  const long n = (long)N * M * K, NM = (long)N * M;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    int t = border_type(x, y, z, N, M, K);
+    int s0 = FLUID_SIDE, s1 = FLUID_SIDE, s2 = FLUID_SIDE;
+    if (in_box(x, y, box)) {
+      const double phi = p[i], phiW = p[i - 1], phiS = p[i - N];
+      const int tT = type_of_side(phi, phiW, p[i - N - 1], phiS);
+      if (z == 0 || z == K - 1) {
+        s0 = type_of_side(phi, phiS, phiS, phi);
+        s1 = type_of_side(phi, phiW, phiW, phi);
+        s2 = tT;
+        t = (tT == 0) ? SolidCell : (z == 0 ? B_Border : T_Border);
+      } else {
+        const double phiSB = p[i - N - NM], phiB = p[i - NM], phiBW = p[i - NM - 1];
+        s0 = type_of_side(phi, phiS, phiSB, phiB);
+        s1 = type_of_side(phi, phiW, phiBW, phiB);
+        s2 = tT;
+        t = tT * 10000 + type_of_side(phiB, phiBW, p[i - N - 1 - NM], phiSB);
+      }
+    }
+    type[i] = t;
+    sx[i] = s0;
+    sy[i] = s1;
+    sz[i] = s2;
+  }
+}
+
+// ---- face fractions ---------------------------------------------------------------------------
+__device__ inline double face_theta(int side, double P0, double P1, double Q0, double Q1) {
+  double t;
+  switch (side) {
+    case FLUID_SIDE: return 1.0;
+    case 0: return 0.0;
+    case 11: case 1100: t = 0.5 * (P0 + P1); break;
+    case 1001: case 110: t = 0.5 * (Q0 + Q1); break;
+    case 1011: t = 1.0 - 0.5 * (1.0 - P1) * (1.0 - Q0); break;
+    case 111: t = 1.0 - 0.5 * (1.0 - P0) * (1.0 - Q0); break;
+    case 1101: t = 1.0 - 0.5 * (1.0 - P1) * (1.0 - Q1); break;
+    case 1110: t = 1.0 - 0.5 * (1.0 - P0) * (1.0 - Q1); break;
+    case 1: t = 0.5 * P0 * Q1; break;
+    case 10: t = 0.5 * P1 * Q1; break;
+    case 1000: t = 0.5 * P0 * Q0; break;
+    case 100: t = 0.5 * P1 * Q0; break;
+    default: return 1.0;
+  }
+  return t < THETA_EQ_ZERO ? 0.0 : t;
+}
+
+__global__ void k_theta(const double* __restrict__ ax, const double* __restrict__ ay, const double* __restrict__ az,
+                        const int* __restrict__ sx, const int* __restrict__ sy, const int* __restrict__ sz, int N, int M,
+                        int K, Box box, double* __restrict__ tx, double* __restrict__ ty, double* __restrict__ tz) {
+  const long n = (long)N * M * K, NM = (long)N * M;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    double t0 = 1.0, t1 = 1.0, t2 = 1.0;
+    if (in_box(x, y, box)) {
+      t2 = face_theta(sz[i], ay[i], ay[i - 1], ax[i], ax[i - N]);
+      if (z == 0 || z == K - 1) {
+        const double a = ay[i], b = ax[i];
+        t0 = a < THETA_EQ_ZERO ? 0.0 : a;
+        t1 = b < THETA_EQ_ZERO ? 0.0 : b;
+      } else {
+        t0 = face_theta(sx[i], az[i], az[i - N], ay[i], ay[i - NM]);
+        t1 = face_theta(sy[i], az[i], az[i - 1], ax[i], ax[i - NM]);
+      }
+    }
+    tx[i] = t0;
+    ty[i] = t1;
+    tz[i] = t2;
+  }
+}
+
+// ---- cut volumes --------------------------------------------------------------------------------
+// Formula families of the reference volume table; each cut code names a family and the
+// alpha/theta entries (component, corner offset) it reads.
+enum Fam : unsigned char { F_TET, F_HEP3, F_PRISM, F_HEP5, F_HEX5, F_HEP6 };
+struct Ref {
+  signed char dx, dy, dz, comp;
+};
+struct Rule {
+  int code;
+  unsigned char fam, axis, sel, pad;
+  Ref a, b, a2, b2;
+};
+
+#define R_(dx, dy, dz, c) Ref{dx, dy, dz, c}
+#define NONE R_(0, 0, 0, 0)
+// sorted by code for binary search
+__constant__ Rule c_rules[] = {
+    {1, F_TET, 0, 0, 0, R_(0, -1, -1, 0), R_(0, 0, 0, 0), NONE, NONE},
+    {10, F_TET, 0, 0, 0, R_(0, -1, -1, 0), R_(-1, 0, 0, 0), NONE, NONE},
+    {11, F_PRISM, 0, 0, 0, NONE, NONE, NONE, NONE},
+    {100, F_TET, 0, 0, 0, R_(0, 0, -1, 0), R_(-1, 0, 0, 0), NONE, NONE},
+    {110, F_PRISM, 1, 0, 0, NONE, NONE, NONE, NONE},
+    {111, F_HEX5, 1, 1, 0, R_(0, 0, -1, 0), R_(0, 0, 0, 1), NONE, NONE},
+    {1000, F_TET, 0, 0, 0, R_(0, 0, -1, 0), R_(0, 0, 0, 0), NONE, NONE},
+    {1001, F_PRISM, 1, 0, 0, NONE, NONE, NONE, NONE},
+    {1011, F_HEX5, 1, 1, 0, R_(0, 0, -1, 0), R_(-1, 0, 0, 1), NONE, NONE},
+    {1100, F_PRISM, 0, 0, 0, NONE, NONE, NONE, NONE},
+    {1101, F_HEX5, 1, 0, 0, R_(0, -1, -1, 0), R_(-1, 0, 0, 1), NONE, NONE},
+    {1110, F_HEX5, 1, 0, 0, R_(0, -1, -1, 0), R_(0, 0, 0, 1), NONE, NONE},
+    {1111, F_PRISM, 0, 0, 0, NONE, NONE, NONE, NONE},
+    {10000, F_TET, 0, 0, 0, R_(0, -1, 0, 0), R_(0, 0, 0, 0), NONE, NONE},
+    {10001, F_PRISM, 2, 0, 0, NONE, NONE, NONE, NONE},
+    {10011, F_HEX5, 0, 0, 0, R_(-1, -1, 0, 2), R_(0, -1, 0, 0), NONE, NONE},
+    {11001, F_HEX5, 2, 1, 0, R_(0, 0, 0, 1), R_(0, 0, 0, 2), NONE, NONE},
+    {11011, F_HEP6, 0, 1, 0, R_(0, 0, 0, 2), R_(0, 0, -1, 0), R_(0, 0, 0, 1), R_(0, -1, 0, 0)},
+    {11111, F_HEP5, 1, 0, 0, R_(0, -1, 0, 0), R_(-1, 0, 0, 1), NONE, NONE},
+    {100000, F_TET, 0, 0, 0, R_(0, -1, 0, 0), R_(-1, 0, 0, 0), NONE, NONE},
+    {100010, F_PRISM, 2, 0, 0, NONE, NONE, NONE, NONE},
+    {100011, F_HEX5, 0, 1, 0, R_(0, -1, 0, 2), R_(0, -1, 0, 0), NONE, NONE},
+    {100110, F_HEX5, 2, 1, 0, R_(-1, 0, 0, 1), R_(-1, 0, 0, 2), NONE, NONE},
+    {100111, F_HEP6, 0, 0, 0, R_(-1, 0, 0, 2), R_(0, 0, -1, 0), R_(-1, 0, 0, 1), R_(0, -1, 0, 0)},
+    {101111, F_HEP5, 1, 0, 0, R_(0, -1, 0, 0), R_(0, 0, 0, 1), NONE, NONE},
+    {110000, F_PRISM, 0, 0, 0, NONE, NONE, NONE, NONE},
+    {110001, F_HEX5, 0, 0, 0, R_(-1, -1, 0, 2), R_(0, -1, -1, 0), NONE, NONE},
+    {110010, F_HEX5, 0, 1, 0, R_(0, -1, 0, 2), R_(0, -1, -1, 0), NONE, NONE},
+    {110011, F_PRISM, 0, 0, 0, NONE, NONE, NONE, NONE},
+    {110111, F_HEP5, 0, 0, 0, R_(-1, 0, 0, 2), R_(0, 0, -1, 0), NONE, NONE},
+    {111011, F_HEP5, 0, 1, 0, R_(0, 0, 0, 2), R_(0, 0, -1, 2), NONE, NONE},
+    {111111, F_PRISM, 0, 0, 0, NONE, NONE, NONE, NONE},
+    {1000000, F_TET, 0, 0, 0, R_(0, 0, 0, 0), R_(-1, 0, 0, 0), NONE, NONE},
+    {1000100, F_PRISM, 2, 0, 0, NONE, NONE, NONE, NONE},
+    {1000110, F_HEX5, 2, 1, 0, R_(-1, 0, 0, 1), R_(-1, -1, 0, 2), NONE, NONE},
+    {1001100, F_HEX5, 0, 1, 0, R_(0, 0, 0, 2), R_(0, 0, 0, 0), NONE, NONE},
+    {1001110, F_HEP6, 0, 0, 0, R_(-1, -1, 0, 2), R_(0, -1, -1, 0), R_(-1, 0, 0, 1), R_(0, 0, 0, 0)},
+    {1001111, F_HEP5, 1, 1, 0, R_(0, 0, 0, 0), R_(-1, 0, 0, 1), NONE, NONE},
+    {1100000, F_PRISM, 1, 0, 0, NONE, NONE, NONE, NONE},
+    {1100010, F_HEX5, 2, 0, 0, R_(-1, 0, -1, 1), R_(-1, 0, 0, 2), NONE, NONE},
+    {1100100, F_HEX5, 2, 0, 0, R_(-1, 0, -1, 1), R_(-1, -1, 0, 2), NONE, NONE},
+    {1100110, F_PRISM, 1, 0, 0, NONE, NONE, NONE, NONE},
+    {1100111, F_HEP5, 2, 0, 0, R_(0, 0, -1, 1), R_(0, -1, 0, 2), NONE, NONE},
+    {1101110, F_HEP5, 2, 0, 0, R_(0, 0, -1, 1), R_(0, 0, 0, 2), NONE, NONE},
+    {1101111, F_PRISM, 1, 0, 0, NONE, NONE, NONE, NONE},
+    {1110000, F_HEX5, 1, 1, 0, R_(0, 0, 0, 0), R_(0, 0, 0, 1), NONE, NONE},
+    {1110010, F_HEP6, 0, 0, 0, R_(-1, 0, 0, 2), R_(0, 0, 0, 0), R_(-1, 0, -1, 1), R_(0, -1, -1, 0)},
+    {1110011, F_HEP5, 0, 0, 0, R_(-1, 0, 0, 2), R_(0, 0, 0, 0), NONE, NONE},
+    {1110110, F_HEP5, 2, 1, 0, R_(0, 0, 0, 1), R_(0, -1, 0, 2), NONE, NONE},
+    {1110111, F_PRISM, 2, 0, 0, NONE, NONE, NONE, NONE},
+    {1111111, F_HEP3, 0, 0, 0, R_(0, 0, 0, 0), R_(0, 0, 0, 0), NONE, NONE},
+    {10000000, F_TET, 0, 0, 0, R_(0, 0, 0, 0), R_(0, 0, 0, 0), NONE, NONE},
+    {10001000, F_PRISM, 2, 0, 0, NONE, NONE, NONE, NONE},
+    {10001001, F_HEX5, 2, 1, 0, R_(0, 0, 0, 1), R_(0, -1, 0, 2), NONE, NONE},
+    {10001100, F_HEX5, 0, 0, 0, R_(-1, 0, 0, 2), R_(0, 0, 0, 0), NONE, NONE},
+    {10001101, F_HEP6, 0, 1, 0, R_(0, -1, 0, 2), R_(0, -1, -1, 0), R_(0, 0, 0, 1), R_(0, 0, 0, 0)},
+    {10001111, F_HEP5, 1, 1, 0, R_(0, 0, 0, 0), R_(0, 0, 0, 1), NONE, NONE},
+    {10010000, F_PRISM, 1, 0, 0, NONE, NONE, NONE, NONE},
+    {10010001, F_HEX5, 2, 0, 0, R_(0, 0, -1, 1), R_(0, 0, 0, 2), NONE, NONE},
+    {10011000, F_HEX5, 2, 0, 0, R_(0, 0, -1, 1), R_(0, -1, 0, 2), NONE, NONE},
+    {10011001, F_PRISM, 1, 0, 0, NONE, NONE, NONE, NONE},
+    {10011011, F_HEP5, 2, 0, 0, R_(-1, 0, -1, 1), R_(-1, -1, 0, 2), NONE, NONE},
+    {10011101, F_HEP5, 2, 0, 0, R_(-1, 0, -1, 1), R_(-1, 0, 0, 2), NONE, NONE},
+    {10011111, F_PRISM, 1, 0, 0, NONE, NONE, NONE, NONE},
+    {10110000, F_HEX5, 1, 1, 0, R_(0, 0, 0, 0), R_(-1, 0, 0, 1), NONE, NONE},
+    {10110001, F_HEP6, 0, 1, 0, R_(0, 0, 0, 2), R_(0, 0, 0, 0), R_(0, 0, -1, 1), R_(0, -1, -1, 0)},
+    {10110011, F_HEP5, 0, 1, 0, R_(0, 0, 0, 2), R_(0, 0, 0, 0), NONE, NONE},
+    {10111001, F_HEP5, 2, 1, 0, R_(-1, 0, 0, 1), R_(-1, -1, 0, 2), NONE, NONE},
+    {10111011, F_PRISM, 2, 0, 0, NONE, NONE, NONE, NONE},
+    {10111111, F_HEP3, 0, 0, 0, R_(0, 0, 0, 0), R_(-1, 0, 0, 0), NONE, NONE},
+    {11000000, F_PRISM, 0, 0, 0, NONE, NONE, NONE, NONE},
+    {11000100, F_HEX5, 0, 1, 0, R_(0, 0, 0, 2), R_(0, 0, -1, 0), NONE, NONE},
+    {11001000, F_HEX5, 0, 0, 0, R_(-1, 0, 0, 2), R_(0, 0, -1, 0), NONE, NONE},
+    {11001100, F_PRISM, 0, 0, 0, NONE, NONE, NONE, NONE},
+    {11001101, F_HEP5, 0, 1, 0, R_(0, -1, 0, 2), R_(0, -1, -1, 0), NONE, NONE},
+    {11001110, F_HEP5, 0, 0, 0, R_(-1, -1, 0, 2), R_(0, -1, -1, 0), NONE, NONE},
+    {11001111, F_PRISM, 0, 0, 0, NONE, NONE, NONE, NONE},
+    {11010000, F_HEX5, 1, 0, 0, R_(0, -1, 0, 0), R_(-1, 0, 0, 1), NONE, NONE},
+    {11011000, F_HEP6, 0, 1, 0, R_(0, -1, 0, 2), R_(0, 0, -1, 0), R_(0, 0, -1, 1), R_(0, -1, 0, 0)},
+    {11011001, F_HEP5, 2, 1, 0, R_(-1, 0, 0, 1), R_(-1, 0, 0, 2), NONE, NONE},
+    {11011100, F_HEP5, 0, 1, 0, R_(0, -1, 0, 2), R_(0, -1, 0, 0), NONE, NONE},
+    {11011101, F_PRISM, 2, 0, 0, NONE, NONE, NONE, NONE},
+    {11011111, F_HEP3, 0, 0, 0, R_(0, -1, 0, 0), R_(-1, 0, 0, 0), NONE, NONE},
+    {11100000, F_HEX5, 1, 0, 0, R_(0, -1, 0, 0), R_(0, 0, 0, 1), NONE, NONE},
+    {11100100, F_HEP6, 0, 0, 0, R_(-1, -1, 0, 2), R_(0, -1, 0, 0), R_(-1, 0, -1, 1), R_(0, 0, -1, 0)},
+    {11100110, F_HEP5, 2, 1, 0, R_(0, 0, 0, 1), R_(0, 0, 0, 2), NONE, NONE},
+    {11101100, F_HEP5, 0, 0, 0, R_(-1, -1, 0, 2), R_(0, -1, 0, 0), NONE, NONE},
+    {11101110, F_PRISM, 2, 0, 0, NONE, NONE, NONE, NONE},
+    {11101111, F_HEP3, 0, 0, 0, R_(0, -1, 0, 0), R_(0, 0, 0, 0), NONE, NONE},
+    {11110000, F_PRISM, 0, 0, 0, NONE, NONE, NONE, NONE},
+    {11110001, F_HEP5, 1, 0, 0, R_(0, -1, -1, 0), R_(0, 0, -1, 1), NONE, NONE},
+    {11110010, F_HEP5, 1, 0, 0, R_(0, -1, -1, 0), R_(-1, 0, -1, 1), NONE, NONE},
+    {11110011, F_PRISM, 0, 0, 0, NONE, NONE, NONE, NONE},
+    {11110100, F_HEP5, 1, 1, 0, R_(0, 0, -1, 0), R_(-1, 0, -1, 1), NONE, NONE},
+    {11110110, F_PRISM, 1, 0, 0, NONE, NONE, NONE, NONE},
+    {11110111, F_HEP3, 0, 0, 0, R_(0, 0, -1, 0), R_(0, 0, 0, 0), NONE, NONE},
+    {11111000, F_HEP5, 1, 1, 0, R_(0, 0, -1, 0), R_(0, 0, -1, 1), NONE, NONE},
+    {11111001, F_PRISM, 1, 0, 0, NONE, NONE, NONE, NONE},
+    {11111011, F_HEP3, 0, 0, 0, R_(0, 0, -1, 0), R_(-1, 0, 0, 0), NONE, NONE},
+    {11111100, F_PRISM, 0, 0, 0, NONE, NONE, NONE, NONE},
+    {11111101, F_HEP3, 0, 0, 0, R_(0, -1, -1, 0), R_(-1, 0, 0, 0), NONE, NONE},
+    {11111110, F_HEP3, 0, 0, 0, R_(0, -1, -1, 0), R_(0, 0, 0, 0), NONE, NONE},
+};
+#undef NONE
+#undef R_
+constexpr int N_RULES = sizeof(c_rules) / sizeof(Rule);
+
+__device__ inline int find_rule(int code) {
+  int lo = 0, hi = N_RULES - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int c = c_rules[mid].code;
+    if (c == code) return mid;
+    if (c < code) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  return -1;
+}
+
+struct Fields {
+  const double* a[3];
+  const double* t[3];
+  int N, NM;
+};
+
+__device__ inline double at(const double* const* arr, const Fields& F, long i, Ref r) {
+  return arr[(int)r.comp][i + r.dx + (long)F.N * r.dy + (long)F.NM * r.dz];
+}
+
+__global__ void k_vof(const int* __restrict__ type, Fields F, const double* __restrict__ dx,
+                      const double* __restrict__ dy, const double* __restrict__ dz, int N, int M, int K,
+                      double* __restrict__ vof) {
+  const long n = (long)N * M * K;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    const double fullV = dx[x] * dy[y] * dz[z];
+    double vol = fullV;
+    const int t = type[i];
+    if (t == B_Border || t == T_Border) {
+      vol *= F.t[2][i];
+    } else if (t == SolidCell) {
+      vol = 0.0;
+    } else if (t > 0 && t != FluidCell) {
+      const int k = find_rule(t);
+      if (k >= 0) {
+        const Rule r = c_rules[k];
+        const long lowOff = r.axis == 0 ? 1 : (r.axis == 1 ? F.N : F.NM);
+        const double tUp = F.t[r.axis][i], tLo = F.t[r.axis][i - lowOff];
+        switch (r.fam) {
+          case F_TET:
+            vol = at(F.a, F, i, r.a) * at(F.t, F, i, r.b) * fullV * ONE_THIRD;
+            break;
+          case F_HEP3:
+            vol = (3.0 - (1.0 - at(F.a, F, i, r.a)) * (1.0 - at(F.t, F, i, r.b))) * fullV * ONE_THIRD;
+            break;
+          case F_PRISM:
+            vol = (tUp + tLo) * fullV * 0.5;
+            break;
+          case F_HEP5: {
+            const double tsel = r.sel ? tUp : tLo;
+            double ap = at(F.a, F, i, r.a);
+            const double denom = 1.0 - ap;
+            double term = 0.0;
+            if (denom > EQ_DOUBLE) {
+              ap /= denom;
+              term = (1.0 - tsel) * (3.0 - 2.0 * at(F.a, F, i, r.b)) * ap * ap * ONE_THIRD;
+            }
+            vol = (tLo + tUp + term) * fullV * 0.5;
+            break;
+          }
+          case F_HEX5: {
+            const double denom = at(F.a, F, i, r.a);
+            double term2 = r.sel ? tUp : tLo;
+            const double other = r.sel ? tLo : tUp;
+            if (denom > EQ_DOUBLE) {
+              const double coef = (1.0 - denom) / denom;
+              term2 *= (1.0 - (1.0 + 2.0 * at(F.a, F, i, r.b)) * coef * coef * ONE_THIRD);
+            }
+            vol = (other + term2) * fullV * 0.5;
+            break;
+          }
+          default: {  // F_HEP6
+            const double tsel = r.sel ? tUp : tLo;
+            const double other = r.sel ? tLo : tUp;
+            double term = 0.0;
+            double al = at(F.a, F, i, r.a);
+            double denom = 1.0 - al;
+            if (denom > EQ_DOUBLE) {
+              al /= denom;
+              term = (3.0 - 2.0 * at(F.a, F, i, r.b)) * al * al;
+            }
+            al = at(F.a, F, i, r.a2);
+            denom = 1.0 - al;
+            if (denom > EQ_DOUBLE) {
+              al /= denom;
+              term += (3.0 - 2.0 * at(F.a, F, i, r.b2)) * al * al;
+            }
+            vol = (other + tsel + (1.0 - tsel) * term * ONE_THIRD) * fullV * 0.5;
+          }
+        }
+      }
+    }
+    vof[i] = vol + PLUS_TINY;
+  }
+}
+
+// ---- masses and operator coefficients -------------------------------------------------------------
+__device__ inline double inv_mass(double v) { return (v < EQ_TINY_DOUBLE) ? 0.0 : (1.0 / v); }
+
+__device__ inline bool no_b(int t) { return t == B_Border || t == BN_Edge || t == BS_Edge || t == BE_Edge || t == BW_Edge || t == BNE_Corner || t == BNW_Corner || t == BSE_Corner || t == BSW_Corner; }
+__device__ inline bool no_t(int t) { return t == T_Border || t == TN_Edge || t == TS_Edge || t == TE_Edge || t == TW_Edge || t == TNE_Corner || t == TNW_Corner || t == TSE_Corner || t == TSW_Corner; }
+__device__ inline bool no_s(int t) { return t == S_Border || t == TS_Edge || t == BS_Edge || t == SE_Edge || t == SW_Edge || t == TSE_Corner || t == TSW_Corner || t == BSE_Corner || t == BSW_Corner; }
+__device__ inline bool no_n(int t) { return t == N_Border || t == TN_Edge || t == BN_Edge || t == NE_Edge || t == NW_Edge || t == TNE_Corner || t == TNW_Corner || t == BNE_Corner || t == BNW_Corner; }
+__device__ inline bool no_w(int t) { return t == W_Border || t == TW_Edge || t == BW_Edge || t == NW_Edge || t == SW_Edge || t == TNW_Corner || t == TSW_Corner || t == BNW_Corner || t == BSW_Corner; }
+__device__ inline bool no_e(int t) { return t == E_Border || t == TE_Edge || t == BE_Edge || t == NE_Edge || t == SE_Edge || t == TNE_Corner || t == TSE_Corner || t == BNE_Corner || t == BSE_Corner; }
+
+// Pressure operator row of each cell as 7 coefficient planes [b,s,w,p,e,n,t] (absent = 0):
+// the values of LaplaceDiscreteOperator::calcLap (laplace_discrete_operator.h:107-686) with the
+// staggered masses M = (V_i + V_nb)/2 inlined, and identity rows at the two pinned cells.
+__global__ void k_laplace7(const int* __restrict__ type, const double* __restrict__ tx, const double* __restrict__ ty,
+                           const double* __restrict__ tz, const double* __restrict__ vof,
+                           const double* __restrict__ dxs, const double* __restrict__ dys,
+                           const double* __restrict__ dzs, int N, int M, int K, int pin, double* __restrict__ coef) {
+  const long n = (long)N * M * K, NM = (long)N * M;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    const int t = type[i];
+    double a_b = 0, a_s = 0, a_w = 0, a_p = 0, a_e = 0, a_n = 0, a_t = 0;
+    if (t == SolidCell) {
+      a_p = PLUS_TINY;
+    } else {
+      const double v = vof[i];
+      const double ddx = dxs[x], ddy = dys[y], ddz = dzs[z];
+      const bool hb = !no_b(t), hs = !no_s(t), hw = !no_w(t), he = !no_e(t), hn = !no_n(t), ht = !no_t(t);
+      // masses of the six faces (only the present ones are read)
+      const double imw = hw ? inv_mass(0.5 * (vof[i - 1] + v)) : 0.0;
+      const double ime = he ? inv_mass(0.5 * (v + vof[i + 1])) : 0.0;
+      const double ims = hs ? inv_mass(0.5 * (vof[i - N] + v)) : 0.0;
+      const double imn = hn ? inv_mass(0.5 * (v + vof[i + N])) : 0.0;
+      const double imb = hb ? inv_mass(0.5 * (vof[i - NM] + v)) : 0.0;
+      const double imt = ht ? inv_mass(0.5 * (v + vof[i + NM])) : 0.0;
+      if (t == FluidCell) {
+        const double dyz = ddy * ddz, dxz = ddx * ddz, dxy = ddy * ddx;
+        double aux = dyz * dyz;
+        a_w = aux * imw;
+        a_e = aux * ime;
+        aux = dxz * dxz;
+        a_s = aux * ims;
+        a_n = aux * imn;
+        aux = dxy * dxy;
+        a_b = aux * imb;
+        a_t = aux * imt;
+        a_p = -a_b - a_s - a_w - a_e - a_n - a_t;
+      } else {
+        const double dyz = ddy * ddz, dxz = ddx * ddz, dxy = ddx * ddy;
+        double aux;
+        if (hw) { aux = tx[i - 1] * dyz; a_w = aux * aux * imw; }
+        if (he) { aux = tx[i] * dyz; a_e = aux * aux * ime; }
+        if (hs) { aux = ty[i - N] * dxz; a_s = aux * aux * ims; }
+        if (hn) { aux = ty[i] * dxz; a_n = aux * aux * imn; }
+        if (hb) { aux = tz[i - NM] * dxy; a_b = aux * aux * imb; }
+        if (ht) { aux = tz[i] * dxy; a_t = aux * aux * imt; }
+        bool first = true;
+        if (hb) { a_p = -a_b; first = false; }
+        if (hs) { a_p = first ? -a_s : a_p - a_s; first = false; }
+        if (hw) { a_p = first ? -a_w : a_p - a_w; first = false; }
+        if (he) { a_p = first ? -a_e : a_p - a_e; first = false; }
+        if (hn) { a_p = first ? -a_n : a_p - a_n; first = false; }
+        if (ht) { a_p = first ? -a_t : a_p - a_t; first = false; }
+      }
+      if (pin && (i == NM - 1 || i == N - 1)) {
+        a_b = a_s = a_w = a_e = a_n = a_t = 0.0;
+        a_p = 1.0;
+      }
+    }
+    coef[0 * n + i] = a_b;
+    coef[1 * n + i] = a_s;
+    coef[2 * n + i] = a_w;
+    coef[3 * n + i] = a_p;
+    coef[4 * n + i] = a_e;
+    coef[5 * n + i] = a_n;
+    coef[6 * n + i] = a_t;
+  }
+}
+
+int grid_1d(long n) {
+  long g = (n + 255) / 256;
+  if (g > 16384) g = 16384;
+  return (int)(g < 1 ? 1 : g);
+}
+
+}  // namespace
+
+int rule_count() { return N_RULES; }
+
+void level_set(const double* cx, const double* cy, const double* cz, int N, int M, int K, Box box, Body body,
+               double* phi, hipStream_t s) {
+  k_level_set<<<grid_1d((long)N * M * K), 256, 0, s>>>(cx, cy, cz, N, M, K, box, body, phi);
+}
+
+void oscillation_fix(double* phi, unsigned char* flag, int N, int M, int K, Box box, hipStream_t s) {
+  k_osc_flags<<<grid_1d((long)N * M * K), 256, 0, s>>>(phi, N, M, K, box, flag);
+  k_osc_replay<<<1, 1024, 0, s>>>(phi, flag, N, M, K);
+}
+
+void alpha(const double* phi, int N, int M, int K, Box box, double* ax, double* ay, double* az, hipStream_t s) {
+  k_alpha<<<grid_1d((long)N * M * K), 256, 0, s>>>(phi, N, M, K, box, ax, ay, az);
+}
+
+void types(const double* phi, int N, int M, int K, Box box, int* type, int* sx, int* sy, int* sz, hipStream_t s) {
+  k_types<<<grid_1d((long)N * M * K), 256, 0, s>>>(phi, N, M, K, box, type, sx, sy, sz);
+}
+
+void theta(const double* ax, const double* ay, const double* az, const int* sx, const int* sy, const int* sz, int N,
+           int M, int K, Box box, double* tx, double* ty, double* tz, hipStream_t s) {
+  k_theta<<<grid_1d((long)N * M * K), 256, 0, s>>>(ax, ay, az, sx, sy, sz, N, M, K, box, tx, ty, tz);
+}
+
+void vof(const int* type, const double* ax, const double* ay, const double* az, const double* tx, const double* ty,
+         const double* tz, const double* dx, const double* dy, const double* dz, int N, int M, int K, double* v,
+         hipStream_t s) {
+  Fields F;
+  F.a[0] = ax;
+  F.a[1] = ay;
+  F.a[2] = az;
+  F.t[0] = tx;
+  F.t[1] = ty;
+  F.t[2] = tz;
+  F.N = N;
+  F.NM = N * M;
+  k_vof<<<grid_1d((long)N * M * K), 256, 0, s>>>(type, F, dx, dy, dz, N, M, K, v);
+}
+
+void laplace7(const int* type, const double* tx, const double* ty, const double* tz, const double* v, const double* dx,
+              const double* dy, const double* dz, int N, int M, int K, int pin, double* coef, hipStream_t s) {
+  k_laplace7<<<grid_1d((long)N * M * K), 256, 0, s>>>(type, tx, ty, tz, v, dx, dy, dz, N, M, K, pin, coef);
+}
+
+}  // namespace geo
+}  // namespace lsx

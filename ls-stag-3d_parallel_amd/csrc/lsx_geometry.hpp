@@ -1,0 +1,50 @@
+// Cut-cell geometry of the pressure grid on the GPU (level set -> edge fractions -> face and
+// cell types -> face fractions -> cut volumes -> operator coefficients).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace lsx {
+namespace geo {
+
+// Reference cell type codes (src/Base/LS_STAG_Mesh/cut_cells.h:26-168): interior cut cells
+// carry an 8-digit decimal code of their fluid corners, domain border cells negative codes.
+enum CellCode : int {
+  SolidCell = 0,
+  FluidCell = 11111111,
+  T_Border = -2, B_Border = -3, N_Border = -4, S_Border = -5, E_Border = -6, W_Border = -7,
+  TN_Edge = -8, TS_Edge = -9, TE_Edge = -10, TW_Edge = -11, BN_Edge = -12, BS_Edge = -13,
+  BE_Edge = -14, BW_Edge = -15, NE_Edge = -16, NW_Edge = -17, SE_Edge = -18, SW_Edge = -19,
+  TNE_Corner = -20, TNW_Corner = -21, TSE_Corner = -22, TSW_Corner = -23,
+  BNE_Corner = -24, BNW_Corner = -25, BSE_Corner = -26, BSW_Corner = -27
+};
+
+enum BodyKind : int { BODY_SPHERE = 0, BODY_ELLIPSOID = 1 };
+
+struct Body {
+  int kind;
+  double c[3];  // centre
+  double a[3];  // radius (sphere: a[0]) or semi-axes
+};
+
+// cut cells may appear for x in [hs, hf), y in [vs, vf), every z plane
+struct Box {
+  int hs, hf, vs, vf;
+};
+
+int rule_count();
+void level_set(const double* cx, const double* cy, const double* cz, int N, int M, int K, Box box, Body body,
+               double* phi, hipStream_t s);
+void oscillation_fix(double* phi, unsigned char* flag, int N, int M, int K, Box box, hipStream_t s);
+void alpha(const double* phi, int N, int M, int K, Box box, double* ax, double* ay, double* az, hipStream_t s);
+void types(const double* phi, int N, int M, int K, Box box, int* type, int* sx, int* sy, int* sz, hipStream_t s);
+void theta(const double* ax, const double* ay, const double* az, const int* sx, const int* sy, const int* sz, int N,
+           int M, int K, Box box, double* tx, double* ty, double* tz, hipStream_t s);
+void vof(const int* type, const double* ax, const double* ay, const double* az, const double* tx, const double* ty,
+         const double* tz, const double* dx, const double* dy, const double* dz, int N, int M, int K, double* v,
+         hipStream_t s);
+void laplace7(const int* type, const double* tx, const double* ty, const double* tz, const double* v, const double* dx,
+              const double* dy, const double* dz, int N, int M, int K, int pin, double* coef, hipStream_t s);
+
+}  // namespace geo
+}  // namespace lsx

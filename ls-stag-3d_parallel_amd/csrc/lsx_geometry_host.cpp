@@ -1,0 +1,129 @@
+// Geometry pipeline driver: grid setup (cell centres and upper-corner coordinates follow the
+// reference's buildDeltaNodes and level-set sweep, ls_stag_uniform_mesh.h:174-185 and
+// flow_sphere_task_description.h:57-74) and the ordered launch of the per-cell kernels.
+
+#include "lsx_geometry_host.hpp"
+
+#include <cmath>
+
+namespace lsx {
+
+namespace {
+// cell centres: a uniform axis uses a + (k + 1/2) h; a stretched axis accumulates widths
+std::vector<double> centres(const std::vector<double>& w, double a) {
+  const int n = (int)w.size();
+  std::vector<double> c(n);
+  bool uniform = true;
+  for (int i = 1; i < n; ++i) uniform = uniform && (w[i] == w[0]);
+  if (uniform) {
+    for (int i = 0; i < n; ++i) c[i] = a + (i + 0.5) * w[0];
+  } else {
+    double e = a;
+    for (int i = 0; i < n; ++i) {
+      c[i] = e + 0.5 * w[i];
+      e += w[i];
+    }
+  }
+  return c;
+}
+}  // namespace
+
+Geometry::Geometry(int N_, int M_, int K_, const double* dx_, const double* dy_, const double* dz_, double x0_,
+                   double y0_, double z0_, hipStream_t s)
+    : N(N_), M(M_), K(K_), x0(x0_), y0(y0_), z0(z0_) {
+  if (N < 3 || M < 3 || K < 3) throw Error(LSX_ERR_ARG, "grid dimensions must be >= 3");
+  if ((long)N * M * K >= (1L << 31)) throw Error(LSX_ERR_ARG, "grid too large for 32-bit cell indices");
+  if (!dx_ || !dy_ || !dz_) throw Error(LSX_ERR_ARG, "null cell width array");
+  hdx.assign(dx_, dx_ + N);
+  hdy.assign(dy_, dy_ + M);
+  hdz.assign(dz_, dz_ + K);
+  for (double w : hdx) if (!(w > 0)) throw Error(LSX_ERR_ARG, "cell widths must be positive");
+  for (double w : hdy) if (!(w > 0)) throw Error(LSX_ERR_ARG, "cell widths must be positive");
+  for (double w : hdz) if (!(w > 0)) throw Error(LSX_ERR_ARG, "cell widths must be positive");
+  if (s) {
+    stream = s;
+  } else {
+    LSX_HIP(hipStreamCreate(&stream));
+    own_stream = true;
+  }
+  dx.alloc(N);
+  dy.alloc(M);
+  dz.alloc(K);
+  LSX_HIP(hipMemcpyAsync(dx.p, hdx.data(), N * sizeof(double), hipMemcpyHostToDevice, stream));
+  LSX_HIP(hipMemcpyAsync(dy.p, hdy.data(), M * sizeof(double), hipMemcpyHostToDevice, stream));
+  LSX_HIP(hipMemcpyAsync(dz.p, hdz.data(), K * sizeof(double), hipMemcpyHostToDevice, stream));
+  LSX_HIP(hipStreamSynchronize(stream));
+}
+
+Geometry::~Geometry() {
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+void Geometry::build(const geo::Body& body, geo::Box box) {
+  if (box.hs < 1 || box.vs < 1 || box.hf > N - 1 || box.vf > M - 1 || box.hs >= box.hf || box.vs >= box.vf)
+    throw Error(LSX_ERR_ARG, "geometry box must lie inside the domain's interior");
+  if (body.kind != geo::BODY_SPHERE && body.kind != geo::BODY_ELLIPSOID) throw Error(LSX_ERR_ARG, "unknown body kind");
+  for (int q = 0; q < (body.kind == geo::BODY_SPHERE ? 1 : 3); ++q)
+    if (!(body.a[q] > 0)) throw Error(LSX_ERR_ARG, "body size must be positive");
+  // upper-corner coordinates: x starts at the first box cell and steps by the next cell's width
+  const std::vector<double> xc = centres(hdx, x0), yc = centres(hdy, y0), zc = centres(hdz, z0);
+  std::vector<double> hcx(N, 0.0), hcy(M), hcz(K);
+  double px = xc[box.hs] + 0.5 * hdx[box.hs];
+  for (int x = box.hs; x < box.hf; ++x) {
+    hcx[x] = px;
+    if (x + 1 < N) px += hdx[x + 1];
+  }
+  for (int y = 0; y < M; ++y) hcy[y] = yc[y] + 0.5 * hdy[y];
+  for (int z = 0; z < K; ++z) hcz[z] = zc[z] + 0.5 * hdz[z];
+  const long nn = n();
+  cx.alloc(N);
+  cy.alloc(M);
+  cz.alloc(K);
+  phi.alloc(nn);
+  flag.alloc(nn);
+  ax.alloc(nn);
+  ay.alloc(nn);
+  az.alloc(nn);
+  tx.alloc(nn);
+  ty.alloc(nn);
+  tz.alloc(nn);
+  vof.alloc(nn);
+  type.alloc(nn);
+  sx.alloc(nn);
+  sy.alloc(nn);
+  sz.alloc(nn);
+  LSX_HIP(hipMemcpyAsync(cx.p, hcx.data(), N * sizeof(double), hipMemcpyHostToDevice, stream));
+  LSX_HIP(hipMemcpyAsync(cy.p, hcy.data(), M * sizeof(double), hipMemcpyHostToDevice, stream));
+  LSX_HIP(hipMemcpyAsync(cz.p, hcz.data(), K * sizeof(double), hipMemcpyHostToDevice, stream));
+  geo::level_set(cx.p, cy.p, cz.p, N, M, K, box, body, phi.p, stream);
+  geo::oscillation_fix(phi.p, flag.p, N, M, K, box, stream);
+  geo::alpha(phi.p, N, M, K, box, ax.p, ay.p, az.p, stream);
+  geo::types(phi.p, N, M, K, box, type.p, sx.p, sy.p, sz.p, stream);
+  geo::theta(ax.p, ay.p, az.p, sx.p, sy.p, sz.p, N, M, K, box, tx.p, ty.p, tz.p, stream);
+  geo::vof(type.p, ax.p, ay.p, az.p, tx.p, ty.p, tz.p, dx.p, dy.p, dz.p, N, M, K, vof.p, stream);
+  LSX_HIP(hipGetLastError());
+  LSX_HIP(hipStreamSynchronize(stream));
+  built = true;
+}
+
+void Geometry::fields(int* htype, double* hthx, double* hthy, double* hthz, double* hvof, double* hphi) const {
+  if (!built) throw Error(LSX_ERR_STATE, "geometry not built");
+  const long nn = n();
+  if (htype) LSX_HIP(hipMemcpyAsync(htype, type.p, nn * sizeof(int), hipMemcpyDeviceToHost, stream));
+  if (hthx) LSX_HIP(hipMemcpyAsync(hthx, tx.p, nn * sizeof(double), hipMemcpyDeviceToHost, stream));
+  if (hthy) LSX_HIP(hipMemcpyAsync(hthy, ty.p, nn * sizeof(double), hipMemcpyDeviceToHost, stream));
+  if (hthz) LSX_HIP(hipMemcpyAsync(hthz, tz.p, nn * sizeof(double), hipMemcpyDeviceToHost, stream));
+  if (hvof) LSX_HIP(hipMemcpyAsync(hvof, vof.p, nn * sizeof(double), hipMemcpyDeviceToHost, stream));
+  if (hphi) LSX_HIP(hipMemcpyAsync(hphi, phi.p, nn * sizeof(double), hipMemcpyDeviceToHost, stream));
+  LSX_HIP(hipStreamSynchronize(stream));
+}
+
+void Geometry::laplace7(int pin, double* coef_dev, hipStream_t s) const {
+  if (!built) throw Error(LSX_ERR_STATE, "geometry not built");
+  // the geometry stream must have finished before another stream reads the fields
+  LSX_HIP(hipStreamSynchronize(stream));
+  geo::laplace7(type.p, tx.p, ty.p, tz.p, vof.p, dx.p, dy.p, dz.p, N, M, K, pin, coef_dev, s);
+  LSX_HIP(hipGetLastError());
+}
+
+}  // namespace lsx

@@ -1,0 +1,38 @@
+// Host side of the GPU cut-cell geometry: grid description, device fields, build pipeline.
+#pragma once
+
+#include <vector>
+
+#include "lsx_geometry.hpp"
+#include "lsx_solver.hpp"
+
+namespace lsx {
+
+class Geometry {
+ public:
+  Geometry(int N, int M, int K, const double* dx, const double* dy, const double* dz, double x0, double y0, double z0,
+           hipStream_t stream);
+  ~Geometry();
+  // Build every field for `body` with cut cells allowed in box [hs,hf) x [vs,vf) x all z.
+  void build(const geo::Body& body, geo::Box box);
+  // Copy fields to host (any pointer may be null).
+  void fields(int* type, double* thx, double* thy, double* thz, double* vof, double* phi) const;
+  // 7 coefficient planes [b,s,w,p,e,n,t] of the pressure operator (device pointer, 7*n doubles).
+  void laplace7(int pin, double* coef_dev, hipStream_t s) const;
+
+  int N, M, K;
+  long n() const { return (long)N * M * K; }
+  bool built = false;
+
+ private:
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::vector<double> hdx, hdy, hdz;
+  double x0, y0, z0;
+  DevBuf<double> dx, dy, dz, cx, cy, cz;
+  DevBuf<double> phi, ax, ay, az, tx, ty, tz, vof;
+  DevBuf<int> type, sx, sy, sz;
+  DevBuf<unsigned char> flag;
+};
+
+}  // namespace lsx

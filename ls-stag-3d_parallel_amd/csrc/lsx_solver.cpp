@@ -3,6 +3,7 @@
 // scalars the Krylov recurrences need (one small copy per Arnoldi step for FGMRES).
 
 #include "lsx_solver.hpp"
+#include "lsx_geometry_host.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -37,7 +38,82 @@ Solver::Solver(const lsx_options& o, hipStream_t s) : opt(o) {
 }
 
 Solver::~Solver() {
+  for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+// -----------------------------------------------------------------------------------------------
+// kernel timing
+// -----------------------------------------------------------------------------------------------
+hipEvent_t Solver::ev_take() {
+  if (ev_used == ev_pool.size()) {
+    hipEvent_t e;
+    LSX_HIP(hipEventCreate(&e));
+    ev_pool.push_back(e);
+  }
+  return ev_pool[ev_used++];
+}
+
+void Solver::tbeg(int k, hipEvent_t* e) {
+  (void)k;
+  *e = nullptr;
+  if (!timing) return;
+  *e = ev_take();
+  LSX_HIP(hipEventRecord(*e, stream));
+}
+
+void Solver::tend(int k, hipEvent_t b) {
+  if (!timing || !b) return;
+  hipEvent_t e = ev_take();
+  LSX_HIP(hipEventRecord(e, stream));
+  ev_rec[k].push_back({b, e});
+  if (ev_used > 4096) drain_events();
+}
+
+void Solver::drain_events() {
+  LSX_HIP(hipStreamSynchronize(stream));
+  for (int k = 0; k < LSX_K_COUNT; ++k) {
+    for (auto& pr : ev_rec[k]) {
+      float ms = 0.f;
+      LSX_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+      ev_ms[k] += ms;
+      ev_count[k] += 1;
+    }
+    ev_rec[k].clear();
+  }
+  ev_used = 0;
+}
+
+void Solver::set_timing(bool on) {
+  if (timing) drain_events();
+  timing = on;
+  for (int k = 0; k < LSX_K_COUNT; ++k) {
+    ev_ms[k] = 0.0;
+    ev_count[k] = 0;
+  }
+}
+
+double Solver::kernel_bytes(int k) const {
+  // compulsory HBM bytes per fine cell: each array read or written once (operator stored as
+  // the four planes a_p, a_e, a_n, a_t of the symmetric 7-point stencil)
+  const double n = (double)d0.n();
+  switch (k) {
+    case LSX_K_RESIDUAL: return 56.0 * n;  // c 8, coefficients 32, r read + write 16
+    case LSX_K_LINE_X:
+    case LSX_K_LINE_Y:
+    case LSX_K_LINE_Z: return 56.0 * n;    // r 8, two factor planes 16, sub coupling 8, ch 16, c 8
+    case LSX_K_OPERATOR: return 48.0 * n;  // z 8, coefficients 32, w 8
+    case LSX_K_ORTHO: return 32.0 * n;     // w read + write 16, v_i 8, v_i+1 8
+    default: return 0.0;
+  }
+}
+
+void Solver::kernel_stats(int k, long* launches, double* total_ms, double* bytes) {
+  if (k < 0 || k >= LSX_K_COUNT) throw Error(LSX_ERR_ARG, "kernel id out of range");
+  if (timing) drain_events();
+  if (launches) *launches = ev_count[k];
+  if (total_ms) *total_ms = ev_ms[k];
+  if (bytes) *bytes = kernel_bytes(k);
 }
 
 void Solver::synchronize() { LSX_HIP(hipStreamSynchronize(stream)); }
@@ -56,25 +132,74 @@ static int level_count(int N, int M, int K) {
   return num;
 }
 
-void Solver::set_csr(int N, int M, int K, const int* rp, const int* cl, const double* vl, long nnz, int mem) {
+static void check_dims(int N, int M, int K) {
   if (N < 3 || M < 3 || K < 3) throw Error(LSX_ERR_ARG, "grid dimensions must be >= 3");
   if ((long)N * M * K >= (1L << 31)) throw Error(LSX_ERR_ARG, "grid too large for 32-bit row indices");
+  const int nmg = level_count(N, M, K);
+  int n = N, m = M, k = K;
+  for (int l = 0; l < nmg; ++l) {
+    if ((n & 1) || (m & 1) || (k & 1))
+      throw Error(LSX_ERR_ARG, "every multigrid level must have even dimensions (even-even-even transfers)");
+    n /= 2;
+    m /= 2;
+    k /= 2;
+  }
+  if ((long)n * m * k > 64) throw Error(LSX_ERR_ARG, "coarsest level exceeds 64 unknowns");
+}
+
+void Solver::setup_levels(int N, int M, int K) {
+  const int nmg = level_count(N, M, K);
+  d0 = Dims{N, M, K};
+  levels.clear();
+  levels.resize(nmg + 1);
+  {
+    int nn = N, mm = M, kk = K;
+    for (int l = 0; l <= nmg; ++l) {
+      levels[l].d = Dims{nn, mm, kk};
+      levels[l].npts = l == 0 ? 7 : 27;
+      nn /= 2;
+      mm /= 2;
+      kk /= 2;
+    }
+  }
+  for (size_t l = 0; l < levels.size(); ++l) {
+    MGLevel& L = levels[l];
+    const long nl = L.d.n();
+    L.coef.alloc((size_t)L.npts * nl);
+    L.r.alloc(nl);
+    if (l > 0) L.ch.alloc(nl);
+    if (l + 1 < levels.size())
+      for (int dir = 0; dir < 3; ++dir) {
+        L.dinv[dir].alloc(nl);
+        L.eu[dir].alloc(nl);
+      }
+  }
+  cwork.alloc(d0.n());
+  const long nc = levels.back().d.n();
+  lu_scale.alloc(nc);
+  lu_perm.alloc(nc);
+  lu_L.alloc(nc * nc);
+  lu_U.alloc(nc * nc);
+  lu_col.alloc(nc);
+}
+
+void Solver::set_geometry(const Geometry& g, int pin) {
+  check_dims(g.N, g.M, g.K);
+  ready = false;
+  setup_levels(g.N, g.M, g.K);
+  g.laplace7(pin, levels[0].coef.p, stream);
+  from_csr = false;
+  build_hierarchy();
+  alloc_krylov();
+  ready = true;
+}
+
+void Solver::set_csr(int N, int M, int K, const int* rp, const int* cl, const double* vl, long nnz, int mem) {
+  check_dims(N, M, K);
   if (!rp || !cl || !vl || nnz <= 0) throw Error(LSX_ERR_ARG, "null CSR array or empty matrix");
   if (mem != LSX_MEM_HOST && mem != LSX_MEM_DEVICE) throw Error(LSX_ERR_ARG, "mem must be LSX_MEM_HOST or LSX_MEM_DEVICE");
-  const int nmg = level_count(N, M, K);
-  {
-    int n = N, m = M, k = K;
-    for (int l = 0; l < nmg; ++l) {
-      if ((n & 1) || (m & 1) || (k & 1))
-        throw Error(LSX_ERR_ARG, "every multigrid level must have even dimensions (even-even-even transfers)");
-      n /= 2;
-      m /= 2;
-      k /= 2;
-    }
-    if ((long)n * m * k > 64) throw Error(LSX_ERR_ARG, "coarsest level exceeds 64 unknowns");
-  }
-  d0 = Dims{N, M, K};
-  const long n = d0.n();
+  ready = false;
+  const long n = (long)N * M * K;
   if (mem == LSX_MEM_HOST) {
     // rows must be well formed before anything touches the device
     if (rp[0] != 0 || rp[n] != nnz) throw Error(LSX_ERR_ARG, "row pointers inconsistent with nnz");
@@ -103,44 +228,15 @@ void Solver::set_csr(int N, int M, int K, const int* rp, const int* cl, const do
     for (long j = 0; j < nnz; ++j)
       if (hcl[j] < 0 || hcl[j] >= n) throw Error(LSX_ERR_ARG, "column index out of range");
   }
-  levels.clear();
-  levels.resize(nmg + 1);
-  {
-    int nn = N, mm = M, kk = K;
-    for (int l = 0; l <= nmg; ++l) {
-      levels[l].d = Dims{nn, mm, kk};
-      levels[l].npts = l == 0 ? 7 : 27;
-      nn /= 2;
-      mm /= 2;
-      kk /= 2;
-    }
-  }
-  for (size_t l = 0; l < levels.size(); ++l) {
-    MGLevel& L = levels[l];
-    const long nl = L.d.n();
-    L.coef.alloc((size_t)L.npts * nl);
-    L.r.alloc(nl);
-    if (l > 0) L.ch.alloc(nl);
-    if (l + 1 < levels.size())
-      for (int dir = 0; dir < 3; ++dir) {
-        L.dinv[dir].alloc(nl);
-        L.eu[dir].alloc(nl);
-      }
-  }
-  cwork.alloc(n);
-  const long nc = levels.back().d.n();
-  lu_scale.alloc(nc);
-  lu_perm.alloc(nc);
-  lu_L.alloc(nc * nc);
-  lu_U.alloc(nc * nc);
-  lu_col.alloc(nc);
+  setup_levels(N, M, K);
+  from_csr = true;
   build_hierarchy();
   alloc_krylov();
   ready = true;
 }
 
 void Solver::update_values(const double* vl, int mem) {
-  if (!ready) throw Error(LSX_ERR_STATE, "update_values before set_csr");
+  if (!ready || !from_csr) throw Error(LSX_ERR_STATE, "update_values needs a matrix set with set_csr");
   const hipMemcpyKind kind = mem == LSX_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
   LSX_HIP(hipMemcpyAsync(val.p, vl, nnz0 * sizeof(double), kind, stream));
 }
@@ -152,12 +248,14 @@ void Solver::rebuild() {
 
 void Solver::build_hierarchy() {
   MGLevel& F = levels[0];
-  LSX_HIP(hipMemsetAsync(err.p, 0, sizeof(int), stream));
-  csr_to_stencil7(rowptr.p, col.p, val.p, F.d, F.coef.p, err.p, stream);
-  int herr = 0;
-  LSX_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, stream));
-  LSX_HIP(hipStreamSynchronize(stream));
-  if (herr) throw Error(LSX_ERR_ARG, "matrix has a column outside the 7-point neighbourhood of its row");
+  if (from_csr) {
+    LSX_HIP(hipMemsetAsync(err.p, 0, sizeof(int), stream));
+    csr_to_stencil7(rowptr.p, col.p, val.p, F.d, F.coef.p, err.p, stream);
+    int herr = 0;
+    LSX_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    LSX_HIP(hipStreamSynchronize(stream));
+    if (herr) throw Error(LSX_ERR_ARG, "matrix has a column outside the 7-point neighbourhood of its row");
+  }
   for (size_t l = 0; l + 1 < levels.size(); ++l) {
     MGLevel& f = levels[l];
     MGLevel& c = levels[l + 1];
@@ -208,9 +306,16 @@ void Solver::mg_apply(const double* v, double* z) {
     prolong_copy(levels[k + 1].ch.p, levels[k + 1].d, ch, cwork.p, lv.d, stream);
     for (int s = 0; s < opt.n_post; ++s)
       for (int dir = 2; dir >= 0; --dir) {  // z, y, x
+        hipEvent_t e0 = nullptr;
+        if (k == 0) tbeg(LSX_K_RESIDUAL, &e0);
         stencil_apply(lv.coef.p, lv.npts, lv.d, cwork.p, nullptr, lv.r.p, ST_SUBR, nullptr, stream);
+        if (k == 0) {
+          tend(LSX_K_RESIDUAL, e0);
+          tbeg(LSX_K_LINE_X + dir, &e0);
+        }
         line_solve(lv.coef.p, lv.npts, lv.d, dir, lv.dinv[dir].p, lv.eu[dir].p, lv.r.p, cwork.p, ch, opt.omega,
                    opt.z_head_fix, stream);
+        if (k == 0) tend(LSX_K_LINE_X + dir, e0);
       }
   }
 }
@@ -271,7 +376,10 @@ int Solver::fgmres(const double* b, double* x) {
       double* zj = Z.p + (size_t)j * n;
       scale_copy(w.p, dnorm, vj, n, stream);
       mg_apply(vj, zj);
+      hipEvent_t e0 = nullptr;
+      tbeg(LSX_K_OPERATOR, &e0);
       stencil_apply(F.coef.p, 7, d0, zj, nullptr, w.p, ST_PROD, nullptr, stream);
+      tend(LSX_K_OPERATOR, e0);
       // modified Gram-Schmidt: h_i = w . v_i ; w -= h_i v_i  (the dot of the next basis vector
       // is fused into the update pass; the last pass yields ||w||^2)
       dot_partial(w.p, V.p, n, partial.p, stream);
@@ -279,7 +387,9 @@ int Solver::fgmres(const double* b, double* x) {
       for (int i = 0; i <= j; ++i) {
         const double* vi = V.p + (size_t)i * n;
         const double* vnext = i < j ? V.p + (size_t)(i + 1) * n : nullptr;
+        if (vnext) tbeg(LSX_K_ORTHO, &e0);
         axpy_dot(w.p, vi, dH + hColStart + i, vnext, n, partial.p, stream);
+        if (vnext) tend(LSX_K_ORTHO, e0);
         if (vnext)
           finalize(partial.p, nb, dH + hColStart + i + 1, 0, stream);
         else
@@ -429,6 +539,13 @@ long Solver::level_dims(int level, int* N, int* M, int* K) const {
   if (M) *M = d.M;
   if (K) *K = d.K;
   return d.n();
+}
+
+void Solver::operator_diagonal(double* host) const {
+  if (!ready) throw Error(LSX_ERR_STATE, "operator_diagonal before the matrix is set");
+  const long n = d0.n();
+  LSX_HIP(hipMemcpyAsync(host, levels[0].coef.p + 3 * n, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+  LSX_HIP(hipStreamSynchronize(stream));
 }
 
 int Solver::level_stencil(int level, double* host, long capacity) const {

@@ -68,12 +68,15 @@ struct MGLevel {
   DevBuf<double> r, ch;         // level right-hand side / correction
 };
 
+class Geometry;
+
 class Solver {
  public:
   Solver(const lsx_options& opt, hipStream_t stream);
   ~Solver();
 
   void set_csr(int N, int M, int K, const int* rowptr, const int* col, const double* val, long nnz, int mem);
+  void set_geometry(const Geometry& g, int pin);
   void update_values(const double* val, int mem);
   void rebuild();
   void solve(const double* b, double* x, int mem, int* it, double* norm_r);
@@ -81,7 +84,11 @@ class Solver {
   void apply_operator(int level, const double* x, double* y, int mem);
   long level_dims(int level, int* N, int* M, int* K) const;
   int level_stencil(int level, double* host, long capacity) const;
+  void operator_diagonal(double* host) const;
   void synchronize();
+
+  void set_timing(bool on);
+  void kernel_stats(int k, long* launches, double* total_ms, double* bytes);
 
   lsx_options opt;
   int it = 0;
@@ -89,7 +96,9 @@ class Solver {
   int num_levels() const { return (int)levels.size(); }
 
  private:
+  void setup_levels(int N, int M, int K);
   void build_hierarchy();
+  bool from_csr = false;
   void alloc_krylov();
   void mg_apply(const double* v_dev, double* z_dev);
   int fgmres(const double* b, double* x);
@@ -115,6 +124,19 @@ class Solver {
   DevBuf<double> hb, hx;                              // staging for host inputs
   std::vector<double> H, cs, sn, y;
   bool ready = false;
+
+  // event timing of fine-level kernel launches
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_rec[LSX_K_COUNT];
+  double ev_ms[LSX_K_COUNT] = {0};
+  long ev_count[LSX_K_COUNT] = {0};
+  hipEvent_t ev_take();
+  void tbeg(int k, hipEvent_t* e);
+  void tend(int k, hipEvent_t b);
+  void drain_events();
+  double kernel_bytes(int k) const;
 };
 
 }  // namespace lsx

@@ -16,6 +16,41 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblsx.so")
 
+K_RESIDUAL, K_LINE_X, K_LINE_Y, K_LINE_Z, K_OPERATOR, K_ORTHO, K_SMOOTH = range(7)
+KERNEL_NAMES = {K_RESIDUAL: "residual_update", K_LINE_X: "line_solve_x", K_LINE_Y: "line_solve_y",
+                K_LINE_Z: "line_solve_z", K_OPERATOR: "operator_apply", K_ORTHO: "mgs_axpy_dot",
+                K_SMOOTH: "smoother_sweep"}
+
+
+def stretched_axis(n, length=1.0, center=0.5, ratio=1.0):
+    """Cell widths of one axis, growing geometrically by `ratio` per cell away from `center`
+    (ratio 1 gives a uniform axis of width length/n)."""
+    if ratio == 1.0:
+        return np.full(n, length / n)
+    k = np.arange(n) + 0.5
+    kc = center / length * n
+    w = ratio ** np.abs(k - kc)
+    return w * (length / w.sum())
+
+
+def random_divergence_rhs(diag, dims, seed=0, pin=True):
+    """Seeded synthetic pressure right-hand side: unit-variance Gaussian on the active cells,
+    mean removed over them, zero elsewhere.  Active cells are those whose operator diagonal is
+    negative (fluid and cut cells with at least one open face); solid cells (diagonal +1e-16),
+    cut cells with every face closed (diagonal 0) and the pinned cells (+1) get 0, as a
+    discrete divergence does."""
+    N, M, K = dims
+    active = diag < 0.0
+    if pin:
+        active[N * M - 1] = False
+        active[N - 1] = False
+    rng = np.random.default_rng(seed)
+    b = np.zeros(diag.size)
+    b[active] = rng.standard_normal(int(active.sum()))
+    b[active] -= b[active].mean()
+    return b
+
+
 KRYLOV_FGMRES, KRYLOV_BICGSTAB = 0, 1
 MG_CASCADE, MG_VCYCLE = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
@@ -63,8 +98,70 @@ def lib():
     L.lsx_level_dims.restype = C.c_long
     L.lsx_level_stencil.argtypes = [vp, C.c_int, vp, C.c_long]
     L.lsx_synchronize.argtypes = [vp]
+    L.lsx_geometry_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_double, C.c_double,
+                                      C.c_double, vp]
+    L.lsx_geometry_destroy.argtypes = [vp]
+    L.lsx_geometry_build.argtypes = [vp, C.POINTER(Body), C.c_int, C.c_int, C.c_int, C.c_int]
+    L.lsx_geometry_fields.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.lsx_set_geometry.argtypes = [vp, vp, C.c_int]
+    L.lsx_operator_diagonal.argtypes = [vp, vp]
+    L.lsx_kernel_timing.argtypes = [vp, C.c_int]
+    L.lsx_kernel_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     _lib = L
     return L
+
+
+class Body(C.Structure):
+    _fields_ = [("kind", C.c_int), ("center", C.c_double * 3), ("axes", C.c_double * 3)]
+
+
+BODY_SPHERE, BODY_ELLIPSOID = 0, 1
+
+
+class Geometry:
+    """Cut-cell geometry of the pressure grid, built on the GPU (reference LSSTAGUniformMesh
+    mesh functions for a static body)."""
+
+    def __init__(self, dx, dy, dz, origin=(0.0, 0.0, 0.0), stream=None):
+        self.dx = np.ascontiguousarray(dx, dtype=np.float64)
+        self.dy = np.ascontiguousarray(dy, dtype=np.float64)
+        self.dz = np.ascontiguousarray(dz, dtype=np.float64)
+        self.dims = (len(self.dx), len(self.dy), len(self.dz))
+        h = C.c_void_p()
+        _check(lib().lsx_geometry_create(C.byref(h), *self.dims, _ptr(self.dx), _ptr(self.dy), _ptr(self.dz),
+                                         *origin, stream))
+        self.h = h
+
+    @property
+    def n(self):
+        N, M, K = self.dims
+        return N * M * K
+
+    def build(self, kind, center, axes, box=None):
+        N, M, K = self.dims
+        hs, hf, vs, vf = box or (1, N - 1, 1, M - 1)
+        a = tuple(axes) + (axes[0],) * (3 - len(axes))
+        b = Body(BODY_SPHERE if kind == "sphere" else BODY_ELLIPSOID, (C.c_double * 3)(*center), (C.c_double * 3)(*a))
+        _check(lib().lsx_geometry_build(self.h, C.byref(b), hs, hf, vs, vf))
+        return self
+
+    def fields(self):
+        n = self.n
+        t = np.zeros(n, np.int32)
+        tx, ty, tz, v, phi = (np.zeros(n) for _ in range(5))
+        _check(lib().lsx_geometry_fields(self.h, _ptr(t), _ptr(tx), _ptr(ty), _ptr(tz), _ptr(v), _ptr(phi)))
+        return dict(type=t, thx=tx, thy=ty, thz=tz, vof=v, phi=phi)
+
+    def close(self):
+        if self.h:
+            lib().lsx_geometry_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _check(status):
@@ -132,6 +229,28 @@ class IterativeSolver:
         self._rhs = rhs
         self._sol = sol
 
+    def set_geometry(self, geometry, rhs, sol, task, pin=True):
+        """Like set(), with the cut-cell operator built on the device from `geometry`."""
+        self.close()
+        opt = Options()
+        lib().lsx_options_default(C.byref(opt))
+        opt.krylov = self.krylov
+        opt.restart = int(task.FGMRES_RESTART_NUM)
+        opt.max_it = int(task.MAX_ITER)
+        opt.eps = float(task.eps)
+        opt.n_post = int(task.n_post)
+        opt.omega = float(task.omega)
+        opt.precision = self.precision
+        opt.mg = self.mg
+        opt.z_head_fix = 1 if self.z_head_fix else 0
+        h = C.c_void_p()
+        _check(lib().lsx_create(C.byref(h), C.byref(opt), self.stream))
+        self.h = h
+        self.eps = float(task.eps)
+        _check(lib().lsx_set_geometry(self.h, geometry.h, 1 if pin else 0))
+        self._rhs = rhs
+        self._sol = sol
+
     def solve(self):
         b = np.ascontiguousarray(self._rhs, dtype=np.float64)
         if not (isinstance(self._sol, np.ndarray) and self._sol.dtype == np.float64 and self._sol.flags.c_contiguous):
@@ -164,6 +283,21 @@ class IterativeSolver:
         y = np.zeros(N * M * K)
         _check(lib().lsx_apply_operator(self.h, level, _ptr(x), _ptr(y), MEM_HOST))
         return y
+
+    def operator_diagonal(self):
+        N, M, K = self.level_dims(0)
+        d = np.zeros(N * M * K)
+        _check(lib().lsx_operator_diagonal(self.h, _ptr(d)))
+        return d
+
+    def kernel_timing(self, enable=True):
+        _check(lib().lsx_kernel_timing(self.h, 1 if enable else 0))
+
+    def kernel_stats(self, kernel):
+        """(launches, total_ms, algorithmic_bytes_per_launch) of a fine-level kernel class."""
+        n, ms, by = C.c_long(), C.c_double(), C.c_double()
+        _check(lib().lsx_kernel_stats(self.h, kernel, C.byref(n), C.byref(ms), C.byref(by)))
+        return n.value, ms.value, by.value
 
     def num_levels(self):
         it, nr, lv = C.c_int(), C.c_double(), C.c_int()

@@ -72,15 +72,44 @@ def synthetic_cut(N, M=None, K=None, center=(0.35, 0.5, 0.5), radius=0.2, seed=0
     return g, g.laplace(pin=pin)
 
 
-def random_rhs(g, seed=0, pin=True):
+def random_rhs(g, seed=0, pin=True, A=None):
+    """Gaussian on active cells (negative operator diagonal), mean removed, zero elsewhere:
+    solid cells (+1e-16 diagonal), cut cells whose faces are all closed (empty rows) and the
+    pinned cells get 0, so the right-hand side is consistent."""
     N, M, K = g.dims
-    rng = np.random.default_rng(seed)
-    t = g.fields()["type"]
-    fluid = t != SOLID
-    b = np.zeros(g.n)
-    b[fluid] = rng.standard_normal(int(fluid.sum()))
-    b[fluid] -= b[fluid].mean()
+    if A is None:
+        A = g.laplace(pin=pin)
+    active = A.to_scipy().diagonal() < 0.0
     if pin:
-        b[N * M - 1] = 0.0
-        b[N - 1] = 0.0
+        active[N * M - 1] = False
+        active[N - 1] = False
+    rng = np.random.default_rng(seed)
+    b = np.zeros(g.n)
+    b[active] = rng.standard_normal(int(active.sum()))
+    b[active] -= b[active].mean()
     return b
+
+
+def sphere(N, M=None, K=None, center=(0.35, 0.5, 0.5), radius=0.2, pin=True):
+    """The reference's sphere level set on a uniform grid of the unit cube (config C1 geometry)."""
+    M = M or N
+    K = K or N
+    g = O.uniform_geometry(N, M, K)
+    g.build_body("sphere", center, (radius,))
+    return g, g.laplace(pin=pin)
+
+
+def ellipsoid_stretched(N, ratio=1.02, axes=(0.25, 0.15, 0.15), pin=True):
+    """Ellipsoid at the centre of a geometrically stretched grid (config C3 geometry)."""
+    w = [stretched_axis(N, 1.0, 0.5, ratio) for _ in range(3)]
+    g = O.Geometry(*w)
+    g.build_body("ellipsoid", (0.5, 0.5, 0.5), axes)
+    return g, g.laplace(pin=pin)
+
+
+def stretched_axis(n, length=1.0, center=0.5, ratio=1.0):
+    if ratio == 1.0:
+        return np.full(n, length / n)
+    k = np.arange(n) + 0.5
+    w = ratio ** np.abs(k - center / length * n)
+    return w * (length / w.sum())
