@@ -68,6 +68,9 @@ class Hip:
     def zero(self, p, nbytes):
         self.check(self.l.hipMemset(p, 0, nbytes), "hipMemset")
 
+    def free(self, p):
+        self.check(self.l.hipFree(p), "hipFree")
+
     def sync(self):
         self.check(self.l.hipDeviceSynchronize(), "hipDeviceSynchronize")
 
@@ -104,15 +107,66 @@ def cpu_baseline(sample_n, gpu_it, gpu_n, restart, krylov):
     }
 
 
+def run_solver(lsx, hip, geom, precision, krylov, restart, steps, warmup):
+    """Set up one solver on the device geometry and time `steps` solves (x0 = 0 each)."""
+    S = lsx.solver(krylov, precision)
+    task = lsx.mixed_task(1.0, restart=restart) if precision == "mixed" else \
+        lsx.Task(eps=1.0, MAX_ITER=1000, FGMRES_RESTART_NUM=restart)
+    t_setup = time.perf_counter()
+    S.set_geometry(geom, None, None, task)
+    hip.sync()
+    setup_s = time.perf_counter() - t_setup
+    b = lsx.random_divergence_rhs(S.operator_diagonal(), geom.dims)
+    nb = float(np.linalg.norm(b))
+    lsx._check(lsx.lib().lsx_set_eps(S.h, 1e-8 * nb))
+    ncell = geom.n
+    db = hip.malloc(ncell * 8)
+    dx = hip.malloc(ncell * 8)
+    hip.h2d(db, b)
+    hip.sync()
+    L = lsx.lib()
+    it = C.c_int()
+    nr = C.c_double()
+
+    def step():
+        hip.zero(dx, ncell * 8)
+        st = L.lsx_solve(S.h, db, dx, lsx.MEM_DEVICE, C.byref(it), C.byref(nr))
+        if st != 0:
+            raise RuntimeError(L.lsx_last_error().decode())
+
+    for _ in range(warmup):
+        step()
+    hip.sync()
+    S.kernel_timing(True)
+    hip.sync()
+    t0 = time.perf_counter()
+    its = []
+    for _ in range(steps):
+        step()
+        its.append(it.value)
+    hip.sync()
+    elapsed = time.perf_counter() - t0
+    stats = {k: S.kernel_stats(k) for k in lsx.KERNEL_NAMES}
+    S.kernel_timing(False)
+    x = np.zeros(ncell)
+    hip.d2h(x, dx)
+    r = b - S.apply_operator(x, 0)
+    hip.free(db)
+    hip.free(dx)
+    return dict(S=S, elapsed=elapsed, its=its, stats=stats, setup_s=setup_s,
+                rel_res=float(np.linalg.norm(r) / nb), x=x, levels=S.num_levels())
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=256)
-    ap.add_argument("--precision", choices=["fp64"], default="fp64")
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--krylov", choices=["fgmres", "bicgstab"], default="fgmres")
     ap.add_argument("--restart", type=int, default=30)
+    ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--cpu-sample-grid", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -125,72 +179,59 @@ def main():
     hip = Hip()
     n = args.grid
     widths, center, axes = workload(n)
-    t_setup = time.perf_counter()
+    t_geo = time.perf_counter()
     geom = lsx.Geometry(*widths)
     geom.build("ellipsoid", center, axes)
-    dims = geom.dims
-    S = lsx.MGFGMRES() if args.krylov == "fgmres" else lsx.MGBiCGStab()
-    S.set_geometry(geom, None, None, lsx.Task(eps=1.0, MAX_ITER=1000, FGMRES_RESTART_NUM=args.restart))
-    b = lsx.random_divergence_rhs(S.operator_diagonal(), dims)
-    nb = float(np.linalg.norm(b))
-    eps = 1e-8 * nb
-    lsx._check(lsx.lib().lsx_set_eps(S.h, eps))
-    ncell = geom.n
-    db = hip.malloc(ncell * 8)
-    dx = hip.malloc(ncell * 8)
-    hip.h2d(db, b)
     hip.sync()
-    setup_s = time.perf_counter() - t_setup
-    L = lsx.lib()
-    it = C.c_int()
-    nr = C.c_double()
+    geo_s = time.perf_counter() - t_geo
+    print(f"[bench] geometry {n}^3 built in {geo_s:.2f} s", file=sys.stderr, flush=True)
 
-    def step():
-        hip.zero(dx, ncell * 8)
-        st = L.lsx_solve(S.h, db, dx, lsx.MEM_DEVICE, C.byref(it), C.byref(nr))
-        if st != 0:
-            raise RuntimeError(L.lsx_last_error().decode())
+    main_run = run_solver(lsx, hip, geom, args.precision, args.krylov, args.restart, args.steps, args.warmup)
+    print(f"[bench] {args.precision}: {args.steps} solves in {main_run['elapsed']:.3f} s, its {main_run['its']}",
+          file=sys.stderr, flush=True)
+    fp64 = None
+    if args.precision == "mixed" and not args.no_fp64_compare:
+        fp64 = run_solver(lsx, hip, geom, "fp64", args.krylov, args.restart, max(1, args.steps // 2), 1)
+        print(f"[bench] fp64: {len(fp64['its'])} solves in {fp64['elapsed']:.3f} s, its {fp64['its']}",
+              file=sys.stderr, flush=True)
 
-    for _ in range(args.warmup):
-        step()
-    hip.sync()
-    S.kernel_timing(True)
-    hip.sync()
-    t0 = time.perf_counter()
-    its = []
-    for _ in range(args.steps):
-        step()
-        its.append(it.value)
-    hip.sync()
-    elapsed = time.perf_counter() - t0
-    stats = {k: S.kernel_stats(k) for k in lsx.KERNEL_NAMES}
-    S.kernel_timing(False)
-
-    # true residual of the last solve, checked on the device operator
-    x = np.zeros(ncell)
-    hip.d2h(x, dx)
-    r = b - S.apply_operator(x, 0)
-    rel_res = float(np.linalg.norm(r) / nb)
-
+    stats, elapsed = main_run["stats"], main_run["elapsed"]
     dom = max(stats, key=lambda k: stats[k][1])
     launches, tot_ms, bytes_per = stats[dom]
     avg_s = tot_ms / launches / 1e3
     achieved = bytes_per / avg_s / 1e9
-    kernels = {lsx.KERNEL_NAMES[k]: {"launches": v[0], "avg_us": (v[1] / v[0] * 1e3) if v[0] else None,
-                                     "GBs": (v[2] / (v[1] / v[0] / 1e3) / 1e9) if v[0] else None,
+    kernels = {lsx.KERNEL_NAMES[k]: {"launches": v[0], "avg_us": v[1] / v[0] * 1e3,
+                                     "GBs": v[2] / (v[1] / v[0] / 1e3) / 1e9,
                                      "share_of_step": v[1] / 1e3 / elapsed}
                for k, v in stats.items() if v[0]}
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
         try:
-            traffic = json.load(open(tfile)).get(f"{lsx.KERNEL_NAMES[dom]}@{n}")
+            traffic = json.load(open(tfile)).get(f"{lsx.KERNEL_NAMES[dom]}@{args.precision}@{n}")
         except (ValueError, OSError):
             traffic = None
-
+    value = args.steps / elapsed
+    cfg = {
+        "workload": (f"{n}^3 stretched grid (ratio 1.02 to centre), ellipsoid body 0.25/0.15/0.15 "
+                     f"(level set of the ellipsoid the Bezier body approximates), {args.krylov.upper()}"
+                     f"({args.restart}), rel tol 1e-8, "
+                     + ("mixed: FP32 V(2,2) red-black GS multigrid, FP16 coarse faces, FP64 Krylov"
+                        if args.precision == "mixed" else "FP64 reference multigrid cascade (ADLJ)")),
+        "precision": args.precision, "krylov": args.krylov, "cells": geom.n,
+        "iterations": main_run["its"], "true_rel_residual": main_run["rel_res"],
+        "geometry_setup_s": geo_s, "solver_setup_s": main_run["setup_s"], "mg_levels": main_run["levels"],
+    }
+    if fp64 is not None:
+        fp64_rate = len(fp64["its"]) / fp64["elapsed"]
+        act = main_run["S"].operator_diagonal() < 0
+        err = float(np.linalg.norm((main_run["x"] - fp64["x"])[act]) / np.linalg.norm(fp64["x"][act]))
+        cfg.update({"fp64_solves_per_s": fp64_rate, "fp64_iterations": fp64["its"],
+                    "fp64_true_rel_residual": fp64["rel_res"], "mixed_over_fp64": value / fp64_rate,
+                    "pressure_rel_diff_vs_fp64": err})
     out = {
         "metric": METRIC,
-        "value": args.steps / elapsed,
+        "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -199,17 +240,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f16 (f64 Krylov)" if args.precision == "mixed" else "f64",
         "data": "synthetic",
-        "config": {
-            "workload": (f"{n}^3 stretched grid (ratio 1.02 to centre), ellipsoid body 0.25/0.15/0.15 "
-                         f"(level set of the ellipsoid the Bezier body approximates), "
-                         f"{args.krylov.upper()}({args.restart}) + reference multigrid cascade (ADLJ), FP64, "
-                         f"rel tol 1e-8"),
-            "precision": args.precision, "krylov": args.krylov, "cells": ncell,
-            "iterations": its, "true_rel_residual": rel_res, "setup_s": setup_s,
-            "mg_levels": S.num_levels(),
-        },
+        "config": cfg,
         "roofline": {"bound": "hbm", "kernel": lsx.KERNEL_NAMES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_per, "avg_launch_us": avg_s * 1e6},
@@ -218,7 +251,8 @@ def main():
     }
     if not args.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline(args.cpu_sample_grid, its[-1], n, args.restart, args.krylov)
+            out["cpu_baseline"] = cpu_baseline(args.cpu_sample_grid, main_run["its"][-1], n, args.restart,
+                                               args.krylov)
         except Exception as e:  # the GPU measurement stands on its own
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out), flush=True)

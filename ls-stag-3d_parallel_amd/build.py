@@ -31,6 +31,7 @@ SOURCES = {
     "lsx_solver.cpp": ["-ffp-contract=off"],
     "lsx_geometry.hip": ["-ffp-contract=off"],
     "lsx_geometry_host.cpp": ["-ffp-contract=off"],
+    "lsx_mixed.hip": [],
     "lsx_capi.cpp": [],
 }
 

@@ -569,9 +569,39 @@ __global__ void k_axpy_dot(double* __restrict__ w, const double* __restrict__ v,
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// y = A x on the fine 7-point operator with an FP32 input vector (mixed-precision Krylov basis)
+__global__ void k_stencil7_prod_f32x(const double* __restrict__ coef, int N, int M, int K,
+                                     const float* __restrict__ x, double* __restrict__ y) {
+  const long n = (long)N * M * K;
+  const int NM = N * M;
+  for (long li = blockIdx.x * (long)blockDim.x + threadIdx.x; li < n; li += (long)gridDim.x * blockDim.x) {
+    const int i = (int)li;
+    int xx, yy, zz;
+    decode(i, N, M, &xx, &yy, &zz);
+    double acc = 0.0;
+    if (zz > 0) acc = acc + coef[0 * n + i] * (double)x[i - NM];
+    if (yy > 0) acc = acc + coef[1 * n + i] * (double)x[i - N];
+    if (xx > 0) acc = acc + coef[2 * n + i] * (double)x[i - 1];
+    acc = acc + coef[3 * n + i] * (double)x[i];
+    if (xx < N - 1) acc = acc + coef[4 * n + i] * (double)x[i + 1];
+    if (yy < M - 1) acc = acc + coef[5 * n + i] * (double)x[i + N];
+    if (zz < K - 1) acc = acc + coef[6 * n + i] * (double)x[i + NM];
+    y[i] = acc;
+  }
+}
+
 struct YCoef {
   double y[64];
 };
+
+__global__ void k_sol_update_f32(double* __restrict__ x, const float* __restrict__ Z, long ldz, YCoef yk, int count,
+                                 long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double aux = x[i];
+    for (int k = count - 1; k >= 0; --k) aux = aux + (double)Z[k * ldz + i] * yk.y[k];
+    x[i] = aux;
+  }
+}
 
 __global__ void k_sol_update(double* __restrict__ x, const double* __restrict__ Z, long ldz, YCoef yk, int count,
                              long n) {
@@ -665,6 +695,12 @@ void diag_absmax(const double* coef, int npts, Dims d, double* partial, double* 
   k_max_finalize<RED_THREADS><<<1, RED_THREADS, 0, s>>>(partial, nb, out);
 }
 
+void absmax(const double* a, long n, double* partial, double* out, hipStream_t s) {
+  const int nb = reduction_blocks(n);
+  k_diag_absmax_partial<RED_THREADS><<<nb, RED_THREADS, 0, s>>>(a, n, partial);
+  k_max_finalize<RED_THREADS><<<1, RED_THREADS, 0, s>>>(partial, nb, out);
+}
+
 void line_factor(const double* coef, int npts, Dims d, int dir, const double* absmax, double* dinv, double* eu,
                  hipStream_t s) {
   const int nlines = dir == 0 ? d.M * d.K : (dir == 1 ? d.N * d.K : d.N * d.M);
@@ -739,6 +775,21 @@ void sol_update(double* x, const double* Z, long ldz, const double* yk, int coun
     YCoef c;
     for (int k = lo; k < hi; ++k) c.y[k - lo] = yk[k];
     k_sol_update<<<grid_1d(n), 256, 0, s>>>(x, Z + (long)lo * ldz, ldz, c, hi - lo, n);
+    hi = lo;
+  }
+}
+
+void stencil7_prod_f32x(const double* coef, Dims d, const float* x, double* y, hipStream_t s) {
+  k_stencil7_prod_f32x<<<grid_1d(d.n()), 256, 0, s>>>(coef, d.N, d.M, d.K, x, y);
+}
+
+void sol_update_f32(double* x, const float* Z, long ldz, const double* yk, int count, long n, hipStream_t s) {
+  int hi = count;
+  while (hi > 0) {
+    const int lo = hi > 64 ? hi - 64 : 0;
+    YCoef c;
+    for (int k = lo; k < hi; ++k) c.y[k - lo] = yk[k];
+    k_sol_update_f32<<<grid_1d(n), 256, 0, s>>>(x, Z + (long)lo * ldz, ldz, c, hi - lo, n);
     hi = lo;
   }
 }

@@ -29,6 +29,8 @@ void csr_to_stencil7(const int* rowptr, const int* col, const double* val, Dims 
                      hipStream_t s);
 // Galerkin coarse operator A_c = R A_f P (even-even-even transfers), 27 planes on the coarse grid.
 void galerkin(const double* coef_f, int npts_f, Dims df, double* coef27_c, Dims dc, hipStream_t s);
+// max |a_i| of an array into out[0]
+void absmax(const double* a, long n, double* partial, double* out, hipStream_t s);
 // max |diag| of an operator into out[0]
 void diag_absmax(const double* coef, int npts, Dims d, double* partial, double* out, hipStream_t s);
 // Thomas LU factors of the line systems of direction dir (0 x, 1 y, 2 z): dinv, eu (scaled upper)
@@ -59,6 +61,9 @@ void axpy_dot(double* w, const double* v, const double* h, const double* vnext, 
               hipStream_t s);
 // x += sum_k y_k z_k in the order k = last..0 (coefficients from host array, up to 64)
 void sol_update(double* x, const double* Z, long ldz, const double* yk, int count, long n, hipStream_t s);
+// mixed precision: y = A x with FP32 x (fine 7-point operator); x += sum_k y_k z_k with FP32 z
+void stencil7_prod_f32x(const double* coef, Dims d, const float* x, double* y, hipStream_t s);
+void sol_update_f32(double* x, const float* Z, long ldz, const double* yk, int count, long n, hipStream_t s);
 // BiCGStab fused updates (bicgstab_interface.h:39-56)
 void bicg_s_update(double* nu, const double* r, double* sv, double alpha, long n, hipStream_t s);
 void bicg_sol_update(const double* sv, double* y, double* z, double* x, double* t, double* r, double alpha, double w,

@@ -1,0 +1,354 @@
+// Mixed-precision multigrid preconditioner: FP32 V-cycle with red-black Gauss-Seidel on
+// 7-point operators stored as face coefficients (FP32 on the finest level, FP16 with a
+// per-level scale on coarse levels), full-sum restriction, trilinear prolongation and an FP64
+// dense LU on the coarsest level.  It preconditions the FP64 flexible Krylov loop; nothing here
+// restates reference arithmetic, so ordinary FMA contraction is allowed.
+//
+// Face storage: ae[i] couples cell i with i+1 (x), an[i] with i+N, at[i] with i+N*M; a face on
+// the domain boundary is 0.  s[i] is 1/a_p (0 for solid and fully closed cells, whose rows the
+// reference keeps singular or near-singular).  The two pinned cells are identity rows.
+
+#include <hip/hip_fp16.h>
+
+#include "lsx_mixed.hpp"
+
+namespace lsx {
+namespace mix {
+
+namespace {
+
+__device__ inline void decode(long i, int N, int M, int* x, int* y, int* z) {
+  const long NM = (long)N * M;
+  *z = (int)(i / NM);
+  const long r = i - (long)*z * NM;
+  *y = (int)(r / N);
+  *x = (int)(r - (long)*y * N);
+}
+
+inline int grid_1d(long n, int bs = 256) {
+  long g = (n + bs - 1) / bs;
+  if (g > 16384) g = 16384;
+  return (int)(g < 1 ? 1 : g);
+}
+
+template <class FT>
+__device__ inline float ld(const FT* p, long i) {
+  return (float)p[i];
+}
+
+// ---- setup -----------------------------------------------------------------------------------
+// faces and inverse diagonal of the finest level from the FP64 7-plane operator [b,s,w,p,e,n,t];
+// a face is read from the row of the non-pinned cell (pinned rows are identity rows).
+__global__ void k_faces_from7(const double* __restrict__ c7, int N, int M, int K, int pin0, int pin1,
+                              double* __restrict__ fe, double* __restrict__ fn, double* __restrict__ ft,
+                              double* __restrict__ dg) {
+  const long n = (long)N * M * K, NM = (long)N * M;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    const bool pin = (i == pin0) || (i == pin1);
+    fe[i] = x < N - 1 ? (pin ? c7[2 * n + i + 1] : c7[4 * n + i]) : 0.0;
+    fn[i] = y < M - 1 ? (pin ? c7[1 * n + i + N] : c7[5 * n + i]) : 0.0;
+    ft[i] = z < K - 1 ? (pin ? c7[0 * n + i + NM] : c7[6 * n + i]) : 0.0;
+    dg[i] = c7[3 * n + i];
+  }
+}
+
+// coarse faces: half the sum of the four fine faces covering each coarse face
+__global__ void k_aggregate(const double* __restrict__ fe, const double* __restrict__ fn, const double* __restrict__ ft,
+                            int Nf, int Mf, double* __restrict__ ce, double* __restrict__ cn, double* __restrict__ ct,
+                            int Nc, int Mc, int Kc) {
+  const long nc = (long)Nc * Mc * Kc, NMf = (long)Nf * Mf;
+  for (long I = blockIdx.x * (long)blockDim.x + threadIdx.x; I < nc; I += (long)gridDim.x * blockDim.x) {
+    int X, Y, Z;
+    decode(I, Nc, Mc, &X, &Y, &Z);
+    const long a = 2L * X + (long)Nf * 2 * Y + NMf * 2 * Z;
+    double se = 0, sn = 0, st = 0;
+    if (X < Nc - 1) {
+      const long f = a + 1;
+      se = fe[f] + fe[f + Nf] + fe[f + NMf] + fe[f + Nf + NMf];
+    }
+    if (Y < Mc - 1) {
+      const long f = a + Nf;
+      sn = fn[f] + fn[f + 1] + fn[f + NMf] + fn[f + 1 + NMf];
+    }
+    if (Z < Kc - 1) {
+      const long f = a + NMf;
+      st = ft[f] + ft[f + 1] + ft[f + Nf] + ft[f + 1 + Nf];
+    }
+    ce[I] = 0.5 * se;
+    cn[I] = 0.5 * sn;
+    ct[I] = 0.5 * st;
+  }
+}
+
+// Neumann diagonal of an aggregated level: -sum of its six faces
+__global__ void k_diag_from_faces(const double* __restrict__ fe, const double* __restrict__ fn,
+                                  const double* __restrict__ ft, int N, int M, int K, double* __restrict__ dg) {
+  const long n = (long)N * M * K, NM = (long)N * M;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    double d = -fe[i] - fn[i] - ft[i];
+    if (x > 0) d -= fe[i - 1];
+    if (y > 0) d -= fn[i - N];
+    if (z > 0) d -= ft[i - NM];
+    dg[i] = d;
+  }
+}
+
+// inverse diagonal with the zero-pivot rule of the line smoother (tiny = max(1e-5 max|d|, 1e-12))
+__global__ void k_inv_diag(const double* __restrict__ dg, long n, const double* __restrict__ absmax, int pin0,
+                           int pin1, float* __restrict__ s) {
+  double tiny = 1e-5 * absmax[0];
+  if (tiny < 1e-12) tiny = 1e-12;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double d = dg[i];
+    float v = fabs(d) < tiny ? 0.0f : (float)(1.0 / d);
+    if (i == pin0 || i == pin1) v = 1.0f;
+    s[i] = v;
+  }
+}
+
+template <class FT>
+__global__ void k_store_faces(const double* __restrict__ fe, const double* __restrict__ fn,
+                              const double* __restrict__ ft, long n, double inv_scale, FT* __restrict__ ae,
+                              FT* __restrict__ an, FT* __restrict__ at) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    ae[i] = (FT)(fe[i] * inv_scale);
+    an[i] = (FT)(fn[i] * inv_scale);
+    at[i] = (FT)(ft[i] * inv_scale);
+  }
+}
+
+__global__ void k_absmax3(const double* __restrict__ a, const double* __restrict__ b, const double* __restrict__ c,
+                          long n, double* __restrict__ partial) {
+  __shared__ double lds[4];
+  double m = 0.0;
+  const long per = (n + gridDim.x - 1) / gridDim.x;
+  const long lo = per * blockIdx.x, hi = lo + per < n ? lo + per : n;
+  for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) m = fmax(m, fmax(fabs(a[i]), fmax(fabs(b[i]), fabs(c[i]))));
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+}
+
+__global__ void k_max_reduce(const double* __restrict__ partial, int nb, double* out) {
+  __shared__ double lds[4];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) m = fmax(m, partial[i]);
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+}
+
+// coarsest level as the 27-plane FP64 layout of the dense LU kernels
+__global__ void k_to27(const double* __restrict__ fe, const double* __restrict__ fn, const double* __restrict__ ft,
+                       const double* __restrict__ dg, int N, int M, int K, int pin0, int pin1, double* __restrict__ c27) {
+  const long n = (long)N * M * K, NM = (long)N * M;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    for (int q = 0; q < 27; ++q) c27[q * n + i] = 0.0;
+    if (i == pin0 || i == pin1) {
+      c27[13 * n + i] = 1.0;
+      continue;
+    }
+    c27[13 * n + i] = dg[i];
+    if (x < N - 1) c27[14 * n + i] = fe[i];
+    if (x > 0) c27[12 * n + i] = fe[i - 1];
+    if (y < M - 1) c27[16 * n + i] = fn[i];
+    if (y > 0) c27[10 * n + i] = fn[i - N];
+    if (z < K - 1) c27[22 * n + i] = ft[i];
+    if (z > 0) c27[4 * n + i] = ft[i - NM];
+  }
+}
+
+// ---- V-cycle ---------------------------------------------------------------------------------
+template <class FT>
+__device__ inline float offdiag(const float* __restrict__ x, const FT* __restrict__ ae, const FT* __restrict__ an,
+                                const FT* __restrict__ at, long i, int xx, int yy, int zz, int N, int M, int K) {
+  const long NM = (long)N * M;
+  float acc = 0.0f;
+  if (xx > 0) acc += ld(ae, i - 1) * x[i - 1];
+  if (xx < N - 1) acc += ld(ae, i) * x[i + 1];
+  if (yy > 0) acc += ld(an, i - N) * x[i - N];
+  if (yy < M - 1) acc += ld(an, i) * x[i + N];
+  if (zz > 0) acc += ld(at, i - NM) * x[i - NM];
+  if (zz < K - 1) acc += ld(at, i) * x[i + NM];
+  return acc;
+}
+
+// one colour of a red-black Gauss-Seidel sweep; thread t owns the t-th cell of that colour
+template <class FT>
+__global__ void k_rbgs(float* __restrict__ x, const float* __restrict__ b, const FT* __restrict__ ae,
+                       const FT* __restrict__ an, const FT* __restrict__ at, const float* __restrict__ s, float fs,
+                       int N, int M, int K, int color, int pin0, int pin1, int zero_guess) {
+  const long half_row = (N + 1) / 2;
+  const long rows = (long)M * K;
+  const long total = half_row * rows;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const long row = t / half_row;
+    const int yy = (int)(row % M), zz = (int)(row / M);
+    const int xx = 2 * (int)(t - row * half_row) + ((yy + zz + color) & 1);
+    if (xx >= N) continue;
+    const long i = (long)xx + (long)N * row;
+    float v;
+    if (i == pin0 || i == pin1) {
+      v = b[i];
+    } else if (zero_guess) {
+      v = b[i] * s[i];
+    } else {
+      v = (b[i] - fs * offdiag(x, ae, an, at, i, xx, yy, zz, N, M, K)) * s[i];
+    }
+    x[i] = v;
+  }
+}
+
+// r = b - A x on the 8 children of each coarse cell, summed into the coarse right-hand side
+template <class FT>
+__global__ void k_resid_restrict(const float* __restrict__ x, const float* __restrict__ b, const FT* __restrict__ ae,
+                                 const FT* __restrict__ an, const FT* __restrict__ at, const float* __restrict__ s,
+                                 float fs, int N, int M, int K, int pin0, int pin1, float* __restrict__ bc) {
+  const int Nc = N / 2, Mc = M / 2, Kc = K / 2;
+  const long nc = (long)Nc * Mc * Kc, NM = (long)N * M;
+  for (long I = blockIdx.x * (long)blockDim.x + threadIdx.x; I < nc; I += (long)gridDim.x * blockDim.x) {
+    int X, Y, Z;
+    decode(I, Nc, Mc, &X, &Y, &Z);
+    float acc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int xx = 2 * X + (c & 1), yy = 2 * Y + ((c >> 1) & 1), zz = 2 * Z + (c >> 2);
+      const long i = xx + (long)N * yy + NM * zz;
+      float r;
+      if (i == pin0 || i == pin1) {
+        r = b[i] - x[i];
+      } else {
+        const float ap = s[i] != 0.0f ? 1.0f / s[i] : 0.0f;
+        r = b[i] - ap * x[i] - fs * offdiag(x, ae, an, at, i, xx, yy, zz, N, M, K);
+      }
+      acc += r;
+    }
+    bc[I] = acc;
+  }
+}
+
+__device__ inline void axis_w(int f, int nf, int* c, float* w, int* cnt) {
+  const int p = f >> 1;
+  if (f == 0 || f == nf - 1) {
+    c[0] = p; w[0] = 1.0f; *cnt = 1;
+  } else if (f & 1) {
+    c[0] = p; w[0] = 0.75f; c[1] = p + 1; w[1] = 0.25f; *cnt = 2;
+  } else {
+    c[0] = p - 1; w[0] = 0.25f; c[1] = p; w[1] = 0.75f; *cnt = 2;
+  }
+}
+
+// x_f += P x_c (trilinear)
+__global__ void k_prolong_add(const float* __restrict__ xc, int Nc, int Mc, float* __restrict__ xf, int Nf, int Mf,
+                              int Kf) {
+  const long nf = (long)Nf * Mf * Kf;
+  const long NMc = (long)Nc * Mc;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nf; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, Nf, Mf, &x, &y, &z);
+    int cx[2], cy[2], cz[2], nx, ny, nz;
+    float wx[2], wy[2], wz[2];
+    axis_w(x, Nf, cx, wx, &nx);
+    axis_w(y, Mf, cy, wy, &ny);
+    axis_w(z, Kf, cz, wz, &nz);
+    float acc = 0.0f;
+    for (int c3 = 0; c3 < nz; ++c3)
+      for (int c2 = 0; c2 < ny; ++c2)
+        for (int c1 = 0; c1 < nx; ++c1) acc += wx[c1] * wy[c2] * wz[c3] * xc[cx[c1] + Nc * cy[c2] + NMc * cz[c3]];
+    xf[i] += acc;
+  }
+}
+
+__global__ void k_f64_to_f32(const double* __restrict__ a, float* __restrict__ b, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = (float)a[i];
+}
+__global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ b, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = (double)a[i];
+}
+
+}  // namespace
+
+// ================================================================================================
+void faces_from7(const double* c7, Dims d, int pin0, int pin1, double* fe, double* fn, double* ft, double* dg,
+                 hipStream_t s) {
+  k_faces_from7<<<grid_1d(d.n()), 256, 0, s>>>(c7, d.N, d.M, d.K, pin0, pin1, fe, fn, ft, dg);
+}
+
+void aggregate(const double* fe, const double* fn, const double* ft, Dims df, double* ce, double* cn, double* ct,
+               Dims dc, hipStream_t s) {
+  k_aggregate<<<grid_1d(dc.n()), 256, 0, s>>>(fe, fn, ft, df.N, df.M, ce, cn, ct, dc.N, dc.M, dc.K);
+}
+
+void diag_from_faces(const double* fe, const double* fn, const double* ft, Dims d, double* dg, hipStream_t s) {
+  k_diag_from_faces<<<grid_1d(d.n()), 256, 0, s>>>(fe, fn, ft, d.N, d.M, d.K, dg);
+}
+
+void face_absmax(const double* fe, const double* fn, const double* ft, long n, double* partial, double* out,
+                 hipStream_t s) {
+  int nb = (int)((n + 1023) / 1024);
+  if (nb > 1024) nb = 1024;
+  if (nb < 1) nb = 1;
+  k_absmax3<<<nb, 256, 0, s>>>(fe, fn, ft, n, partial);
+  k_max_reduce<<<1, 256, 0, s>>>(partial, nb, out);
+}
+
+void inv_diag(const double* dg, long n, const double* absmax, int pin0, int pin1, float* sv, hipStream_t s) {
+  k_inv_diag<<<grid_1d(n), 256, 0, s>>>(dg, n, absmax, pin0, pin1, sv);
+}
+
+void store_faces_f32(const double* fe, const double* fn, const double* ft, long n, double inv_scale, float* ae,
+                     float* an, float* at, hipStream_t s) {
+  k_store_faces<float><<<grid_1d(n), 256, 0, s>>>(fe, fn, ft, n, inv_scale, ae, an, at);
+}
+
+void store_faces_f16(const double* fe, const double* fn, const double* ft, long n, double inv_scale, _Float16* ae,
+                     _Float16* an, _Float16* at, hipStream_t s) {
+  k_store_faces<_Float16><<<grid_1d(n), 256, 0, s>>>(fe, fn, ft, n, inv_scale, ae, an, at);
+}
+
+void to27(const double* fe, const double* fn, const double* ft, const double* dg, Dims d, int pin0, int pin1,
+          double* c27, hipStream_t s) {
+  k_to27<<<grid_1d(d.n()), 256, 0, s>>>(fe, fn, ft, dg, d.N, d.M, d.K, pin0, pin1, c27);
+}
+
+template <class FT>
+static void rbgs_t(float* x, const float* b, const FT* ae, const FT* an, const FT* at, const float* sv, float fs,
+                   Dims d, int color, int pin0, int pin1, int zero_guess, hipStream_t s) {
+  const long total = (long)((d.N + 1) / 2) * d.M * d.K;
+  k_rbgs<FT><<<grid_1d(total), 256, 0, s>>>(x, b, ae, an, at, sv, fs, d.N, d.M, d.K, color, pin0, pin1, zero_guess);
+}
+
+void rbgs(const Level& L, float* x, const float* b, int color, int zero_guess, hipStream_t s) {
+  if (L.f16)
+    rbgs_t<_Float16>(x, b, L.h_e, L.h_n, L.h_t, L.s, L.fscale, L.d, color, L.pin0, L.pin1, zero_guess, s);
+  else
+    rbgs_t<float>(x, b, L.f_e, L.f_n, L.f_t, L.s, L.fscale, L.d, color, L.pin0, L.pin1, zero_guess, s);
+}
+
+void resid_restrict(const Level& L, const float* x, const float* b, float* bc, hipStream_t s) {
+  const long nc = (long)(L.d.N / 2) * (L.d.M / 2) * (L.d.K / 2);
+  if (L.f16)
+    k_resid_restrict<_Float16><<<grid_1d(nc), 256, 0, s>>>(x, b, L.h_e, L.h_n, L.h_t, L.s, L.fscale, L.d.N, L.d.M,
+                                                           L.d.K, L.pin0, L.pin1, bc);
+  else
+    k_resid_restrict<float><<<grid_1d(nc), 256, 0, s>>>(x, b, L.f_e, L.f_n, L.f_t, L.s, L.fscale, L.d.N, L.d.M, L.d.K,
+                                                        L.pin0, L.pin1, bc);
+}
+
+void prolong_add(const float* xc, Dims dc, float* xf, Dims df, hipStream_t s) {
+  k_prolong_add<<<grid_1d(df.n()), 256, 0, s>>>(xc, dc.N, dc.M, xf, df.N, df.M, df.K);
+}
+
+void f64_to_f32(const double* a, float* b, long n, hipStream_t s) { k_f64_to_f32<<<grid_1d(n), 256, 0, s>>>(a, b, n); }
+void f32_to_f64(const float* a, double* b, long n, hipStream_t s) { k_f32_to_f64<<<grid_1d(n), 256, 0, s>>>(a, b, n); }
+
+}  // namespace mix
+}  // namespace lsx

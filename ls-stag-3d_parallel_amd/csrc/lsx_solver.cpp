@@ -24,8 +24,13 @@ Solver::Solver(const lsx_options& o, hipStream_t s) : opt(o) {
   if (opt.max_it < 1) throw Error(LSX_ERR_ARG, "max_it must be >= 1");
   if (opt.n_post < 0) throw Error(LSX_ERR_ARG, "n_post must be >= 0");
   if (opt.krylov != LSX_KRYLOV_FGMRES && opt.krylov != LSX_KRYLOV_BICGSTAB) throw Error(LSX_ERR_ARG, "unknown krylov");
-  if (opt.mg != LSX_MG_CASCADE) throw Error(LSX_ERR_ARG, "only the cascade multigrid is available in this build");
-  if (opt.precision != LSX_PREC_FP64) throw Error(LSX_ERR_ARG, "only FP64 precision is available in this build");
+  if (opt.precision == LSX_PREC_FP64 && opt.mg != LSX_MG_CASCADE)
+    throw Error(LSX_ERR_ARG, "FP64 precision uses the reference cascade multigrid (LSX_MG_CASCADE)");
+  if (opt.precision == LSX_PREC_MIXED && opt.mg != LSX_MG_VCYCLE)
+    throw Error(LSX_ERR_ARG, "mixed precision uses the V-cycle multigrid (LSX_MG_VCYCLE)");
+  if (opt.precision != LSX_PREC_FP64 && opt.precision != LSX_PREC_MIXED) throw Error(LSX_ERR_ARG, "unknown precision");
+  if (opt.mg == LSX_MG_VCYCLE && (opt.n_pre < 1 || opt.n_post < 1))
+    throw Error(LSX_ERR_ARG, "the V-cycle needs n_pre >= 1 and n_post >= 1");
   if (s) {
     stream = s;
   } else {
@@ -104,6 +109,8 @@ double Solver::kernel_bytes(int k) const {
     case LSX_K_LINE_Z: return 56.0 * n;    // r 8, two factor planes 16, sub coupling 8, ch 16, c 8
     case LSX_K_OPERATOR: return 48.0 * n;  // z 8, coefficients 32, w 8
     case LSX_K_ORTHO: return 32.0 * n;     // w read + write 16, v_i 8, v_i+1 8
+    case LSX_K_SMOOTH:                     // n_pre red-black sweeps, FP32: x 4 + faces 12 + 1/a_p 4
+      return 28.0 * n * opt.n_pre;         //   + b 4 + x write 4 per sweep
     default: return 0.0;
   }
 }
@@ -165,6 +172,10 @@ void Solver::setup_levels(int N, int M, int K) {
   for (size_t l = 0; l < levels.size(); ++l) {
     MGLevel& L = levels[l];
     const long nl = L.d.n();
+    if (mixed()) {  // the V-cycle keeps its own compact levels; only the FP64 fine operator here
+      if (l == 0) L.coef.alloc((size_t)L.npts * nl);
+      continue;
+    }
     L.coef.alloc((size_t)L.npts * nl);
     L.r.alloc(nl);
     if (l > 0) L.ch.alloc(nl);
@@ -174,7 +185,7 @@ void Solver::setup_levels(int N, int M, int K) {
         L.eu[dir].alloc(nl);
       }
   }
-  cwork.alloc(d0.n());
+  if (!mixed()) cwork.alloc(d0.n());
   const long nc = levels.back().d.n();
   lu_scale.alloc(nc);
   lu_perm.alloc(nc);
@@ -256,6 +267,10 @@ void Solver::build_hierarchy() {
     LSX_HIP(hipStreamSynchronize(stream));
     if (herr) throw Error(LSX_ERR_ARG, "matrix has a column outside the 7-point neighbourhood of its row");
   }
+  if (mixed()) {
+    build_mixed();
+    return;
+  }
   for (size_t l = 0; l + 1 < levels.size(); ++l) {
     MGLevel& f = levels[l];
     MGLevel& c = levels[l + 1];
@@ -270,11 +285,143 @@ void Solver::build_hierarchy() {
   LSX_HIP(hipStreamSynchronize(stream));
 }
 
+// -----------------------------------------------------------------------------------------------
+// mixed-precision V-cycle hierarchy: faces of the fine operator (FP32), aggregated coarse faces
+// (half the sum of the four fine faces of each coarse face, stored FP16 with a per-level scale),
+// Neumann diagonals, the two pinned cells carried to every level, FP64 LU on the coarsest.
+// -----------------------------------------------------------------------------------------------
+void Solver::build_mixed() {
+  const int L = (int)levels.size() - 1;
+  mlev.clear();
+  mlev.resize(L + 1);
+  std::vector<DevBuf<double>> fe(2), fn(2), ft(2), dg(2);
+  int cur = 0;
+  {
+    const long n = d0.n();
+    fe[0].alloc(n);
+    fn[0].alloc(n);
+    ft[0].alloc(n);
+    dg[0].alloc(n);
+    fe[1].alloc(std::max<long>(1, n / 8));
+    fn[1].alloc(std::max<long>(1, n / 8));
+    ft[1].alloc(std::max<long>(1, n / 8));
+    dg[1].alloc(std::max<long>(1, n / 8));
+  }
+  for (int l = 0; l <= L; ++l) {
+    const Dims d = levels[l].d;
+    const long nl = d.n();
+    const int pin0 = d.N - 1, pin1 = d.N * d.M - 1;
+    MixStore& S = mlev[l];
+    S.L.d = d;
+    S.L.pin0 = pin0;
+    S.L.pin1 = pin1;
+    if (l == 0) {
+      mix::faces_from7(levels[0].coef.p, d, pin0, pin1, fe[cur].p, fn[cur].p, ft[cur].p, dg[cur].p, stream);
+    } else {
+      const Dims df = levels[l - 1].d;
+      const int nxt = cur ^ 1;
+      mix::aggregate(fe[cur].p, fn[cur].p, ft[cur].p, df, fe[nxt].p, fn[nxt].p, ft[nxt].p, d, stream);
+      mix::diag_from_faces(fe[nxt].p, fn[nxt].p, ft[nxt].p, d, dg[nxt].p, stream);
+      cur = nxt;
+    }
+    if (l == L) {  // coarsest: FP64 LU of the 7-point operator
+      DevBuf<double> c27;
+      c27.alloc(27 * nl);
+      mix::to27(fe[cur].p, fn[cur].p, ft[cur].p, dg[cur].p, d, pin0, pin1, c27.p, stream);
+      lu_factor(c27.p, d, lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, lu_col.p, stream);
+      S.b.alloc(nl);
+      S.x.alloc(nl);
+      cb64.alloc(nl);
+      cx64.alloc(nl);
+      LSX_HIP(hipStreamSynchronize(stream));
+      break;
+    }
+    S.s.alloc(nl);
+    S.b.alloc(nl);
+    if (l > 0) S.x.alloc(nl);  // level 0 writes into the caller's output
+    absmax(dg[cur].p, nl, partial.p, scalars.p + SC_TINY, stream);
+    mix::inv_diag(dg[cur].p, nl, scalars.p + SC_TINY, pin0, pin1, S.s.p, stream);
+    if (l == 0) {
+      S.fe.alloc(nl);
+      S.fn.alloc(nl);
+      S.ft.alloc(nl);
+      mix::store_faces_f32(fe[cur].p, fn[cur].p, ft[cur].p, nl, 1.0, S.fe.p, S.fn.p, S.ft.p, stream);
+      S.L.f16 = false;
+      S.L.f_e = S.fe.p;
+      S.L.f_n = S.fn.p;
+      S.L.f_t = S.ft.p;
+      S.L.fscale = 1.0f;
+    } else {
+      mix::face_absmax(fe[cur].p, fn[cur].p, ft[cur].p, nl, partial.p, scalars.p + SC_TMP, stream);
+      double amax = reduce_to_host(scalars.p + SC_TMP);
+      const double scale = amax > 0 ? amax / 16384.0 : 1.0;  // stored faces within [0, 16384]
+      S.he.alloc(nl);
+      S.hn.alloc(nl);
+      S.ht.alloc(nl);
+      mix::store_faces_f16(fe[cur].p, fn[cur].p, ft[cur].p, nl, 1.0 / scale, S.he.p, S.hn.p, S.ht.p, stream);
+      S.L.f16 = true;
+      S.L.h_e = S.he.p;
+      S.L.h_n = S.hn.p;
+      S.L.h_t = S.ht.p;
+      S.L.fscale = (float)scale;
+    }
+    S.L.s = S.s.p;
+    LSX_HIP(hipStreamSynchronize(stream));  // the ping-pong temporaries are reused next level
+  }
+  LSX_HIP(hipGetLastError());
+  ztmp.alloc(d0.n());
+}
+
+void Solver::vcycle_apply(const double* v, float* z) {
+  const int L = (int)mlev.size() - 1;
+  mix::f64_to_f32(v, mlev[0].b.p, d0.n(), stream);
+  for (int l = 0; l < L; ++l) {
+    MixStore& S = mlev[l];
+    float* x = l == 0 ? z : S.x.p;
+    hipEvent_t e0 = nullptr;
+    if (l == 0) tbeg(LSX_K_SMOOTH, &e0);
+    for (int s = 0; s < opt.n_pre; ++s) {
+      mix::rbgs(S.L, x, S.b.p, 0, s == 0, stream);
+      mix::rbgs(S.L, x, S.b.p, 1, 0, stream);
+    }
+    if (l == 0) tend(LSX_K_SMOOTH, e0);
+    mix::resid_restrict(S.L, x, S.b.p, mlev[l + 1].b.p, stream);
+  }
+  {
+    MixStore& C = mlev[L];
+    const long nc = C.L.d.n();
+    mix::f32_to_f64(C.b.p, cb64.p, nc, stream);
+    lu_solve(lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, (int)nc, cb64.p, cx64.p, stream);
+    mix::f64_to_f32(cx64.p, C.x.p, nc, stream);
+  }
+  for (int l = L - 1; l >= 0; --l) {
+    MixStore& S = mlev[l];
+    float* x = l == 0 ? z : S.x.p;
+    mix::prolong_add(mlev[l + 1].x.p, mlev[l + 1].L.d, x, S.L.d, stream);
+    for (int s = 0; s < opt.n_post; ++s) {
+      mix::rbgs(S.L, x, S.b.p, 0, 0, stream);
+      mix::rbgs(S.L, x, S.b.p, 1, 0, stream);
+    }
+  }
+}
+
+void Solver::precond(const double* v, double* z64) {
+  if (mixed()) {
+    vcycle_apply(v, ztmp.p);
+    mix::f32_to_f64(ztmp.p, z64, d0.n(), stream);
+  } else {
+    mg_apply(v, z64);
+  }
+}
+
 void Solver::alloc_krylov() {
   const long n = d0.n();
   if (opt.krylov == LSX_KRYLOV_FGMRES) {
     V.alloc((size_t)opt.restart * n);
-    Z.alloc((size_t)opt.restart * n);
+    if (mixed())
+      Z32.alloc((size_t)opt.restart * n);
+    else
+      Z.alloc((size_t)opt.restart * n);
     w.alloc(n);
     H.assign(opt.restart * (opt.restart + 1) / 2, 0.0);
     cs.assign(opt.restart, 0.0);
@@ -357,7 +504,10 @@ int Solver::fgmres(const double* b, double* x) {
       }
       ycoef[k] = curY;
     }
-    sol_update(x, Z.p, n, ycoef.data(), last + 1, n, stream);
+    if (mixed())
+      sol_update_f32(x, Z32.p, n, ycoef.data(), last + 1, n, stream);
+    else
+      sol_update(x, Z.p, n, ycoef.data(), last + 1, n, stream);
   };
 
   stencil_apply(F.coef.p, 7, d0, x, b, w.p, ST_RESID, partial.p, stream);
@@ -373,12 +523,19 @@ int Solver::fgmres(const double* b, double* x) {
     LSX_HIP(hipMemcpyAsync(dnorm, &normW, sizeof(double), hipMemcpyHostToDevice, stream));
     for (int j = 0, hColStart = 0, hNum = 0; j < m; ++j, hColStart = hNum, ++iters) {
       double* vj = V.p + (size_t)j * n;
-      double* zj = Z.p + (size_t)j * n;
+      double* zj = mixed() ? nullptr : Z.p + (size_t)j * n;
+      float* zj32 = mixed() ? Z32.p + (size_t)j * n : nullptr;
       scale_copy(w.p, dnorm, vj, n, stream);
-      mg_apply(vj, zj);
+      if (mixed())
+        vcycle_apply(vj, zj32);
+      else
+        mg_apply(vj, zj);
       hipEvent_t e0 = nullptr;
       tbeg(LSX_K_OPERATOR, &e0);
-      stencil_apply(F.coef.p, 7, d0, zj, nullptr, w.p, ST_PROD, nullptr, stream);
+      if (mixed())
+        stencil7_prod_f32x(F.coef.p, d0, zj32, w.p, stream);
+      else
+        stencil_apply(F.coef.p, 7, d0, zj, nullptr, w.p, ST_PROD, nullptr, stream);
       tend(LSX_K_OPERATOR, e0);
       // modified Gram-Schmidt: h_i = w . v_i ; w -= h_i v_i  (the dot of the next basis vector
       // is fused into the update pass; the last pass yields ||w||^2)
@@ -448,13 +605,13 @@ int Solver::bicgstab(const double* b, double* x) {
     copy(br.p, brr.p, n, stream);
     copy(br.p, bp.p, n, stream);
     do {
-      mg_apply(bp.p, by.p);
+      precond(bp.p, by.p);
       stencil_apply(F.coef.p, 7, d0, by.p, nullptr, bnu.p, ST_PROD, nullptr, stream);
       double vs = dot(brr.p, br.p, n, tmp);
       double coef = dot(brr.p, bnu.p, n, tmp);
       const double alpha = vs / coef;
       bicg_s_update(bnu.p, br.p, bs.p, alpha, n, stream);
-      mg_apply(bs.p, bz.p);
+      precond(bs.p, bz.p);
       stencil_apply(F.coef.p, 7, d0, bz.p, nullptr, bt.p, ST_PROD, nullptr, stream);
       double wv = dot(bt.p, bs.p, n, tmp);
       coef = dot(bt.p, bt.p, n, tmp);
@@ -505,10 +662,10 @@ void Solver::precondition(const double* v, double* z, int mem) {
     hb.alloc(n);
     hx.alloc(n);
     LSX_HIP(hipMemcpyAsync(hb.p, v, n * sizeof(double), hipMemcpyHostToDevice, stream));
-    mg_apply(hb.p, hx.p);
+    precond(hb.p, hx.p);
     LSX_HIP(hipMemcpyAsync(z, hx.p, n * sizeof(double), hipMemcpyDeviceToHost, stream));
   } else {
-    mg_apply(v, z);
+    precond(v, z);
   }
   LSX_HIP(hipGetLastError());
   LSX_HIP(hipStreamSynchronize(stream));

@@ -9,6 +9,7 @@
 
 #include "../../include/lsx.h"
 #include "lsx_kernels.hpp"
+#include "lsx_mixed.hpp"
 
 namespace lsx {
 
@@ -124,6 +125,20 @@ class Solver {
   DevBuf<double> hb, hx;                              // staging for host inputs
   std::vector<double> H, cs, sn, y;
   bool ready = false;
+
+  // mixed-precision V-cycle hierarchy
+  struct MixStore {
+    mix::Level L;
+    DevBuf<float> fe, fn, ft, s, x, b;
+    DevBuf<_Float16> he, hn, ht;
+  };
+  std::vector<MixStore> mlev;
+  DevBuf<float> Z32, ztmp;
+  DevBuf<double> cb64, cx64;
+  bool mixed() const { return opt.precision == LSX_PREC_MIXED; }
+  void build_mixed();
+  void vcycle_apply(const double* v, float* z);
+  void precond(const double* v, double* z64);  // either preconditioner, FP64 output
 
   // event timing of fine-level kernel launches
   bool timing = false;

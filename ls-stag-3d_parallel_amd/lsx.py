@@ -176,7 +176,8 @@ def _ptr(a):
 class Task:
     """The solver fields of the reference BaseTaskDescription (base_task_description.h:41-46)."""
 
-    def __init__(self, eps=1e-5, MAX_ITER=200, FGMRES_RESTART_NUM=13, n_post=3, omega=0.805):
+    def __init__(self, eps=1e-5, MAX_ITER=200, FGMRES_RESTART_NUM=13, n_post=3, omega=0.805, n_pre=0):
+        self.n_pre = n_pre
         self.eps = eps
         self.MAX_ITER = MAX_ITER
         self.FGMRES_RESTART_NUM = FGMRES_RESTART_NUM
@@ -212,6 +213,7 @@ class IterativeSolver:
         opt.max_it = int(task.MAX_ITER)
         opt.eps = float(task.eps)
         opt.n_post = int(task.n_post)
+        opt.n_pre = int(getattr(task, "n_pre", 0))
         opt.omega = float(task.omega)
         opt.precision = self.precision
         opt.mg = self.mg
@@ -239,6 +241,7 @@ class IterativeSolver:
         opt.max_it = int(task.MAX_ITER)
         opt.eps = float(task.eps)
         opt.n_post = int(task.n_post)
+        opt.n_pre = int(getattr(task, "n_pre", 0))
         opt.omega = float(task.omega)
         opt.precision = self.precision
         opt.mg = self.mg
@@ -340,3 +343,16 @@ class MGFGMRES(IterativeSolver):
 class MGBiCGStab(IterativeSolver):
     """Reference MGBiCGStab (src/Base/Solvers/BiCGStab/mg_bicgstab.h:33-52)."""
     krylov = KRYLOV_BICGSTAB
+
+
+def mixed_task(eps, restart=30, max_it=1000, n_pre=2, n_post=2):
+    """Task for the mixed-precision solvers: FP32 V(n_pre, n_post) red-black Gauss-Seidel cycle."""
+    return Task(eps=eps, MAX_ITER=max_it, FGMRES_RESTART_NUM=restart, n_post=n_post, n_pre=n_pre)
+
+
+def solver(kind="fgmres", precision="fp64", **kw):
+    """MGFGMRES / MGBiCGStab in the reference FP64 mode or the mixed-precision V-cycle mode."""
+    cls = MGFGMRES if kind == "fgmres" else MGBiCGStab
+    if precision == "mixed":
+        return cls(precision=PREC_MIXED, mg=MG_VCYCLE, **kw)
+    return cls(**kw)

@@ -16,7 +16,7 @@ def axes_for(lsx, dims, ratio):
 CASES = [
     ("sphere16", (16, 16, 16), 1.0, "sphere", (0.35, 0.5, 0.5), (0.2,)),
     ("sphere32", (32, 32, 32), 1.0, "sphere", (0.35, 0.5, 0.5), (0.2,)),
-    ("sphere48x32x40", (48, 32, 40), 1.0, "sphere", (0.4, 0.45, 0.55), (0.22,)),
+    ("sphere48x32x32", (48, 32, 32), 1.0, "sphere", (0.4, 0.45, 0.55), (0.22,)),
     ("ellipsoid32_stretched", (32, 32, 32), 1.02, "ellipsoid", (0.5, 0.5, 0.5), (0.25, 0.15, 0.15)),
     ("ellipsoid64_stretched", (64, 64, 64), 1.02, "ellipsoid", (0.5, 0.5, 0.5), (0.25, 0.15, 0.15)),
 ]
