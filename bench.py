@@ -2,8 +2,11 @@
 """Pressure-solve throughput on one MI355X.
 
 A step is one pressure Poisson solve of the cut-cell operator to relative residual 1e-8
-(absolute eps = 1e-8 ||b||, x0 = 0), FGMRES(30) (or BiCGStab) preconditioned by geometric
-multigrid.  Workload: 256^3 cells on the unit cube, per-axis geometric stretching ratio 1.02
+(absolute eps = 1e-8 ||b||, x0 = 0), FGMRES(30) (or BiCGStab) preconditioned by a geometric
+multigrid V-cycle: mixed precision (FP32 red-black Gauss-Seidel, FP16 coarse faces) by
+default, compared with the same V-cycle in FP64 (the plain FP64 GPU solve) and with the
+reference's FP64 multigrid cascade, which is reported at an iteration cap because it does not
+reach 1e-8 on this stretched grid.  Workload: 256^3 cells on the unit cube, per-axis geometric stretching ratio 1.02
 towards the body centre, an ellipsoidal body (semi-axes 0.25/0.15/0.15) at the centre, the
 reference's pinned-outlet pressure boundary conditions, a seeded unit-variance Gaussian
 right-hand side on fluid cells with its mean removed.  Geometry, operator and multigrid
@@ -81,9 +84,11 @@ def workload(n, ratio=1.02, center=(0.5, 0.5, 0.5), axes=(0.25, 0.15, 0.15)):
     return widths, center, axes
 
 
-def cpu_baseline(sample_n, gpu_it, gpu_n, restart, krylov):
-    """The CPU oracle timed on this host on the same workload at sample_n^3 (solve only), scaled to
-    one gpu_n^3 solve by the cell ratio and the iteration-count ratio."""
+def cpu_baseline(sample_n, gpu_n, restart, krylov):
+    """The CPU oracle (the reference's FP64 cascade algorithm) timed on this host, solving the same
+    workload at sample_n^3 to 1e-8, scaled to gpu_n^3 by the cell count at the same iteration
+    count.  At 256^3 the reference algorithm does not reach 1e-8 at all, so this scaled rate is an
+    upper bound on what the CPU path achieves there."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py as O
     lsx = load_lsx()
@@ -97,21 +102,21 @@ def cpu_baseline(sample_n, gpu_it, gpu_n, restart, krylov):
     t0 = time.perf_counter()
     x, it, nr = S.solve(b)
     t = time.perf_counter() - t0
-    scale = (gpu_n / sample_n) ** 3 * (gpu_it / max(it, 1))
+    scale = (gpu_n / sample_n) ** 3
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
         "value": 1.0 / (t * scale), "unit": "solves/s", "cores": cores, "kind": "port",
-        "sample": (f"oracle {krylov.upper()} solve at {sample_n}^3 ({it} iterations, {t:.2f} s, "
-                   f"{cores} OpenMP threads), scaled to {gpu_n}^3 by cells x{(gpu_n / sample_n) ** 3:.0f} "
-                   f"and iterations {gpu_it}/{it}"),
+        "sample": (f"oracle (reference FP64 cascade) {krylov.upper()} solve at {sample_n}^3 to 1e-8: {it} iterations "
+                   f"in {t:.2f} s on {cores} OpenMP threads, scaled to {gpu_n}^3 by cells x{scale:.0f} at the same "
+                   f"iteration count (an upper bound: the reference algorithm does not reach 1e-8 at {gpu_n}^3)"),
     }
 
 
-def run_solver(lsx, hip, geom, precision, krylov, restart, steps, warmup):
+def run_solver(lsx, hip, geom, precision, krylov, restart, steps, warmup, max_it=1000):
     """Set up one solver on the device geometry and time `steps` solves (x0 = 0 each)."""
     S = lsx.solver(krylov, precision)
-    task = lsx.mixed_task(1.0, restart=restart) if precision == "mixed" else \
-        lsx.Task(eps=1.0, MAX_ITER=1000, FGMRES_RESTART_NUM=restart)
+    task = lsx.Task(eps=1.0, MAX_ITER=max_it, FGMRES_RESTART_NUM=restart) if precision == "fp64" else \
+        lsx.mixed_task(1.0, restart=restart, max_it=max_it)
     t_setup = time.perf_counter()
     S.set_geometry(geom, None, None, task)
     hip.sync()
@@ -163,10 +168,12 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=256)
-    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--precision", choices=["mixed", "fp64-vcycle", "fp64"], default="mixed")
     ap.add_argument("--krylov", choices=["fgmres", "bicgstab"], default="fgmres")
     ap.add_argument("--restart", type=int, default=30)
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--cascade-max-it", type=int, default=200,
+                    help="iteration cap of the reference-cascade comparison solve (0 = skip it)")
     ap.add_argument("--cpu-sample-grid", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -191,9 +198,14 @@ def main():
           file=sys.stderr, flush=True)
     fp64 = None
     if args.precision == "mixed" and not args.no_fp64_compare:
-        fp64 = run_solver(lsx, hip, geom, "fp64", args.krylov, args.restart, max(1, args.steps // 2), 1)
-        print(f"[bench] fp64: {len(fp64['its'])} solves in {fp64['elapsed']:.3f} s, its {fp64['its']}",
+        fp64 = run_solver(lsx, hip, geom, "fp64-vcycle", args.krylov, args.restart, max(1, args.steps // 2), 1)
+        print(f"[bench] fp64-vcycle: {len(fp64['its'])} solves in {fp64['elapsed']:.3f} s, its {fp64['its']}",
               file=sys.stderr, flush=True)
+    cascade = None
+    if args.precision != "fp64" and args.cascade_max_it > 0:
+        cascade = run_solver(lsx, hip, geom, "fp64", args.krylov, args.restart, 1, 0, max_it=args.cascade_max_it)
+        print(f"[bench] fp64 reference cascade: its {cascade['its']} in {cascade['elapsed']:.3f} s, "
+              f"true rel res {cascade['rel_res']:.3e}", file=sys.stderr, flush=True)
 
     stats, elapsed = main_run["stats"], main_run["elapsed"]
     dom = max(stats, key=lambda k: stats[k][1])
@@ -216,8 +228,9 @@ def main():
         "workload": (f"{n}^3 stretched grid (ratio 1.02 to centre), ellipsoid body 0.25/0.15/0.15 "
                      f"(level set of the ellipsoid the Bezier body approximates), {args.krylov.upper()}"
                      f"({args.restart}), rel tol 1e-8, "
-                     + ("mixed: FP32 V(2,2) red-black GS multigrid, FP16 coarse faces, FP64 Krylov"
-                        if args.precision == "mixed" else "FP64 reference multigrid cascade (ADLJ)")),
+                     + {"mixed": "mixed: FP32 V(2,2) red-black GS multigrid, FP16 coarse faces, FP64 Krylov",
+                        "fp64-vcycle": "FP64 V(2,2) red-black GS multigrid",
+                        "fp64": "FP64 reference multigrid cascade (ADLJ)"}[args.precision]),
         "precision": args.precision, "krylov": args.krylov, "cells": geom.n,
         "iterations": main_run["its"], "true_rel_residual": main_run["rel_res"],
         "geometry_setup_s": geo_s, "solver_setup_s": main_run["setup_s"], "mg_levels": main_run["levels"],
@@ -229,6 +242,12 @@ def main():
         cfg.update({"fp64_solves_per_s": fp64_rate, "fp64_iterations": fp64["its"],
                     "fp64_true_rel_residual": fp64["rel_res"], "mixed_over_fp64": value / fp64_rate,
                     "pressure_rel_diff_vs_fp64": err})
+    if cascade is not None:
+        cit = cascade["its"][0]
+        cfg["reference_cascade_fp64"] = {
+            "iterations": cit, "iteration_cap": args.cascade_max_it, "true_rel_residual": cascade["rel_res"],
+            "ms_per_iteration": cascade["elapsed"] / max(cit, 1) * 1e3,
+            "reached_1e-8": bool(cascade["rel_res"] <= 1e-8 * 1.05)}
     out = {
         "metric": METRIC,
         "value": value,
@@ -251,8 +270,7 @@ def main():
     }
     if not args.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline(args.cpu_sample_grid, main_run["its"][-1], n, args.restart,
-                                               args.krylov)
+            out["cpu_baseline"] = cpu_baseline(args.cpu_sample_grid, n, args.restart, args.krylov)
         except Exception as e:  # the GPU measurement stands on its own
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out), flush=True)

@@ -34,10 +34,12 @@ extern "C" {
 
 #define LSX_MG_CASCADE 0 /* reference cascade: restrict RHS, coarsest LU, prolong + ADLJ
                             post-smoothing (multigrid_solver.h:48-75) */
-#define LSX_MG_VCYCLE 1  /* V-cycle with red-black Gauss-Seidel (mixed-precision mode) */
+#define LSX_MG_VCYCLE 1  /* V-cycle: red-black Gauss-Seidel, rediscretised face levels, pinned
+                            cells as Dirichlet sinks on coarse levels (no reference counterpart) */
 
-#define LSX_PREC_FP64 0  /* everything in FP64 */
-#define LSX_PREC_MIXED 1 /* FP32 multigrid, FP16 coarse face coefficients, FP64 Krylov */
+#define LSX_PREC_FP64 0  /* everything in FP64 (cascade or V-cycle) */
+#define LSX_PREC_MIXED 1 /* V-cycle only: FP32 multigrid, FP16 coarse face coefficients, FP64
+                            Krylov loop, operator and coarsest LU */
 
 #define LSX_MEM_HOST 0
 #define LSX_MEM_DEVICE 1
@@ -110,6 +112,12 @@ int lsx_level_stencil(const lsx_solver* s, int level, double* coef_host, long ca
 /* Copy the diagonal of the fine-level operator (N*M*K values) to host memory.  Rows whose
  * diagonal is 0 are cut cells with every face closed; a consistent right-hand side is 0 there. */
 int lsx_operator_diagonal(const lsx_solver* s, double* diag_host);
+
+/* Storage of the fine operator used by the solve: 4 = symmetric planes [a_p, a_e, a_n, a_t]
+ * (operator symmetric off its pinned identity rows), 7 = general [b,s,w,p,e,n,t].  No
+ * reference counterpart (the reference keeps CSR, sparse_matrix.h:33-229).  Returns the plane
+ * count or a negative status. */
+int lsx_fine_storage(const lsx_solver* s);
 
 /* Synchronise the handle's stream. */
 int lsx_synchronize(lsx_solver* s);

@@ -141,6 +141,13 @@ int lsx_operator_diagonal(const lsx_solver* s, double* diag_host) {
   return guarded([&] { s->impl.operator_diagonal(diag_host); });
 }
 
+int lsx_fine_storage(const lsx_solver* s) {
+  if (!s) return LSX_ERR_ARG;
+  int planes = 0;
+  const int st = guarded([&] { planes = s->impl.fine_storage(); });
+  return st == LSX_OK ? planes : st;
+}
+
 int lsx_synchronize(lsx_solver* s) {
   if (!s) return LSX_ERR_ARG;
   return guarded([&] { s->impl.synchronize(); });

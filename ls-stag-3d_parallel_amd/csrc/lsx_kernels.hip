@@ -382,6 +382,126 @@ __global__ void k_lu_factor(const double* __restrict__ coef, int N, int M, int K
   }
 }
 
+// ---- symmetric storage of the fine operator ----------------------------------------------------
+// The reference pressure operator is symmetric except its identity (pinned) rows, whose
+// neighbours still couple to them.  A row is an exception when its own off-diagonals are all 0
+// while a neighbour's coupling to it is not; each face then takes the coupled side's value.
+__device__ inline bool row_decoupled(const double* c7, long n, long i) {
+  return c7[0 * n + i] == 0.0 && c7[1 * n + i] == 0.0 && c7[2 * n + i] == 0.0 && c7[4 * n + i] == 0.0 &&
+         c7[5 * n + i] == 0.0 && c7[6 * n + i] == 0.0;
+}
+
+__global__ void k_sym_check(const double* __restrict__ c7, int N, int M, int K, unsigned char* __restrict__ exc,
+                            int* __restrict__ asym) {
+  const long n = (long)N * M * K, NM = (long)N * M;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode((int)i, N, M, &x, &y, &z);
+    unsigned char e = 0;
+    const bool di = row_decoupled(c7, n, i);
+    // faces towards lower neighbours: (this row's w/s/b) versus (neighbour's e/n/t)
+    const long nb[3] = {i - 1, i - N, i - NM};
+    const bool has[3] = {x > 0, y > 0, z > 0};
+    const int lo_plane[3] = {2, 1, 0}, hi_plane[3] = {4, 5, 6};
+    for (int d = 0; d < 3; ++d) {
+      if (!has[d]) continue;
+      const double mine = c7[lo_plane[d] * n + i], theirs = c7[hi_plane[d] * n + nb[d]];
+      if (mine == theirs) continue;
+      if (di && mine == 0.0) e = 1;                                   // this row is an identity-like row
+      else if (mine != 0.0 && theirs == 0.0 && row_decoupled(c7, n, nb[d])) continue;  // flagged by that row
+      else atomicOr(asym, 1);
+    }
+    const long up[3] = {i + 1, i + N, i + NM};
+    const bool hasu[3] = {x < N - 1, y < M - 1, z < K - 1};
+    for (int d = 0; d < 3; ++d) {
+      if (!hasu[d]) continue;
+      const double mine = c7[hi_plane[d] * n + i], theirs = c7[lo_plane[d] * n + up[d]];
+      if (mine != theirs && di && mine == 0.0) e = 1;
+    }
+    exc[i] = e;
+  }
+}
+
+__global__ void k_sym_build(const double* __restrict__ c7, const unsigned char* __restrict__ exc, int N, int M, int K,
+                            double* __restrict__ s4) {
+  const long n = (long)N * M * K, NM = (long)N * M;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode((int)i, N, M, &x, &y, &z);
+    s4[0 * n + i] = c7[3 * n + i];
+    s4[1 * n + i] = x < N - 1 ? (exc[i] ? c7[2 * n + i + 1] : c7[4 * n + i]) : 0.0;
+    s4[2 * n + i] = y < M - 1 ? (exc[i] ? c7[1 * n + i + N] : c7[5 * n + i]) : 0.0;
+    s4[3 * n + i] = z < K - 1 ? (exc[i] ? c7[0 * n + i + NM] : c7[6 * n + i]) : 0.0;
+  }
+}
+
+// Row i of the symmetric fine operator, terms in the reference CSR order [b,s,w,p,e,n,t]
+// (identical values and order to the 7-plane row, hence bit-identical results).
+template <class XT>
+__device__ inline double row_apply_sym(const double* __restrict__ s4, long n, int i, int x, int y, int z, int N,
+                                       int M, int K, const XT* __restrict__ xv, double acc, bool subtract,
+                                       const SymExc& ex) {
+  const int NM = N * M;
+  bool exc_row = false;
+#pragma unroll
+  for (int q = 0; q < SymExc::CAP; ++q) exc_row |= (q < ex.count) && (ex.row[q] == i);
+  double t;
+  if (!exc_row) {
+    if (z > 0) { t = s4[3 * n + i - NM] * (double)xv[i - NM]; acc = subtract ? acc - t : acc + t; }
+    if (y > 0) { t = s4[2 * n + i - N] * (double)xv[i - N]; acc = subtract ? acc - t : acc + t; }
+    if (x > 0) { t = s4[1 * n + i - 1] * (double)xv[i - 1]; acc = subtract ? acc - t : acc + t; }
+  }
+  t = s4[i] * (double)xv[i];
+  acc = subtract ? acc - t : acc + t;
+  if (!exc_row) {
+    if (x < N - 1) { t = s4[1 * n + i] * (double)xv[i + 1]; acc = subtract ? acc - t : acc + t; }
+    if (y < M - 1) { t = s4[2 * n + i] * (double)xv[i + N]; acc = subtract ? acc - t : acc + t; }
+    if (z < K - 1) { t = s4[3 * n + i] * (double)xv[i + NM]; acc = subtract ? acc - t : acc + t; }
+  }
+  return acc;
+}
+
+template <int MODE, int BS>
+__global__ void k_stencil_sym(const double* __restrict__ s4, int N, int M, int K, SymExc ex,
+                              const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ y,
+                              double* __restrict__ partial) {
+  const long n = (long)N * M * K;
+  long lo, hi;
+  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
+  double ss = 0.0;
+  for (long li = lo + threadIdx.x; li < hi; li += BS) {
+    const int i = (int)li;
+    int xx, yy, zz;
+    decode(i, N, M, &xx, &yy, &zz);
+    if (MODE == ST_PROD) {
+      y[i] = row_apply_sym(s4, n, i, xx, yy, zz, N, M, K, x, 0.0, false, ex);
+    } else if (MODE == ST_RESID) {
+      const double r = row_apply_sym(s4, n, i, xx, yy, zz, N, M, K, x, b[i], true, ex);
+      y[i] = r;
+      ss += r * r;
+    } else {
+      const double q = row_apply_sym(s4, n, i, xx, yy, zz, N, M, K, x, 0.0, false, ex);
+      y[i] = y[i] - q;
+    }
+  }
+  if (MODE == ST_RESID) {
+    ss = block_sum<BS>(ss);
+    if (threadIdx.x == 0) partial[blockIdx.x] = ss;
+  }
+}
+
+// y = A x with an FP32 x (mixed-precision Krylov products)
+__global__ void k_stencil_sym_f32x(const double* __restrict__ s4, int N, int M, int K, SymExc ex,
+                                   const float* __restrict__ x, double* __restrict__ y) {
+  const long n = (long)N * M * K;
+  for (long li = blockIdx.x * (long)blockDim.x + threadIdx.x; li < n; li += (long)gridDim.x * blockDim.x) {
+    const int i = (int)li;
+    int xx, yy, zz;
+    decode(i, N, M, &xx, &yy, &zz);
+    y[i] = row_apply_sym(s4, n, i, xx, yy, zz, N, M, K, x, 0.0, false, ex);
+  }
+}
+
 // ---- kernels: solve phase ---------------------------------------------------------------------
 template <int NPTS, int MODE, int BS>
 __global__ void k_stencil(const double* __restrict__ coef, int N, int M, int K, const double* __restrict__ x,
@@ -728,6 +848,25 @@ void stencil_apply(const double* coef, int npts, Dims d, const double* x, const 
 #undef LSX_ST
 }
 
+void sym_check(const double* coef7, Dims d, unsigned char* exc, int* asym, hipStream_t s) {
+  k_sym_check<<<grid_1d(d.n()), 256, 0, s>>>(coef7, d.N, d.M, d.K, exc, asym);
+}
+
+void sym_build(const double* coef7, const unsigned char* exc, Dims d, double* s4, hipStream_t s) {
+  k_sym_build<<<grid_1d(d.n()), 256, 0, s>>>(coef7, exc, d.N, d.M, d.K, s4);
+}
+
+void stencil_apply_sym(const double* s4, const SymExc& ex, Dims d, const double* x, const double* b, double* y,
+                       int mode, double* partial, hipStream_t s) {
+  const int nb = reduction_blocks(d.n());
+  if (mode == ST_PROD)
+    k_stencil_sym<ST_PROD, RED_THREADS><<<nb, RED_THREADS, 0, s>>>(s4, d.N, d.M, d.K, ex, x, b, y, partial);
+  else if (mode == ST_RESID)
+    k_stencil_sym<ST_RESID, RED_THREADS><<<nb, RED_THREADS, 0, s>>>(s4, d.N, d.M, d.K, ex, x, b, y, partial);
+  else
+    k_stencil_sym<ST_SUBR, RED_THREADS><<<nb, RED_THREADS, 0, s>>>(s4, d.N, d.M, d.K, ex, x, b, y, partial);
+}
+
 void restrict_full(const double* rf, Dims df, double* rc, Dims dc, hipStream_t s) {
   k_restrict<<<grid_1d(dc.n()), 256, 0, s>>>(rf, df.N, df.M, rc, dc.N, dc.M, dc.K);
 }
@@ -777,6 +916,10 @@ void sol_update(double* x, const double* Z, long ldz, const double* yk, int coun
     k_sol_update<<<grid_1d(n), 256, 0, s>>>(x, Z + (long)lo * ldz, ldz, c, hi - lo, n);
     hi = lo;
   }
+}
+
+void stencil_prod_sym_f32x(const double* s4, const SymExc& ex, Dims d, const float* x, double* y, hipStream_t s) {
+  k_stencil_sym_f32x<<<grid_1d(d.n()), 256, 0, s>>>(s4, d.N, d.M, d.K, ex, x, y);
 }
 
 void stencil7_prod_f32x(const double* coef, Dims d, const float* x, double* y, hipStream_t s) {

@@ -18,6 +18,20 @@ struct Dims {
 //  npts == 27: index (dx+1) + 3(dy+1) + 9(dz+1)            (Galerkin coarse levels)
 enum StencilMode { ST_PROD = 0, ST_RESID = 1, ST_SUBR = 2 };
 
+// Identity-like rows of a symmetric fine operator (the pinned cells), at most CAP.
+struct SymExc {
+  static constexpr int CAP = 8;
+  int count = 0;
+  int row[CAP] = {-1, -1, -1, -1, -1, -1, -1, -1};
+};
+
+// symmetric fine storage: planes [a_p, a_e, a_n, a_t]; exc[i] = 1 for identity-like rows;
+// asym[0] set when the operator is not symmetric off those rows
+void sym_check(const double* coef7, Dims d, unsigned char* exc, int* asym, hipStream_t s);
+void sym_build(const double* coef7, const unsigned char* exc, Dims d, double* s4, hipStream_t s);
+void stencil_apply_sym(const double* s4, const SymExc& ex, Dims d, const double* x, const double* b, double* y,
+                       int mode, double* partial, hipStream_t s);
+
 // Deterministic reductions: partial sums per block into `partial`, then finalize sums them in
 // a fixed order into out[out_index] (device memory).
 constexpr int RED_THREADS = 256;
@@ -63,6 +77,7 @@ void axpy_dot(double* w, const double* v, const double* h, const double* vnext, 
 void sol_update(double* x, const double* Z, long ldz, const double* yk, int count, long n, hipStream_t s);
 // mixed precision: y = A x with FP32 x (fine 7-point operator); x += sum_k y_k z_k with FP32 z
 void stencil7_prod_f32x(const double* coef, Dims d, const float* x, double* y, hipStream_t s);
+void stencil_prod_sym_f32x(const double* s4, const SymExc& ex, Dims d, const float* x, double* y, hipStream_t s);
 void sol_update_f32(double* x, const float* Z, long ldz, const double* yk, int count, long n, hipStream_t s);
 // BiCGStab fused updates (bicgstab_interface.h:39-56)
 void bicg_s_update(double* nu, const double* r, double* sv, double alpha, long n, hipStream_t s);

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace lsx {
 
@@ -16,6 +17,7 @@ namespace {
 constexpr int SC_NORM = 0;    // residual / ||w|| results
 constexpr int SC_TINY = 1;    // max |diag| of a level during setup
 constexpr int SC_TMP = 2;     // scratch dot results
+constexpr int SC_SIGMA = 4;   // the two pinned cells' face sums (V-cycle setup)
 constexpr int SC_H = 8;       // Hessenberg column entries H[hNum]
 }  // namespace
 
@@ -24,8 +26,7 @@ Solver::Solver(const lsx_options& o, hipStream_t s) : opt(o) {
   if (opt.max_it < 1) throw Error(LSX_ERR_ARG, "max_it must be >= 1");
   if (opt.n_post < 0) throw Error(LSX_ERR_ARG, "n_post must be >= 0");
   if (opt.krylov != LSX_KRYLOV_FGMRES && opt.krylov != LSX_KRYLOV_BICGSTAB) throw Error(LSX_ERR_ARG, "unknown krylov");
-  if (opt.precision == LSX_PREC_FP64 && opt.mg != LSX_MG_CASCADE)
-    throw Error(LSX_ERR_ARG, "FP64 precision uses the reference cascade multigrid (LSX_MG_CASCADE)");
+  if (opt.mg != LSX_MG_CASCADE && opt.mg != LSX_MG_VCYCLE) throw Error(LSX_ERR_ARG, "unknown multigrid");
   if (opt.precision == LSX_PREC_MIXED && opt.mg != LSX_MG_VCYCLE)
     throw Error(LSX_ERR_ARG, "mixed precision uses the V-cycle multigrid (LSX_MG_VCYCLE)");
   if (opt.precision != LSX_PREC_FP64 && opt.precision != LSX_PREC_MIXED) throw Error(LSX_ERR_ARG, "unknown precision");
@@ -107,10 +108,11 @@ double Solver::kernel_bytes(int k) const {
     case LSX_K_LINE_X:
     case LSX_K_LINE_Y:
     case LSX_K_LINE_Z: return 56.0 * n;    // r 8, two factor planes 16, sub coupling 8, ch 16, c 8
-    case LSX_K_OPERATOR: return 48.0 * n;  // z 8, coefficients 32, w 8
+    case LSX_K_OPERATOR:                   // coefficients 32, w 8, z 8 (FP64) or 4 (FP32)
+      return (mixed() ? 44.0 : 48.0) * n;
     case LSX_K_ORTHO: return 32.0 * n;     // w read + write 16, v_i 8, v_i+1 8
-    case LSX_K_SMOOTH:                     // n_pre red-black sweeps, FP32: x 4 + faces 12 + 1/a_p 4
-      return 28.0 * n * opt.n_pre;         //   + b 4 + x write 4 per sweep
+    case LSX_K_SMOOTH:                     // n_pre red-black sweeps; per sweep x, three faces,
+      return (mixed() ? 28.0 : 56.0) * n * opt.n_pre;  // 1/a_p, b read and x written once
     default: return 0.0;
   }
 }
@@ -172,7 +174,7 @@ void Solver::setup_levels(int N, int M, int K) {
   for (size_t l = 0; l < levels.size(); ++l) {
     MGLevel& L = levels[l];
     const long nl = L.d.n();
-    if (mixed()) {  // the V-cycle keeps its own compact levels; only the FP64 fine operator here
+    if (vcyc()) {  // the V-cycle keeps its own compact levels; only the FP64 fine operator here
       if (l == 0) L.coef.alloc((size_t)L.npts * nl);
       continue;
     }
@@ -185,7 +187,7 @@ void Solver::setup_levels(int N, int M, int K) {
         L.eu[dir].alloc(nl);
       }
   }
-  if (!mixed()) cwork.alloc(d0.n());
+  if (!vcyc()) cwork.alloc(d0.n());
   const long nc = levels.back().d.n();
   lu_scale.alloc(nc);
   lu_perm.alloc(nc);
@@ -267,8 +269,9 @@ void Solver::build_hierarchy() {
     LSX_HIP(hipStreamSynchronize(stream));
     if (herr) throw Error(LSX_ERR_ARG, "matrix has a column outside the 7-point neighbourhood of its row");
   }
-  if (mixed()) {
-    build_mixed();
+  sym_setup();
+  if (vcyc()) {
+    build_vcycle();
     return;
   }
   for (size_t l = 0; l + 1 < levels.size(); ++l) {
@@ -286,14 +289,15 @@ void Solver::build_hierarchy() {
 }
 
 // -----------------------------------------------------------------------------------------------
-// mixed-precision V-cycle hierarchy: faces of the fine operator (FP32), aggregated coarse faces
-// (half the sum of the four fine faces of each coarse face, stored FP16 with a per-level scale),
-// Neumann diagonals, the two pinned cells carried to every level, FP64 LU on the coarsest.
+// V-cycle hierarchy: faces of the fine operator, aggregated coarse faces (half the sum of the
+// four fine faces of each coarse face), Neumann diagonals, the pinned cells as Dirichlet sinks
+// on coarse levels, FP64 LU on the coarsest.  Mixed precision stores the finest faces in FP32
+// and coarse faces in FP16 divided by a per-level scale (stored values within [0, 16384]).
 // -----------------------------------------------------------------------------------------------
-void Solver::build_mixed() {
+void Solver::build_vcycle() {
   const int L = (int)levels.size() - 1;
-  mlev.clear();
-  mlev.resize(L + 1);
+  vlev.clear();
+  vlev.resize(L + 1);
   std::vector<DevBuf<double>> fe(2), fn(2), ft(2), dg(2);
   int cur = 0;
   {
@@ -307,110 +311,171 @@ void Solver::build_mixed() {
     ft[1].alloc(std::max<long>(1, n / 8));
     dg[1].alloc(std::max<long>(1, n / 8));
   }
+  const bool m32 = mixed();
+  double* sigma = scalars.p + SC_SIGMA;
+  if (pins.empty() || pins.size() > 2)
+    throw Error(LSX_ERR_ARG, "the V-cycle needs one or two pinned (identity) rows to fix the pressure level");
+  auto coarse_of = [&](int p, int l) {  // the level-l cell that contains fine cell p
+    if (p < 0) return -1;
+    const int x = p % d0.N, y = (p / d0.N) % d0.M, z = p / (d0.N * d0.M);
+    const Dims& d = levels[l].d;
+    return (x >> l) + d.N * ((y >> l) + d.M * (z >> l));
+  };
+  const int fp0 = pins[0], fp1 = pins.size() > 1 ? pins[1] : -1;
   for (int l = 0; l <= L; ++l) {
     const Dims d = levels[l].d;
     const long nl = d.n();
-    const int pin0 = d.N - 1, pin1 = d.N * d.M - 1;
-    MixStore& S = mlev[l];
+    // the finest level keeps the pinned rows as identity rows; coarse levels take them as sinks
+    const int pin0 = l == 0 ? fp0 : -1, pin1 = l == 0 ? fp1 : -1;
+    VStore& S = vlev[l];
     S.L.d = d;
     S.L.pin0 = pin0;
     S.L.pin1 = pin1;
     if (l == 0) {
-      mix::faces_from7(levels[0].coef.p, d, pin0, pin1, fe[cur].p, fn[cur].p, ft[cur].p, dg[cur].p, stream);
+      vc::faces_from7(levels[0].coef.p, d, pin0, pin1, fe[cur].p, fn[cur].p, ft[cur].p, dg[cur].p, stream);
+      vc::pin_sigma(fe[cur].p, fn[cur].p, ft[cur].p, d, pin0, pin1, sigma, stream);
     } else {
       const Dims df = levels[l - 1].d;
       const int nxt = cur ^ 1;
-      mix::aggregate(fe[cur].p, fn[cur].p, ft[cur].p, df, fe[nxt].p, fn[nxt].p, ft[nxt].p, d, stream);
-      mix::diag_from_faces(fe[nxt].p, fn[nxt].p, ft[nxt].p, d, dg[nxt].p, stream);
+      vc::aggregate(fe[cur].p, fn[cur].p, ft[cur].p, df, fe[nxt].p, fn[nxt].p, ft[nxt].p, d, stream);
+      vc::diag_from_faces(fe[nxt].p, fn[nxt].p, ft[nxt].p, d, dg[nxt].p, stream);
+      vc::pin_sink(dg[nxt].p, coarse_of(fp0, l), coarse_of(fp1, l), sigma, stream);
       cur = nxt;
     }
     if (l == L) {  // coarsest: FP64 LU of the 7-point operator
       DevBuf<double> c27;
       c27.alloc(27 * nl);
-      mix::to27(fe[cur].p, fn[cur].p, ft[cur].p, dg[cur].p, d, pin0, pin1, c27.p, stream);
+      vc::to27(fe[cur].p, fn[cur].p, ft[cur].p, dg[cur].p, d, pin0, pin1, c27.p, stream);
       lu_factor(c27.p, d, lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, lu_col.p, stream);
-      S.b.alloc(nl);
-      S.x.alloc(nl);
-      cb64.alloc(nl);
-      cx64.alloc(nl);
+      S.db.alloc(nl);
+      S.dx.alloc(nl);
+      if (m32) {
+        S.fb.alloc(nl);
+        S.fx.alloc(nl);
+      }
       LSX_HIP(hipStreamSynchronize(stream));
       break;
     }
-    S.s.alloc(nl);
-    S.b.alloc(nl);
-    if (l > 0) S.x.alloc(nl);  // level 0 writes into the caller's output
     absmax(dg[cur].p, nl, partial.p, scalars.p + SC_TINY, stream);
-    mix::inv_diag(dg[cur].p, nl, scalars.p + SC_TINY, pin0, pin1, S.s.p, stream);
-    if (l == 0) {
+    if (m32) {
+      S.fs.alloc(nl);
+      S.fb.alloc(nl);
+      if (l > 0) S.fx.alloc(nl);  // level 0 writes into the caller's output
+      vc::inv_diag<float>(dg[cur].p, nl, scalars.p + SC_TINY, pin0, pin1, S.fs.p, stream);
+      S.L.s = S.fs.p;
+    } else {
+      S.ds.alloc(nl);
+      if (l > 0) {  // level 0 reads the caller's input and writes the caller's output
+        S.db.alloc(nl);
+        S.dx.alloc(nl);
+      }
+      vc::inv_diag<double>(dg[cur].p, nl, scalars.p + SC_TINY, pin0, pin1, S.ds.p, stream);
+      S.L.s = S.ds.p;
+    }
+    if (!m32) {
+      S.de.alloc(nl);
+      S.dn.alloc(nl);
+      S.dt.alloc(nl);
+      vc::store_faces<double>(fe[cur].p, fn[cur].p, ft[cur].p, nl, 1.0, S.de.p, S.dn.p, S.dt.p, stream);
+      S.L.ftype = vc::FACE_F64;
+      S.L.f_e = S.de.p;
+      S.L.f_n = S.dn.p;
+      S.L.f_t = S.dt.p;
+      S.L.fscale = 1.0;
+    } else if (l == 0) {
       S.fe.alloc(nl);
       S.fn.alloc(nl);
       S.ft.alloc(nl);
-      mix::store_faces_f32(fe[cur].p, fn[cur].p, ft[cur].p, nl, 1.0, S.fe.p, S.fn.p, S.ft.p, stream);
-      S.L.f16 = false;
+      vc::store_faces<float>(fe[cur].p, fn[cur].p, ft[cur].p, nl, 1.0, S.fe.p, S.fn.p, S.ft.p, stream);
+      S.L.ftype = vc::FACE_F32;
       S.L.f_e = S.fe.p;
       S.L.f_n = S.fn.p;
       S.L.f_t = S.ft.p;
-      S.L.fscale = 1.0f;
+      S.L.fscale = 1.0;
     } else {
-      mix::face_absmax(fe[cur].p, fn[cur].p, ft[cur].p, nl, partial.p, scalars.p + SC_TMP, stream);
-      double amax = reduce_to_host(scalars.p + SC_TMP);
-      const double scale = amax > 0 ? amax / 16384.0 : 1.0;  // stored faces within [0, 16384]
+      vc::face_absmax(fe[cur].p, fn[cur].p, ft[cur].p, nl, partial.p, scalars.p + SC_TMP, stream);
+      const double amax = reduce_to_host(scalars.p + SC_TMP);
+      const double scale = amax > 0 ? amax / 16384.0 : 1.0;
       S.he.alloc(nl);
       S.hn.alloc(nl);
       S.ht.alloc(nl);
-      mix::store_faces_f16(fe[cur].p, fn[cur].p, ft[cur].p, nl, 1.0 / scale, S.he.p, S.hn.p, S.ht.p, stream);
-      S.L.f16 = true;
-      S.L.h_e = S.he.p;
-      S.L.h_n = S.hn.p;
-      S.L.h_t = S.ht.p;
+      vc::store_faces<_Float16>(fe[cur].p, fn[cur].p, ft[cur].p, nl, 1.0 / scale, S.he.p, S.hn.p, S.ht.p, stream);
+      S.L.ftype = vc::FACE_F16;
+      S.L.f_e = S.he.p;
+      S.L.f_n = S.hn.p;
+      S.L.f_t = S.ht.p;
       S.L.fscale = (float)scale;
     }
-    S.L.s = S.s.p;
     LSX_HIP(hipStreamSynchronize(stream));  // the ping-pong temporaries are reused next level
   }
   LSX_HIP(hipGetLastError());
-  ztmp.alloc(d0.n());
+  if (m32) ztmp32.alloc(d0.n());
 }
 
-void Solver::vcycle_apply(const double* v, float* z) {
-  const int L = (int)mlev.size() - 1;
-  mix::f64_to_f32(v, mlev[0].b.p, d0.n(), stream);
+template <>
+float* Solver::vx<float>(VStore& S) { return S.fx.p; }
+template <>
+double* Solver::vx<double>(VStore& S) { return S.dx.p; }
+template <>
+float* Solver::vb<float>(VStore& S) { return S.fb.p; }
+template <>
+double* Solver::vb<double>(VStore& S) { return S.db.p; }
+
+template <class T>
+void Solver::vcycle_apply(const double* v, T* z) {
+  const int L = (int)vlev.size() - 1;
+  // level 0 right-hand side: the input itself in FP64, an FP32 copy in mixed precision
+  const T* b0;
+  if constexpr (std::is_same<T, float>::value) {
+    vc::convert<double, float>(v, vlev[0].fb.p, d0.n(), stream);
+    b0 = vlev[0].fb.p;
+  } else {
+    b0 = v;
+  }
   for (int l = 0; l < L; ++l) {
-    MixStore& S = mlev[l];
-    float* x = l == 0 ? z : S.x.p;
+    VStore& S = vlev[l];
+    T* x = l == 0 ? z : vx<T>(S);
+    const T* b = l == 0 ? b0 : vb<T>(S);
     hipEvent_t e0 = nullptr;
     if (l == 0) tbeg(LSX_K_SMOOTH, &e0);
     for (int s = 0; s < opt.n_pre; ++s) {
-      mix::rbgs(S.L, x, S.b.p, 0, s == 0, stream);
-      mix::rbgs(S.L, x, S.b.p, 1, 0, stream);
+      vc::rbgs<T>(S.L, x, b, 0, s == 0, stream);
+      vc::rbgs<T>(S.L, x, b, 1, 0, stream);
     }
     if (l == 0) tend(LSX_K_SMOOTH, e0);
-    mix::resid_restrict(S.L, x, S.b.p, mlev[l + 1].b.p, stream);
+    vc::resid_restrict<T>(S.L, x, b, vb<T>(vlev[l + 1]), stream);
   }
   {
-    MixStore& C = mlev[L];
+    VStore& C = vlev[L];
     const long nc = C.L.d.n();
-    mix::f32_to_f64(C.b.p, cb64.p, nc, stream);
-    lu_solve(lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, (int)nc, cb64.p, cx64.p, stream);
-    mix::f64_to_f32(cx64.p, C.x.p, nc, stream);
+    if constexpr (std::is_same<T, float>::value) {
+      vc::convert<float, double>(C.fb.p, C.db.p, nc, stream);
+      lu_solve(lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, (int)nc, C.db.p, C.dx.p, stream);
+      vc::convert<double, float>(C.dx.p, C.fx.p, nc, stream);
+    } else {
+      lu_solve(lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, (int)nc, C.db.p, C.dx.p, stream);
+    }
   }
   for (int l = L - 1; l >= 0; --l) {
-    MixStore& S = mlev[l];
-    float* x = l == 0 ? z : S.x.p;
-    mix::prolong_add(mlev[l + 1].x.p, mlev[l + 1].L.d, x, S.L.d, stream);
+    VStore& S = vlev[l];
+    T* x = l == 0 ? z : vx<T>(S);
+    const T* b = l == 0 ? b0 : vb<T>(S);
+    vc::prolong_add<T>(vx<T>(vlev[l + 1]), vlev[l + 1].L.d, x, S.L.d, stream);
     for (int s = 0; s < opt.n_post; ++s) {
-      mix::rbgs(S.L, x, S.b.p, 0, 0, stream);
-      mix::rbgs(S.L, x, S.b.p, 1, 0, stream);
+      vc::rbgs<T>(S.L, x, b, 0, 0, stream);
+      vc::rbgs<T>(S.L, x, b, 1, 0, stream);
     }
   }
 }
 
 void Solver::precond(const double* v, double* z64) {
-  if (mixed()) {
-    vcycle_apply(v, ztmp.p);
-    mix::f32_to_f64(ztmp.p, z64, d0.n(), stream);
-  } else {
+  if (!vcyc()) {
     mg_apply(v, z64);
+  } else if (mixed()) {
+    vcycle_apply<float>(v, ztmp32.p);
+    vc::convert<float, double>(ztmp32.p, z64, d0.n(), stream);
+  } else {
+    vcycle_apply<double>(v, z64);
   }
 }
 
@@ -440,6 +505,43 @@ void Solver::alloc_krylov() {
 }
 
 // -----------------------------------------------------------------------------------------------
+// fine operator: symmetric four-plane storage [a_p, a_e, a_n, a_t] when the operator is symmetric
+// off its identity-like (pinned) rows, which take their faces from the coupled side; the same
+// values in the same order as the 7-plane rows, so results are bit-identical.
+// -----------------------------------------------------------------------------------------------
+void Solver::sym_setup() {
+  const long n = d0.n();
+  sym_ok = false;
+  sym_ex = SymExc{};
+  pins.clear();
+  DevBuf<unsigned char> exc;
+  exc.alloc(n);
+  LSX_HIP(hipMemsetAsync(err.p, 0, sizeof(int), stream));
+  sym_check(levels[0].coef.p, d0, exc.p, err.p, stream);
+  std::vector<unsigned char> h(n);
+  int asym = 0;
+  LSX_HIP(hipMemcpyAsync(h.data(), exc.p, n, hipMemcpyDeviceToHost, stream));
+  LSX_HIP(hipMemcpyAsync(&asym, err.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+  LSX_HIP(hipStreamSynchronize(stream));
+  for (long i = 0; i < n; ++i)
+    if (h[i]) pins.push_back((int)i);
+  if (asym || (int)pins.size() > SymExc::CAP) return;
+  for (size_t q = 0; q < pins.size(); ++q) sym_ex.row[q] = pins[q];
+  sym_ex.count = (int)pins.size();
+  s4.alloc(4 * n);
+  sym_build(levels[0].coef.p, exc.p, d0, s4.p, stream);
+  LSX_HIP(hipStreamSynchronize(stream));
+  sym_ok = true;
+}
+
+void Solver::fine_apply(const double* x, const double* b, double* y, int mode) {
+  if (sym_ok)
+    stencil_apply_sym(s4.p, sym_ex, d0, x, b, y, mode, partial.p, stream);
+  else
+    stencil_apply(levels[0].coef.p, 7, d0, x, b, y, mode, partial.p, stream);
+}
+
+// -----------------------------------------------------------------------------------------------
 // multigrid cascade — multigrid_solver.h:48-75
 // -----------------------------------------------------------------------------------------------
 void Solver::mg_apply(const double* v, double* z) {
@@ -455,7 +557,10 @@ void Solver::mg_apply(const double* v, double* z) {
       for (int dir = 2; dir >= 0; --dir) {  // z, y, x
         hipEvent_t e0 = nullptr;
         if (k == 0) tbeg(LSX_K_RESIDUAL, &e0);
-        stencil_apply(lv.coef.p, lv.npts, lv.d, cwork.p, nullptr, lv.r.p, ST_SUBR, nullptr, stream);
+        if (k == 0)
+          fine_apply(cwork.p, nullptr, lv.r.p, ST_SUBR);
+        else
+          stencil_apply(lv.coef.p, lv.npts, lv.d, cwork.p, nullptr, lv.r.p, ST_SUBR, nullptr, stream);
         if (k == 0) {
           tend(LSX_K_RESIDUAL, e0);
           tbeg(LSX_K_LINE_X + dir, &e0);
@@ -493,6 +598,7 @@ int Solver::fgmres(const double* b, double* x) {
   const double eps = opt.eps;
   const MGLevel& F = levels[0];
 
+  const bool m32 = vcyc() && mixed();
   auto sol_upd = [&](int last, int lastH) {
     std::vector<double> ycoef(last + 1);
     int hNum = lastH;
@@ -504,13 +610,13 @@ int Solver::fgmres(const double* b, double* x) {
       }
       ycoef[k] = curY;
     }
-    if (mixed())
+    if (m32)
       sol_update_f32(x, Z32.p, n, ycoef.data(), last + 1, n, stream);
     else
       sol_update(x, Z.p, n, ycoef.data(), last + 1, n, stream);
   };
 
-  stencil_apply(F.coef.p, 7, d0, x, b, w.p, ST_RESID, partial.p, stream);
+  fine_apply(x, b, w.p, ST_RESID);
   finalize(partial.p, nb, dnorm, 1, stream);
   norm_r = reduce_to_host(dnorm);
   if (!(norm_r > eps)) return 0;
@@ -523,19 +629,25 @@ int Solver::fgmres(const double* b, double* x) {
     LSX_HIP(hipMemcpyAsync(dnorm, &normW, sizeof(double), hipMemcpyHostToDevice, stream));
     for (int j = 0, hColStart = 0, hNum = 0; j < m; ++j, hColStart = hNum, ++iters) {
       double* vj = V.p + (size_t)j * n;
-      double* zj = mixed() ? nullptr : Z.p + (size_t)j * n;
-      float* zj32 = mixed() ? Z32.p + (size_t)j * n : nullptr;
+      double* zj = m32 ? nullptr : Z.p + (size_t)j * n;
+      float* zj32 = m32 ? Z32.p + (size_t)j * n : nullptr;
       scale_copy(w.p, dnorm, vj, n, stream);
-      if (mixed())
-        vcycle_apply(vj, zj32);
+      if (m32)
+        vcycle_apply<float>(vj, zj32);
+      else if (vcyc())
+        vcycle_apply<double>(vj, zj);
       else
         mg_apply(vj, zj);
       hipEvent_t e0 = nullptr;
       tbeg(LSX_K_OPERATOR, &e0);
-      if (mixed())
-        stencil7_prod_f32x(F.coef.p, d0, zj32, w.p, stream);
-      else
-        stencil_apply(F.coef.p, 7, d0, zj, nullptr, w.p, ST_PROD, nullptr, stream);
+      if (m32) {
+        if (sym_ok)
+          stencil_prod_sym_f32x(s4.p, sym_ex, d0, zj32, w.p, stream);
+        else
+          stencil7_prod_f32x(F.coef.p, d0, zj32, w.p, stream);
+      } else {
+        fine_apply(zj, nullptr, w.p, ST_PROD);
+      }
       tend(LSX_K_OPERATOR, e0);
       // modified Gram-Schmidt: h_i = w . v_i ; w -= h_i v_i  (the dot of the next basis vector
       // is fused into the update pass; the last pass yields ||w||^2)
@@ -583,7 +695,7 @@ int Solver::fgmres(const double* b, double* x) {
     }
     sol_upd(m - 1, m_size - 1);
     // w = b - A x starts the next cycle; its norm is the carried estimate |gamma|
-    stencil_apply(F.coef.p, 7, d0, x, b, w.p, ST_RESID, partial.p, stream);
+    fine_apply(x, b, w.p, ST_RESID);
     normW = curH;
   }
 }
@@ -596,9 +708,8 @@ int Solver::bicgstab(const double* b, double* x) {
   const int nb = reduction_blocks(n);
   double* tmp = scalars.p + SC_TMP;
   double* dnorm = scalars.p + SC_NORM;
-  const MGLevel& F = levels[0];
   int iters = 1;
-  stencil_apply(F.coef.p, 7, d0, x, nullptr, bz.p, ST_PROD, nullptr, stream);
+  fine_apply(x, nullptr, bz.p, ST_PROD);
   sub_vec(b, bz.p, br.p, n, stream);
   norm_r = std::sqrt(dot(br.p, br.p, n, tmp));
   if (norm_r > opt.eps) {
@@ -606,13 +717,13 @@ int Solver::bicgstab(const double* b, double* x) {
     copy(br.p, bp.p, n, stream);
     do {
       precond(bp.p, by.p);
-      stencil_apply(F.coef.p, 7, d0, by.p, nullptr, bnu.p, ST_PROD, nullptr, stream);
+      fine_apply(by.p, nullptr, bnu.p, ST_PROD);
       double vs = dot(brr.p, br.p, n, tmp);
       double coef = dot(brr.p, bnu.p, n, tmp);
       const double alpha = vs / coef;
       bicg_s_update(bnu.p, br.p, bs.p, alpha, n, stream);
       precond(bs.p, bz.p);
-      stencil_apply(F.coef.p, 7, d0, bz.p, nullptr, bt.p, ST_PROD, nullptr, stream);
+      fine_apply(bz.p, nullptr, bt.p, ST_PROD);
       double wv = dot(bt.p, bs.p, n, tmp);
       coef = dot(bt.p, bt.p, n, tmp);
       wv /= coef;
@@ -674,14 +785,20 @@ void Solver::precondition(const double* v, double* z, int mem) {
 void Solver::apply_operator(int level, const double* x, double* yv, int mem) {
   if (!ready) throw Error(LSX_ERR_STATE, "apply_operator before set_csr");
   if (level < 0 || level >= (int)levels.size()) throw Error(LSX_ERR_ARG, "level out of range");
+  if (vcyc() && level > 0) throw Error(LSX_ERR_STATE, "the V-cycle keeps coarse levels as face coefficients only");
   const MGLevel& L = levels[level];
   const long n = L.d.n();
   if (mem == LSX_MEM_HOST) {
     hb.alloc(d0.n());
     hx.alloc(d0.n());
     LSX_HIP(hipMemcpyAsync(hb.p, x, n * sizeof(double), hipMemcpyHostToDevice, stream));
-    stencil_apply(L.coef.p, L.npts, L.d, hb.p, nullptr, hx.p, ST_PROD, nullptr, stream);
+    if (level == 0)
+      fine_apply(hb.p, nullptr, hx.p, ST_PROD);
+    else
+      stencil_apply(L.coef.p, L.npts, L.d, hb.p, nullptr, hx.p, ST_PROD, nullptr, stream);
     LSX_HIP(hipMemcpyAsync(yv, hx.p, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+  } else if (level == 0) {
+    fine_apply(x, nullptr, yv, ST_PROD);
   } else {
     stencil_apply(L.coef.p, L.npts, L.d, x, nullptr, yv, ST_PROD, nullptr, stream);
   }
@@ -707,6 +824,7 @@ void Solver::operator_diagonal(double* host) const {
 
 int Solver::level_stencil(int level, double* host, long capacity) const {
   if (level < 0 || level >= (int)levels.size()) throw Error(LSX_ERR_ARG, "level out of range");
+  if (vcyc() && level > 0) throw Error(LSX_ERR_STATE, "the V-cycle keeps coarse levels as face coefficients only");
   const MGLevel& L = levels[level];
   const long need = (long)L.npts * L.d.n();
   if (capacity < need) throw Error(LSX_ERR_ARG, "capacity too small for the level stencil");

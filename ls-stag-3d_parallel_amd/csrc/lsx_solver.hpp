@@ -9,7 +9,7 @@
 
 #include "../../include/lsx.h"
 #include "lsx_kernels.hpp"
-#include "lsx_mixed.hpp"
+#include "lsx_vcycle.hpp"
 
 namespace lsx {
 
@@ -86,6 +86,10 @@ class Solver {
   long level_dims(int level, int* N, int* M, int* K) const;
   int level_stencil(int level, double* host, long capacity) const;
   void operator_diagonal(double* host) const;
+  int fine_storage() const {
+    if (!ready) throw Error(LSX_ERR_STATE, "fine_storage before the matrix is set");
+    return sym_ok ? 4 : 7;
+  }
   void synchronize();
 
   void set_timing(bool on);
@@ -126,19 +130,33 @@ class Solver {
   std::vector<double> H, cs, sn, y;
   bool ready = false;
 
-  // mixed-precision V-cycle hierarchy
-  struct MixStore {
-    mix::Level L;
-    DevBuf<float> fe, fn, ft, s, x, b;
+  // fine operator in symmetric four-plane storage (when the operator allows it)
+  DevBuf<double> s4;
+  SymExc sym_ex;
+  bool sym_ok = false;
+  std::vector<int> pins;  // identity-like rows of the fine operator (pinned cells)
+  void sym_setup();
+  void fine_apply(const double* x, const double* b, double* y, int mode);  // StencilMode on level 0
+
+  // V-cycle hierarchy (FP64, or FP32 values with FP32/FP16 faces in mixed precision)
+  struct VStore {
+    vc::Level L;
+    DevBuf<double> de, dn, dt, ds, dx, db;
+    DevBuf<float> fe, fn, ft, fs, fx, fb;
     DevBuf<_Float16> he, hn, ht;
   };
-  std::vector<MixStore> mlev;
-  DevBuf<float> Z32, ztmp;
-  DevBuf<double> cb64, cx64;
+  std::vector<VStore> vlev;
+  DevBuf<float> Z32, ztmp32;
   bool mixed() const { return opt.precision == LSX_PREC_MIXED; }
-  void build_mixed();
-  void vcycle_apply(const double* v, float* z);
-  void precond(const double* v, double* z64);  // either preconditioner, FP64 output
+  bool vcyc() const { return opt.mg == LSX_MG_VCYCLE; }
+  void build_vcycle();
+  template <class T>
+  void vcycle_apply(const double* v, T* z);
+  template <class T>
+  T* vx(VStore& S);
+  template <class T>
+  T* vb(VStore& S);
+  void precond(const double* v, double* z64);  // the configured preconditioner, FP64 in and out
 
   // event timing of fine-level kernel launches
   bool timing = false;

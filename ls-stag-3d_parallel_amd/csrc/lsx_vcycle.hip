@@ -1,19 +1,19 @@
-// Mixed-precision multigrid preconditioner: FP32 V-cycle with red-black Gauss-Seidel on
-// 7-point operators stored as face coefficients (FP32 on the finest level, FP16 with a
-// per-level scale on coarse levels), full-sum restriction, trilinear prolongation and an FP64
-// dense LU on the coarsest level.  It preconditions the FP64 flexible Krylov loop; nothing here
-// restates reference arithmetic, so ordinary FMA contraction is allowed.
+// Geometric V-cycle preconditioner: red-black Gauss-Seidel on 7-point operators stored as face
+// coefficients, full-sum restriction, trilinear prolongation, FP64 dense LU on the coarsest
+// level.  It preconditions the FP64 flexible Krylov loop; nothing here restates reference
+// arithmetic, so ordinary FMA contraction is allowed.
 //
-// Face storage: ae[i] couples cell i with i+1 (x), an[i] with i+N, at[i] with i+N*M; a face on
-// the domain boundary is 0.  s[i] is 1/a_p (0 for solid and fully closed cells, whose rows the
-// reference keeps singular or near-singular).  The two pinned cells are identity rows.
+// Coarse levels are rediscretised: a coarse face is half the sum of the four fine faces it
+// covers (area x4, distance x2), the diagonal is the Neumann -sum of faces.  The finest level's
+// two pinned cells are identity rows there; a coarse level sees each as a Dirichlet sink on the
+// coarse cell that contains it, of conductance equal to the sum of the pinned cell's faces.
 
 #include <hip/hip_fp16.h>
 
-#include "lsx_mixed.hpp"
+#include "lsx_vcycle.hpp"
 
 namespace lsx {
-namespace mix {
+namespace vc {
 
 namespace {
 
@@ -31,14 +31,12 @@ inline int grid_1d(long n, int bs = 256) {
   return (int)(g < 1 ? 1 : g);
 }
 
-template <class FT>
-__device__ inline float ld(const FT* p, long i) {
-  return (float)p[i];
+template <class T, class FT>
+__device__ inline T ld(const FT* p, long i) {
+  return (T)p[i];
 }
 
 // ---- setup -----------------------------------------------------------------------------------
-// faces and inverse diagonal of the finest level from the FP64 7-plane operator [b,s,w,p,e,n,t];
-// a face is read from the row of the non-pinned cell (pinned rows are identity rows).
 __global__ void k_faces_from7(const double* __restrict__ c7, int N, int M, int K, int pin0, int pin1,
                               double* __restrict__ fe, double* __restrict__ fn, double* __restrict__ ft,
                               double* __restrict__ dg) {
@@ -54,7 +52,6 @@ __global__ void k_faces_from7(const double* __restrict__ c7, int N, int M, int K
   }
 }
 
-// coarse faces: half the sum of the four fine faces covering each coarse face
 __global__ void k_aggregate(const double* __restrict__ fe, const double* __restrict__ fn, const double* __restrict__ ft,
                             int Nf, int Mf, double* __restrict__ ce, double* __restrict__ cn, double* __restrict__ ct,
                             int Nc, int Mc, int Kc) {
@@ -82,7 +79,6 @@ __global__ void k_aggregate(const double* __restrict__ fe, const double* __restr
   }
 }
 
-// Neumann diagonal of an aggregated level: -sum of its six faces
 __global__ void k_diag_from_faces(const double* __restrict__ fe, const double* __restrict__ fn,
                                   const double* __restrict__ ft, int N, int M, int K, double* __restrict__ dg) {
   const long n = (long)N * M * K, NM = (long)N * M;
@@ -97,16 +93,40 @@ __global__ void k_diag_from_faces(const double* __restrict__ fe, const double* _
   }
 }
 
+__global__ void k_pin_sigma(const double* __restrict__ fe, const double* __restrict__ fn, const double* __restrict__ ft,
+                            int N, int M, int K, int pin0, int pin1, double* __restrict__ sigma) {
+  const int q = threadIdx.x;
+  if (blockIdx.x != 0 || q > 1) return;
+  const long NM = (long)N * M;
+  const long i = q == 0 ? pin0 : pin1;
+  double sgm = 0.0;
+  if (i >= 0) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    sgm = fe[i] + fn[i] + ft[i];
+    if (x > 0) sgm += fe[i - 1];
+    if (y > 0) sgm += fn[i - N];
+    if (z > 0) sgm += ft[i - NM];
+  }
+  sigma[q] = sgm;
+}
+
+__global__ void k_pin_sink(double* __restrict__ dg, int c0, int c1, const double* __restrict__ sigma) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  if (c0 >= 0) dg[c0] -= sigma[0];
+  if (c1 >= 0) dg[c1] -= sigma[1];
+}
+
 // inverse diagonal with the zero-pivot rule of the line smoother (tiny = max(1e-5 max|d|, 1e-12))
+template <class T>
 __global__ void k_inv_diag(const double* __restrict__ dg, long n, const double* __restrict__ absmax, int pin0,
-                           int pin1, float* __restrict__ s) {
+                           int pin1, T* __restrict__ s) {
   double tiny = 1e-5 * absmax[0];
   if (tiny < 1e-12) tiny = 1e-12;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const double d = dg[i];
-    float v = fabs(d) < tiny ? 0.0f : (float)(1.0 / d);
-    if (i == pin0 || i == pin1) v = 1.0f;
-    s[i] = v;
+    const bool pin = i == pin0 || i == pin1;  // identity-like rows keep their own pivot
+    s[i] = (fabs(d) < tiny && !pin) ? (T)0 : (T)(1.0 / d);
   }
 }
 
@@ -144,7 +164,6 @@ __global__ void k_max_reduce(const double* __restrict__ partial, int nb, double*
   if (threadIdx.x == 0) out[0] = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
 }
 
-// coarsest level as the 27-plane FP64 layout of the dense LU kernels
 __global__ void k_to27(const double* __restrict__ fe, const double* __restrict__ fn, const double* __restrict__ ft,
                        const double* __restrict__ dg, int N, int M, int K, int pin0, int pin1, double* __restrict__ c27) {
   const long n = (long)N * M * K, NM = (long)N * M;
@@ -152,11 +171,8 @@ __global__ void k_to27(const double* __restrict__ fe, const double* __restrict__
     int x, y, z;
     decode(i, N, M, &x, &y, &z);
     for (int q = 0; q < 27; ++q) c27[q * n + i] = 0.0;
-    if (i == pin0 || i == pin1) {
-      c27[13 * n + i] = 1.0;
-      continue;
-    }
     c27[13 * n + i] = dg[i];
+    if (i == pin0 || i == pin1) continue;
     if (x < N - 1) c27[14 * n + i] = fe[i];
     if (x > 0) c27[12 * n + i] = fe[i - 1];
     if (y < M - 1) c27[16 * n + i] = fn[i];
@@ -166,26 +182,26 @@ __global__ void k_to27(const double* __restrict__ fe, const double* __restrict__
   }
 }
 
-// ---- V-cycle ---------------------------------------------------------------------------------
-template <class FT>
-__device__ inline float offdiag(const float* __restrict__ x, const FT* __restrict__ ae, const FT* __restrict__ an,
-                                const FT* __restrict__ at, long i, int xx, int yy, int zz, int N, int M, int K) {
+// ---- cycle -----------------------------------------------------------------------------------
+template <class T, class FT>
+__device__ inline T offdiag(const T* __restrict__ x, const FT* __restrict__ ae, const FT* __restrict__ an,
+                            const FT* __restrict__ at, long i, int xx, int yy, int zz, int N, int M, int K) {
   const long NM = (long)N * M;
-  float acc = 0.0f;
-  if (xx > 0) acc += ld(ae, i - 1) * x[i - 1];
-  if (xx < N - 1) acc += ld(ae, i) * x[i + 1];
-  if (yy > 0) acc += ld(an, i - N) * x[i - N];
-  if (yy < M - 1) acc += ld(an, i) * x[i + N];
-  if (zz > 0) acc += ld(at, i - NM) * x[i - NM];
-  if (zz < K - 1) acc += ld(at, i) * x[i + NM];
+  T acc = 0;
+  if (xx > 0) acc += ld<T>(ae, i - 1) * x[i - 1];
+  if (xx < N - 1) acc += ld<T>(ae, i) * x[i + 1];
+  if (yy > 0) acc += ld<T>(an, i - N) * x[i - N];
+  if (yy < M - 1) acc += ld<T>(an, i) * x[i + N];
+  if (zz > 0) acc += ld<T>(at, i - NM) * x[i - NM];
+  if (zz < K - 1) acc += ld<T>(at, i) * x[i + NM];
   return acc;
 }
 
-// one colour of a red-black Gauss-Seidel sweep; thread t owns the t-th cell of that colour
-template <class FT>
-__global__ void k_rbgs(float* __restrict__ x, const float* __restrict__ b, const FT* __restrict__ ae,
-                       const FT* __restrict__ an, const FT* __restrict__ at, const float* __restrict__ s, float fs,
-                       int N, int M, int K, int color, int pin0, int pin1, int zero_guess) {
+// thread t owns the t-th cell of the colour
+template <class T, class FT>
+__global__ void k_rbgs(T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae, const FT* __restrict__ an,
+                       const FT* __restrict__ at, const T* __restrict__ s, T fs, int N, int M, int K, int color,
+                       int pin0, int pin1, int zero_guess) {
   const long half_row = (N + 1) / 2;
   const long rows = (long)M * K;
   const long total = half_row * rows;
@@ -195,40 +211,35 @@ __global__ void k_rbgs(float* __restrict__ x, const float* __restrict__ b, const
     const int xx = 2 * (int)(t - row * half_row) + ((yy + zz + color) & 1);
     if (xx >= N) continue;
     const long i = (long)xx + (long)N * row;
-    float v;
+    T v;
     if (i == pin0 || i == pin1) {
-      v = b[i];
+      v = b[i] * s[i];
     } else if (zero_guess) {
       v = b[i] * s[i];
     } else {
-      v = (b[i] - fs * offdiag(x, ae, an, at, i, xx, yy, zz, N, M, K)) * s[i];
+      v = (b[i] - fs * offdiag<T>(x, ae, an, at, i, xx, yy, zz, N, M, K)) * s[i];
     }
     x[i] = v;
   }
 }
 
-// r = b - A x on the 8 children of each coarse cell, summed into the coarse right-hand side
-template <class FT>
-__global__ void k_resid_restrict(const float* __restrict__ x, const float* __restrict__ b, const FT* __restrict__ ae,
-                                 const FT* __restrict__ an, const FT* __restrict__ at, const float* __restrict__ s,
-                                 float fs, int N, int M, int K, int pin0, int pin1, float* __restrict__ bc) {
+template <class T, class FT>
+__global__ void k_resid_restrict(const T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
+                                 const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s, T fs,
+                                 int N, int M, int K, int pin0, int pin1, T* __restrict__ bc) {
   const int Nc = N / 2, Mc = M / 2, Kc = K / 2;
   const long nc = (long)Nc * Mc * Kc, NM = (long)N * M;
   for (long I = blockIdx.x * (long)blockDim.x + threadIdx.x; I < nc; I += (long)gridDim.x * blockDim.x) {
     int X, Y, Z;
     decode(I, Nc, Mc, &X, &Y, &Z);
-    float acc = 0.0f;
+    T acc = 0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const int xx = 2 * X + (c & 1), yy = 2 * Y + ((c >> 1) & 1), zz = 2 * Z + (c >> 2);
       const long i = xx + (long)N * yy + NM * zz;
-      float r;
-      if (i == pin0 || i == pin1) {
-        r = b[i] - x[i];
-      } else {
-        const float ap = s[i] != 0.0f ? 1.0f / s[i] : 0.0f;
-        r = b[i] - ap * x[i] - fs * offdiag(x, ae, an, at, i, xx, yy, zz, N, M, K);
-      }
+      const T ap = s[i] != (T)0 ? (T)1 / s[i] : (T)0;
+      T r = b[i] - ap * x[i];
+      if (i != pin0 && i != pin1) r -= fs * offdiag<T>(x, ae, an, at, i, xx, yy, zz, N, M, K);
       acc += r;
     }
     bc[I] = acc;
@@ -246,9 +257,8 @@ __device__ inline void axis_w(int f, int nf, int* c, float* w, int* cnt) {
   }
 }
 
-// x_f += P x_c (trilinear)
-__global__ void k_prolong_add(const float* __restrict__ xc, int Nc, int Mc, float* __restrict__ xf, int Nf, int Mf,
-                              int Kf) {
+template <class T>
+__global__ void k_prolong_add(const T* __restrict__ xc, int Nc, int Mc, T* __restrict__ xf, int Nf, int Mf, int Kf) {
   const long nf = (long)Nf * Mf * Kf;
   const long NMc = (long)Nc * Mc;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nf; i += (long)gridDim.x * blockDim.x) {
@@ -259,19 +269,18 @@ __global__ void k_prolong_add(const float* __restrict__ xc, int Nc, int Mc, floa
     axis_w(x, Nf, cx, wx, &nx);
     axis_w(y, Mf, cy, wy, &ny);
     axis_w(z, Kf, cz, wz, &nz);
-    float acc = 0.0f;
+    T acc = 0;
     for (int c3 = 0; c3 < nz; ++c3)
       for (int c2 = 0; c2 < ny; ++c2)
-        for (int c1 = 0; c1 < nx; ++c1) acc += wx[c1] * wy[c2] * wz[c3] * xc[cx[c1] + Nc * cy[c2] + NMc * cz[c3]];
+        for (int c1 = 0; c1 < nx; ++c1)
+          acc += (T)(wx[c1] * wy[c2] * wz[c3]) * xc[cx[c1] + Nc * cy[c2] + NMc * cz[c3]];
     xf[i] += acc;
   }
 }
 
-__global__ void k_f64_to_f32(const double* __restrict__ a, float* __restrict__ b, long n) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = (float)a[i];
-}
-__global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ b, long n) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = (double)a[i];
+template <class A, class B>
+__global__ void k_convert(const A* __restrict__ a, B* __restrict__ b, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = (B)a[i];
 }
 
 }  // namespace
@@ -291,6 +300,15 @@ void diag_from_faces(const double* fe, const double* fn, const double* ft, Dims 
   k_diag_from_faces<<<grid_1d(d.n()), 256, 0, s>>>(fe, fn, ft, d.N, d.M, d.K, dg);
 }
 
+void pin_sigma(const double* fe, const double* fn, const double* ft, Dims d, int pin0, int pin1, double* sigma,
+               hipStream_t s) {
+  k_pin_sigma<<<1, 64, 0, s>>>(fe, fn, ft, d.N, d.M, d.K, pin0, pin1, sigma);
+}
+
+void pin_sink(double* dg, int c0, int c1, const double* sigma, hipStream_t s) {
+  k_pin_sink<<<1, 64, 0, s>>>(dg, c0, c1, sigma);
+}
+
 void face_absmax(const double* fe, const double* fn, const double* ft, long n, double* partial, double* out,
                  hipStream_t s) {
   int nb = (int)((n + 1023) / 1024);
@@ -300,55 +318,83 @@ void face_absmax(const double* fe, const double* fn, const double* ft, long n, d
   k_max_reduce<<<1, 256, 0, s>>>(partial, nb, out);
 }
 
-void inv_diag(const double* dg, long n, const double* absmax, int pin0, int pin1, float* sv, hipStream_t s) {
-  k_inv_diag<<<grid_1d(n), 256, 0, s>>>(dg, n, absmax, pin0, pin1, sv);
+template <class T>
+void inv_diag(const double* dg, long n, const double* absmax, int pin0, int pin1, T* sv, hipStream_t s) {
+  k_inv_diag<T><<<grid_1d(n), 256, 0, s>>>(dg, n, absmax, pin0, pin1, sv);
 }
+template void inv_diag<float>(const double*, long, const double*, int, int, float*, hipStream_t);
+template void inv_diag<double>(const double*, long, const double*, int, int, double*, hipStream_t);
 
-void store_faces_f32(const double* fe, const double* fn, const double* ft, long n, double inv_scale, float* ae,
-                     float* an, float* at, hipStream_t s) {
-  k_store_faces<float><<<grid_1d(n), 256, 0, s>>>(fe, fn, ft, n, inv_scale, ae, an, at);
+template <class FT>
+void store_faces(const double* fe, const double* fn, const double* ft, long n, double inv_scale, FT* ae, FT* an,
+                 FT* at, hipStream_t s) {
+  k_store_faces<FT><<<grid_1d(n), 256, 0, s>>>(fe, fn, ft, n, inv_scale, ae, an, at);
 }
-
-void store_faces_f16(const double* fe, const double* fn, const double* ft, long n, double inv_scale, _Float16* ae,
-                     _Float16* an, _Float16* at, hipStream_t s) {
-  k_store_faces<_Float16><<<grid_1d(n), 256, 0, s>>>(fe, fn, ft, n, inv_scale, ae, an, at);
-}
+template void store_faces<double>(const double*, const double*, const double*, long, double, double*, double*,
+                                  double*, hipStream_t);
+template void store_faces<float>(const double*, const double*, const double*, long, double, float*, float*, float*,
+                                 hipStream_t);
+template void store_faces<_Float16>(const double*, const double*, const double*, long, double, _Float16*, _Float16*,
+                                    _Float16*, hipStream_t);
 
 void to27(const double* fe, const double* fn, const double* ft, const double* dg, Dims d, int pin0, int pin1,
           double* c27, hipStream_t s) {
   k_to27<<<grid_1d(d.n()), 256, 0, s>>>(fe, fn, ft, dg, d.N, d.M, d.K, pin0, pin1, c27);
 }
 
-template <class FT>
-static void rbgs_t(float* x, const float* b, const FT* ae, const FT* an, const FT* at, const float* sv, float fs,
-                   Dims d, int color, int pin0, int pin1, int zero_guess, hipStream_t s) {
-  const long total = (long)((d.N + 1) / 2) * d.M * d.K;
-  k_rbgs<FT><<<grid_1d(total), 256, 0, s>>>(x, b, ae, an, at, sv, fs, d.N, d.M, d.K, color, pin0, pin1, zero_guess);
+template <class T, class FT>
+static void rbgs_t(const Level& L, T* x, const T* b, int color, int zero_guess, hipStream_t s) {
+  const long total = (long)((L.d.N + 1) / 2) * L.d.M * L.d.K;
+  k_rbgs<T, FT><<<grid_1d(total), 256, 0, s>>>(x, b, (const FT*)L.f_e, (const FT*)L.f_n, (const FT*)L.f_t,
+                                               (const T*)L.s, (T)L.fscale, L.d.N, L.d.M, L.d.K, color, L.pin0,
+                                               L.pin1, zero_guess);
 }
 
-void rbgs(const Level& L, float* x, const float* b, int color, int zero_guess, hipStream_t s) {
-  if (L.f16)
-    rbgs_t<_Float16>(x, b, L.h_e, L.h_n, L.h_t, L.s, L.fscale, L.d, color, L.pin0, L.pin1, zero_guess, s);
+template <>
+void rbgs<double>(const Level& L, double* x, const double* b, int color, int zero_guess, hipStream_t s) {
+  rbgs_t<double, double>(L, x, b, color, zero_guess, s);
+}
+template <>
+void rbgs<float>(const Level& L, float* x, const float* b, int color, int zero_guess, hipStream_t s) {
+  if (L.ftype == FACE_F16)
+    rbgs_t<float, _Float16>(L, x, b, color, zero_guess, s);
   else
-    rbgs_t<float>(x, b, L.f_e, L.f_n, L.f_t, L.s, L.fscale, L.d, color, L.pin0, L.pin1, zero_guess, s);
+    rbgs_t<float, float>(L, x, b, color, zero_guess, s);
 }
 
-void resid_restrict(const Level& L, const float* x, const float* b, float* bc, hipStream_t s) {
+template <class T, class FT>
+static void resid_restrict_t(const Level& L, const T* x, const T* b, T* bc, hipStream_t s) {
   const long nc = (long)(L.d.N / 2) * (L.d.M / 2) * (L.d.K / 2);
-  if (L.f16)
-    k_resid_restrict<_Float16><<<grid_1d(nc), 256, 0, s>>>(x, b, L.h_e, L.h_n, L.h_t, L.s, L.fscale, L.d.N, L.d.M,
-                                                           L.d.K, L.pin0, L.pin1, bc);
+  k_resid_restrict<T, FT><<<grid_1d(nc), 256, 0, s>>>(x, b, (const FT*)L.f_e, (const FT*)L.f_n, (const FT*)L.f_t,
+                                                      (const T*)L.s, (T)L.fscale, L.d.N, L.d.M, L.d.K, L.pin0,
+                                                      L.pin1, bc);
+}
+
+template <>
+void resid_restrict<double>(const Level& L, const double* x, const double* b, double* bc, hipStream_t s) {
+  resid_restrict_t<double, double>(L, x, b, bc, s);
+}
+template <>
+void resid_restrict<float>(const Level& L, const float* x, const float* b, float* bc, hipStream_t s) {
+  if (L.ftype == FACE_F16)
+    resid_restrict_t<float, _Float16>(L, x, b, bc, s);
   else
-    k_resid_restrict<float><<<grid_1d(nc), 256, 0, s>>>(x, b, L.f_e, L.f_n, L.f_t, L.s, L.fscale, L.d.N, L.d.M, L.d.K,
-                                                        L.pin0, L.pin1, bc);
+    resid_restrict_t<float, float>(L, x, b, bc, s);
 }
 
-void prolong_add(const float* xc, Dims dc, float* xf, Dims df, hipStream_t s) {
-  k_prolong_add<<<grid_1d(df.n()), 256, 0, s>>>(xc, dc.N, dc.M, xf, df.N, df.M, df.K);
+template <class T>
+void prolong_add(const T* xc, Dims dc, T* xf, Dims df, hipStream_t s) {
+  k_prolong_add<T><<<grid_1d(df.n()), 256, 0, s>>>(xc, dc.N, dc.M, xf, df.N, df.M, df.K);
 }
+template void prolong_add<float>(const float*, Dims, float*, Dims, hipStream_t);
+template void prolong_add<double>(const double*, Dims, double*, Dims, hipStream_t);
 
-void f64_to_f32(const double* a, float* b, long n, hipStream_t s) { k_f64_to_f32<<<grid_1d(n), 256, 0, s>>>(a, b, n); }
-void f32_to_f64(const float* a, double* b, long n, hipStream_t s) { k_f32_to_f64<<<grid_1d(n), 256, 0, s>>>(a, b, n); }
+template <class A, class B>
+void convert(const A* a, B* b, long n, hipStream_t s) {
+  k_convert<A, B><<<grid_1d(n), 256, 0, s>>>(a, b, n);
+}
+template void convert<double, float>(const double*, float*, long, hipStream_t);
+template void convert<float, double>(const float*, double*, long, hipStream_t);
 
-}  // namespace mix
+}  // namespace vc
 }  // namespace lsx

@@ -1,0 +1,80 @@
+// Geometric V-cycle preconditioner on face-coefficient storage: device kernels.
+//
+// Two instantiations share the kernels:
+//   FP64 : FP64 values and faces on every level (the plain FP64 V-cycle);
+//   mixed: FP32 values, FP32 faces on the finest level and FP16 faces (divided by a per-level
+//          scale) on coarse levels.
+// The coarsest level is always solved by the FP64 dense LU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lsx_kernels.hpp"
+
+namespace lsx {
+namespace vc {
+
+enum FaceType { FACE_F64 = 0, FACE_F32 = 1, FACE_F16 = 2 };
+
+// One level.  Faces: fe[i] couples cell i with i+1 (x), fn[i] with i+N, ft[i] with i+N*M; a
+// face on the domain boundary is 0.  s[i] = 1 / a_p (0 where the zero-pivot rule freezes the
+// cell).  pin0/pin1 are identity rows (the finest level's pinned cells), -1 on coarse levels.
+struct Level {
+  Dims d{0, 0, 0};
+  int ftype = FACE_F64;
+  const void *f_e = nullptr, *f_n = nullptr, *f_t = nullptr;
+  const void* s = nullptr;  // value type of the hierarchy
+  double fscale = 1.0;      // faces are stored divided by fscale
+  int pin0 = -1, pin1 = -1;
+};
+
+// ---- setup (FP64 temporaries) --------------------------------------------------------------
+// faces and diagonal of the finest level from the FP64 7-plane operator [b,s,w,p,e,n,t]; a face
+// is read from the row of the non-pinned cell (pinned rows are identity rows)
+void faces_from7(const double* c7, Dims d, int pin0, int pin1, double* fe, double* fn, double* ft, double* dg,
+                 hipStream_t s);
+// coarse faces: half the sum of the four fine faces covering each coarse face
+void aggregate(const double* fe, const double* fn, const double* ft, Dims df, double* ce, double* cn, double* ct,
+               Dims dc, hipStream_t s);
+// Neumann diagonal (-sum of the six faces)
+void diag_from_faces(const double* fe, const double* fn, const double* ft, Dims d, double* dg, hipStream_t s);
+// sigma[q] = sum of the faces of pinned cell q (q = 0, 1) on the finest level
+void pin_sigma(const double* fe, const double* fn, const double* ft, Dims d, int pin0, int pin1, double* sigma,
+               hipStream_t s);
+// dg[c0] -= sigma[0]; dg[c1] -= sigma[1]  (Dirichlet sinks of the pins on a coarse level)
+void pin_sink(double* dg, int c0, int c1, const double* sigma, hipStream_t s);
+void face_absmax(const double* fe, const double* fn, const double* ft, long n, double* partial, double* out,
+                 hipStream_t s);
+template <class T>
+void inv_diag(const double* dg, long n, const double* absmax, int pin0, int pin1, T* sv, hipStream_t s);
+template <class FT>
+void store_faces(const double* fe, const double* fn, const double* ft, long n, double inv_scale, FT* ae, FT* an,
+                 FT* at, hipStream_t s);
+// coarsest level as the 27-plane FP64 layout of the dense LU kernels
+void to27(const double* fe, const double* fn, const double* ft, const double* dg, Dims d, int pin0, int pin1,
+          double* c27, hipStream_t s);
+
+// ---- cycle (T = double with FACE_F64 levels, T = float with FACE_F32 / FACE_F16 levels) -------
+// one colour of a red-black Gauss-Seidel sweep (zero_guess: x = 0 before this half-sweep)
+template <class T>
+void rbgs(const Level& L, T* x, const T* b, int color, int zero_guess, hipStream_t s);
+// bc = sum over the 8 children of (b - A x)
+template <class T>
+void resid_restrict(const Level& L, const T* x, const T* b, T* bc, hipStream_t s);
+// x_f += P x_c (trilinear)
+template <class T>
+void prolong_add(const T* xc, Dims dc, T* xf, Dims df, hipStream_t s);
+template <class A, class B>
+void convert(const A* a, B* b, long n, hipStream_t s);
+
+template <>
+void rbgs<double>(const Level& L, double* x, const double* b, int color, int zero_guess, hipStream_t s);
+template <>
+void rbgs<float>(const Level& L, float* x, const float* b, int color, int zero_guess, hipStream_t s);
+template <>
+void resid_restrict<double>(const Level& L, const double* x, const double* b, double* bc, hipStream_t s);
+template <>
+void resid_restrict<float>(const Level& L, const float* x, const float* b, float* bc, hipStream_t s);
+
+}  // namespace vc
+}  // namespace lsx

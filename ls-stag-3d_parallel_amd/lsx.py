@@ -105,6 +105,7 @@ def lib():
     L.lsx_geometry_fields.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.lsx_set_geometry.argtypes = [vp, vp, C.c_int]
     L.lsx_operator_diagonal.argtypes = [vp, vp]
+    L.lsx_fine_storage.argtypes = [vp]
     L.lsx_kernel_timing.argtypes = [vp, C.c_int]
     L.lsx_kernel_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     _lib = L
@@ -293,6 +294,13 @@ class IterativeSolver:
         _check(lib().lsx_operator_diagonal(self.h, _ptr(d)))
         return d
 
+    def fine_storage(self):
+        """4 when the solve uses the symmetric four-plane fine operator, 7 for the general one."""
+        r = lib().lsx_fine_storage(self.h)
+        if r < 0:
+            _check(r)
+        return r
+
     def kernel_timing(self, enable=True):
         _check(lib().lsx_kernel_timing(self.h, 1 if enable else 0))
 
@@ -351,8 +359,15 @@ def mixed_task(eps, restart=30, max_it=1000, n_pre=2, n_post=2):
 
 
 def solver(kind="fgmres", precision="fp64", **kw):
-    """MGFGMRES / MGBiCGStab in the reference FP64 mode or the mixed-precision V-cycle mode."""
+    """MGFGMRES / MGBiCGStab with one of three preconditioners:
+    "fp64"        the reference FP64 multigrid cascade (ADLJ line smoothing, Galerkin levels);
+    "fp64-vcycle" the FP64 V-cycle (red-black Gauss-Seidel, rediscretised face levels);
+    "mixed"       the same V-cycle in FP32 with FP16 coarse faces."""
     cls = MGFGMRES if kind == "fgmres" else MGBiCGStab
     if precision == "mixed":
         return cls(precision=PREC_MIXED, mg=MG_VCYCLE, **kw)
+    if precision == "fp64-vcycle":
+        return cls(precision=PREC_FP64, mg=MG_VCYCLE, **kw)
+    if precision != "fp64":
+        raise ValueError(f"unknown precision {precision!r}")
     return cls(**kw)

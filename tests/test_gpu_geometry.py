@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import oracle_py as O
+import problems as P
 
 pytestmark = pytest.mark.gpu
 
@@ -62,12 +63,7 @@ def test_solve_on_gpu_geometry_matches_oracle(lsx):
     dims, ratio = (32, 32, 32), 1.02
     g_ref, g_gpu = build_both(lsx, dims, ratio, "ellipsoid", (0.5, 0.5, 0.5), (0.25, 0.15, 0.15))
     A = g_ref.laplace()
-    t = g_ref.fields()["type"]
-    rng = np.random.default_rng(3)
-    b = np.where(t != 0, rng.standard_normal(A.n), 0.0)
-    b[t != 0] -= b[t != 0].mean()
-    N, M, K = dims
-    b[N * M - 1] = b[N - 1] = 0.0
+    b = P.random_rhs(g_ref, seed=3, A=A)
     eps = 1e-8 * np.linalg.norm(b)
     x_ref, it_ref, _ = O.Solver(A, eps=eps, restart=30, max_it=500).solve(b)
     S = lsx.MGFGMRES()

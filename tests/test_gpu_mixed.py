@@ -1,6 +1,8 @@
-"""Mixed-precision mode (FP32 V-cycle, FP16 coarse faces, FP64 flexible Krylov loop): it must
-reach the same relative residual 1e-8 as the FP64 path, checked with the oracle's FP64 CSR
-operator, and its pressure is reported against the FP64 oracle solution of the same system."""
+"""V-cycle preconditioner (red-black Gauss-Seidel, rediscretised face levels, pinned cells as
+Dirichlet sinks on coarse levels) in FP64 and in mixed precision (FP32 values, FP16 coarse
+faces) inside the FP64 flexible Krylov loop.  Both must reach the relative residual 1e-8 that
+the FP64 reference path reaches, checked with the oracle's FP64 CSR operator, and the pressure
+is reported against the FP64 oracle solution of the same system."""
 import numpy as np
 import pytest
 
@@ -10,11 +12,11 @@ import problems as P
 pytestmark = pytest.mark.gpu
 
 
-def run_mixed(lsx, A, b, kind, restart=30):
+def run_vcycle(lsx, A, b, kind, precision, restart=30, max_it=500):
     eps = 1e-8 * np.linalg.norm(b)
-    S = lsx.solver(kind, "mixed")
+    S = lsx.solver(kind, precision)
     x = np.zeros(A.n)
-    S.set(A, b, x, lsx.mixed_task(eps, restart=restart, max_it=500))
+    S.set(A, b, x, lsx.mixed_task(eps, restart=restart, max_it=max_it))
     it = S.solve()
     return S, x, it, eps
 
@@ -22,36 +24,55 @@ def run_mixed(lsx, A, b, kind, restart=30):
 CASES = [
     ("sphere64", lambda: P.sphere(64)),
     ("ellipsoid64_stretched", lambda: P.ellipsoid_stretched(64)),
+    ("ellipsoid64_stretched_1.08", lambda: P.ellipsoid_stretched(64, ratio=1.08)),
     ("cut32x16x24", lambda: P.synthetic_cut(32, 16, 24, seed=3)),
 ]
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64-vcycle"])
 @pytest.mark.parametrize("kind", ["fgmres", "bicgstab"])
 @pytest.mark.parametrize("name,mk", CASES, ids=[c[0] for c in CASES])
-def test_mixed_reaches_fp64_tolerance(lsx, name, mk, kind):
+def test_vcycle_reaches_fp64_tolerance(lsx, name, mk, kind, precision):
     g, A = mk()
     b = P.random_rhs(g, seed=2, A=A)
-    S, x, it, eps = run_mixed(lsx, A, b, kind)
+    S, x, it, eps = run_vcycle(lsx, A, b, kind, precision)
     true_res = np.linalg.norm(b - A.matvec(x))
     x64, it64, _ = O.Solver(A, kind=kind, eps=eps, restart=30, max_it=500).solve(b)
     active = A.to_scipy().diagonal() < 0
     err = np.linalg.norm((x - x64)[active]) / np.linalg.norm(x64[active])
-    print(f"case={name} kind={kind} mixed_it={it} fp64_it={it64} true_rel_res={true_res / np.linalg.norm(b):.3e} "
-          f"p_err_vs_fp64={err:.3e}", flush=True)
+    print(f"case={name} kind={kind} precision={precision} it={it} fp64_cascade_it={it64} "
+          f"true_rel_res={true_res / np.linalg.norm(b):.3e} p_err_vs_fp64={err:.3e}", flush=True)
     assert S.norm_r <= eps
-    assert true_res <= 1.05 * eps
-    assert it < 100
+    # BiCGStab's recursively updated residual drifts from the true one by a few percent of eps
+    assert true_res <= 1.2 * eps
+    assert it < 60
     # both solutions satisfy ||b - A p|| <= 1e-8 ||b||; their difference is bounded by the
     # conditioning of the pinned Neumann operator
     assert err <= 1e-4
 
 
-def test_mixed_geometry_path_256_like_smaller(lsx):
+def test_mixed_and_fp64_vcycle_agree_on_one_application(lsx):
+    """One preconditioner application: the mixed V-cycle is the FP64 V-cycle up to FP32/FP16
+    rounding (relative difference well below the 1e-2 a wrong level or transfer would give)."""
+    g, A = P.ellipsoid_stretched(32)
+    v = P.random_rhs(g, seed=4, A=A)
+    z = {}
+    for precision in ("mixed", "fp64-vcycle"):
+        S = lsx.solver("fgmres", precision)
+        S.set(A, np.zeros(A.n), np.zeros(A.n), lsx.mixed_task(1.0))
+        z[precision] = S.precondition(v)
+    rel = np.linalg.norm(z["mixed"] - z["fp64-vcycle"]) / np.linalg.norm(z["fp64-vcycle"])
+    print(f"mixed vs fp64 V-cycle rel={rel:.3e}", flush=True)
+    assert rel <= 1e-3
+
+
+def test_vcycle_geometry_path_64(lsx):
     """The benchmark's construction (device geometry -> operator -> mixed solve) at 64^3."""
     w = [lsx.stretched_axis(64, 1.0, 0.5, 1.02) for _ in range(3)]
     geom = lsx.Geometry(*w).build("ellipsoid", (0.5, 0.5, 0.5), (0.25, 0.15, 0.15))
     S = lsx.solver("fgmres", "mixed")
     S.set_geometry(geom, None, None, lsx.mixed_task(1.0))
+    assert S.fine_storage() == 4
     b = lsx.random_divergence_rhs(S.operator_diagonal(), geom.dims)
     S.eps = 1e-8 * np.linalg.norm(b)
     x = np.zeros(geom.n)
@@ -59,4 +80,11 @@ def test_mixed_geometry_path_256_like_smaller(lsx):
     it = S.solve()
     r = b - S.apply_operator(x, 0)
     print(f"it={it} rel_res={np.linalg.norm(r) / np.linalg.norm(b):.3e}", flush=True)
-    assert np.linalg.norm(r) <= 1.05 * S.eps and it < 60
+    assert np.linalg.norm(r) <= 1.05 * S.eps and it < 40
+
+
+def test_vcycle_needs_a_pinned_row(lsx):
+    g, A = P.body_free(16, pin=False)
+    S = lsx.solver("fgmres", "mixed")
+    with pytest.raises(lsx.LsxError):
+        S.set(A, np.zeros(A.n), np.zeros(A.n), lsx.mixed_task(1.0))
