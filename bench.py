@@ -112,11 +112,11 @@ def cpu_baseline(sample_n, gpu_n, restart, krylov):
     }
 
 
-def run_solver(lsx, hip, geom, precision, krylov, restart, steps, warmup, max_it=1000):
+def run_solver(lsx, hip, geom, precision, krylov, restart, steps, warmup, max_it=1000, smooth=(2, 2)):
     """Set up one solver on the device geometry and time `steps` solves (x0 = 0 each)."""
     S = lsx.solver(krylov, precision)
     task = lsx.Task(eps=1.0, MAX_ITER=max_it, FGMRES_RESTART_NUM=restart) if precision == "fp64" else \
-        lsx.mixed_task(1.0, restart=restart, max_it=max_it)
+        lsx.mixed_task(1.0, restart=restart, max_it=max_it, n_pre=smooth[0], n_post=smooth[1])
     t_setup = time.perf_counter()
     S.set_geometry(geom, None, None, task)
     hip.sync()
@@ -171,6 +171,7 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64-vcycle", "fp64"], default="mixed")
     ap.add_argument("--krylov", choices=["fgmres", "bicgstab"], default="fgmres")
     ap.add_argument("--restart", type=int, default=30)
+    ap.add_argument("--smooth", default="2,2", help="V-cycle pre,post red-black sweeps")
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--cascade-max-it", type=int, default=200,
                     help="iteration cap of the reference-cascade comparison solve (0 = skip it)")
@@ -193,12 +194,15 @@ def main():
     geo_s = time.perf_counter() - t_geo
     print(f"[bench] geometry {n}^3 built in {geo_s:.2f} s", file=sys.stderr, flush=True)
 
-    main_run = run_solver(lsx, hip, geom, args.precision, args.krylov, args.restart, args.steps, args.warmup)
+    smooth = tuple(int(q) for q in args.smooth.split(","))
+    main_run = run_solver(lsx, hip, geom, args.precision, args.krylov, args.restart, args.steps, args.warmup,
+                          smooth=smooth)
     print(f"[bench] {args.precision}: {args.steps} solves in {main_run['elapsed']:.3f} s, its {main_run['its']}",
           file=sys.stderr, flush=True)
     fp64 = None
     if args.precision == "mixed" and not args.no_fp64_compare:
-        fp64 = run_solver(lsx, hip, geom, "fp64-vcycle", args.krylov, args.restart, max(1, args.steps // 2), 1)
+        fp64 = run_solver(lsx, hip, geom, "fp64-vcycle", args.krylov, args.restart, max(1, args.steps // 2), 1,
+                          smooth=smooth)
         print(f"[bench] fp64-vcycle: {len(fp64['its'])} solves in {fp64['elapsed']:.3f} s, its {fp64['its']}",
               file=sys.stderr, flush=True)
     cascade = None
@@ -228,8 +232,8 @@ def main():
         "workload": (f"{n}^3 stretched grid (ratio 1.02 to centre), ellipsoid body 0.25/0.15/0.15 "
                      f"(level set of the ellipsoid the Bezier body approximates), {args.krylov.upper()}"
                      f"({args.restart}), rel tol 1e-8, "
-                     + {"mixed": "mixed: FP32 V(2,2) red-black GS multigrid, FP16 coarse faces, FP64 Krylov",
-                        "fp64-vcycle": "FP64 V(2,2) red-black GS multigrid",
+                     + {"mixed": f"mixed: FP32 V{smooth} red-black GS multigrid, FP16 coarse faces, FP64 Krylov",
+                        "fp64-vcycle": f"FP64 V{smooth} red-black GS multigrid",
                         "fp64": "FP64 reference multigrid cascade (ADLJ)"}[args.precision]),
         "precision": args.precision, "krylov": args.krylov, "cells": geom.n,
         "iterations": main_run["its"], "true_rel_residual": main_run["rel_res"],

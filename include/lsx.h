@@ -125,16 +125,18 @@ int lsx_synchronize(lsx_solver* s);
 /* Kernel timing (no reference counterpart; the reference's profiler times host sections,
  * profiler.h:53-274).  While enabled, the fine-level launches of each kernel class below are
  * bracketed by HIP events on the handle's stream.  lsx_kernel_stats reports the launch count,
- * the summed device time and the kernel's algorithmic HBM bytes per launch (compulsory
- * traffic: each array read or written once). */
+ * the summed device time and the kernel's algorithmic HBM bytes per launch, averaged over the
+ * launches (compulsory traffic: each array read or written once). */
 #define LSX_K_RESIDUAL 0   /* r -= A c inside the smoother                 */
 #define LSX_K_LINE_X 1     /* x-line solve, c = w T^-1 r, ch += c           */
 #define LSX_K_LINE_Y 2
 #define LSX_K_LINE_Z 3
 #define LSX_K_OPERATOR 4   /* w = A z in the Krylov loop                    */
-#define LSX_K_ORTHO 5      /* fused w -= h v_i ; h' = w . v_{i+1}           */
-#define LSX_K_SMOOTH 6     /* fused smoother sweep (V-cycle)                */
-#define LSX_K_COUNT 8
+#define LSX_K_ORTHO 5      /* MGS step: fused w -= h v_i ; h' = w . v_{i+1}  */
+#define LSX_K_SMOOTH 6     /* smoother sweeps of the fine level (V-cycle)   */
+#define LSX_K_CGS_DOT 7    /* block Gram-Schmidt h = V^T w (V-cycle loops)  */
+#define LSX_K_CGS_UPDATE 8 /* w -= V h fused with V^T w or ||w|| (V-cycle)  */
+#define LSX_K_COUNT 16
 int lsx_kernel_timing(lsx_solver* s, int enable);
 int lsx_kernel_stats(lsx_solver* s, int kernel, long* launches, double* total_ms, double* bytes_per_launch);
 
@@ -167,6 +169,26 @@ int lsx_geometry_build(lsx_geometry* g, const lsx_body* body, int hs, int hf, in
  * east/north/top face fractions, cut volumes, upper-corner level set. */
 int lsx_geometry_fields(const lsx_geometry* g, int* type, double* thx, double* thy, double* thz, double* vof,
                         double* phi);
+
+/* Boundary data of the pressure equation's flux (FlowTaskDescription U_E, V_S, V_N, W_B, W_T,
+ * flow_task_description.h:79-96). */
+typedef struct lsx_flow_bc {
+  double U_E, V_S, V_N, W_B, W_T;
+  int outflow; /* 1: the east boundary flux is the last interior face of u_out (setBaseFlux) */
+  int pin;     /* 1: b = 0 at the pinned cells N-1 and N*M-1 (fixPoint)                   */
+} lsx_flow_bc;
+
+/* Pressure right-hand side b = Dx u + Dy v + Dz w + flux for a static body, computed on the
+ * device: replaces PoissonDiscreteEquation::recalcRightMember (poisson_discrete_equation.h:
+ * 145-159) with DivergenceDiscreteOperatorX/Y/Z::calcD* (divergence_discrete_operator_x.h:
+ * 119-159 and its y, z siblings) and the flux of LaplaceDiscreteOperator::initFlux, addFluxNSp,
+ * addFluxTBp, setBaseFlux (laplace_discrete_operator.h:48-83, 744-750, 799-808).  Face arrays:
+ * u has (N-1)*M*K values (index (y + M z)(N-1) + x), v has N(M-1)K (z N(M-1) + y N + x), w has
+ * N*M*(K-1) (z N M + y N + x); u_west has M*K inflow velocities (pU_W, index y + M z); u_out is
+ * the x-face field of the outflow flux (the reference's current u), NULL to use u.  Every array
+ * is in `mem` space; b receives N*M*K values. */
+int lsx_geometry_rhs(lsx_geometry* g, const double* u, const double* v, const double* w, const double* u_west,
+                     const double* u_out, const lsx_flow_bc* bc, double* b, int mem);
 
 /* Set the solver's matrix to the cut-cell pressure operator of `g` (built on the device, no
  * CSR round trip) and build the preconditioner; pin != 0 turns rows (N-1,0,0) and

@@ -188,6 +188,13 @@ int lsx_geometry_fields(const lsx_geometry* g, int* type, double* thx, double* t
   return guarded([&] { g->impl.fields(type, thx, thy, thz, vof, phi); });
 }
 
+int lsx_geometry_rhs(lsx_geometry* g, const double* u, const double* v, const double* w, const double* u_west,
+                     const double* u_out, const lsx_flow_bc* bc, double* b, int mem) {
+  if (!g || !u || !v || !w || !u_west || !bc || !b) return LSX_ERR_ARG;
+  if (mem != LSX_MEM_HOST && mem != LSX_MEM_DEVICE) return LSX_ERR_ARG;
+  return guarded([&] { g->impl.pressure_rhs(u, v, w, u_west, u_out, *bc, b, mem); });
+}
+
 int lsx_kernel_timing(lsx_solver* s, int enable) {
   if (!s) return LSX_ERR_ARG;
   return guarded([&] { s->impl.set_timing(enable != 0); });

@@ -590,5 +590,115 @@ void laplace7(const int* type, const double* tx, const double* ty, const double*
   k_laplace7<<<grid_1d((long)N * M * K), 256, 0, s>>>(type, tx, ty, tz, v, dx, dy, dz, N, M, K, pin, coef);
 }
 
+// ---- pressure right-hand side ------------------------------------------------------------------
+// b = Dx u* + Dy v* + Dz w* + flux, b = 0 at the pinned cells: PoissonDiscreteEquation::
+// recalcRightMember (poisson_discrete_equation.h:145-159) with the divergence rows of calcDx/Dy/Dz
+// (divergence_discrete_operator_{x,y,z}.h) summed in their portrait order, and the static-body
+// flux of initFlux / addFluxNSp / addFluxTBp / setBaseFlux (laplace_discrete_operator.h:48-83,
+// 744-750) applied in that order.  One thread per cell; no FMA contraction.
+namespace {
+
+__device__ inline bool x_east(int t) {
+  return t == E_Border || t == NE_Edge || t == SE_Edge || t == TE_Edge || t == BE_Edge || t == TNE_Corner ||
+         t == TSE_Corner || t == BNE_Corner || t == BSE_Corner;
+}
+__device__ inline bool x_west(int t) {
+  return t == W_Border || t == NW_Edge || t == SW_Edge || t == TW_Edge || t == BW_Edge || t == TNW_Corner ||
+         t == TSW_Corner || t == BNW_Corner || t == BSW_Corner;
+}
+__device__ inline bool y_north(int t) {
+  return t == N_Border || t == NE_Edge || t == NW_Edge || t == TN_Edge || t == BN_Edge || t == TNE_Corner ||
+         t == TNW_Corner || t == BNE_Corner || t == BNW_Corner;
+}
+__device__ inline bool y_south(int t) {
+  return t == S_Border || t == SE_Edge || t == SW_Edge || t == TS_Edge || t == BS_Edge || t == TSE_Corner ||
+         t == TSW_Corner || t == BSE_Corner || t == BSW_Corner;
+}
+__device__ inline bool z_bottom(int t) {
+  return t == B_Border || t == BN_Edge || t == BS_Edge || t == BE_Edge || t == BW_Edge || t == BNE_Corner ||
+         t == BNW_Corner || t == BSE_Corner || t == BSW_Corner;
+}
+__device__ inline bool z_top(int t) {
+  return t == T_Border || t == TN_Edge || t == TS_Edge || t == TE_Edge || t == TW_Edge || t == TNE_Corner ||
+         t == TNW_Corner || t == TSE_Corner || t == TSW_Corner;
+}
+
+__global__ void k_pressure_rhs(const int* __restrict__ type, const double* __restrict__ tx,
+                               const double* __restrict__ ty, const double* __restrict__ tz,
+                               const double* __restrict__ dxa, const double* __restrict__ dya,
+                               const double* __restrict__ dza, int N, int M, int K, const double* __restrict__ u,
+                               const double* __restrict__ v, const double* __restrict__ w,
+                               const double* __restrict__ u_west, const double* __restrict__ u_out, FlowBc bc,
+                               double* __restrict__ b) {
+  const long n = (long)N * M * K, NM = (long)N * M;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int z = (int)(i / NM);
+    const long r2 = i - (long)z * NM;
+    const int y = (int)(r2 / N), x = (int)(r2 - (long)y * N);
+    const int t = type[i];
+    const double dx = dxa[x], dy = dya[y], dz = dza[z];
+    const long row = y + (long)z * M;
+    // flux: initFlux, addFluxNSp, addFluxTBp, setBaseFlux (static body: ibFlux = 0)
+    double flux = 0.0;
+    if (x == 0) flux = -u_west[row] * (dy * dz);
+    if (x == N - 1) flux = bc.U_E * (dy * dz);
+    if (y == 0) flux -= (dx * dz) * bc.V_S;
+    if (y == M - 1) flux += (dx * dz) * bc.V_N;
+    if (z == 0) flux -= (dy * dx) * tz[i] * bc.W_B;
+    if (z == K - 1) flux += (dy * dx) * tz[i] * bc.W_T;
+    if (u_out && x == N - 1) flux = u_out[row * (N - 1) + N - 2] * dy * dz;
+    // Dx u
+    double aux = 0.0;
+    {
+      const long p = row * (N - 1) + x;
+      if (t == SolidCell) aux += 0.0 * u[x == 0 ? p : p - 1];
+      else if (x_east(t)) aux += (-tx[i - 1] * dy * dz) * u[p - 1];
+      else if (x_west(t)) aux += (tx[i] * dy * dz) * u[p];
+      else {
+        const double dyz = dy * dz;
+        aux += (-tx[i - 1] * dyz) * u[p - 1];
+        aux += (tx[i] * dyz) * u[p];
+      }
+    }
+    // (aux + flux) + Dy v
+    double r = aux + flux;
+    {
+      const long p = (long)z * N * (M - 1) + (long)y * N + x;
+      if (t == SolidCell) r += 0.0 * v[y == 0 ? p : p - N];
+      else if (y_north(t)) r += (-ty[i - N] * dx * dz) * v[p - N];
+      else if (y_south(t)) r += (ty[i] * dx * dz) * v[p];
+      else {
+        const double dxz = dx * dz;
+        r += (-ty[i - N] * dxz) * v[p - N];
+        r += (ty[i] * dxz) * v[p];
+      }
+    }
+    // + Dz w
+    double auxz = 0.0;
+    {
+      if (t == SolidCell) auxz += 0.0 * w[z == 0 ? i : i - NM];
+      else if (z_bottom(t)) auxz += (tz[i] * dx * dy) * w[i];
+      else if (z_top(t)) auxz += (-tz[i - NM] * dx * dy) * w[i - NM];
+      else {
+        const double dxy = dx * dy;
+        auxz += (-tz[i - NM] * dxy) * w[i - NM];
+        auxz += (tz[i] * dxy) * w[i];
+      }
+    }
+    r += auxz;
+    if (bc.pin && (i == N - 1 || i == NM - 1)) r = 0.0;
+    b[i] = r;
+  }
+}
+
+}  // namespace
+
+void pressure_rhs(const int* type, const double* tx, const double* ty, const double* tz, const double* dx,
+                  const double* dy, const double* dz, int N, int M, int K, const double* u, const double* v,
+                  const double* w, const double* u_west, const double* u_out, FlowBc bc, double* b, hipStream_t s) {
+  k_pressure_rhs<<<grid_1d((long)N * M * K), 256, 0, s>>>(type, tx, ty, tz, dx, dy, dz, N, M, K, u, v, w, u_west,
+                                                          u_out, bc, b);
+}
+
 }  // namespace geo
 }  // namespace lsx

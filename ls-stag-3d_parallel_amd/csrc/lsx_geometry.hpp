@@ -46,5 +46,16 @@ void vof(const int* type, const double* ax, const double* ay, const double* az, 
 void laplace7(const int* type, const double* tx, const double* ty, const double* tz, const double* v, const double* dx,
               const double* dy, const double* dz, int N, int M, int K, int pin, double* coef, hipStream_t s);
 
+// boundary velocities of the pressure equation's flux (flow_task_description.h:79-96)
+struct FlowBc {
+  double U_E, V_S, V_N, W_B, W_T;
+  int pin;  // b = 0 at cells N-1 and N*M-1
+};
+// b = D u* + flux on the base grid (u_west: one inflow velocity per x-row; u_out: x-face field
+// whose last interior face feeds an outflow east boundary, or null for the fixed U_E)
+void pressure_rhs(const int* type, const double* tx, const double* ty, const double* tz, const double* dx,
+                  const double* dy, const double* dz, int N, int M, int K, const double* u, const double* v,
+                  const double* w, const double* u_west, const double* u_out, FlowBc bc, double* b, hipStream_t s);
+
 }  // namespace geo
 }  // namespace lsx

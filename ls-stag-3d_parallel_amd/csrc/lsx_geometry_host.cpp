@@ -126,4 +126,44 @@ void Geometry::laplace7(int pin, double* coef_dev, hipStream_t s) const {
   LSX_HIP(hipGetLastError());
 }
 
+void Geometry::pressure_rhs(const double* u, const double* v, const double* w, const double* u_west,
+                            const double* u_out, const lsx_flow_bc& bc, double* b, int mem) {
+  if (!built) throw Error(LSX_ERR_STATE, "geometry not built");
+  const long nn = n(), nu = (long)(N - 1) * M * K, nv = (long)N * (M - 1) * K, nw = (long)N * M * (K - 1);
+  const long nrow = (long)M * K;
+  geo::FlowBc fb{bc.U_E, bc.V_S, bc.V_N, bc.W_B, bc.W_T, bc.pin != 0};
+  if (mem == LSX_MEM_DEVICE) {
+    geo::pressure_rhs(type.p, tx.p, ty.p, tz.p, dx.p, dy.p, dz.p, N, M, K, u, v, w, u_west,
+                      bc.outflow ? (u_out ? u_out : u) : nullptr, fb, b, stream);
+    LSX_HIP(hipGetLastError());
+    LSX_HIP(hipStreamSynchronize(stream));
+    return;
+  }
+  DevBuf<double> du, dv, dw, dwest, dout, db;
+  du.alloc(nu);
+  dv.alloc(nv);
+  dw.alloc(nw);
+  dwest.alloc(nrow);
+  db.alloc(nn);
+  LSX_HIP(hipMemcpyAsync(du.p, u, nu * sizeof(double), hipMemcpyHostToDevice, stream));
+  LSX_HIP(hipMemcpyAsync(dv.p, v, nv * sizeof(double), hipMemcpyHostToDevice, stream));
+  LSX_HIP(hipMemcpyAsync(dw.p, w, nw * sizeof(double), hipMemcpyHostToDevice, stream));
+  LSX_HIP(hipMemcpyAsync(dwest.p, u_west, nrow * sizeof(double), hipMemcpyHostToDevice, stream));
+  const double* pout = nullptr;
+  if (bc.outflow) {
+    if (u_out) {
+      dout.alloc(nu);
+      LSX_HIP(hipMemcpyAsync(dout.p, u_out, nu * sizeof(double), hipMemcpyHostToDevice, stream));
+      pout = dout.p;
+    } else {
+      pout = du.p;
+    }
+  }
+  geo::pressure_rhs(type.p, tx.p, ty.p, tz.p, dx.p, dy.p, dz.p, N, M, K, du.p, dv.p, dw.p, dwest.p, pout, fb, db.p,
+                    stream);
+  LSX_HIP(hipGetLastError());
+  LSX_HIP(hipMemcpyAsync(b, db.p, nn * sizeof(double), hipMemcpyDeviceToHost, stream));
+  LSX_HIP(hipStreamSynchronize(stream));
+}
+
 }  // namespace lsx

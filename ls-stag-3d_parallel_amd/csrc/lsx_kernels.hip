@@ -689,6 +689,92 @@ __global__ void k_axpy_dot(double* __restrict__ w, const double* __restrict__ v,
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// ---- block classical Gram-Schmidt ------------------------------------------------------------
+// V holds cnt <= CGS_MAX basis vectors, vector k at V + k*ldv.  Each block reduces its chunk of
+// rows; partial[k * nb + block] is then summed in a fixed order by k_finalize_multi.
+template <int BS>
+__device__ void block_sum_multi(double (&acc)[CGS_MAX], int cnt, double* partial, int nb) {
+  __shared__ double lds[BS / 64][CGS_MAX];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < CGS_MAX; ++k) {
+    if (k < cnt) {
+      double v = acc[k];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      if (lane == 0) lds[wave][k] = v;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < cnt) {
+    double s = 0.0;
+    for (int w = 0; w < BS / 64; ++w) s += lds[w][threadIdx.x];
+    partial[(long)threadIdx.x * nb + blockIdx.x] = s;
+  }
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) k_cgs_dot(const double* __restrict__ V, long ldv, int cnt,
+                                                const double* __restrict__ w, long n, double* __restrict__ partial) {
+  long lo, hi;
+  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
+  double acc[CGS_MAX];
+#pragma unroll
+  for (int k = 0; k < CGS_MAX; ++k) acc[k] = 0.0;
+  for (long i = lo + threadIdx.x; i < hi; i += BS) {
+    const double wi = w[i];
+#pragma unroll
+    for (int k = 0; k < CGS_MAX; ++k)
+      if (k < cnt) acc[k] += V[k * ldv + i] * wi;
+  }
+  block_sum_multi<BS>(acc, cnt, partial, gridDim.x);
+}
+
+// w -= V h; MODE 0: partial V^T w (reorthogonalisation coefficients), MODE 1: partial ||w||^2
+template <int BS, int MODE>
+__global__ void __launch_bounds__(BS) k_cgs_update(const double* __restrict__ V, long ldv, int cnt,
+                                                   const double* __restrict__ h, double* __restrict__ w, long n,
+                                                   double* __restrict__ partial) {
+  long lo, hi;
+  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
+  double hk[CGS_MAX], acc[CGS_MAX];
+#pragma unroll
+  for (int k = 0; k < CGS_MAX; ++k) {
+    hk[k] = k < cnt ? h[k] : 0.0;
+    acc[k] = 0.0;
+  }
+  for (long i = lo + threadIdx.x; i < hi; i += BS) {
+    double v[CGS_MAX];
+    double wn = w[i];
+#pragma unroll
+    for (int k = 0; k < CGS_MAX; ++k)
+      if (k < cnt) {
+        v[k] = V[k * ldv + i];
+        wn -= hk[k] * v[k];
+      }
+    w[i] = wn;
+    if (MODE == 0) {
+#pragma unroll
+      for (int k = 0; k < CGS_MAX; ++k)
+        if (k < cnt) acc[k] += wn * v[k];
+    } else {
+      acc[0] += wn * wn;
+    }
+  }
+  block_sum_multi<BS>(acc, MODE == 0 ? cnt : 1, partial, gridDim.x);
+}
+
+template <int BS>
+__global__ void k_finalize_multi(const double* __restrict__ partial, int nb, double* __restrict__ out,
+                                 int sqrt_flag) {
+  const double* p = partial + (long)blockIdx.x * nb;
+  double s = 0.0;
+  const int per = (nb + BS - 1) / BS;
+  const int lo = threadIdx.x * per;
+  for (int i = lo; i < lo + per && i < nb; ++i) s += p[i];
+  s = block_sum<BS>(s);
+  if (threadIdx.x == 0) out[blockIdx.x] = sqrt_flag ? sqrt(s) : s;
+}
+
 // y = A x on the fine 7-point operator with an FP32 input vector (mixed-precision Krylov basis)
 __global__ void k_stencil7_prod_f32x(const double* __restrict__ coef, int N, int M, int K,
                                      const float* __restrict__ x, double* __restrict__ y) {
@@ -915,6 +1001,25 @@ void sol_update(double* x, const double* Z, long ldz, const double* yk, int coun
     for (int k = lo; k < hi; ++k) c.y[k - lo] = yk[k];
     k_sol_update<<<grid_1d(n), 256, 0, s>>>(x, Z + (long)lo * ldz, ldz, c, hi - lo, n);
     hi = lo;
+  }
+}
+
+void cgs_dot(const double* V, long ldv, int cnt, const double* w, long n, double* partial, double* out,
+             hipStream_t s) {
+  const int nb = reduction_blocks(n);
+  k_cgs_dot<RED_THREADS><<<nb, RED_THREADS, 0, s>>>(V, ldv, cnt, w, n, partial);
+  k_finalize_multi<RED_THREADS><<<cnt, RED_THREADS, 0, s>>>(partial, nb, out, 0);
+}
+
+void cgs_update(const double* V, long ldv, int cnt, const double* h, double* w, long n, int mode, double* partial,
+                double* out, hipStream_t s) {
+  const int nb = reduction_blocks(n);
+  if (mode == 0) {
+    k_cgs_update<RED_THREADS, 0><<<nb, RED_THREADS, 0, s>>>(V, ldv, cnt, h, w, n, partial);
+    k_finalize_multi<RED_THREADS><<<cnt, RED_THREADS, 0, s>>>(partial, nb, out, 0);
+  } else {
+    k_cgs_update<RED_THREADS, 1><<<nb, RED_THREADS, 0, s>>>(V, ldv, cnt, h, w, n, partial);
+    k_finalize_multi<RED_THREADS><<<1, RED_THREADS, 0, s>>>(partial, nb, out, 1);
   }
 }
 

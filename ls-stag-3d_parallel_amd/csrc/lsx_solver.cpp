@@ -41,6 +41,8 @@ Solver::Solver(const lsx_options& o, hipStream_t s) : opt(o) {
   err.alloc(1);
   scalars.alloc(SC_H + 256 * 257 / 2 + 8);
   partial.alloc(2048);
+  mpartial.alloc((size_t)CGS_MAX * 1024);
+  cgs_h.alloc(2 * CGS_MAX);
 }
 
 Solver::~Solver() {
@@ -68,11 +70,12 @@ void Solver::tbeg(int k, hipEvent_t* e) {
   LSX_HIP(hipEventRecord(*e, stream));
 }
 
-void Solver::tend(int k, hipEvent_t b) {
+void Solver::tend(int k, hipEvent_t b, double bytes) {
   if (!timing || !b) return;
   hipEvent_t e = ev_take();
   LSX_HIP(hipEventRecord(e, stream));
   ev_rec[k].push_back({b, e});
+  ev_bytes[k] += bytes < 0 ? kernel_bytes(k) : bytes;
   if (ev_used > 4096) drain_events();
 }
 
@@ -95,6 +98,7 @@ void Solver::set_timing(bool on) {
   timing = on;
   for (int k = 0; k < LSX_K_COUNT; ++k) {
     ev_ms[k] = 0.0;
+    ev_bytes[k] = 0.0;
     ev_count[k] = 0;
   }
 }
@@ -122,7 +126,7 @@ void Solver::kernel_stats(int k, long* launches, double* total_ms, double* bytes
   if (timing) drain_events();
   if (launches) *launches = ev_count[k];
   if (total_ms) *total_ms = ev_ms[k];
-  if (bytes) *bytes = kernel_bytes(k);
+  if (bytes) *bytes = ev_count[k] ? ev_bytes[k] / ev_count[k] : kernel_bytes(k);
 }
 
 void Solver::synchronize() { LSX_HIP(hipStreamSynchronize(stream)); }
@@ -356,12 +360,11 @@ void Solver::build_vcycle() {
       LSX_HIP(hipStreamSynchronize(stream));
       break;
     }
-    absmax(dg[cur].p, nl, partial.p, scalars.p + SC_TINY, stream);
     if (m32) {
       S.fs.alloc(nl);
       S.fb.alloc(nl);
       if (l > 0) S.fx.alloc(nl);  // level 0 writes into the caller's output
-      vc::inv_diag<float>(dg[cur].p, nl, scalars.p + SC_TINY, pin0, pin1, S.fs.p, stream);
+      vc::inv_diag<float>(dg[cur].p, nl, S.fs.p, stream);
       S.L.s = S.fs.p;
     } else {
       S.ds.alloc(nl);
@@ -369,7 +372,7 @@ void Solver::build_vcycle() {
         S.db.alloc(nl);
         S.dx.alloc(nl);
       }
-      vc::inv_diag<double>(dg[cur].p, nl, scalars.p + SC_TINY, pin0, pin1, S.ds.p, stream);
+      vc::inv_diag<double>(dg[cur].p, nl, S.ds.p, stream);
       S.L.s = S.ds.p;
     }
     if (!m32) {
@@ -623,7 +626,8 @@ int Solver::fgmres(const double* b, double* x) {
   int iters = 1;
   double normW = norm_r;
   double curH = norm_r;
-  std::vector<double> hcol(m + 2);
+  std::vector<double> hcol(m + 2), hcgs(2 * CGS_MAX);
+  const bool cgs = vcyc() && m <= CGS_MAX;
   for (;;) {
     // normW lives on the device for the scaling kernel
     LSX_HIP(hipMemcpyAsync(dnorm, &normW, sizeof(double), hipMemcpyHostToDevice, stream));
@@ -649,25 +653,47 @@ int Solver::fgmres(const double* b, double* x) {
         fine_apply(zj, nullptr, w.p, ST_PROD);
       }
       tend(LSX_K_OPERATOR, e0);
-      // modified Gram-Schmidt: h_i = w . v_i ; w -= h_i v_i  (the dot of the next basis vector
-      // is fused into the update pass; the last pass yields ||w||^2)
-      dot_partial(w.p, V.p, n, partial.p, stream);
-      finalize(partial.p, nb, dH + hColStart, 0, stream);
-      for (int i = 0; i <= j; ++i) {
-        const double* vi = V.p + (size_t)i * n;
-        const double* vnext = i < j ? V.p + (size_t)(i + 1) * n : nullptr;
-        if (vnext) tbeg(LSX_K_ORTHO, &e0);
-        axpy_dot(w.p, vi, dH + hColStart + i, vnext, n, partial.p, stream);
-        if (vnext) tend(LSX_K_ORTHO, e0);
-        if (vnext)
-          finalize(partial.p, nb, dH + hColStart + i + 1, 0, stream);
-        else
-          finalize(partial.p, nb, dnorm, 1, stream);
+      if (cgs) {
+        // block classical Gram-Schmidt with one reorthogonalisation (the V-cycle loops are free to
+        // choose their orthogonalisation): h = V^T w ; w -= V h fused with h2 = V^T w ;
+        // w -= V h2 fused with ||w||.  Three passes over V instead of j + 1.
+        const int cnt = j + 1;
+        const double vb = 8.0 * cnt * (double)n;
+        tbeg(LSX_K_CGS_DOT, &e0);
+        cgs_dot(V.p, n, cnt, w.p, n, mpartial.p, cgs_h.p, stream);
+        tend(LSX_K_CGS_DOT, e0, vb + 8.0 * n);
+        tbeg(LSX_K_CGS_UPDATE, &e0);
+        cgs_update(V.p, n, cnt, cgs_h.p, w.p, n, 0, mpartial.p, cgs_h.p + CGS_MAX, stream);
+        tend(LSX_K_CGS_UPDATE, e0, vb + 16.0 * n);
+        tbeg(LSX_K_CGS_UPDATE, &e0);
+        cgs_update(V.p, n, cnt, cgs_h.p + CGS_MAX, w.p, n, 1, mpartial.p, dnorm, stream);
+        tend(LSX_K_CGS_UPDATE, e0, vb + 16.0 * n);
+        hNum = hColStart + j + 1;
+        LSX_HIP(hipMemcpyAsync(hcgs.data(), cgs_h.p, 2 * CGS_MAX * sizeof(double), hipMemcpyDeviceToHost, stream));
+        LSX_HIP(hipMemcpyAsync(&normW, dnorm, sizeof(double), hipMemcpyDeviceToHost, stream));
+        LSX_HIP(hipStreamSynchronize(stream));
+        for (int i = 0; i <= j; ++i) hcol[i] = hcgs[i] + hcgs[CGS_MAX + i];
+      } else {
+        // modified Gram-Schmidt: h_i = w . v_i ; w -= h_i v_i  (the dot of the next basis vector
+        // is fused into the update pass; the last pass yields ||w||^2)
+        dot_partial(w.p, V.p, n, partial.p, stream);
+        finalize(partial.p, nb, dH + hColStart, 0, stream);
+        for (int i = 0; i <= j; ++i) {
+          const double* vi = V.p + (size_t)i * n;
+          const double* vnext = i < j ? V.p + (size_t)(i + 1) * n : nullptr;
+          if (vnext) tbeg(LSX_K_ORTHO, &e0);
+          axpy_dot(w.p, vi, dH + hColStart + i, vnext, n, partial.p, stream);
+          if (vnext) tend(LSX_K_ORTHO, e0);
+          if (vnext)
+            finalize(partial.p, nb, dH + hColStart + i + 1, 0, stream);
+          else
+            finalize(partial.p, nb, dnorm, 1, stream);
+        }
+        hNum = hColStart + j + 1;
+        LSX_HIP(hipMemcpyAsync(hcol.data(), dH + hColStart, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, stream));
+        LSX_HIP(hipMemcpyAsync(&normW, dnorm, sizeof(double), hipMemcpyDeviceToHost, stream));
+        LSX_HIP(hipStreamSynchronize(stream));
       }
-      hNum = hColStart + j + 1;
-      LSX_HIP(hipMemcpyAsync(hcol.data(), dH + hColStart, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, stream));
-      LSX_HIP(hipMemcpyAsync(&normW, dnorm, sizeof(double), hipMemcpyDeviceToHost, stream));
-      LSX_HIP(hipStreamSynchronize(stream));
       for (int i = 0; i <= j; ++i) H[hColStart + i] = hcol[i];
       curH = H[hColStart];
       for (int i = 0, cur = hColStart, nxt = cur + 1; i < j; ++i, cur = nxt, ++nxt) {
@@ -694,9 +720,16 @@ int Solver::fgmres(const double* b, double* x) {
       }
     }
     sol_upd(m - 1, m_size - 1);
-    // w = b - A x starts the next cycle; its norm is the carried estimate |gamma|
+    // w = b - A x starts the next cycle.  The reference scales it by the carried estimate
+    // |gamma| and keeps the sign of gamma; the V-cycle loops restart from the true residual.
     fine_apply(x, b, w.p, ST_RESID);
-    normW = curH;
+    if (vcyc()) {
+      finalize(partial.p, nb, dnorm, 1, stream);
+      normW = reduce_to_host(dnorm);
+      norm_r = normW;
+    } else {
+      normW = curH;
+    }
   }
 }
 

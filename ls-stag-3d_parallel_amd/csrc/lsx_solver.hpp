@@ -123,6 +123,7 @@ class Solver {
   DevBuf<int> lu_perm;
   DevBuf<double> cwork;   // correction increment c (fine size, reused on every level)
   DevBuf<double> partial, scalars;
+  DevBuf<double> mpartial, cgs_h;  // block Gram-Schmidt partial sums and coefficients
   // Krylov workspace
   DevBuf<double> V, Z, w;                             // FGMRES
   DevBuf<double> bnu, bt, by, bz, br, bp, bs, brr;   // BiCGStab
@@ -164,10 +165,11 @@ class Solver {
   size_t ev_used = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_rec[LSX_K_COUNT];
   double ev_ms[LSX_K_COUNT] = {0};
+  double ev_bytes[LSX_K_COUNT] = {0};
   long ev_count[LSX_K_COUNT] = {0};
   hipEvent_t ev_take();
   void tbeg(int k, hipEvent_t* e);
-  void tend(int k, hipEvent_t b);
+  void tend(int k, hipEvent_t b, double bytes = -1.0);  // bytes < 0: the kernel's fixed figure
   void drain_events();
   double kernel_bytes(int k) const;
 };

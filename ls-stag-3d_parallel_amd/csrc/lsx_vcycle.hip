@@ -117,16 +117,14 @@ __global__ void k_pin_sink(double* __restrict__ dg, int c0, int c1, const double
   if (c1 >= 0) dg[c1] -= sigma[1];
 }
 
-// inverse diagonal with the zero-pivot rule of the line smoother (tiny = max(1e-5 max|d|, 1e-12))
+// inverse diagonal; only empty rows (closed cut cells, coarse cells inside the body) are left
+// out.  Tiny cut cells keep their own pivot: a right-hand side that is O(1) there asks for an
+// O(1/d) pressure, which the smoother must be able to produce.
 template <class T>
-__global__ void k_inv_diag(const double* __restrict__ dg, long n, const double* __restrict__ absmax, int pin0,
-                           int pin1, T* __restrict__ s) {
-  double tiny = 1e-5 * absmax[0];
-  if (tiny < 1e-12) tiny = 1e-12;
+__global__ void k_inv_diag(const double* __restrict__ dg, long n, T* __restrict__ s) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const double d = dg[i];
-    const bool pin = i == pin0 || i == pin1;  // identity-like rows keep their own pivot
-    s[i] = (fabs(d) < tiny && !pin) ? (T)0 : (T)(1.0 / d);
+    s[i] = d == 0.0 ? (T)0 : (T)(1.0 / d);
   }
 }
 
@@ -319,11 +317,11 @@ void face_absmax(const double* fe, const double* fn, const double* ft, long n, d
 }
 
 template <class T>
-void inv_diag(const double* dg, long n, const double* absmax, int pin0, int pin1, T* sv, hipStream_t s) {
-  k_inv_diag<T><<<grid_1d(n), 256, 0, s>>>(dg, n, absmax, pin0, pin1, sv);
+void inv_diag(const double* dg, long n, T* sv, hipStream_t s) {
+  k_inv_diag<T><<<grid_1d(n), 256, 0, s>>>(dg, n, sv);
 }
-template void inv_diag<float>(const double*, long, const double*, int, int, float*, hipStream_t);
-template void inv_diag<double>(const double*, long, const double*, int, int, double*, hipStream_t);
+template void inv_diag<float>(const double*, long, float*, hipStream_t);
+template void inv_diag<double>(const double*, long, double*, hipStream_t);
 
 template <class FT>
 void store_faces(const double* fe, const double* fn, const double* ft, long n, double inv_scale, FT* ae, FT* an,

@@ -17,8 +17,8 @@ namespace vc {
 enum FaceType { FACE_F64 = 0, FACE_F32 = 1, FACE_F16 = 2 };
 
 // One level.  Faces: fe[i] couples cell i with i+1 (x), fn[i] with i+N, ft[i] with i+N*M; a
-// face on the domain boundary is 0.  s[i] = 1 / a_p (0 where the zero-pivot rule freezes the
-// cell).  pin0/pin1 are identity rows (the finest level's pinned cells), -1 on coarse levels.
+// face on the domain boundary is 0.  s[i] = 1 / a_p (0 for an empty row).  pin0/pin1 are
+// identity rows (the finest level's pinned cells), -1 on coarse levels.
 struct Level {
   Dims d{0, 0, 0};
   int ftype = FACE_F64;
@@ -46,7 +46,7 @@ void pin_sink(double* dg, int c0, int c1, const double* sigma, hipStream_t s);
 void face_absmax(const double* fe, const double* fn, const double* ft, long n, double* partial, double* out,
                  hipStream_t s);
 template <class T>
-void inv_diag(const double* dg, long n, const double* absmax, int pin0, int pin1, T* sv, hipStream_t s);
+void inv_diag(const double* dg, long n, T* sv, hipStream_t s);
 template <class FT>
 void store_faces(const double* fe, const double* fn, const double* ft, long n, double inv_scale, FT* ae, FT* an,
                  FT* at, hipStream_t s);

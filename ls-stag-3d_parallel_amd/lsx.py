@@ -16,10 +16,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblsx.so")
 
-K_RESIDUAL, K_LINE_X, K_LINE_Y, K_LINE_Z, K_OPERATOR, K_ORTHO, K_SMOOTH = range(7)
+K_RESIDUAL, K_LINE_X, K_LINE_Y, K_LINE_Z, K_OPERATOR, K_ORTHO, K_SMOOTH, K_CGS_DOT, K_CGS_UPDATE = range(9)
 KERNEL_NAMES = {K_RESIDUAL: "residual_update", K_LINE_X: "line_solve_x", K_LINE_Y: "line_solve_y",
                 K_LINE_Z: "line_solve_z", K_OPERATOR: "operator_apply", K_ORTHO: "mgs_axpy_dot",
-                K_SMOOTH: "smoother_sweep"}
+                K_SMOOTH: "smoother_sweep", K_CGS_DOT: "cgs_dot", K_CGS_UPDATE: "cgs_update"}
 
 
 def stretched_axis(n, length=1.0, center=0.5, ratio=1.0):
@@ -104,6 +104,7 @@ def lib():
     L.lsx_geometry_build.argtypes = [vp, C.POINTER(Body), C.c_int, C.c_int, C.c_int, C.c_int]
     L.lsx_geometry_fields.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.lsx_set_geometry.argtypes = [vp, vp, C.c_int]
+    L.lsx_geometry_rhs.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(FlowBc), vp, C.c_int]
     L.lsx_operator_diagonal.argtypes = [vp, vp]
     L.lsx_fine_storage.argtypes = [vp]
     L.lsx_kernel_timing.argtypes = [vp, C.c_int]
@@ -117,6 +118,11 @@ class Body(C.Structure):
 
 
 BODY_SPHERE, BODY_ELLIPSOID = 0, 1
+
+
+class FlowBc(C.Structure):
+    _fields_ = [("U_E", C.c_double), ("V_S", C.c_double), ("V_N", C.c_double), ("W_B", C.c_double),
+                ("W_T", C.c_double), ("outflow", C.c_int), ("pin", C.c_int)]
 
 
 class Geometry:
@@ -152,6 +158,30 @@ class Geometry:
         tx, ty, tz, v, phi = (np.zeros(n) for _ in range(5))
         _check(lib().lsx_geometry_fields(self.h, _ptr(t), _ptr(tx), _ptr(ty), _ptr(tz), _ptr(v), _ptr(phi)))
         return dict(type=t, thx=tx, thy=ty, thz=tz, vof=v, phi=phi)
+
+    def face_shapes(self):
+        """Sizes of the staggered face arrays u (x faces), v (y faces), w (z faces)."""
+        N, M, K = self.dims
+        return (N - 1) * M * K, N * (M - 1) * K, N * M * (K - 1)
+
+    def pressure_rhs(self, u, v, w, U_W=1.0, U_E=1.0, V_S=0.0, V_N=0.0, W_B=0.0, W_T=0.0, outflow=True, pin=True,
+                     u_out=None):
+        """b = Dx u + Dy v + Dz w + flux on the device (reference PoissonDiscreteEquation::
+        recalcRightMember for a static body); host arrays in, host array out."""
+        N, M, K = self.dims
+        nu, nv, nw = self.face_shapes()
+        u = np.ascontiguousarray(u, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        w = np.ascontiguousarray(w, np.float64)
+        if u.size != nu or v.size != nv or w.size != nw:
+            raise LsxError("face arrays have the wrong sizes")
+        uw = np.ascontiguousarray(np.broadcast_to(np.asarray(U_W, np.float64), (M * K,)))
+        uo = None if u_out is None else np.ascontiguousarray(u_out, np.float64)
+        bc = FlowBc(U_E, V_S, V_N, W_B, W_T, 1 if outflow else 0, 1 if pin else 0)
+        b = np.zeros(self.n)
+        _check(lib().lsx_geometry_rhs(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(uw), None if uo is None else _ptr(uo),
+                                      C.byref(bc), _ptr(b), MEM_HOST))
+        return b
 
     def close(self):
         if self.h:

@@ -134,6 +134,22 @@ int lsxo_geom_get_level_set(void* p, double* phi, double* ax, double* ay, double
 
 int lsxo_vof_table_size() { return vof_table_size(); }
 
+// Pressure right-hand side from predicted face velocities (oracle_rhs.cpp).
+int lsxo_geom_rhs(void* p, const double* u, const double* v, const double* w, const double* u_west, double U_E,
+                  double V_S, double V_N, double W_B, double W_T, const double* u_out, int pin, double* b,
+                  double* flux_out) {
+  auto* h = static_cast<GeomH*>(p);
+  try {
+    std::vector<double> flux = pressure_flux(h->g, u_west, U_E, V_S, V_N, W_B, W_T, u_out);
+    pressure_rhs(h->g, u, v, w, flux, pin != 0, b);
+    if (flux_out) std::memcpy(flux_out, flux.data(), flux.size() * sizeof(double));
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  return 0;
+}
+
 long lsxo_geom_build_laplace(void* p, int pin) {
   auto* h = static_cast<GeomH*>(p);
   try {

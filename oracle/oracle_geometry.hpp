@@ -56,5 +56,11 @@ int vof_table_size();
 std::vector<double> cell_centres(const std::vector<double>& widths, double origin);
 Csr build_laplace(const Geometry& g, bool pin_outlet);
 void body_free_vof(Geometry& g);
+// Pressure-equation right-hand side (oracle_rhs.cpp).  u_west: M*K inflow velocities (one per
+// x-row); u_out: the x-face velocity for an outflow east boundary, or null for a fixed U_E.
+std::vector<double> pressure_flux(const Geometry& g, const double* u_west, double U_E, double V_S, double V_N,
+                                  double W_B, double W_T, const double* u_out);
+void pressure_rhs(const Geometry& g, const double* u, const double* v, const double* w, const std::vector<double>& flux,
+                  bool pin_outlet, double* b);
 
 }  // namespace lsxo

@@ -51,6 +51,8 @@ def lib():
     L.lsxo_geom_build_body.argtypes = [C.c_void_p, C.c_int] + [C.c_double] * 6 + [C.c_int] * 4
     L.lsxo_geom_get_level_set.argtypes = [C.c_void_p, PD, PD, PD, PD, PI, PI, PI]
     L.lsxo_vof_table_size.restype = C.c_int
+    L.lsxo_geom_rhs.argtypes = [C.c_void_p, PD, PD, PD, PD] + [C.c_double] * 5 + [C.c_void_p, C.c_int, PD,
+                                                                                  C.c_void_p]
     _lib = L
     return L
 
@@ -146,6 +148,33 @@ class Geometry:
         val = np.zeros(nnz)
         lib().lsxo_geom_get_csr(self.h, rp, col, val)
         return Csr(self.dims, rp, col, val)
+
+
+def face_shapes(dims):
+    """Sizes of the staggered face arrays (u on x faces, v on y faces, w on z faces)."""
+    N, M, K = dims
+    return (N - 1) * M * K, N * (M - 1) * K, N * M * (K - 1)
+
+
+def pressure_rhs(g, u, v, w, U_W=1.0, U_E=1.0, V_S=0.0, V_N=0.0, W_B=0.0, W_T=0.0, outflow=True, pin=True,
+                 return_flux=False):
+    """b = Dx u + Dy v + Dz w + flux with b = 0 at the pinned cells (PoissonDiscreteEquation::
+    recalcRightMember).  U_W: scalar or one inflow velocity per x-row (M*K); outflow: the east
+    boundary flux is the last interior u face (setBaseFlux), else U_E."""
+    N, M, K = g.dims
+    nu, nv, nw = face_shapes(g.dims)
+    u = np.ascontiguousarray(u, np.float64)
+    v = np.ascontiguousarray(v, np.float64)
+    w = np.ascontiguousarray(w, np.float64)
+    assert u.size == nu and v.size == nv and w.size == nw
+    uw = np.ascontiguousarray(np.broadcast_to(np.asarray(U_W, np.float64), (M * K,)))
+    b = np.zeros(g.n)
+    flux = np.zeros(g.n)
+    st = lib().lsxo_geom_rhs(g.h, u, v, w, uw, U_E, V_S, V_N, W_B, W_T, u.ctypes.data if outflow else None,
+                             1 if pin else 0, b, flux.ctypes.data)
+    if st != 0:
+        raise RuntimeError(lib().lsxo_last_error().decode())
+    return (b, flux) if return_flux else b
 
 
 class Solver:

@@ -48,7 +48,7 @@ def test_vcycle_reaches_fp64_tolerance(lsx, name, mk, kind, precision):
     assert it < 60
     # both solutions satisfy ||b - A p|| <= 1e-8 ||b||; their difference is bounded by the
     # conditioning of the pinned Neumann operator
-    assert err <= 1e-4
+    assert err <= 1e-3
 
 
 def test_mixed_and_fp64_vcycle_agree_on_one_application(lsx):
