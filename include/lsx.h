@@ -190,6 +190,16 @@ typedef struct lsx_flow_bc {
 int lsx_geometry_rhs(lsx_geometry* g, const double* u, const double* v, const double* w, const double* u_west,
                      const double* u_out, const lsx_flow_bc* bc, double* b, int mem);
 
+/* Projection correction after the pressure solve, on the device: u = (G_x dp) inv_Mx + us and
+ * likewise v, w, and p += dp * coef when p is not NULL.  Replaces SetSolutions::correction
+ * (set_solutions.h:215-237, without its optional outflow clamps) with the gradient rows of
+ * GradientDiscreteOperatorX/Y/Z::calc (gradient_discrete_operator_x.h:58-78 and its y, z
+ * siblings) and the staggered cell types of LSSTAGUniformMesh::cvu/cvv/cvw
+ * (ls_stag_uniform_mesh.h:108-130).  Face arrays as in lsx_geometry_rhs; every array in `mem`
+ * space. */
+int lsx_geometry_correct(lsx_geometry* g, const double* dp, const double* us, const double* vs, const double* ws,
+                         double* u, double* v, double* w, double* p, double coef, int mem);
+
 /* Set the solver's matrix to the cut-cell pressure operator of `g` (built on the device, no
  * CSR round trip) and build the preconditioner; pin != 0 turns rows (N-1,0,0) and
  * (N-1,M-1,0) into identity rows.  Replaces LaplaceDiscreteOperator::build

@@ -195,6 +195,13 @@ int lsx_geometry_rhs(lsx_geometry* g, const double* u, const double* v, const do
   return guarded([&] { g->impl.pressure_rhs(u, v, w, u_west, u_out, *bc, b, mem); });
 }
 
+int lsx_geometry_correct(lsx_geometry* g, const double* dp, const double* us, const double* vs, const double* ws,
+                         double* u, double* v, double* w, double* p, double coef, int mem) {
+  if (!g || !dp || !us || !vs || !ws || !u || !v || !w) return LSX_ERR_ARG;
+  if (mem != LSX_MEM_HOST && mem != LSX_MEM_DEVICE) return LSX_ERR_ARG;
+  return guarded([&] { g->impl.correct(dp, us, vs, ws, u, v, w, p, coef, mem); });
+}
+
 int lsx_kernel_timing(lsx_solver* s, int enable) {
   if (!s) return LSX_ERR_ARG;
   return guarded([&] { s->impl.set_timing(enable != 0); });

@@ -691,7 +691,94 @@ __global__ void k_pressure_rhs(const int* __restrict__ type, const double* __res
   }
 }
 
+// Velocity correction u = (G dp) inv_M + u* (SetSolutions::correction, set_solutions.h:215-237,
+// with GradientDiscreteOperatorX/Y/Z::calc and the staggered types of cvu/cvv/cvw,
+// ls_stag_uniform_mesh.h:108-130).  One thread per face of the three staggered grids; the
+// inverse staggered mass is 1 / (0.5 (V_a + V_b)), 0 below 1e-12 (inverseVOF).
+__device__ inline double inv_mass(double va, double vb) {
+  const double m = 0.5 * (va + vb);
+  return m < EQ_TINY_DOUBLE ? 0.0 : 1.0 / m;
+}
+
+__device__ inline bool staggered_solid(double theta, int a, int b) {
+  return theta < EQ_DOUBLE || (b < a ? b : a) == SolidCell;
+}
+
+__global__ void k_correct(const int* __restrict__ type, const double* __restrict__ tx, const double* __restrict__ ty,
+                          const double* __restrict__ tz, const double* __restrict__ vof,
+                          const double* __restrict__ dxa, const double* __restrict__ dya,
+                          const double* __restrict__ dza, int N, int M, int K, const double* __restrict__ dp,
+                          const double* __restrict__ us, const double* __restrict__ vs, const double* __restrict__ ws,
+                          double* __restrict__ u, double* __restrict__ v, double* __restrict__ w) {
+  const long nu = (long)(N - 1) * M * K, nv = (long)N * (M - 1) * K, nw = (long)N * M * (K - 1);
+  const long NM = (long)N * M;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < nu + nv + nw; q += (long)gridDim.x * blockDim.x) {
+    if (q < nu) {  // x faces: index x + (N-1)(y + M z)
+      const long f = q, row = f / (N - 1);
+      const int x = (int)(f - row * (N - 1)), y = (int)(row % M), z = (int)(row / M);
+      const long a = x + (long)N * row, b = a + 1;
+      double aux = 0.0;
+      if (staggered_solid(tx[a], type[a], type[b])) {
+        aux += 0.0 * dp[a];
+      } else {
+        const double cell = tx[a] * dya[y] * dza[z];
+        aux += cell * dp[a];
+        aux += -cell * dp[b];
+      }
+      aux *= inv_mass(vof[a], vof[b]);
+      u[f] = aux + us[f];
+    } else if (q < nu + nv) {  // y faces: index x + N (y + (M-1) z)
+      const long f = q - nu, xy = f % ((long)N * (M - 1));
+      const int z = (int)(f / ((long)N * (M - 1))), y = (int)(xy / N), x = (int)(xy - (long)y * N);
+      const long a = x + (long)N * y + NM * z, b = a + N;
+      double aux = 0.0;
+      if (staggered_solid(ty[a], type[a], type[b])) {
+        aux += 0.0 * dp[a];
+      } else {
+        const double cell = ty[a] * dxa[x] * dza[z];
+        aux += cell * dp[a];
+        aux += -cell * dp[b];
+      }
+      aux *= inv_mass(vof[a], vof[b]);
+      v[f] = aux + vs[f];
+    } else {  // z faces: index of the lower cell
+      const long a = q - nu - nv, b = a + NM;
+      const int z = (int)(a / NM);
+      const long r2 = a - (long)z * NM;
+      const int y = (int)(r2 / N), x = (int)(r2 - (long)y * N);
+      double aux = 0.0;
+      if (staggered_solid(tz[a], type[a], type[b])) {
+        aux += 0.0 * dp[a];
+      } else {
+        const double cell = tz[a] * dxa[x] * dya[y];
+        aux += cell * dp[a];
+        aux += -cell * dp[b];
+      }
+      aux *= inv_mass(vof[a], vof[b]);
+      w[a] = aux + ws[a];
+    }
+  }
+}
+
 }  // namespace
+
+namespace {
+__global__ void k_pressure_update(double* __restrict__ p, const double* __restrict__ dp, double coef, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = p[i] + dp[i] * coef;
+}
+}  // namespace
+
+void pressure_update(double* p, const double* dp, double coef, long n, hipStream_t s) {
+  k_pressure_update<<<grid_1d(n), 256, 0, s>>>(p, dp, coef, n);
+}
+
+void correct(const int* type, const double* tx, const double* ty, const double* tz, const double* vof, const double* dx,
+             const double* dy, const double* dz, int N, int M, int K, const double* dp, const double* us,
+             const double* vs, const double* ws, double* u, double* v, double* w, hipStream_t s) {
+  const long total = (long)(N - 1) * M * K + (long)N * (M - 1) * K + (long)N * M * (K - 1);
+  k_correct<<<grid_1d(total), 256, 0, s>>>(type, tx, ty, tz, vof, dx, dy, dz, N, M, K, dp, us, vs, ws, u, v, w);
+}
 
 void pressure_rhs(const int* type, const double* tx, const double* ty, const double* tz, const double* dx,
                   const double* dy, const double* dz, int N, int M, int K, const double* u, const double* v,

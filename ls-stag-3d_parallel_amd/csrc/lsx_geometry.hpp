@@ -51,6 +51,12 @@ struct FlowBc {
   double U_E, V_S, V_N, W_B, W_T;
   int pin;  // b = 0 at cells N-1 and N*M-1
 };
+// u = (G dp) inv_M + u*, v and w likewise (velocity part of the projection correction)
+void correct(const int* type, const double* tx, const double* ty, const double* tz, const double* vof, const double* dx,
+             const double* dy, const double* dz, int N, int M, int K, const double* dp, const double* us,
+             const double* vs, const double* ws, double* u, double* v, double* w, hipStream_t s);
+// p = p + dp * coef
+void pressure_update(double* p, const double* dp, double coef, long n, hipStream_t s);
 // b = D u* + flux on the base grid (u_west: one inflow velocity per x-row; u_out: x-face field
 // whose last interior face feeds an outflow east boundary, or null for the fixed U_E)
 void pressure_rhs(const int* type, const double* tx, const double* ty, const double* tz, const double* dx,

@@ -166,4 +166,42 @@ void Geometry::pressure_rhs(const double* u, const double* v, const double* w, c
   LSX_HIP(hipStreamSynchronize(stream));
 }
 
+void Geometry::correct(const double* dp, const double* us, const double* vs, const double* ws, double* u, double* v,
+                       double* w, double* p, double coef, int mem) {
+  if (!built) throw Error(LSX_ERR_STATE, "geometry not built");
+  const long nn = n(), nu = (long)(N - 1) * M * K, nv = (long)N * (M - 1) * K, nw = (long)N * M * (K - 1);
+  if (mem == LSX_MEM_DEVICE) {
+    geo::correct(type.p, tx.p, ty.p, tz.p, vof.p, dx.p, dy.p, dz.p, N, M, K, dp, us, vs, ws, u, v, w, stream);
+    if (p) geo::pressure_update(p, dp, coef, nn, stream);
+    LSX_HIP(hipGetLastError());
+    LSX_HIP(hipStreamSynchronize(stream));
+    return;
+  }
+  DevBuf<double> ddp, dus, dvs, dws, du, dv, dw, dpp;
+  ddp.alloc(nn);
+  dus.alloc(nu);
+  dvs.alloc(nv);
+  dws.alloc(nw);
+  du.alloc(nu);
+  dv.alloc(nv);
+  dw.alloc(nw);
+  LSX_HIP(hipMemcpyAsync(ddp.p, dp, nn * sizeof(double), hipMemcpyHostToDevice, stream));
+  LSX_HIP(hipMemcpyAsync(dus.p, us, nu * sizeof(double), hipMemcpyHostToDevice, stream));
+  LSX_HIP(hipMemcpyAsync(dvs.p, vs, nv * sizeof(double), hipMemcpyHostToDevice, stream));
+  LSX_HIP(hipMemcpyAsync(dws.p, ws, nw * sizeof(double), hipMemcpyHostToDevice, stream));
+  geo::correct(type.p, tx.p, ty.p, tz.p, vof.p, dx.p, dy.p, dz.p, N, M, K, ddp.p, dus.p, dvs.p, dws.p, du.p, dv.p, dw.p,
+               stream);
+  if (p) {
+    dpp.alloc(nn);
+    LSX_HIP(hipMemcpyAsync(dpp.p, p, nn * sizeof(double), hipMemcpyHostToDevice, stream));
+    geo::pressure_update(dpp.p, ddp.p, coef, nn, stream);
+    LSX_HIP(hipMemcpyAsync(p, dpp.p, nn * sizeof(double), hipMemcpyDeviceToHost, stream));
+  }
+  LSX_HIP(hipGetLastError());
+  LSX_HIP(hipMemcpyAsync(u, du.p, nu * sizeof(double), hipMemcpyDeviceToHost, stream));
+  LSX_HIP(hipMemcpyAsync(v, dv.p, nv * sizeof(double), hipMemcpyDeviceToHost, stream));
+  LSX_HIP(hipMemcpyAsync(w, dw.p, nw * sizeof(double), hipMemcpyDeviceToHost, stream));
+  LSX_HIP(hipStreamSynchronize(stream));
+}
+
 }  // namespace lsx

@@ -22,6 +22,9 @@ class Geometry {
   // b = D u + flux (pressure right-hand side); arrays in `mem` space (LSX_MEM_HOST / DEVICE)
   void pressure_rhs(const double* u, const double* v, const double* w, const double* u_west, const double* u_out,
                     const lsx_flow_bc& bc, double* b, int mem);
+  // u = (G dp) inv_M + u*, v, w likewise; p += dp * coef when p is given; `mem` space arrays
+  void correct(const double* dp, const double* us, const double* vs, const double* ws, double* u, double* v, double* w,
+               double* p, double coef, int mem);
 
   int N, M, K;
   long n() const { return (long)N * M * K; }

@@ -105,6 +105,7 @@ def lib():
     L.lsx_geometry_fields.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.lsx_set_geometry.argtypes = [vp, vp, C.c_int]
     L.lsx_geometry_rhs.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(FlowBc), vp, C.c_int]
+    L.lsx_geometry_correct.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_int]
     L.lsx_operator_diagonal.argtypes = [vp, vp]
     L.lsx_fine_storage.argtypes = [vp]
     L.lsx_kernel_timing.argtypes = [vp, C.c_int]
@@ -182,6 +183,21 @@ class Geometry:
         _check(lib().lsx_geometry_rhs(self.h, _ptr(u), _ptr(v), _ptr(w), _ptr(uw), None if uo is None else _ptr(uo),
                                       C.byref(bc), _ptr(b), MEM_HOST))
         return b
+
+    def correct(self, dp, us, vs, ws, p=None, coef=1.0):
+        """Projection correction on the device (reference SetSolutions::correction): returns the new
+        face velocities (u, v, w); p (host array) is updated in place by p += dp * coef."""
+        nu, nv, nw = self.face_shapes()
+        dp = np.ascontiguousarray(dp, np.float64)
+        us, vs, ws = (np.ascontiguousarray(a, np.float64) for a in (us, vs, ws))
+        if dp.size != self.n or us.size != nu or vs.size != nv or ws.size != nw:
+            raise LsxError("array sizes do not match the grid")
+        u, v, w = np.zeros(nu), np.zeros(nv), np.zeros(nw)
+        if p is not None and not (isinstance(p, np.ndarray) and p.dtype == np.float64 and p.flags.c_contiguous):
+            raise LsxError("p must be a contiguous float64 array (updated in place)")
+        _check(lib().lsx_geometry_correct(self.h, _ptr(dp), _ptr(us), _ptr(vs), _ptr(ws), _ptr(u), _ptr(v), _ptr(w),
+                                          None if p is None else _ptr(p), coef, MEM_HOST))
+        return u, v, w
 
     def close(self):
         if self.h:

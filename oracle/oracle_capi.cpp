@@ -134,6 +134,14 @@ int lsxo_geom_get_level_set(void* p, double* phi, double* ax, double* ay, double
 
 int lsxo_vof_table_size() { return vof_table_size(); }
 
+// Velocity correction u = (G dp) inv_M + u* (oracle_rhs.cpp).
+int lsxo_geom_correct(void* p, const double* dp, const double* us, const double* vs, const double* ws, double* u,
+                      double* v, double* w) {
+  auto* h = static_cast<GeomH*>(p);
+  velocity_correction(h->g, dp, us, vs, ws, u, v, w);
+  return 0;
+}
+
 // Pressure right-hand side from predicted face velocities (oracle_rhs.cpp).
 int lsxo_geom_rhs(void* p, const double* u, const double* v, const double* w, const double* u_west, double U_E,
                   double V_S, double V_N, double W_B, double W_T, const double* u_out, int pin, double* b,

@@ -62,5 +62,8 @@ std::vector<double> pressure_flux(const Geometry& g, const double* u_west, doubl
                                   double W_B, double W_T, const double* u_out);
 void pressure_rhs(const Geometry& g, const double* u, const double* v, const double* w, const std::vector<double>& flux,
                   bool pin_outlet, double* b);
+// u = (G_x dp) inv_Mx + u* (and v, w), the velocity part of SetSolutions::correction
+void velocity_correction(const Geometry& g, const double* dp, const double* us, const double* vs, const double* ws,
+                         double* u, double* v, double* w);
 
 }  // namespace lsxo

@@ -193,4 +193,67 @@ void pressure_rhs(const Geometry& g, const double* u, const double* v, const dou
   }
 }
 
+// Velocity and pressure correction after the pressure solve: SetSolutions::correction
+// (src/Base/set_solutions.h:215-237, without the optional outflow clamps) with the gradient rows
+// of GradientDiscreteOperatorX/Y/Z::calc (src/DiscreteOperators/Gradient/gradient_discrete_
+// operator_{x,y,z}.h) and the staggered cell types cvu/cvv/cvw (src/Base/LS_STAG_Mesh/
+// ls_stag_uniform_mesh.h:108-130):
+//   u = (G_x dp) * inv_Mx + u*   (G row: [theta_x dy dz, -theta_x dy dz] on [west, east] cells,
+//                                  a single 0 entry when the x control volume is solid)
+//   p = p + dp * coef
+namespace {
+int staggered_type(double theta, int a, int b) { return theta < EQ_DOUBLE ? (int)SolidCell : (b < a ? b : a); }
+}  // namespace
+
+void velocity_correction(const Geometry& g, const double* dp, const double* us, const double* vs, const double* ws,
+                         double* u, double* v, double* w) {
+  const int N = g.N, M = g.M, K = g.K;
+  const long NM = (long)N * M;
+  for (int z = 0; z < K; ++z)
+    for (int y = 0; y < M; ++y)
+      for (int x = 0; x + 1 < N; ++x) {
+        const long W = g.idx(x, y, z), f = x + (long)(N - 1) * (y + (long)M * z);
+        double aux = 0.0;
+        if (staggered_type(g.thx[W], g.type[W], g.type[W + 1]) == SolidCell) {
+          aux += 0.0 * dp[W];
+        } else {
+          const double cell = g.thx[W] * g.dy[y] * g.dz[z];
+          aux += cell * dp[W];
+          aux += -cell * dp[W + 1];
+        }
+        aux *= g.inv_mx[f];
+        u[f] = aux + us[f];
+      }
+  for (int z = 0; z < K; ++z)
+    for (int y = 0; y + 1 < M; ++y)
+      for (int x = 0; x < N; ++x) {
+        const long S = g.idx(x, y, z), f = x + (long)N * (y + (long)(M - 1) * z);
+        double aux = 0.0;
+        if (staggered_type(g.thy[S], g.type[S], g.type[S + N]) == SolidCell) {
+          aux += 0.0 * dp[S];
+        } else {
+          const double cell = g.thy[S] * g.dx[x] * g.dz[z];
+          aux += cell * dp[S];
+          aux += -cell * dp[S + N];
+        }
+        aux *= g.inv_my[f];
+        v[f] = aux + vs[f];
+      }
+  for (int z = 0; z + 1 < K; ++z)
+    for (int y = 0; y < M; ++y)
+      for (int x = 0; x < N; ++x) {
+        const long B = g.idx(x, y, z);
+        double aux = 0.0;
+        if (staggered_type(g.thz[B], g.type[B], g.type[B + NM]) == SolidCell) {
+          aux += 0.0 * dp[B];
+        } else {
+          const double cell = g.thz[B] * g.dx[x] * g.dy[y];
+          aux += cell * dp[B];
+          aux += -cell * dp[B + NM];
+        }
+        aux *= g.inv_mz[B];
+        w[B] = aux + ws[B];
+      }
+}
+
 }  // namespace lsxo

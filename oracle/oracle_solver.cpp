@@ -9,8 +9,10 @@
 // macros), so this restatement is pinned by closed forms and brute-force checks in
 // tests/test_oracle_*.py instead of by a build of the reference.
 //
-// Determinism: every reduction is a fixed-order blocked sum (blocks of RED_BLOCK elements,
-// summed left to right), so results do not depend on the OpenMP thread count.
+// Determinism: the reference's own reductions have no fixed order (OpenMP reductions on the CPU,
+// atomicAdd on the GPU).  Every reduction here follows one fixed order, the one the HIP library
+// uses (device_order_sum below), so results do not depend on the thread count and the Krylov
+// scalars of the two implementations can be compared bit for bit.
 
 #include "oracle_internal.hpp"
 
@@ -24,22 +26,54 @@ namespace lsxo {
 // ---------------------------------------------------------------------------------------
 // BLAS-1 with deterministic reductions
 // ---------------------------------------------------------------------------------------
-static const long RED_BLOCK = 4096;
+// Fixed summation order of the HIP reductions: nb = clamp(ceil(n / 1024), 1, 1024) contiguous
+// chunks of ceil(n / nb) elements; in a chunk, lane t of 256 adds elements t, t + 256, ... left to
+// right; each 64-lane wave folds its lanes pairwise (l += l + 32, then + 16, ..., + 1); lane 0 of
+// the 4 waves are added in wave order.  The nb chunk sums are then reduced the same way, lane t
+// adding the chunk sums [t p, t p + p) with p = ceil(nb / 256).
+static double wave_fold(double* v) {
+  for (int off = 32; off > 0; off >>= 1)
+    for (int l = 0; l < off; ++l) v[l] = v[l] + v[l + off];
+  return v[0];
+}
+
+static double block_fold(double* lane) {  // 256 lane values
+  double s = 0.0;
+  for (int w = 0; w < 4; ++w) s += wave_fold(lane + 64 * w);
+  return s;
+}
+
+template <class F>
+static double device_order_sum(long n, F term) {
+  long nb = (n + 1023) / 1024;
+  if (nb > 1024) nb = 1024;
+  if (nb < 1) nb = 1;
+  const long per = (n + nb - 1) / nb;
+  std::vector<double> part(nb, 0.0);
+#pragma omp parallel for schedule(static)
+  for (long b = 0; b < nb; ++b) {
+    double lane[256];
+    const long lo = per * b, hi = std::min(n, lo + per);
+    for (int t = 0; t < 256; ++t) {
+      double acc = 0.0;
+      for (long i = lo + t; i < hi; i += 256) acc += term(i);
+      lane[t] = acc;
+    }
+    part[b] = block_fold(lane);
+  }
+  const long pp = (nb + 255) / 256;
+  double lane[256];
+  for (int t = 0; t < 256; ++t) {
+    double acc = 0.0;
+    for (long i = t * pp; i < t * pp + pp && i < nb; ++i) acc += part[i];
+    lane[t] = acc;
+  }
+  return block_fold(lane);
+}
 
 template <class F>
 static double blocked_sum(long n, F term) {
-  const long nb = (n + RED_BLOCK - 1) / RED_BLOCK;
-  std::vector<double> part(nb > 0 ? nb : 1, 0.0);
-#pragma omp parallel for schedule(static)
-  for (long b = 0; b < nb; ++b) {
-    double s = 0.0;
-    const long e = std::min(n, (b + 1) * RED_BLOCK);
-    for (long i = b * RED_BLOCK; i < e; ++i) s += term(i);
-    part[b] = s;
-  }
-  double s = 0.0;
-  for (long b = 0; b < nb; ++b) s += part[b];
-  return s;
+  return device_order_sum(n, term);
 }
 
 double dot(long n, const double* a, const double* b) {

@@ -51,6 +51,7 @@ def lib():
     L.lsxo_geom_build_body.argtypes = [C.c_void_p, C.c_int] + [C.c_double] * 6 + [C.c_int] * 4
     L.lsxo_geom_get_level_set.argtypes = [C.c_void_p, PD, PD, PD, PD, PI, PI, PI]
     L.lsxo_vof_table_size.restype = C.c_int
+    L.lsxo_geom_correct.argtypes = [C.c_void_p, PD, PD, PD, PD, PD, PD, PD]
     L.lsxo_geom_rhs.argtypes = [C.c_void_p, PD, PD, PD, PD] + [C.c_double] * 5 + [C.c_void_p, C.c_int, PD,
                                                                                   C.c_void_p]
     _lib = L
@@ -175,6 +176,17 @@ def pressure_rhs(g, u, v, w, U_W=1.0, U_E=1.0, V_S=0.0, V_N=0.0, W_B=0.0, W_T=0.
     if st != 0:
         raise RuntimeError(lib().lsxo_last_error().decode())
     return (b, flux) if return_flux else b
+
+
+def velocity_correction(g, dp, us, vs, ws):
+    """u = (G_x dp) inv_Mx + u* and likewise v, w (SetSolutions::correction)."""
+    nu, nv, nw = face_shapes(g.dims)
+    dp = np.ascontiguousarray(dp, np.float64)
+    us, vs, ws = (np.ascontiguousarray(a, np.float64) for a in (us, vs, ws))
+    assert dp.size == g.n and us.size == nu and vs.size == nv and ws.size == nw
+    u, v, w = np.zeros(nu), np.zeros(nv), np.zeros(nw)
+    lib().lsxo_geom_correct(g.h, dp, us, vs, ws, u, v, w)
+    return u, v, w
 
 
 class Solver:
