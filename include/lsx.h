@@ -58,7 +58,13 @@ typedef struct lsx_options {
   int precision;     /* LSX_PREC_*                                            */
   int levels;        /* 0 = automatic level count; >0 = cap (V-cycle)        */
   int z_head_fix;    /* 0 = reproduce the reference z-line head read (default) */
+  int ortho;         /* LSX_ORTHO_*: FGMRES orthogonalisation                    */
 } lsx_options;
+
+#define LSX_ORTHO_AUTO 0 /* MGS with the cascade (reference), CGS with the V-cycle           */
+#define LSX_ORTHO_MGS 1  /* modified Gram-Schmidt (fgmres_interface.h:97-104)               */
+#define LSX_ORTHO_CGS 2  /* classical Gram-Schmidt, two passes over the basis per step     */
+#define LSX_ORTHO_CGS2 3 /* classical Gram-Schmidt with one reorthogonalisation            */
 
 /* Fill defaults: FGMRES(13), max_it 200, eps 1e-5, cascade MG, n_post 3, omega 0.805, FP64
  * (base_task_description.h:41-46, ls_stag_real_constants.h:47, 62). */

@@ -58,6 +58,7 @@ void lsx_options_default(lsx_options* o) {
   o->precision = LSX_PREC_FP64;
   o->levels = 0;
   o->z_head_fix = 0;
+  o->ortho = LSX_ORTHO_AUTO;
 }
 
 int lsx_create(lsx_solver** out, const lsx_options* opt, void* stream) {

@@ -689,78 +689,70 @@ __global__ void k_axpy_dot(double* __restrict__ w, const double* __restrict__ v,
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// ---- block classical Gram-Schmidt ------------------------------------------------------------
-// V holds cnt <= CGS_MAX basis vectors, vector k at V + k*ldv.  Each block reduces its chunk of
-// rows; partial[k * nb + block] is then summed in a fixed order by k_finalize_multi.
-template <int BS>
-__device__ void block_sum_multi(double (&acc)[CGS_MAX], int cnt, double* partial, int nb) {
-  __shared__ double lds[BS / 64][CGS_MAX];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < CGS_MAX; ++k) {
-    if (k < cnt) {
-      double v = acc[k];
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-      if (lane == 0) lds[wave][k] = v;
-    }
-  }
-  __syncthreads();
-  if ((int)threadIdx.x < cnt) {
-    double s = 0.0;
-    for (int w = 0; w < BS / 64; ++w) s += lds[w][threadIdx.x];
-    partial[(long)threadIdx.x * nb + blockIdx.x] = s;
-  }
-}
+// ---- classical Gram-Schmidt passes ------------------------------------------------------------
+// V holds cnt <= CGS_MAX basis vectors, vector k at V + k*ldv.  The dot pass runs on a 2D grid
+// (row chunks x groups of CGS_GROUP vectors) so each thread keeps only CGS_GROUP accumulators and
+// pointers; partial[k * nb + chunk] is summed in a fixed order by k_finalize_multi.  The update
+// pass keeps the coefficients in LDS and streams the vectors row by row.
+constexpr int CGS_GROUP = 8;
 
 template <int BS>
 __global__ void __launch_bounds__(BS) k_cgs_dot(const double* __restrict__ V, long ldv, int cnt,
                                                 const double* __restrict__ w, long n, double* __restrict__ partial) {
+  __shared__ double lds[BS / 64][CGS_GROUP];
+  const int nb = gridDim.x;
+  const int k0 = blockIdx.y * CGS_GROUP;
+  const int kc = cnt - k0 < CGS_GROUP ? cnt - k0 : CGS_GROUP;
   long lo, hi;
-  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
-  double acc[CGS_MAX];
+  chunk_of(n, blockIdx.x, nb, &lo, &hi);
+  const double* Vg = V + (long)k0 * ldv;
+  double acc[CGS_GROUP];
 #pragma unroll
-  for (int k = 0; k < CGS_MAX; ++k) acc[k] = 0.0;
+  for (int k = 0; k < CGS_GROUP; ++k) acc[k] = 0.0;
   for (long i = lo + threadIdx.x; i < hi; i += BS) {
     const double wi = w[i];
 #pragma unroll
-    for (int k = 0; k < CGS_MAX; ++k)
-      if (k < cnt) acc[k] += V[k * ldv + i] * wi;
+    for (int k = 0; k < CGS_GROUP; ++k)
+      if (k < kc) acc[k] += Vg[k * ldv + i] * wi;
   }
-  block_sum_multi<BS>(acc, cnt, partial, gridDim.x);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < CGS_GROUP; ++k) {
+    double v = acc[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) lds[wave][k] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < kc) {
+    double s = 0.0;
+    for (int q = 0; q < BS / 64; ++q) s += lds[q][threadIdx.x];
+    partial[(long)(k0 + threadIdx.x) * nb + blockIdx.x] = s;
+  }
 }
 
-// w -= V h; MODE 0: partial V^T w (reorthogonalisation coefficients), MODE 1: partial ||w||^2
-template <int BS, int MODE>
+// w -= sum_k h_k v_k ; with_norm: partial ||w||^2 per block
+template <int BS>
 __global__ void __launch_bounds__(BS) k_cgs_update(const double* __restrict__ V, long ldv, int cnt,
                                                    const double* __restrict__ h, double* __restrict__ w, long n,
-                                                   double* __restrict__ partial) {
+                                                   int with_norm, double* __restrict__ partial) {
+  __shared__ double hk[CGS_MAX];
+  if ((int)threadIdx.x < cnt) hk[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
   long lo, hi;
   chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
-  double hk[CGS_MAX], acc[CGS_MAX];
-#pragma unroll
-  for (int k = 0; k < CGS_MAX; ++k) {
-    hk[k] = k < cnt ? h[k] : 0.0;
-    acc[k] = 0.0;
-  }
+  double ss = 0.0;
   for (long i = lo + threadIdx.x; i < hi; i += BS) {
-    double v[CGS_MAX];
     double wn = w[i];
-#pragma unroll
-    for (int k = 0; k < CGS_MAX; ++k)
-      if (k < cnt) {
-        v[k] = V[k * ldv + i];
-        wn -= hk[k] * v[k];
-      }
+    const double* vp = V + i;
+#pragma unroll 8
+    for (int k = 0; k < cnt; ++k) wn -= hk[k] * vp[k * ldv];
     w[i] = wn;
-    if (MODE == 0) {
-#pragma unroll
-      for (int k = 0; k < CGS_MAX; ++k)
-        if (k < cnt) acc[k] += wn * v[k];
-    } else {
-      acc[0] += wn * wn;
-    }
+    ss += wn * wn;
   }
-  block_sum_multi<BS>(acc, MODE == 0 ? cnt : 1, partial, gridDim.x);
+  if (with_norm) {
+    ss = block_sum<BS>(ss);
+    if (threadIdx.x == 0) partial[blockIdx.x] = ss;
+  }
 }
 
 template <int BS>
@@ -1007,20 +999,16 @@ void sol_update(double* x, const double* Z, long ldz, const double* yk, int coun
 void cgs_dot(const double* V, long ldv, int cnt, const double* w, long n, double* partial, double* out,
              hipStream_t s) {
   const int nb = reduction_blocks(n);
-  k_cgs_dot<RED_THREADS><<<nb, RED_THREADS, 0, s>>>(V, ldv, cnt, w, n, partial);
+  const dim3 grid(nb, (cnt + CGS_GROUP - 1) / CGS_GROUP);
+  k_cgs_dot<RED_THREADS><<<grid, RED_THREADS, 0, s>>>(V, ldv, cnt, w, n, partial);
   k_finalize_multi<RED_THREADS><<<cnt, RED_THREADS, 0, s>>>(partial, nb, out, 0);
 }
 
-void cgs_update(const double* V, long ldv, int cnt, const double* h, double* w, long n, int mode, double* partial,
-                double* out, hipStream_t s) {
+void cgs_update(const double* V, long ldv, int cnt, const double* h, double* w, long n, double* partial,
+                double* norm_out, hipStream_t s) {
   const int nb = reduction_blocks(n);
-  if (mode == 0) {
-    k_cgs_update<RED_THREADS, 0><<<nb, RED_THREADS, 0, s>>>(V, ldv, cnt, h, w, n, partial);
-    k_finalize_multi<RED_THREADS><<<cnt, RED_THREADS, 0, s>>>(partial, nb, out, 0);
-  } else {
-    k_cgs_update<RED_THREADS, 1><<<nb, RED_THREADS, 0, s>>>(V, ldv, cnt, h, w, n, partial);
-    k_finalize_multi<RED_THREADS><<<1, RED_THREADS, 0, s>>>(partial, nb, out, 1);
-  }
+  k_cgs_update<RED_THREADS><<<nb, RED_THREADS, 0, s>>>(V, ldv, cnt, h, w, n, norm_out != nullptr, partial);
+  if (norm_out) k_finalize_multi<RED_THREADS><<<1, RED_THREADS, 0, s>>>(partial, nb, norm_out, 1);
 }
 
 void stencil_prod_sym_f32x(const double* s4, const SymExc& ex, Dims d, const float* x, double* y, hipStream_t s) {

@@ -76,13 +76,13 @@ void axpy_dot(double* w, const double* v, const double* h, const double* vnext, 
               hipStream_t s);
 // x += sum_k y_k z_k in the order k = last..0 (coefficients from host array, up to 64)
 void sol_update(double* x, const double* Z, long ldz, const double* yk, int count, long n, hipStream_t s);
-// block classical Gram-Schmidt over cnt <= CGS_MAX basis vectors (vector k at V + k*ldv):
-// cgs_dot: out[k] = v_k . w;  cgs_update: w -= sum_k h_k v_k, then out[k] = v_k . w (mode 0) or
-// out[0] = ||w|| (mode 1).  partial needs CGS_MAX * reduction_blocks(n) doubles.
+// classical Gram-Schmidt passes over cnt <= CGS_MAX basis vectors (vector k at V + k*ldv):
+// cgs_dot: out[k] = v_k . w (k < cnt);  cgs_update: w -= sum_k h_k v_k, and norm_out[0] = ||w||
+// when norm_out is given.  partial needs CGS_MAX * reduction_blocks(n) doubles.
 void cgs_dot(const double* V, long ldv, int cnt, const double* w, long n, double* partial, double* out,
              hipStream_t s);
-void cgs_update(const double* V, long ldv, int cnt, const double* h, double* w, long n, int mode, double* partial,
-                double* out, hipStream_t s);
+void cgs_update(const double* V, long ldv, int cnt, const double* h, double* w, long n, double* partial,
+                double* norm_out, hipStream_t s);
 // mixed precision: y = A x with FP32 x (fine 7-point operator); x += sum_k y_k z_k with FP32 z
 void stencil7_prod_f32x(const double* coef, Dims d, const float* x, double* y, hipStream_t s);
 void stencil_prod_sym_f32x(const double* s4, const SymExc& ex, Dims d, const float* x, double* y, hipStream_t s);

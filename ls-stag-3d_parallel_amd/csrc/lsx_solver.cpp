@@ -30,6 +30,7 @@ Solver::Solver(const lsx_options& o, hipStream_t s) : opt(o) {
   if (opt.precision == LSX_PREC_MIXED && opt.mg != LSX_MG_VCYCLE)
     throw Error(LSX_ERR_ARG, "mixed precision uses the V-cycle multigrid (LSX_MG_VCYCLE)");
   if (opt.precision != LSX_PREC_FP64 && opt.precision != LSX_PREC_MIXED) throw Error(LSX_ERR_ARG, "unknown precision");
+  if (opt.ortho < LSX_ORTHO_AUTO || opt.ortho > LSX_ORTHO_CGS2) throw Error(LSX_ERR_ARG, "unknown ortho");
   if (opt.mg == LSX_MG_VCYCLE && (opt.n_pre < 1 || opt.n_post < 1))
     throw Error(LSX_ERR_ARG, "the V-cycle needs n_pre >= 1 and n_post >= 1");
   if (s) {
@@ -627,7 +628,13 @@ int Solver::fgmres(const double* b, double* x) {
   double normW = norm_r;
   double curH = norm_r;
   std::vector<double> hcol(m + 2), hcgs(2 * CGS_MAX);
-  const bool cgs = vcyc() && m <= CGS_MAX;
+  // orthogonalisation: the reference's MGS for the cascade; classical Gram-Schmidt (one pass, or
+  // two for CGS2) for the V-cycle loops, whose windows must fit CGS_MAX
+  int ortho = opt.ortho;
+  if (ortho == LSX_ORTHO_AUTO) ortho = vcyc() ? LSX_ORTHO_CGS : LSX_ORTHO_MGS;
+  if (ortho != LSX_ORTHO_MGS && m > CGS_MAX) ortho = LSX_ORTHO_MGS;
+  const int cgs_passes = ortho == LSX_ORTHO_CGS ? 1 : (ortho == LSX_ORTHO_CGS2 ? 2 : 0);
+  bool resume = false;  // a V-cycle loop whose estimate converged but whose true residual did not
   for (;;) {
     // normW lives on the device for the scaling kernel
     LSX_HIP(hipMemcpyAsync(dnorm, &normW, sizeof(double), hipMemcpyHostToDevice, stream));
@@ -653,26 +660,26 @@ int Solver::fgmres(const double* b, double* x) {
         fine_apply(zj, nullptr, w.p, ST_PROD);
       }
       tend(LSX_K_OPERATOR, e0);
-      if (cgs) {
-        // block classical Gram-Schmidt with one reorthogonalisation (the V-cycle loops are free to
-        // choose their orthogonalisation): h = V^T w ; w -= V h fused with h2 = V^T w ;
-        // w -= V h2 fused with ||w||.  Three passes over V instead of j + 1.
+      if (cgs_passes) {
+        // classical Gram-Schmidt: h = V^T w ; w -= V h (twice for CGS2), the last update pass
+        // yielding ||w||.  Two or four streaming passes over V instead of j + 1 MGS passes.
         const int cnt = j + 1;
         const double vb = 8.0 * cnt * (double)n;
-        tbeg(LSX_K_CGS_DOT, &e0);
-        cgs_dot(V.p, n, cnt, w.p, n, mpartial.p, cgs_h.p, stream);
-        tend(LSX_K_CGS_DOT, e0, vb + 8.0 * n);
-        tbeg(LSX_K_CGS_UPDATE, &e0);
-        cgs_update(V.p, n, cnt, cgs_h.p, w.p, n, 0, mpartial.p, cgs_h.p + CGS_MAX, stream);
-        tend(LSX_K_CGS_UPDATE, e0, vb + 16.0 * n);
-        tbeg(LSX_K_CGS_UPDATE, &e0);
-        cgs_update(V.p, n, cnt, cgs_h.p + CGS_MAX, w.p, n, 1, mpartial.p, dnorm, stream);
-        tend(LSX_K_CGS_UPDATE, e0, vb + 16.0 * n);
+        for (int pass = 0; pass < cgs_passes; ++pass) {
+          const bool last = pass + 1 == cgs_passes;
+          double* hp = cgs_h.p + pass * CGS_MAX;
+          tbeg(LSX_K_CGS_DOT, &e0);
+          cgs_dot(V.p, n, cnt, w.p, n, mpartial.p, hp, stream);
+          tend(LSX_K_CGS_DOT, e0, vb + 8.0 * n);
+          tbeg(LSX_K_CGS_UPDATE, &e0);
+          cgs_update(V.p, n, cnt, hp, w.p, n, mpartial.p, last ? dnorm : nullptr, stream);
+          tend(LSX_K_CGS_UPDATE, e0, vb + 16.0 * n);
+        }
         hNum = hColStart + j + 1;
         LSX_HIP(hipMemcpyAsync(hcgs.data(), cgs_h.p, 2 * CGS_MAX * sizeof(double), hipMemcpyDeviceToHost, stream));
         LSX_HIP(hipMemcpyAsync(&normW, dnorm, sizeof(double), hipMemcpyDeviceToHost, stream));
         LSX_HIP(hipStreamSynchronize(stream));
-        for (int i = 0; i <= j; ++i) hcol[i] = hcgs[i] + hcgs[CGS_MAX + i];
+        for (int i = 0; i <= j; ++i) hcol[i] = cgs_passes == 2 ? hcgs[i] + hcgs[CGS_MAX + i] : hcgs[i];
       } else {
         // modified Gram-Schmidt: h_i = w . v_i ; w -= h_i v_i  (the dot of the next basis vector
         // is fused into the update pass; the last pass yields ||w||^2)
@@ -716,8 +723,22 @@ int Solver::fgmres(const double* b, double* x) {
       } else {
         norm_r = curH;
         sol_upd(j, hNum - 1);
-        return iters;
+        if (!vcyc()) return iters;
+        // the V-cycle loops confirm convergence on the true residual and carry on from it when
+        // rounding in the basis has let the estimate drift below it
+        fine_apply(x, b, w.p, ST_RESID);
+        finalize(partial.p, nb, dnorm, 1, stream);
+        norm_r = reduce_to_host(dnorm);
+        if (!(norm_r > eps) || iters >= opt.max_it) return iters;
+        resume = true;
+        break;
       }
+    }
+    if (resume) {
+      resume = false;
+      normW = norm_r;
+      ++iters;
+      continue;
     }
     sol_upd(m - 1, m_size - 1);
     // w = b - A x starts the next cycle.  The reference scales it by the carried estimate

@@ -54,6 +54,7 @@ def random_divergence_rhs(diag, dims, seed=0, pin=True):
 KRYLOV_FGMRES, KRYLOV_BICGSTAB = 0, 1
 MG_CASCADE, MG_VCYCLE = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
+ORTHO_AUTO, ORTHO_MGS, ORTHO_CGS, ORTHO_CGS2 = 0, 1, 2, 3
 MEM_HOST, MEM_DEVICE = 0, 1
 
 
@@ -65,7 +66,7 @@ class Options(C.Structure):
     _fields_ = [
         ("krylov", C.c_int), ("restart", C.c_int), ("max_it", C.c_int), ("eps", C.c_double),
         ("mg", C.c_int), ("n_post", C.c_int), ("n_pre", C.c_int), ("omega", C.c_double),
-        ("precision", C.c_int), ("levels", C.c_int), ("z_head_fix", C.c_int),
+        ("precision", C.c_int), ("levels", C.c_int), ("z_head_fix", C.c_int), ("ortho", C.c_int),
     ]
 
 
@@ -237,7 +238,8 @@ class IterativeSolver:
 
     krylov = KRYLOV_FGMRES
 
-    def __init__(self, precision=PREC_FP64, mg=MG_CASCADE, z_head_fix=False, stream=None):
+    def __init__(self, precision=PREC_FP64, mg=MG_CASCADE, z_head_fix=False, stream=None, ortho=0):
+        self.ortho = ortho
         self.precision = precision
         self.mg = mg
         self.z_head_fix = z_head_fix
@@ -265,6 +267,7 @@ class IterativeSolver:
         opt.precision = self.precision
         opt.mg = self.mg
         opt.z_head_fix = 1 if self.z_head_fix else 0
+        opt.ortho = int(self.ortho)
         h = C.c_void_p()
         _check(lib().lsx_create(C.byref(h), C.byref(opt), self.stream))
         self.h = h
@@ -293,6 +296,7 @@ class IterativeSolver:
         opt.precision = self.precision
         opt.mg = self.mg
         opt.z_head_fix = 1 if self.z_head_fix else 0
+        opt.ortho = int(self.ortho)
         h = C.c_void_p()
         _check(lib().lsx_create(C.byref(h), C.byref(opt), self.stream))
         self.h = h

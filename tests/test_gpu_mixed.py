@@ -88,3 +88,21 @@ def test_vcycle_needs_a_pinned_row(lsx):
     S = lsx.solver("fgmres", "mixed")
     with pytest.raises(lsx.LsxError):
         S.set(A, np.zeros(A.n), np.zeros(A.n), lsx.mixed_task(1.0))
+
+
+@pytest.mark.parametrize("ortho", ["mgs", "cgs", "cgs2"])
+def test_vcycle_orthogonalisations(lsx, ortho):
+    """The V-cycle FGMRES reaches the tolerance on the true residual with each orthogonalisation;
+    the reported norm is the true residual."""
+    g, A = P.ellipsoid_stretched(64, ratio=1.08)
+    b = P.random_rhs(g, seed=9, A=A)
+    eps = 1e-8 * np.linalg.norm(b)
+    code = {"mgs": lsx.ORTHO_MGS, "cgs": lsx.ORTHO_CGS, "cgs2": lsx.ORTHO_CGS2}[ortho]
+    S = lsx.solver("fgmres", "mixed", ortho=code)
+    x = np.zeros(A.n)
+    S.set(A, b, x, lsx.mixed_task(eps, restart=30, max_it=300))
+    it = S.solve()
+    true_res = np.linalg.norm(b - A.matvec(x))
+    print(f"ortho={ortho} it={it} true_rel={true_res / np.linalg.norm(b):.3e} norm_r={S.norm_r:.3e}", flush=True)
+    assert true_res <= 1.05 * eps and it < 100
+    assert abs(S.norm_r - true_res) <= 1e-3 * true_res
