@@ -118,6 +118,33 @@ int lsxo_geom_build_body(void* p, int kind, double cx, double cy, double cz, dou
   return 0;
 }
 
+// Six-patch Bezier body: fit, point distances and the cut-cell geometry of a control net.
+void lsxo_bezier_ellipsoid(const double* center, const double* axes, double* ctrl) {
+  bezier_ellipsoid(center, axes, ctrl);
+}
+
+void lsxo_bezier_phi(const double* ctrl, const double* center, const double* axes, const double* pts, long n,
+                     double* out) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) out[i] = bezier_level_set(ctrl, center, axes, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+}
+
+int lsxo_geom_build_bezier(void* p, const double* ctrl, const double* center, const double* axes, int hs, int hf,
+                           int vs, int vf) {
+  auto* h = static_cast<GeomH*>(p);
+  try {
+    std::vector<double> cp(ctrl, ctrl + 288);
+    const double c0[3] = {center[0], center[1], center[2]}, a0[3] = {axes[0], axes[1], axes[2]};
+    build_cut_geometry(h->g, Box{hs, hf, vs, vf}, [cp, c0, a0](double x, double y, double z) {
+      return bezier_level_set(cp.data(), c0, a0, x, y, z);
+    });
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  return 0;
+}
+
 int lsxo_geom_get_level_set(void* p, double* phi, double* ax, double* ay, double* az, int* sx, int* sy, int* sz) {
   auto* h = static_cast<GeomH*>(p);
   const long n = h->g.ncells();

@@ -56,6 +56,10 @@ int vof_table_size();
 std::vector<double> cell_centres(const std::vector<double>& widths, double origin);
 Csr build_laplace(const Geometry& g, bool pin_outlet);
 void body_free_vof(Geometry& g);
+// Six-patch bicubic Bezier body (oracle_bezier.cpp): ellipsoid fit into 288 control values, and
+// the signed distance (positive inside) at a point.
+void bezier_ellipsoid(const double* center, const double* axes, double* ctrl);
+double bezier_level_set(const double* ctrl, const double* center, const double* axes, double x, double y, double z);
 // Pressure-equation right-hand side (oracle_rhs.cpp).  u_west: M*K inflow velocities (one per
 // x-row); u_out: the x-face velocity for an outflow east boundary, or null for a fixed U_E.
 std::vector<double> pressure_flux(const Geometry& g, const double* u_west, double U_E, double V_S, double V_N,

@@ -51,6 +51,9 @@ def lib():
     L.lsxo_geom_build_body.argtypes = [C.c_void_p, C.c_int] + [C.c_double] * 6 + [C.c_int] * 4
     L.lsxo_geom_get_level_set.argtypes = [C.c_void_p, PD, PD, PD, PD, PI, PI, PI]
     L.lsxo_vof_table_size.restype = C.c_int
+    L.lsxo_bezier_ellipsoid.argtypes = [PD, PD, PD]
+    L.lsxo_bezier_phi.argtypes = [PD, PD, PD, PD, C.c_long, PD]
+    L.lsxo_geom_build_bezier.argtypes = [C.c_void_p, PD, PD, PD] + [C.c_int] * 4
     L.lsxo_geom_correct.argtypes = [C.c_void_p, PD, PD, PD, PD, PD, PD, PD]
     L.lsxo_geom_rhs.argtypes = [C.c_void_p, PD, PD, PD, PD] + [C.c_double] * 5 + [C.c_void_p, C.c_int, PD,
                                                                                   C.c_void_p]
@@ -132,6 +135,16 @@ class Geometry:
             raise RuntimeError(lib().lsxo_last_error().decode())
         return self
 
+    def build_bezier(self, ctrl, center, axes, box=None):
+        """Cut-cell geometry of the six-patch Bezier body with control points ctrl (6, 4, 4, 3)."""
+        N, M, K = self.dims
+        hs, hf, vs, vf = box or (1, N - 1, 1, M - 1)
+        st = lib().lsxo_geom_build_bezier(self.h, np.ascontiguousarray(ctrl, np.float64).ravel(),
+                                          np.asarray(center, np.float64), np.asarray(axes, np.float64), hs, hf, vs, vf)
+        if st != 0:
+            raise RuntimeError(lib().lsxo_last_error().decode())
+        return self
+
     def level_set(self):
         n = self.n
         phi, ax, ay, az = (np.zeros(n) for _ in range(4))
@@ -149,6 +162,22 @@ class Geometry:
         val = np.zeros(nnz)
         lib().lsxo_geom_get_csr(self.h, rp, col, val)
         return Csr(self.dims, rp, col, val)
+
+
+def bezier_ellipsoid(center, axes):
+    """Control points (6, 4, 4, 3) of the six-patch bicubic Bezier fit of an ellipsoid."""
+    ctrl = np.zeros(288)
+    lib().lsxo_bezier_ellipsoid(np.asarray(center, np.float64), np.asarray(axes, np.float64), ctrl)
+    return ctrl.reshape(6, 4, 4, 3)
+
+
+def bezier_phi(ctrl, center, axes, pts):
+    """Signed distance (positive inside) of points (n, 3) to the Bezier body."""
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    out = np.zeros(len(pts))
+    lib().lsxo_bezier_phi(np.ascontiguousarray(ctrl, np.float64).ravel(), np.asarray(center, np.float64),
+                          np.asarray(axes, np.float64), pts, len(pts), out)
+    return out
 
 
 def face_shapes(dims):
