@@ -156,12 +156,24 @@ typedef struct lsx_geometry lsx_geometry;
 
 #define LSX_BODY_SPHERE 0    /* phi = r - |x - c|                                  */
 #define LSX_BODY_ELLIPSOID 1 /* phi = min(a) (1 - |(x - c) / a|), axis-aligned axes a */
+#define LSX_BODY_BEZIER6 2   /* phi = signed distance to a six-patch bicubic Bezier surface */
 
 typedef struct lsx_body {
   int kind;
   double center[3];
-  double axes[3]; /* sphere: axes[0] = radius */
+  double axes[3];     /* sphere: axes[0] = radius; Bezier: the ellipsoid the search starts from */
+  const double* ctrl; /* Bezier: 288 host values, patch f, point (i, j), coordinate c at
+                         ((f * 16 + i * 4 + j) * 3 + c); patches on the cube faces +x, -x, +y,
+                         -y, +z, -z with (u, v) along (y, z), (z, y), (z, x), (x, z), (x, y),
+                         (y, x); copies of shared edge and corner points must be equal */
 } lsx_body;
+
+/* Six-patch bicubic Bezier fit of the ellipsoid with centre c and semi-axes a into ctrl[288]:
+ * each cube edge curve is the least-squares cubic fit of the cube-sphere image of that edge
+ * with the corners fixed, each patch interior the least-squares fit of the face image with the
+ * boundary fixed.  No reference counterpart (the reference's Bezier bodies are 2D curves,
+ * src/Bezier/bezier_approximation.h:39-100).  Host only. */
+int lsx_bezier_ellipsoid(const double center[3], const double axes[3], double ctrl[288]);
 
 /* Tensor grid with per-cell widths dx[N], dy[M], dz[K] (host arrays) and lower corner
  * (x0, y0, z0).  A uniform axis places centres at x0 + (k + 1/2) h as the reference does. */

@@ -32,6 +32,7 @@ SOURCES = {
     "lsx_geometry.hip": ["-ffp-contract=off"],
     "lsx_geometry_host.cpp": ["-ffp-contract=off"],
     "lsx_vcycle.hip": [],
+    "lsx_bezier.cpp": [],
     "lsx_capi.cpp": [],
 }
 

@@ -170,6 +170,11 @@ int lsx_geometry_destroy(lsx_geometry* g) {
   return guarded([&] { delete g; });
 }
 
+int lsx_bezier_ellipsoid(const double center[3], const double axes[3], double ctrl[288]) {
+  if (!center || !axes || !ctrl) return LSX_ERR_ARG;
+  return guarded([&] { lsx::bezier_ellipsoid(center, axes, ctrl); });
+}
+
 int lsx_geometry_build(lsx_geometry* g, const lsx_body* body, int hs, int hf, int vs, int vf) {
   if (!g || !body) return LSX_ERR_ARG;
   return guarded([&] {
@@ -179,7 +184,7 @@ int lsx_geometry_build(lsx_geometry* g, const lsx_body* body, int hs, int hf, in
       b.c[q] = body->center[q];
       b.a[q] = body->axes[q];
     }
-    g->impl.build(b, lsx::geo::Box{hs, hf, vs, vf});
+    g->impl.build(b, lsx::geo::Box{hs, hf, vs, vf}, body->kind == LSX_BODY_BEZIER6 ? body->ctrl : nullptr);
   });
 }
 

@@ -36,6 +36,126 @@ __device__ inline void decode(long i, int N, int M, int* x, int* y, int* z) {
 
 __device__ inline bool in_box(int x, int y, const Box& b) { return x >= b.hs && x < b.hf && y >= b.vs && y < b.vf; }
 
+// ---- six-patch bicubic Bezier body ----------------------------------------------------------------
+// Patches on the six cube faces: face f has normal axis BZ_A[f] with sign BZ_S[f], u along BZ_U[f]
+// and v along BZ_V[f] (S_u x S_v outwards); control point (i, j) of face f, coordinate c, at
+// ctrl[(f * 16 + i * 4 + j) * 3 + c].  phi = +distance inside, -distance outside: per patch 12
+// Newton steps from the cube-sphere inverse of the point (a coordinate at its bound with the
+// gradient pointing out holds there while the other takes a 1D step; Gauss-Newton where the
+// Hessian is not positive definite), smallest distance wins, side from (x - S) . (S_u x S_v).
+__constant__ int BZ_A[6] = {0, 0, 1, 1, 2, 2};
+__constant__ double BZ_S[6] = {1.0, -1.0, 1.0, -1.0, 1.0, -1.0};
+__constant__ int BZ_U[6] = {1, 2, 2, 0, 0, 1};
+__constant__ int BZ_V[6] = {2, 1, 0, 2, 1, 0};
+
+struct BzEval {
+  double S[3], Su[3], Sv[3], Suu[3], Suv[3], Svv[3];
+};
+
+__device__ inline void bz_basis(double t, double* b, double* db, double* ddb) {
+  const double s = 1.0 - t;
+  b[0] = s * s * s;
+  b[1] = 3.0 * t * s * s;
+  b[2] = 3.0 * t * t * s;
+  b[3] = t * t * t;
+  db[0] = -3.0 * s * s;
+  db[1] = 3.0 * s * s - 6.0 * t * s;
+  db[2] = 6.0 * t * s - 3.0 * t * t;
+  db[3] = 3.0 * t * t;
+  ddb[0] = 6.0 * s;
+  ddb[1] = 18.0 * t - 12.0;
+  ddb[2] = 6.0 - 18.0 * t;
+  ddb[3] = 6.0 * t;
+}
+
+__device__ inline void bz_eval(const double* __restrict__ P, double u, double v, BzEval& e) {
+  double bu[4], dbu[4], ddbu[4], bv[4], dbv[4], ddbv[4];
+  bz_basis(u, bu, dbu, ddbu);
+  bz_basis(v, bv, dbv, ddbv);
+  for (int c = 0; c < 3; ++c) {
+    double s = 0.0, su = 0.0, sv = 0.0, suu = 0.0, suv = 0.0, svv = 0.0;
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        const double p = P[(i * 4 + j) * 3 + c];
+        s += bu[i] * bv[j] * p;
+        su += dbu[i] * bv[j] * p;
+        sv += bu[i] * dbv[j] * p;
+        suu += ddbu[i] * bv[j] * p;
+        suv += dbu[i] * dbv[j] * p;
+        svv += bu[i] * ddbv[j] * p;
+      }
+    e.S[c] = s;
+    e.Su[c] = su;
+    e.Sv[c] = sv;
+    e.Suu[c] = suu;
+    e.Suv[c] = suv;
+    e.Svv[c] = svv;
+  }
+}
+
+__device__ inline double bz_dot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+__device__ inline double bz_clamp(double t) { return t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t); }
+
+__device__ double bezier_phi(const double* __restrict__ ctrl, const double* c0, const double* a0, double x, double y,
+                             double z) {
+  const double X[3] = {x, y, z};
+  double d[3];
+  for (int c = 0; c < 3; ++c) d[c] = (X[c] - c0[c]) / a0[c];
+  double best = -1.0, side = 0.0;
+  for (int f = 0; f < 6; ++f) {
+    const double* P = ctrl + f * 48;
+    const double da = fabs(d[BZ_A[f]]);
+    double u = 0.5, v = 0.5;
+    if (da > 0.0) {
+      u = bz_clamp(0.5 * (d[BZ_U[f]] / da + 1.0));
+      v = bz_clamp(0.5 * (d[BZ_V[f]] / da + 1.0));
+    }
+    BzEval e;
+    for (int it = 0; it < 12; ++it) {
+      bz_eval(P, u, v, e);
+      double r[3];
+      for (int c = 0; c < 3; ++c) r[c] = e.S[c] - X[c];
+      const double gu = bz_dot(e.Su, r), gv = bz_dot(e.Sv, r);
+      const double juu = bz_dot(e.Su, e.Su), juv = bz_dot(e.Su, e.Sv), jvv = bz_dot(e.Sv, e.Sv);
+      double huu = juu + bz_dot(e.Suu, r), huv = juv + bz_dot(e.Suv, r), hvv = jvv + bz_dot(e.Svv, r);
+      double det = huu * hvv - huv * huv;
+      if (!(det > 0.0 && huu > 0.0)) {
+        huu = juu;
+        huv = juv;
+        hvv = jvv;
+        det = huu * hvv - huv * huv;
+      }
+      const bool hold_u = (u >= 1.0 && gu < 0.0) || (u <= 0.0 && gu > 0.0);
+      const bool hold_v = (v >= 1.0 && gv < 0.0) || (v <= 0.0 && gv > 0.0);
+      double du = 0.0, dv = 0.0;
+      if (hold_u && hold_v) break;
+      if (hold_u) {
+        dv = -gv / (hvv > 0.0 ? hvv : jvv);
+      } else if (hold_v) {
+        du = -gu / (huu > 0.0 ? huu : juu);
+      } else {
+        if (!(det > 0.0)) break;
+        du = -(hvv * gu - huv * gv) / det;
+        dv = -(huu * gv - huv * gu) / det;
+      }
+      u = bz_clamp(u + du);
+      v = bz_clamp(v + dv);
+    }
+    bz_eval(P, u, v, e);
+    double r[3];
+    for (int c = 0; c < 3; ++c) r[c] = X[c] - e.S[c];
+    const double dist2 = bz_dot(r, r);
+    if (best < 0.0 || dist2 < best) {
+      best = dist2;
+      const double nrm[3] = {e.Su[1] * e.Sv[2] - e.Su[2] * e.Sv[1], e.Su[2] * e.Sv[0] - e.Su[0] * e.Sv[2],
+                             e.Su[0] * e.Sv[1] - e.Su[1] * e.Sv[0]};
+      side = bz_dot(r, nrm);
+    }
+  }
+  const double dist = sqrt(best);
+  return side > 0.0 ? -dist : dist;
+}
+
 // ---- level set -------------------------------------------------------------------------------
 __global__ void k_level_set(const double* __restrict__ cx, const double* __restrict__ cy, const double* __restrict__ cz,
                             int N, int M, int K, Box box, Body body, double* __restrict__ phi) {
@@ -48,6 +168,8 @@ __global__ void k_level_set(const double* __restrict__ cx, const double* __restr
       if (body.kind == BODY_SPHERE) {
         const double qx = cx[x] - body.c[0], qy = cy[y] - body.c[1], qz = cz[z] - body.c[2];
         v = body.a[0] - sqrt(qx * qx + qy * qy + qz * qz);
+      } else if (body.kind == BODY_BEZIER6) {
+        v = bezier_phi(body.ctrl, body.c, body.a, cx[x], cy[y], cz[z]);
       } else {
         const double s = fmin(body.a[0], fmin(body.a[1], body.a[2]));
         const double qx = (cx[x] - body.c[0]) / body.a[0], qy = (cy[y] - body.c[1]) / body.a[1],

@@ -19,12 +19,13 @@ enum CellCode : int {
   BNE_Corner = -24, BNW_Corner = -25, BSE_Corner = -26, BSW_Corner = -27
 };
 
-enum BodyKind : int { BODY_SPHERE = 0, BODY_ELLIPSOID = 1 };
+enum BodyKind : int { BODY_SPHERE = 0, BODY_ELLIPSOID = 1, BODY_BEZIER6 = 2 };
 
 struct Body {
   int kind;
-  double c[3];  // centre
-  double a[3];  // radius (sphere: a[0]) or semi-axes
+  double c[3];                   // centre
+  double a[3];                   // radius (sphere: a[0]) or semi-axes (Bezier: start of the search)
+  const double* ctrl = nullptr;  // Bezier body: 288 control values in device memory
 };
 
 // cut cells may appear for x in [hs, hf), y in [vs, vf), every z plane

@@ -59,12 +59,22 @@ Geometry::~Geometry() {
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
-void Geometry::build(const geo::Body& body, geo::Box box) {
+void Geometry::build(const geo::Body& body_in, geo::Box box, const double* host_ctrl) {
   if (box.hs < 1 || box.vs < 1 || box.hf > N - 1 || box.vf > M - 1 || box.hs >= box.hf || box.vs >= box.vf)
     throw Error(LSX_ERR_ARG, "geometry box must lie inside the domain's interior");
-  if (body.kind != geo::BODY_SPHERE && body.kind != geo::BODY_ELLIPSOID) throw Error(LSX_ERR_ARG, "unknown body kind");
-  for (int q = 0; q < (body.kind == geo::BODY_SPHERE ? 1 : 3); ++q)
-    if (!(body.a[q] > 0)) throw Error(LSX_ERR_ARG, "body size must be positive");
+  if (body_in.kind != geo::BODY_SPHERE && body_in.kind != geo::BODY_ELLIPSOID && body_in.kind != geo::BODY_BEZIER6)
+    throw Error(LSX_ERR_ARG, "unknown body kind");
+  for (int q = 0; q < (body_in.kind == geo::BODY_SPHERE ? 1 : 3); ++q)
+    if (!(body_in.a[q] > 0)) throw Error(LSX_ERR_ARG, "body size must be positive");
+  geo::Body body = body_in;
+  if (body.kind == geo::BODY_BEZIER6) {
+    if (!host_ctrl) throw Error(LSX_ERR_ARG, "a Bezier body needs its 288 control values");
+    for (int q = 0; q < 288; ++q)
+      if (!std::isfinite(host_ctrl[q])) throw Error(LSX_ERR_ARG, "Bezier control values must be finite");
+    bez.alloc(288);
+    LSX_HIP(hipMemcpyAsync(bez.p, host_ctrl, 288 * sizeof(double), hipMemcpyHostToDevice, stream));
+    body.ctrl = bez.p;
+  }
   // upper-corner coordinates: x starts at the first box cell and steps by the next cell's width
   const std::vector<double> xc = centres(hdx, x0), yc = centres(hdy, y0), zc = centres(hdz, z0);
   std::vector<double> hcx(N, 0.0), hcy(M), hcz(K);

@@ -8,13 +8,16 @@
 
 namespace lsx {
 
+// six-patch bicubic Bezier fit of an ellipsoid into ctrl[288] (lsx_bezier.cpp)
+void bezier_ellipsoid(const double* center, const double* axes, double* ctrl);
+
 class Geometry {
  public:
   Geometry(int N, int M, int K, const double* dx, const double* dy, const double* dz, double x0, double y0, double z0,
            hipStream_t stream);
   ~Geometry();
   // Build every field for `body` with cut cells allowed in box [hs,hf) x [vs,vf) x all z.
-  void build(const geo::Body& body, geo::Box box);
+  void build(const geo::Body& body, geo::Box box, const double* host_ctrl = nullptr);
   // Copy fields to host (any pointer may be null).
   void fields(int* type, double* thx, double* thy, double* thz, double* vof, double* phi) const;
   // 7 coefficient planes [b,s,w,p,e,n,t] of the pressure operator (device pointer, 7*n doubles).
@@ -39,6 +42,7 @@ class Geometry {
   DevBuf<double> phi, ax, ay, az, tx, ty, tz, vof;
   DevBuf<int> type, sx, sy, sz;
   DevBuf<unsigned char> flag;
+  DevBuf<double> bez;  // Bezier control values (device)
 };
 
 }  // namespace lsx

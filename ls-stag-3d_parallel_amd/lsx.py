@@ -109,6 +109,7 @@ def lib():
     L.lsx_geometry_correct.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_int]
     L.lsx_operator_diagonal.argtypes = [vp, vp]
     L.lsx_fine_storage.argtypes = [vp]
+    L.lsx_bezier_ellipsoid.argtypes = [vp, vp, vp]
     L.lsx_kernel_timing.argtypes = [vp, C.c_int]
     L.lsx_kernel_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_long), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     _lib = L
@@ -116,10 +117,43 @@ def lib():
 
 
 class Body(C.Structure):
-    _fields_ = [("kind", C.c_int), ("center", C.c_double * 3), ("axes", C.c_double * 3)]
+    _fields_ = [("kind", C.c_int), ("center", C.c_double * 3), ("axes", C.c_double * 3),
+                ("ctrl", C.c_void_p)]
 
 
-BODY_SPHERE, BODY_ELLIPSOID = 0, 1
+BODY_SPHERE, BODY_ELLIPSOID, BODY_BEZIER6 = 0, 1, 2
+
+# Bezier patches on the cube faces +x, -x, +y, -y, +z, -z: (normal axis, sign, u axis, v axis)
+BEZIER_FACES = [(0, 1, 1, 2), (0, -1, 2, 1), (1, 1, 2, 0), (1, -1, 0, 2), (2, 1, 0, 1), (2, -1, 1, 0)]
+
+
+def bezier_ellipsoid(center, axes):
+    """Control points (6, 4, 4, 3) of the six-patch bicubic Bezier fit of an ellipsoid."""
+    ctrl = np.zeros(288)
+    c = np.ascontiguousarray(center, np.float64)
+    a = np.ascontiguousarray(axes, np.float64)
+    _check(lib().lsx_bezier_ellipsoid(_ptr(c), _ptr(a), _ptr(ctrl)))
+    return ctrl.reshape(6, 4, 4, 3)
+
+
+def bezier_perturb(ctrl, amplitude, seed):
+    """Seeded U(-amplitude, amplitude) perturbation of every coordinate of the 56 distinct
+    control points (lattice order), applied to all patch copies of a point so the surface stays
+    closed."""
+    ctrl = np.array(ctrl, np.float64).reshape(6, 4, 4, 3)
+    copies = {}
+    for f, (a, s, bu, bv) in enumerate(BEZIER_FACES):
+        for i in range(4):
+            for j in range(4):
+                l = [0, 0, 0]
+                l[a], l[bu], l[bv] = (3 if s > 0 else 0), i, j
+                copies.setdefault(tuple(l), []).append((f, i, j))
+    rng = np.random.default_rng(seed)
+    for key in sorted(copies):
+        d = rng.uniform(-amplitude, amplitude, 3)
+        for (f, i, j) in copies[key]:
+            ctrl[f, i, j] += d
+    return ctrl
 
 
 class FlowBc(C.Structure):
@@ -146,11 +180,19 @@ class Geometry:
         N, M, K = self.dims
         return N * M * K
 
-    def build(self, kind, center, axes, box=None):
+    def build(self, kind, center, axes, box=None, ctrl=None):
+        """kind 'sphere' (axes = (r,)), 'ellipsoid' (semi-axes) or 'bezier' (ctrl: (6, 4, 4, 3)
+        control points; axes: the ellipsoid the closest-point search starts from)."""
         N, M, K = self.dims
         hs, hf, vs, vf = box or (1, N - 1, 1, M - 1)
         a = tuple(axes) + (axes[0],) * (3 - len(axes))
-        b = Body(BODY_SPHERE if kind == "sphere" else BODY_ELLIPSOID, (C.c_double * 3)(*center), (C.c_double * 3)(*a))
+        code = {"sphere": BODY_SPHERE, "ellipsoid": BODY_ELLIPSOID, "bezier": BODY_BEZIER6}[kind]
+        cp = None
+        if code == BODY_BEZIER6:
+            cp = np.ascontiguousarray(ctrl, np.float64).ravel()
+            if cp.size != 288:
+                raise LsxError("a Bezier body needs 288 control values")
+        b = Body(code, (C.c_double * 3)(*center), (C.c_double * 3)(*a), None if cp is None else cp.ctypes.data)
         _check(lib().lsx_geometry_build(self.h, C.byref(b), hs, hf, vs, vf))
         return self
 
