@@ -6,9 +6,10 @@ A step is one pressure Poisson solve of the cut-cell operator to relative residu
 multigrid V-cycle: mixed precision (FP32 red-black Gauss-Seidel, FP16 coarse faces) by
 default, compared with the same V-cycle in FP64 (the plain FP64 GPU solve) and with the
 reference's FP64 multigrid cascade, which is reported at an iteration cap because it does not
-reach 1e-8 on this stretched grid.  Workload: 256^3 cells on the unit cube, per-axis geometric stretching ratio 1.02
-towards the body centre, an ellipsoidal body (semi-axes 0.25/0.15/0.15) at the centre, the
-reference's pinned-outlet pressure boundary conditions, a seeded unit-variance Gaussian
+reach 1e-8 on this stretched grid.  Workload: 256^3 cells on the unit cube, per-axis geometric
+stretching ratio 1.02 towards the body centre, a six-patch bicubic Bezier body fitted to the
+ellipsoid with semi-axes 0.25/0.15/0.15 at the centre (control points moved by a seeded
+U(-0.01, 0.01)), the reference's pinned-outlet pressure boundary conditions, a seeded unit-variance Gaussian
 right-hand side on fluid cells with its mean removed.  Geometry, operator and multigrid
 hierarchy are built on the GPU before the timed region; b and x live in HBM.
 
@@ -84,7 +85,24 @@ def workload(n, ratio=1.02, center=(0.5, 0.5, 0.5), axes=(0.25, 0.15, 0.15)):
     return widths, center, axes
 
 
-def cpu_baseline(sample_n, gpu_n, restart, krylov):
+def body_ctrl(center, axes, seed=0):
+    """Control points of the benchmark body: six-patch bicubic Bezier fit of the ellipsoid, every
+    coordinate of its 56 distinct control points moved by a seeded U(-0.01, 0.01)."""
+    lsx = load_lsx()
+    return lsx.bezier_perturb(lsx.bezier_ellipsoid(center, axes), 0.01, seed)
+
+
+def build_body_geometry(geom, body, center, axes):
+    if body == "bezier":
+        if hasattr(geom, "build_bezier"):  # oracle geometry
+            return geom.build_bezier(body_ctrl(center, axes), center, axes)
+        return geom.build("bezier", center, axes, ctrl=body_ctrl(center, axes))
+    if hasattr(geom, "build_body"):
+        return geom.build_body("ellipsoid", center, axes)
+    return geom.build("ellipsoid", center, axes)
+
+
+def cpu_baseline(sample_n, gpu_n, restart, krylov, body="bezier"):
     """The CPU oracle (the reference's FP64 cascade algorithm) timed on this host, solving the same
     workload at sample_n^3 to 1e-8, scaled to gpu_n^3 by the cell count at the same iteration
     count.  At 256^3 the reference algorithm does not reach 1e-8 at all, so this scaled rate is an
@@ -94,7 +112,7 @@ def cpu_baseline(sample_n, gpu_n, restart, krylov):
     lsx = load_lsx()
     widths, center, axes = workload(sample_n)
     g = O.Geometry(*widths)
-    g.build_body("ellipsoid", center, axes)
+    build_body_geometry(g, body, center, axes)
     A = g.laplace()
     b = lsx.random_divergence_rhs(A.to_scipy().diagonal(), g.dims)
     eps = 1e-8 * float(np.linalg.norm(b))
@@ -171,6 +189,7 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64-vcycle", "fp64"], default="mixed")
     ap.add_argument("--krylov", choices=["fgmres", "bicgstab"], default="fgmres")
     ap.add_argument("--restart", type=int, default=30)
+    ap.add_argument("--body", choices=["bezier", "ellipsoid"], default="bezier")
     ap.add_argument("--smooth", default="2,2", help="V-cycle pre,post red-black sweeps")
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--cascade-max-it", type=int, default=200,
@@ -189,7 +208,7 @@ def main():
     widths, center, axes = workload(n)
     t_geo = time.perf_counter()
     geom = lsx.Geometry(*widths)
-    geom.build("ellipsoid", center, axes)
+    build_body_geometry(geom, args.body, center, axes)
     hip.sync()
     geo_s = time.perf_counter() - t_geo
     print(f"[bench] geometry {n}^3 built in {geo_s:.2f} s", file=sys.stderr, flush=True)
@@ -229,8 +248,10 @@ def main():
             traffic = None
     value = args.steps / elapsed
     cfg = {
-        "workload": (f"{n}^3 stretched grid (ratio 1.02 to centre), ellipsoid body 0.25/0.15/0.15 "
-                     f"(level set of the ellipsoid the Bezier body approximates), {args.krylov.upper()}"
+        "workload": (f"{n}^3 stretched grid (ratio 1.02 to centre), "
+                     + ("six-patch bicubic Bezier body fitted to the ellipsoid 0.25/0.15/0.15, control points "
+                        "+ U(-0.01, 0.01) (seed 0)" if args.body == "bezier" else "ellipsoid body 0.25/0.15/0.15")
+                     + f", {args.krylov.upper()}"
                      f"({args.restart}), rel tol 1e-8, "
                      + {"mixed": f"mixed: FP32 V{smooth} red-black GS multigrid, FP16 coarse faces, FP64 Krylov",
                         "fp64-vcycle": f"FP64 V{smooth} red-black GS multigrid",
@@ -274,7 +295,7 @@ def main():
     }
     if not args.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline(args.cpu_sample_grid, n, args.restart, args.krylov)
+            out["cpu_baseline"] = cpu_baseline(args.cpu_sample_grid, n, args.restart, args.krylov, args.body)
         except Exception as e:  # the GPU measurement stands on its own
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out), flush=True)

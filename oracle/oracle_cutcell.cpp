@@ -214,6 +214,7 @@ void corner_level_set(Geometry& g, const Box& box, const std::function<double(do
   const long n = g.ncells();
   g.phi.assign(n, 0.0);
   const std::vector<double> xs = cell_centres(g.dx, g.x0), ys = cell_centres(g.dy, g.y0), zs = cell_centres(g.dz, g.z0);
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
   for (int z = 0; z < g.K; ++z)
     for (int y = box.vs; y < box.vf; ++y) {
       // corner of the first box cell, then step by the next cell's width
