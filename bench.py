@@ -180,8 +180,24 @@ def run_solver(lsx, hip, geom, precision, krylov, restart, steps, warmup, max_it
                 rel_res=float(np.linalg.norm(r) / nb), x=x, levels=S.num_levels())
 
 
+def copy_calibration(gib):
+    """Five device-to-device copies of `gib` GiB (torch's elementwise copy kernel): the known
+    traffic against which the FETCH_SIZE / WRITE_SIZE counters of the solver kernels are scaled."""
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "copies": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="only run 5 device copies of this many GiB (counter calibration) and exit")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -197,6 +213,9 @@ def main():
     ap.add_argument("--cpu-sample-grid", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the solve runs on a single "
               "MI355X; multi-GPU sharding is a design note only)", file=sys.stderr)
