@@ -59,6 +59,7 @@ void lsx_options_default(lsx_options* o) {
   o->levels = 0;
   o->z_head_fix = 0;
   o->ortho = LSX_ORTHO_AUTO;
+  o->coarse_fuse = 1;
 }
 
 int lsx_create(lsx_solver** out, const lsx_options* opt, void* stream) {

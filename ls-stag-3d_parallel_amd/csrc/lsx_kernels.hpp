@@ -10,7 +10,7 @@ namespace lsx {
 
 struct Dims {
   int N, M, K;
-  long n() const { return (long)N * M * K; }
+  __host__ __device__ long n() const { return (long)N * M * K; }
 };
 
 // Operator storage: npts SoA coefficient planes of n values each.

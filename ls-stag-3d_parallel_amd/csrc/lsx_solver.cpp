@@ -436,7 +436,16 @@ void Solver::vcycle_apply(const double* v, T* z) {
   } else {
     b0 = v;
   }
-  for (int l = 0; l < L; ++l) {
+  // levels from `lc` down are small enough for one workgroup: a single launch does their cycle
+  int lc = L + 1;
+  for (int l = 1; l <= L && opt.coarse_fuse; ++l)
+    if (vlev[l].L.d.n() <= vc::COARSE_CELLS && L - l + 1 <= vc::COARSE_MAX) {
+      lc = l;
+      break;
+    }
+  if (lc == L) lc = L + 1;  // the LU level alone gains nothing
+  const int top = lc <= L ? lc : L;
+  for (int l = 0; l < top; ++l) {
     VStore& S = vlev[l];
     T* x = l == 0 ? z : vx<T>(S);
     const T* b = l == 0 ? b0 : vb<T>(S);
@@ -449,7 +458,16 @@ void Solver::vcycle_apply(const double* v, T* z) {
     if (l == 0) tend(LSX_K_SMOOTH, e0);
     vc::resid_restrict<T>(S.L, x, b, vb<T>(vlev[l + 1]), stream);
   }
-  {
+  if (lc <= L) {
+    vc::CoarseLevels<T> C;
+    C.count = L - lc + 1;
+    for (int q = 0; q < C.count; ++q) {
+      C.L[q] = vlev[lc + q].L;
+      C.x[q] = vx<T>(vlev[lc + q]);
+      C.b[q] = vb<T>(vlev[lc + q]);
+    }
+    vc::coarse_cycle<T>(C, opt.n_pre, opt.n_post, lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, stream);
+  } else {
     VStore& C = vlev[L];
     const long nc = C.L.d.n();
     if constexpr (std::is_same<T, float>::value) {
@@ -460,7 +478,7 @@ void Solver::vcycle_apply(const double* v, T* z) {
       lu_solve(lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, (int)nc, C.db.p, C.dx.p, stream);
     }
   }
-  for (int l = L - 1; l >= 0; --l) {
+  for (int l = top - 1; l >= 0; --l) {
     VStore& S = vlev[l];
     T* x = l == 0 ? z : vx<T>(S);
     const T* b = l == 0 ? b0 : vb<T>(S);

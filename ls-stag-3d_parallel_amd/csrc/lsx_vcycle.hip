@@ -195,15 +195,16 @@ __device__ inline T offdiag(const T* __restrict__ x, const FT* __restrict__ ae, 
   return acc;
 }
 
-// thread t owns the t-th cell of the colour
+// one colour of a red-black sweep; work item t (from start, by stride) owns the t-th cell of it
 template <class T, class FT>
-__global__ void k_rbgs(T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae, const FT* __restrict__ an,
-                       const FT* __restrict__ at, const T* __restrict__ s, T fs, int N, int M, int K, int color,
-                       int pin0, int pin1, int zero_guess) {
+__device__ inline void rbgs_body(T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
+                                 const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s, T fs,
+                                 int N, int M, int K, int color, int pin0, int pin1, int zero_guess, long start,
+                                 long stride) {
   const long half_row = (N + 1) / 2;
   const long rows = (long)M * K;
   const long total = half_row * rows;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+  for (long t = start; t < total; t += stride) {
     const long row = t / half_row;
     const int yy = (int)(row % M), zz = (int)(row / M);
     const int xx = 2 * (int)(t - row * half_row) + ((yy + zz + color) & 1);
@@ -222,12 +223,21 @@ __global__ void k_rbgs(T* __restrict__ x, const T* __restrict__ b, const FT* __r
 }
 
 template <class T, class FT>
-__global__ void k_resid_restrict(const T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
-                                 const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s, T fs,
-                                 int N, int M, int K, int pin0, int pin1, T* __restrict__ bc) {
+__global__ void k_rbgs(T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae, const FT* __restrict__ an,
+                       const FT* __restrict__ at, const T* __restrict__ s, T fs, int N, int M, int K, int color,
+                       int pin0, int pin1, int zero_guess) {
+  rbgs_body<T, FT>(x, b, ae, an, at, s, fs, N, M, K, color, pin0, pin1, zero_guess,
+                   blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x);
+}
+
+template <class T, class FT>
+__device__ inline void resid_restrict_body(const T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
+                                           const FT* __restrict__ an, const FT* __restrict__ at,
+                                           const T* __restrict__ s, T fs, int N, int M, int K, int pin0, int pin1,
+                                           T* __restrict__ bc, long start, long stride) {
   const int Nc = N / 2, Mc = M / 2, Kc = K / 2;
   const long nc = (long)Nc * Mc * Kc, NM = (long)N * M;
-  for (long I = blockIdx.x * (long)blockDim.x + threadIdx.x; I < nc; I += (long)gridDim.x * blockDim.x) {
+  for (long I = start; I < nc; I += stride) {
     int X, Y, Z;
     decode(I, Nc, Mc, &X, &Y, &Z);
     T acc = 0;
@@ -244,6 +254,14 @@ __global__ void k_resid_restrict(const T* __restrict__ x, const T* __restrict__ 
   }
 }
 
+template <class T, class FT>
+__global__ void k_resid_restrict(const T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
+                                 const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s, T fs,
+                                 int N, int M, int K, int pin0, int pin1, T* __restrict__ bc) {
+  resid_restrict_body<T, FT>(x, b, ae, an, at, s, fs, N, M, K, pin0, pin1, bc,
+                             blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x);
+}
+
 __device__ inline void axis_w(int f, int nf, int* c, float* w, int* cnt) {
   const int p = f >> 1;
   if (f == 0 || f == nf - 1) {
@@ -256,10 +274,11 @@ __device__ inline void axis_w(int f, int nf, int* c, float* w, int* cnt) {
 }
 
 template <class T>
-__global__ void k_prolong_add(const T* __restrict__ xc, int Nc, int Mc, T* __restrict__ xf, int Nf, int Mf, int Kf) {
+__device__ inline void prolong_add_body(const T* __restrict__ xc, int Nc, int Mc, T* __restrict__ xf, int Nf, int Mf,
+                                        int Kf, long start, long stride) {
   const long nf = (long)Nf * Mf * Kf;
   const long NMc = (long)Nc * Mc;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nf; i += (long)gridDim.x * blockDim.x) {
+  for (long i = start; i < nf; i += stride) {
     int x, y, z;
     decode(i, Nf, Mf, &x, &y, &z);
     int cx[2], cy[2], cz[2], nx, ny, nz;
@@ -273,6 +292,76 @@ __global__ void k_prolong_add(const T* __restrict__ xc, int Nc, int Mc, T* __res
         for (int c1 = 0; c1 < nx; ++c1)
           acc += (T)(wx[c1] * wy[c2] * wz[c3]) * xc[cx[c1] + Nc * cy[c2] + NMc * cz[c3]];
     xf[i] += acc;
+  }
+}
+
+template <class T>
+__global__ void k_prolong_add(const T* __restrict__ xc, int Nc, int Mc, T* __restrict__ xf, int Nf, int Mf, int Kf) {
+  prolong_add_body<T>(xc, Nc, Mc, xf, Nf, Mf, Kf, blockIdx.x * (long)blockDim.x + threadIdx.x,
+                      (long)gridDim.x * blockDim.x);
+}
+
+// The small levels of the cycle in one workgroup: pre-smoothing and restriction down to the
+// coarsest level, its dense LU solve (FP64, thread 0, the factors of lu_factor), then
+// prolongation and post-smoothing back up; barriers separate the phases (one launch instead of
+// ~10 per level).  Level q of C is levels[first + q]; the last one is the LU level.
+template <class T, class FT>
+__global__ void __launch_bounds__(COARSE_THREADS) k_coarse_cycle(CoarseLevels<T> C, int n_pre, int n_post,
+                                                                const double* __restrict__ lu_scale,
+                                                                const int* __restrict__ lu_perm,
+                                                                const double* __restrict__ Lf,
+                                                                const double* __restrict__ Uf) {
+  const long tid = threadIdx.x, nth = blockDim.x;
+  const int last = C.count - 1;
+  for (int q = 0; q < last; ++q) {
+    const Level& L = C.L[q];
+    const FT* e = (const FT*)L.f_e;
+    const FT* nn = (const FT*)L.f_n;
+    const FT* t = (const FT*)L.f_t;
+    const T* s = (const T*)L.s;
+    for (int sw = 0; sw < n_pre; ++sw) {
+      rbgs_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, 0, L.pin0, L.pin1, sw == 0,
+                       tid, nth);
+      __syncthreads();
+      rbgs_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, 1, L.pin0, L.pin1, 0, tid, nth);
+      __syncthreads();
+    }
+    resid_restrict_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, L.pin0, L.pin1,
+                               C.b[q + 1], tid, nth);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const int n = (int)C.L[last].d.n();
+    const T* bb = C.b[last];
+    double bs[64], yv[64];
+    for (int i = 0; i < n; ++i) bs[i] = lu_scale[lu_perm[i]] * (double)bb[lu_perm[i]];
+    for (int i = 0; i < n; ++i) {
+      double aux = bs[i];
+      for (int k = 0; k < i; ++k) aux -= Lf[i * n + k] * yv[k];
+      yv[i] = aux;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double aux = yv[i];
+      for (int j = i + 1; j < n; ++j) aux -= Uf[i * n + j] * bs[j];
+      bs[i] = aux / Uf[i * n + i];
+    }
+    for (int i = 0; i < n; ++i) C.x[last][i] = (T)(lu_scale[i] * bs[i]);
+  }
+  __syncthreads();
+  for (int q = last - 1; q >= 0; --q) {
+    const Level& L = C.L[q];
+    const FT* e = (const FT*)L.f_e;
+    const FT* nn = (const FT*)L.f_n;
+    const FT* t = (const FT*)L.f_t;
+    const T* s = (const T*)L.s;
+    prolong_add_body<T>(C.x[q + 1], C.L[q + 1].d.N, C.L[q + 1].d.M, C.x[q], L.d.N, L.d.M, L.d.K, tid, nth);
+    __syncthreads();
+    for (int sw = 0; sw < n_post; ++sw) {
+      rbgs_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, 0, L.pin0, L.pin1, 0, tid, nth);
+      __syncthreads();
+      rbgs_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, 1, L.pin0, L.pin1, 0, tid, nth);
+      __syncthreads();
+    }
   }
 }
 
@@ -381,11 +470,27 @@ void resid_restrict<float>(const Level& L, const float* x, const float* b, float
 }
 
 template <class T>
+void coarse_cycle(const CoarseLevels<T>& C, int n_pre, int n_post, const double* lu_scale, const int* lu_perm,
+                  const double* L, const double* U, hipStream_t s) {
+  if (C.count < 2 || C.count > COARSE_MAX) return;
+  if (C.L[0].ftype == FACE_F16)
+    k_coarse_cycle<T, _Float16><<<1, COARSE_THREADS, 0, s>>>(C, n_pre, n_post, lu_scale, lu_perm, L, U);
+  else if (C.L[0].ftype == FACE_F32)
+    k_coarse_cycle<T, float><<<1, COARSE_THREADS, 0, s>>>(C, n_pre, n_post, lu_scale, lu_perm, L, U);
+  else
+    k_coarse_cycle<T, double><<<1, COARSE_THREADS, 0, s>>>(C, n_pre, n_post, lu_scale, lu_perm, L, U);
+}
+
+template <class T>
 void prolong_add(const T* xc, Dims dc, T* xf, Dims df, hipStream_t s) {
   k_prolong_add<T><<<grid_1d(df.n()), 256, 0, s>>>(xc, dc.N, dc.M, xf, df.N, df.M, df.K);
 }
 template void prolong_add<float>(const float*, Dims, float*, Dims, hipStream_t);
 template void prolong_add<double>(const double*, Dims, double*, Dims, hipStream_t);
+template void coarse_cycle<float>(const CoarseLevels<float>&, int, int, const double*, const int*, const double*,
+                                  const double*, hipStream_t);
+template void coarse_cycle<double>(const CoarseLevels<double>&, int, int, const double*, const int*, const double*,
+                                   const double*, hipStream_t);
 
 template <class A, class B>
 void convert(const A* a, B* b, long n, hipStream_t s) {

@@ -28,6 +28,19 @@ struct Level {
   int pin0 = -1, pin1 = -1;
 };
 
+// The small levels handled by one workgroup in a single launch (k_coarse_cycle): levels with at
+// most COARSE_CELLS cells, down to and including the LU level.
+constexpr int COARSE_MAX = 8;
+constexpr int COARSE_THREADS = 1024;
+constexpr long COARSE_CELLS = 32768;
+template <class T>
+struct CoarseLevels {
+  int count = 0;
+  Level L[COARSE_MAX];
+  T* x[COARSE_MAX] = {};
+  T* b[COARSE_MAX] = {};
+};
+
 // ---- setup (FP64 temporaries) --------------------------------------------------------------
 // faces and diagonal of the finest level from the FP64 7-plane operator [b,s,w,p,e,n,t]; a face
 // is read from the row of the non-pinned cell (pinned rows are identity rows)
@@ -66,6 +79,11 @@ template <class T>
 void prolong_add(const T* xc, Dims dc, T* xf, Dims df, hipStream_t s);
 template <class A, class B>
 void convert(const A* a, B* b, long n, hipStream_t s);
+// the small levels of the cycle in one launch: C.b[0] holds the right-hand side of the first of
+// them, C.x[0] receives its correction (lu_*: the dense LU factors of the last level)
+template <class T>
+void coarse_cycle(const CoarseLevels<T>& C, int n_pre, int n_post, const double* lu_scale, const int* lu_perm,
+                  const double* L, const double* U, hipStream_t s);
 
 template <>
 void rbgs<double>(const Level& L, double* x, const double* b, int color, int zero_guess, hipStream_t s);

@@ -67,6 +67,7 @@ class Options(C.Structure):
         ("krylov", C.c_int), ("restart", C.c_int), ("max_it", C.c_int), ("eps", C.c_double),
         ("mg", C.c_int), ("n_post", C.c_int), ("n_pre", C.c_int), ("omega", C.c_double),
         ("precision", C.c_int), ("levels", C.c_int), ("z_head_fix", C.c_int), ("ortho", C.c_int),
+        ("coarse_fuse", C.c_int),
     ]
 
 
@@ -280,8 +281,9 @@ class IterativeSolver:
 
     krylov = KRYLOV_FGMRES
 
-    def __init__(self, precision=PREC_FP64, mg=MG_CASCADE, z_head_fix=False, stream=None, ortho=0):
+    def __init__(self, precision=PREC_FP64, mg=MG_CASCADE, z_head_fix=False, stream=None, ortho=0, coarse_fuse=True):
         self.ortho = ortho
+        self.coarse_fuse = coarse_fuse
         self.precision = precision
         self.mg = mg
         self.z_head_fix = z_head_fix
@@ -310,6 +312,7 @@ class IterativeSolver:
         opt.mg = self.mg
         opt.z_head_fix = 1 if self.z_head_fix else 0
         opt.ortho = int(self.ortho)
+        opt.coarse_fuse = 1 if self.coarse_fuse else 0
         h = C.c_void_p()
         _check(lib().lsx_create(C.byref(h), C.byref(opt), self.stream))
         self.h = h
@@ -339,6 +342,7 @@ class IterativeSolver:
         opt.mg = self.mg
         opt.z_head_fix = 1 if self.z_head_fix else 0
         opt.ortho = int(self.ortho)
+        opt.coarse_fuse = 1 if self.coarse_fuse else 0
         h = C.c_void_p()
         _check(lib().lsx_create(C.byref(h), C.byref(opt), self.stream))
         self.h = h

@@ -106,3 +106,18 @@ def test_vcycle_orthogonalisations(lsx, ortho):
     print(f"ortho={ortho} it={it} true_rel={true_res / np.linalg.norm(b):.3e} norm_r={S.norm_r:.3e}", flush=True)
     assert true_res <= 1.05 * eps and it < 100
     assert abs(S.norm_r - true_res) <= 1e-3 * true_res
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64-vcycle"])
+def test_coarse_levels_in_one_launch_match_per_level_launches(lsx, precision):
+    """The one-workgroup cycle over the small levels performs the same per-cell operations in the
+    same order as the per-level launches: one preconditioner application is bit-identical."""
+    g, A = P.ellipsoid_stretched(64)
+    v = P.random_rhs(g, seed=12, A=A)
+    z = []
+    for fuse in (True, False):
+        S = lsx.solver("fgmres", precision, coarse_fuse=fuse)
+        S.set(A, np.zeros(A.n), np.zeros(A.n), lsx.mixed_task(1.0))
+        z.append(S.precondition(v))
+    print(f"precision={precision} max|diff|={np.abs(z[0] - z[1]).max():.3e}", flush=True)
+    assert np.array_equal(z[0], z[1])
