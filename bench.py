@@ -202,7 +202,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=256)
-    ap.add_argument("--precision", choices=["mixed", "fp64-vcycle", "fp64"], default="mixed")
+    ap.add_argument("--precision", choices=["mixed", "mixed16", "fp64-vcycle", "fp64"], default="mixed")
     ap.add_argument("--krylov", choices=["fgmres", "bicgstab"], default="fgmres")
     ap.add_argument("--restart", type=int, default=30)
     ap.add_argument("--body", choices=["bezier", "ellipsoid"], default="bezier")
@@ -238,7 +238,7 @@ def main():
     print(f"[bench] {args.precision}: {args.steps} solves in {main_run['elapsed']:.3f} s, its {main_run['its']}",
           file=sys.stderr, flush=True)
     fp64 = None
-    if args.precision == "mixed" and not args.no_fp64_compare:
+    if args.precision in ("mixed", "mixed16") and not args.no_fp64_compare:
         fp64 = run_solver(lsx, hip, geom, "fp64-vcycle", args.krylov, args.restart, max(1, args.steps // 2), 1,
                           smooth=smooth)
         print(f"[bench] fp64-vcycle: {len(fp64['its'])} solves in {fp64['elapsed']:.3f} s, its {fp64['its']}",
@@ -273,6 +273,7 @@ def main():
                      + f", {args.krylov.upper()}"
                      f"({args.restart}), rel tol 1e-8, "
                      + {"mixed": f"mixed: FP32 V{smooth} red-black GS multigrid, FP16 coarse faces, FP64 Krylov",
+                        "mixed16": f"mixed: FP32 V{smooth} red-black GS multigrid, FP16 faces, FP64 Krylov",
                         "fp64-vcycle": f"FP64 V{smooth} red-black GS multigrid",
                         "fp64": "FP64 reference multigrid cascade (ADLJ)"}[args.precision]),
         "precision": args.precision, "krylov": args.krylov, "cells": geom.n,
@@ -303,7 +304,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f16 (f64 Krylov)" if args.precision == "mixed" else "f64",
+        "dtype": "f32+f16 (f64 Krylov)" if args.precision in ("mixed", "mixed16") else "f64",
         "data": "synthetic",
         "config": cfg,
         "roofline": {"bound": "hbm", "kernel": lsx.KERNEL_NAMES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,

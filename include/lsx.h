@@ -40,6 +40,7 @@ extern "C" {
 #define LSX_PREC_FP64 0  /* everything in FP64 (cascade or V-cycle) */
 #define LSX_PREC_MIXED 1 /* V-cycle only: FP32 multigrid, FP16 coarse face coefficients, FP64
                             Krylov loop, operator and coarsest LU */
+#define LSX_PREC_MIXED16 2 /* as LSX_PREC_MIXED with FP16 faces on the finest level too */
 
 #define LSX_MEM_HOST 0
 #define LSX_MEM_DEVICE 1

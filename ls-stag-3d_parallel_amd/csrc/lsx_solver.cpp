@@ -27,9 +27,9 @@ Solver::Solver(const lsx_options& o, hipStream_t s) : opt(o) {
   if (opt.n_post < 0) throw Error(LSX_ERR_ARG, "n_post must be >= 0");
   if (opt.krylov != LSX_KRYLOV_FGMRES && opt.krylov != LSX_KRYLOV_BICGSTAB) throw Error(LSX_ERR_ARG, "unknown krylov");
   if (opt.mg != LSX_MG_CASCADE && opt.mg != LSX_MG_VCYCLE) throw Error(LSX_ERR_ARG, "unknown multigrid");
-  if (opt.precision == LSX_PREC_MIXED && opt.mg != LSX_MG_VCYCLE)
+  if (mixed() && opt.mg != LSX_MG_VCYCLE)
     throw Error(LSX_ERR_ARG, "mixed precision uses the V-cycle multigrid (LSX_MG_VCYCLE)");
-  if (opt.precision != LSX_PREC_FP64 && opt.precision != LSX_PREC_MIXED) throw Error(LSX_ERR_ARG, "unknown precision");
+  if (opt.precision != LSX_PREC_FP64 && !mixed()) throw Error(LSX_ERR_ARG, "unknown precision");
   if (opt.ortho < LSX_ORTHO_AUTO || opt.ortho > LSX_ORTHO_CGS2) throw Error(LSX_ERR_ARG, "unknown ortho");
   if (opt.mg == LSX_MG_VCYCLE && (opt.n_pre < 1 || opt.n_post < 1))
     throw Error(LSX_ERR_ARG, "the V-cycle needs n_pre >= 1 and n_post >= 1");
@@ -117,7 +117,7 @@ double Solver::kernel_bytes(int k) const {
       return (mixed() ? 44.0 : 48.0) * n;
     case LSX_K_ORTHO: return 32.0 * n;     // w read + write 16, v_i 8, v_i+1 8
     case LSX_K_SMOOTH:                     // n_pre red-black sweeps; per sweep x, three faces,
-      return (mixed() ? 28.0 : 56.0) * n * opt.n_pre;  // 1/a_p, b read and x written once
+      return (opt.precision == LSX_PREC_MIXED16 ? 22.0 : (mixed() ? 28.0 : 56.0)) * n * opt.n_pre;
     default: return 0.0;
   }
 }
@@ -386,7 +386,7 @@ void Solver::build_vcycle() {
       S.L.f_n = S.dn.p;
       S.L.f_t = S.dt.p;
       S.L.fscale = 1.0;
-    } else if (l == 0) {
+    } else if (l == 0 && opt.precision != LSX_PREC_MIXED16) {
       S.fe.alloc(nl);
       S.fn.alloc(nl);
       S.ft.alloc(nl);

@@ -148,7 +148,7 @@ class Solver {
   };
   std::vector<VStore> vlev;
   DevBuf<float> Z32, ztmp32;
-  bool mixed() const { return opt.precision == LSX_PREC_MIXED; }
+  bool mixed() const { return opt.precision == LSX_PREC_MIXED || opt.precision == LSX_PREC_MIXED16; }
   bool vcyc() const { return opt.mg == LSX_MG_VCYCLE; }
   void build_vcycle();
   template <class T>

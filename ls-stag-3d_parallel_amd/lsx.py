@@ -53,7 +53,7 @@ def random_divergence_rhs(diag, dims, seed=0, pin=True):
 
 KRYLOV_FGMRES, KRYLOV_BICGSTAB = 0, 1
 MG_CASCADE, MG_VCYCLE = 0, 1
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED16 = 0, 1, 2
 ORTHO_AUTO, ORTHO_MGS, ORTHO_CGS, ORTHO_CGS2 = 0, 1, 2, 3
 MEM_HOST, MEM_DEVICE = 0, 1
 
@@ -462,6 +462,8 @@ def solver(kind="fgmres", precision="fp64", **kw):
     cls = MGFGMRES if kind == "fgmres" else MGBiCGStab
     if precision == "mixed":
         return cls(precision=PREC_MIXED, mg=MG_VCYCLE, **kw)
+    if precision == "mixed16":
+        return cls(precision=PREC_MIXED16, mg=MG_VCYCLE, **kw)
     if precision == "fp64-vcycle":
         return cls(precision=PREC_FP64, mg=MG_VCYCLE, **kw)
     if precision != "fp64":

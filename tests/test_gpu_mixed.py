@@ -29,7 +29,7 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp64-vcycle"])
+@pytest.mark.parametrize("precision", ["mixed", "mixed16", "fp64-vcycle"])
 @pytest.mark.parametrize("kind", ["fgmres", "bicgstab"])
 @pytest.mark.parametrize("name,mk", CASES, ids=[c[0] for c in CASES])
 def test_vcycle_reaches_fp64_tolerance(lsx, name, mk, kind, precision):
