@@ -60,7 +60,7 @@ typedef struct lsx_options {
   int levels;        /* 0 = automatic level count; >0 = cap (V-cycle)        */
   int z_head_fix;    /* 0 = reproduce the reference z-line head read (default) */
   int ortho;         /* LSX_ORTHO_*: FGMRES orthogonalisation                    */
-  int coarse_fuse;   /* V-cycle: 1 = levels of <= 32768 cells in one launch (default), 0 = per level */
+  int coarse_fuse;   /* V-cycle: 1 = levels of <= 4096 cells in one launch (default), 0 = per level */
 } lsx_options;
 
 #define LSX_ORTHO_AUTO 0 /* MGS with the cascade (reference), CGS with the V-cycle           */

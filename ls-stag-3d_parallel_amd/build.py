@@ -31,7 +31,7 @@ SOURCES = {
     "lsx_solver.cpp": ["-ffp-contract=off"],
     "lsx_geometry.hip": ["-ffp-contract=off"],
     "lsx_geometry_host.cpp": ["-ffp-contract=off"],
-    "lsx_vcycle.hip": [],
+    "lsx_vcycle.hip": ["-ffp-contract=off"],
     "lsx_bezier.cpp": [],
     "lsx_capi.cpp": [],
 }

@@ -1,7 +1,7 @@
 // Geometric V-cycle preconditioner: red-black Gauss-Seidel on 7-point operators stored as face
 // coefficients, full-sum restriction, trilinear prolongation, FP64 dense LU on the coarsest
-// level.  It preconditions the FP64 flexible Krylov loop; nothing here restates reference
-// arithmetic, so ordinary FMA contraction is allowed.
+// level.  It preconditions the FP64 flexible Krylov loop.  Built without FMA contraction so a
+// cycle gives the same bits whichever kernels (per-level or fused) carry it out.
 //
 // Coarse levels are rediscretised: a coarse face is half the sum of the four fine faces it
 // covers (area x4, distance x2), the diagonal is the Neumann -sum of faces.  The finest level's
