@@ -130,9 +130,9 @@ def cpu_baseline(sample_n, gpu_n, restart, krylov, body="bezier"):
     }
 
 
-def run_solver(lsx, hip, geom, precision, krylov, restart, steps, warmup, max_it=1000, smooth=(2, 2)):
+def run_solver(lsx, hip, geom, precision, krylov, restart, steps, warmup, max_it=1000, smooth=(2, 2), graphs=False):
     """Set up one solver on the device geometry and time `steps` solves (x0 = 0 each)."""
-    S = lsx.solver(krylov, precision)
+    S = lsx.solver(krylov, precision, graphs=graphs) if precision != "fp64" else lsx.solver(krylov, precision)
     task = lsx.Task(eps=1.0, MAX_ITER=max_it, FGMRES_RESTART_NUM=restart) if precision == "fp64" else \
         lsx.mixed_task(1.0, restart=restart, max_it=max_it, n_pre=smooth[0], n_post=smooth[1])
     t_setup = time.perf_counter()
@@ -194,6 +194,24 @@ def copy_calibration(gib):
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "copies": 5}), flush=True)
 
 
+def projection_run(steps):
+    """Time-stepped projection: 384x256x256 uniform grid on [0,1.5]x[0,1]^2, sphere r = 0.1 at
+    (0.5, 0.5, 0.5), Re = 300, dt = 0.002, operator and hierarchy built once, V-cycles replayed
+    from captured HIP graphs, mixed solve per step with the FP64 V-cycle on the same right-hand
+    side, one adjoint solve of the final drag."""
+    load_lsx()
+    proj = sys.modules["lsx_pkg"].projection
+
+    def log(r):
+        print(f"[projection] step {r['step']}: it {r['iterations']} ({r['solve_s'] * 1e3:.1f} ms), fp64 it "
+              f"{r.get('fp64_iterations')} ({r.get('fp64_solve_s', 0) * 1e3:.1f} ms), divergence "
+              f"{r['divergence_before']:.3e} -> {r['divergence_after']:.3e}", file=sys.stderr, flush=True)
+
+    out = proj.run(steps=steps, graphs=True, log=log)
+    out["summary"] = proj.summary(out)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--copy-calibration", type=float, default=0.0,
@@ -212,9 +230,16 @@ def main():
                     help="iteration cap of the reference-cascade comparison solve (0 = skip it)")
     ap.add_argument("--cpu-sample-grid", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graphs", action="store_true", help="replay the V-cycles from captured HIP graphs")
+    ap.add_argument("--projection-steps", type=int, default=0,
+                    help="instead of the solve benchmark: this many time steps of the 384x256x256 sphere "
+                         "projection run (one JSON line of per-step results)")
     args = ap.parse_args()
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
+        return
+    if args.projection_steps > 0:
+        projection_run(args.projection_steps)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the solve runs on a single "
@@ -234,13 +259,13 @@ def main():
 
     smooth = tuple(int(q) for q in args.smooth.split(","))
     main_run = run_solver(lsx, hip, geom, args.precision, args.krylov, args.restart, args.steps, args.warmup,
-                          smooth=smooth)
+                          smooth=smooth, graphs=args.graphs)
     print(f"[bench] {args.precision}: {args.steps} solves in {main_run['elapsed']:.3f} s, its {main_run['its']}",
           file=sys.stderr, flush=True)
     fp64 = None
     if args.precision in ("mixed", "mixed16") and not args.no_fp64_compare:
         fp64 = run_solver(lsx, hip, geom, "fp64-vcycle", args.krylov, args.restart, max(1, args.steps // 2), 1,
-                          smooth=smooth)
+                          smooth=smooth, graphs=args.graphs)
         print(f"[bench] fp64-vcycle: {len(fp64['its'])} solves in {fp64['elapsed']:.3f} s, its {fp64['its']}",
               file=sys.stderr, flush=True)
     cascade = None

@@ -61,7 +61,13 @@ typedef struct lsx_options {
   int z_head_fix;    /* 0 = reproduce the reference z-line head read (default) */
   int ortho;         /* LSX_ORTHO_*: FGMRES orthogonalisation                    */
   int coarse_fuse;   /* V-cycle: 1 = levels of <= 4096 cells in one launch (default), 0 = per level */
+  int graphs;        /* V-cycle: 1 = replay each cycle from a captured HIP graph (one per basis slot) */
+  int smoother;      /* V-cycle: LSX_SMOOTH_RBGS (default) or LSX_SMOOTH_LINE                  */
 } lsx_options;
+
+#define LSX_SMOOTH_RBGS 0 /* red-black Gauss-Seidel                                             */
+#define LSX_SMOOTH_LINE 1 /* line relaxation: x, y and z lines solved exactly in turn (line     \
+                             Jacobi), for grids whose cells are long in some direction          */
 
 #define LSX_ORTHO_AUTO 0 /* MGS with the cascade (reference), CGS with the V-cycle           */
 #define LSX_ORTHO_MGS 1  /* modified Gram-Schmidt (fgmres_interface.h:97-104)               */
@@ -107,6 +113,23 @@ int lsx_precondition(lsx_solver* s, const double* v, double* z, int mem);
 /* y = A_level x on multigrid level `level` (0 = fine): replaces
  * RealValuedMeshFunction::setProduct (real_valued_mesh_function.h:81-88, 262-272). */
 int lsx_apply_operator(lsx_solver* s, int level, const double* x, double* y, int mem);
+
+/* ---- adjoint solves ------------------------------------------------------------------------
+ * The reference has no adjoint; these serve gradient computations through the pressure solve.
+ * lsx_solve_adjoint: lambda = A^{-T} g in place (lambda holds the initial guess), by the
+ * configured Krylov method (FGMRES or BiCGStab) on A^T, right-preconditioned by the transpose of
+ * the V-cycle, to the same absolute tolerance eps.  Needs LSX_MG_VCYCLE and a fine operator
+ * symmetric off its pinned rows (lsx_fine_storage == 4); LSX_ERR_STATE otherwise.
+ * lsx_precondition_transpose: z = (M^{-1})^T v for the V-cycle M^{-1} of lsx_precondition.
+ * lsx_apply_operator_transpose: y = A^T x on the fine level (a pinned row of A, an identity
+ * row, becomes a full row of A^T; the other rows drop their couplings to pinned cells). */
+int lsx_solve_adjoint(lsx_solver* s, const double* g, double* lambda, int mem, int* it, double* norm_r);
+int lsx_precondition_transpose(lsx_solver* s, const double* v, double* z, int mem);
+int lsx_apply_operator_transpose(lsx_solver* s, const double* x, double* y, int mem);
+
+/* `sweeps` sweeps of the V-cycle's smoother (opt.smoother) on the finest level, x in place;
+ * zero_guess: x is taken as 0.  Exposed for parity tests of the smoothers. */
+int lsx_smooth(lsx_solver* s, const double* b, double* x, int sweeps, int zero_guess, int mem);
 
 /* Dimensions of level `level`; returns its cell count or a negative status. */
 long lsx_level_dims(const lsx_solver* s, int level, int* N, int* M, int* K);
@@ -226,6 +249,44 @@ int lsx_geometry_correct(lsx_geometry* g, const double* dp, const double* us, co
  * (laplace_discrete_operator.h:752-768, calcLap :107-686, fixPoint :809-821) followed by
  * IterativeSolverInterface::set. */
 int lsx_set_geometry(lsx_solver* s, const lsx_geometry* g, int pin);
+
+/* ---- objective and shape gradient ----------------------------------------------------------
+ * The reference computes loads but has no adjoint; these entries have no counterpart there. */
+#define LSX_OBJ_NONE 0   /* J depends on the geometry only through p                           */
+#define LSX_OBJ_DRAG_X 1 /* pressure drag below: J also reads theta_x of the cut cells         */
+
+/* Pressure drag J = sum over cut cells of (theta_x[i-1] - theta_x[i]) dy dz p[i]: the pressure
+ * term of LoadsCalculator::calc (loads_calculator.h:97-116).  dJdp (may be NULL) gets dJ/dp. */
+int lsx_geometry_drag(lsx_geometry* g, const double* p, int mem, double* J, double* dJdp);
+
+/* Shape gradient of an objective at the solution p of A p = b, given lambda = A^{-T} dJ/dp:
+ * dJ/dphi per level-set corner (dJdphi: N*M*K values in `mem` space, may be NULL) and, for a
+ * Bezier body, dJ/dctrl (288 host doubles, may be NULL).  u == NULL: b does not depend on the
+ * geometry; otherwise b = lsx_geometry_rhs(u, v, w, u_west, u_out, bc) and its change with the
+ * face fractions enters.  pin: as in lsx_set_geometry.  h_rel: level-set step of the central
+ * differences of the cut-cell geometry, as a fraction of the smallest cell width (1e-7 is a
+ * good default). */
+int lsx_geometry_shape_gradient(lsx_geometry* g, const double* p, const double* lambda, int objective,
+                                const double* u, const double* v, const double* w, const double* u_west,
+                                const double* u_out, const lsx_flow_bc* bc, int pin, double h_rel, int mem,
+                                double* dJdphi, double* dJdctrl);
+
+/* Forward/backward pair for automatic differentiation through the pressure solve.
+ * lsx_adjoint_forward: set the solver's operator from `g` (unless it already is), solve
+ * A p = b in place like lsx_solve, and record p, g and its build on a new tape.
+ * lsx_adjoint_backward: dJdb = A^{-T} dJdp (the adjoint solve, initial guess 0) and, when
+ * dJdctrl is not NULL, dJ/dctrl through the geometry (objective, flow fields and h_rel as in
+ * lsx_geometry_shape_gradient).  When the solver's operator was set from another geometry in
+ * between, the backward call rebuilds the operator and the preconditioner hierarchy from the
+ * taped geometry instead of having kept them.  LSX_ERR_STATE when the geometry was rebuilt
+ * after the forward call. */
+typedef struct lsx_tape lsx_tape;
+int lsx_adjoint_forward(lsx_solver* s, lsx_geometry* g, int pin, const double* b, double* p, int mem, lsx_tape** tape,
+                        int* it, double* norm_r);
+int lsx_adjoint_backward(lsx_tape* t, const double* dJdp, int objective, const double* u, const double* v,
+                         const double* w, const double* u_west, const double* u_out, const lsx_flow_bc* bc,
+                         double h_rel, int mem, double* dJdb, double* dJdctrl, int* it, double* norm_r);
+int lsx_tape_destroy(lsx_tape* t);
 
 /* Thread-local message of the last failure ("" if none). */
 const char* lsx_last_error(void);

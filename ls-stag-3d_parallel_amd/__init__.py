@@ -4,3 +4,4 @@ The package directory name is not a Python identifier; load it with ``load_packa
 ``__graft_entry__`` or ``importlib`` (``tests/conftest.py`` shows how).
 """
 from . import lsx  # noqa: F401
+from . import projection  # noqa: F401

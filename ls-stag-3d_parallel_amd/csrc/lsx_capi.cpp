@@ -21,6 +21,14 @@ struct lsx_geometry {
       : impl(N, M, K, dx, dy, dz, x0, y0, z0, s) {}
 };
 
+struct lsx_tape {
+  lsx_solver* s;
+  lsx_geometry* g;
+  long version;
+  int pin;
+  lsx::DevBuf<double> p;  // the forward solution
+};
+
 namespace {
 thread_local std::string g_last_error;
 
@@ -60,6 +68,8 @@ void lsx_options_default(lsx_options* o) {
   o->z_head_fix = 0;
   o->ortho = LSX_ORTHO_AUTO;
   o->coarse_fuse = 1;
+  o->graphs = 0;
+  o->smoother = LSX_SMOOTH_RBGS;
 }
 
 int lsx_create(lsx_solver** out, const lsx_options* opt, void* stream) {
@@ -122,6 +132,27 @@ int lsx_precondition(lsx_solver* s, const double* v, double* z, int mem) {
 int lsx_apply_operator(lsx_solver* s, int level, const double* x, double* y, int mem) {
   if (!s) return LSX_ERR_ARG;
   return guarded([&] { s->impl.apply_operator(level, x, y, mem); });
+}
+
+int lsx_solve_adjoint(lsx_solver* s, const double* g, double* lambda, int mem, int* it, double* norm_r) {
+  if (!s) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.solve_adjoint(g, lambda, mem, it, norm_r); });
+}
+
+int lsx_precondition_transpose(lsx_solver* s, const double* v, double* z, int mem) {
+  if (!s) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.precondition_transpose(v, z, mem); });
+}
+
+int lsx_apply_operator_transpose(lsx_solver* s, const double* x, double* y, int mem) {
+  if (!s) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.apply_operator_transpose(x, y, mem); });
+}
+
+int lsx_smooth(lsx_solver* s, const double* b, double* x, int sweeps, int zero_guess, int mem) {
+  if (!s) return LSX_ERR_ARG;
+  if (mem != LSX_MEM_HOST && mem != LSX_MEM_DEVICE) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.smooth_fine(b, x, sweeps, zero_guess, mem); });
 }
 
 long lsx_level_dims(const lsx_solver* s, int level, int* N, int* M, int* K) {
@@ -217,6 +248,87 @@ int lsx_kernel_timing(lsx_solver* s, int enable) {
 int lsx_kernel_stats(lsx_solver* s, int kernel, long* launches, double* total_ms, double* bytes_per_launch) {
   if (!s) return LSX_ERR_ARG;
   return guarded([&] { s->impl.kernel_stats(kernel, launches, total_ms, bytes_per_launch); });
+}
+
+int lsx_geometry_drag(lsx_geometry* g, const double* p, int mem, double* J, double* dJdp) {
+  if (!g || !p || !J) return LSX_ERR_ARG;
+  if (mem != LSX_MEM_HOST && mem != LSX_MEM_DEVICE) return LSX_ERR_ARG;
+  return guarded([&] { *J = g->impl.drag(p, dJdp, mem); });
+}
+
+int lsx_geometry_shape_gradient(lsx_geometry* g, const double* p, const double* lambda, int objective,
+                                const double* u, const double* v, const double* w, const double* u_west,
+                                const double* u_out, const lsx_flow_bc* bc, int pin, double h_rel, int mem,
+                                double* dJdphi, double* dJdctrl) {
+  if (!g || !p || !lambda) return LSX_ERR_ARG;
+  if (mem != LSX_MEM_HOST && mem != LSX_MEM_DEVICE) return LSX_ERR_ARG;
+  if (u && !bc) return LSX_ERR_ARG;
+  return guarded([&] {
+    lsx::FlowStage st;
+    if (u) g->impl.stage_flow(u, v, w, u_west, u_out, *bc, mem, st);
+    g->impl.shape_gradient(p, lambda, objective, u ? &st.args : nullptr, pin, h_rel, dJdphi, dJdctrl, mem);
+  });
+}
+
+int lsx_adjoint_forward(lsx_solver* s, lsx_geometry* g, int pin, const double* b, double* p, int mem, lsx_tape** tape,
+                        int* it, double* norm_r) {
+  if (!s || !g || !b || !p || !tape) return LSX_ERR_ARG;
+  if (mem != LSX_MEM_HOST && mem != LSX_MEM_DEVICE) return LSX_ERR_ARG;
+  *tape = nullptr;
+  return guarded([&] {
+    if (!s->impl.operator_from(&g->impl, g->impl.version(), pin)) s->impl.set_geometry(g->impl, pin);
+    s->impl.solve(b, p, mem, it, norm_r);
+    lsx_tape* t = new lsx_tape{s, g, g->impl.version(), pin, {}};
+    const long n = g->impl.n();
+    try {
+      t->p.alloc(n);
+      LSX_HIP(hipMemcpy(t->p.p, p, n * sizeof(double),
+                        mem == LSX_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice));
+    } catch (...) {
+      delete t;
+      throw;
+    }
+    *tape = t;
+  });
+}
+
+int lsx_adjoint_backward(lsx_tape* t, const double* dJdp, int objective, const double* u, const double* v,
+                         const double* w, const double* u_west, const double* u_out, const lsx_flow_bc* bc,
+                         double h_rel, int mem, double* dJdb, double* dJdctrl, int* it, double* norm_r) {
+  if (!t || !dJdp || !dJdb) return LSX_ERR_ARG;
+  if (mem != LSX_MEM_HOST && mem != LSX_MEM_DEVICE) return LSX_ERR_ARG;
+  if (u && !bc) return LSX_ERR_ARG;
+  return guarded([&] {
+    lsx::Geometry& G = t->g->impl;
+    lsx::Solver& S = t->s->impl;
+    if (G.version() != t->version)
+      throw lsx::Error(LSX_ERR_STATE, "the geometry was rebuilt after the forward call");
+    // recompute the operator and its multigrid hierarchy when they were replaced meanwhile
+    if (!S.operator_from(&G, t->version, t->pin)) S.set_geometry(G, t->pin);
+    const long n = G.n();
+    if (mem == LSX_MEM_HOST)
+      for (long i = 0; i < n; ++i) dJdb[i] = 0.0;
+    else
+      LSX_HIP(hipMemset(dJdb, 0, n * sizeof(double)));
+    S.solve_adjoint(dJdp, dJdb, mem, it, norm_r);
+    if (dJdctrl) {
+      lsx::FlowStage st;
+      if (u) G.stage_flow(u, v, w, u_west, u_out, *bc, mem, st);
+      lsx::DevBuf<double> lam;
+      const double* dl = dJdb;
+      if (mem == LSX_MEM_HOST) {
+        lam.alloc(n);
+        LSX_HIP(hipMemcpy(lam.p, dJdb, n * sizeof(double), hipMemcpyHostToDevice));
+        dl = lam.p;
+      }
+      G.shape_gradient(t->p.p, dl, objective, u ? &st.args : nullptr, t->pin, h_rel, nullptr, dJdctrl,
+                       LSX_MEM_DEVICE);
+    }
+  });
+}
+
+int lsx_tape_destroy(lsx_tape* t) {
+  return guarded([&] { delete t; });
 }
 
 int lsx_set_geometry(lsx_solver* s, const lsx_geometry* g, int pin) {

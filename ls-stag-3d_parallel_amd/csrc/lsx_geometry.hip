@@ -1,16 +1,18 @@
 // Cut-cell geometry of the pressure grid and the pressure operator's coefficients, on the GPU.
 //
-// One thread per node/cell for every stage except the level-set oscillation fix, whose outcome
-// depends on the order nodes are visited (reference ls_stag_uniform_mesh.h:2906-2916 inside
-// the serial sweep flow_task_description.h:102-137).  A node can only change there if its
-// level set is positive and one of its upper neighbours is negative, so a flag pass marks
-// those few nodes and a single workgroup replays the checks for them in index order; every
-// other node keeps its value and all later stages read final values only.
+// One thread per node/cell.  The level-set oscillation fix depends on the order nodes are
+// visited (reference ls_stag_uniform_mesh.h:2906-2916 inside the serial sweep
+// flow_task_description.h:102-137).  A node can only change there if its level set is positive
+// and one of its upper neighbours is negative, so a flag pass marks those few nodes and
+// repeated parallel passes over them reach the ordered sweep's result (k_osc_iter); every other
+// node keeps its value and all later stages read final values only.
 //
 // Built with -ffp-contract=off: every formula is evaluated with the operation order of the
 // reference expressions it restates.
 
 #include <hip/hip_runtime.h>
+
+#include <cmath>
 
 #include "lsx_geometry.hpp"
 
@@ -96,8 +98,14 @@ __device__ inline void bz_eval(const double* __restrict__ P, double u, double v,
 __device__ inline double bz_dot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 __device__ inline double bz_clamp(double t) { return t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t); }
 
-__device__ double bezier_phi(const double* __restrict__ ctrl, const double* c0, const double* a0, double x, double y,
-                             double z) {
+// closest point of the winning patch: phi = sgn * |X - S(u, v)| with r = X - S
+struct BzHit {
+  int f;
+  double u, v, r[3], dist, sgn;
+};
+
+__device__ double bezier_closest(const double* __restrict__ ctrl, const double* c0, const double* a0, double x,
+                                 double y, double z, BzHit& hit) {
   const double X[3] = {x, y, z};
   double d[3];
   for (int c = 0; c < 3; ++c) d[c] = (X[c] - c0[c]) / a0[c];
@@ -150,10 +158,22 @@ __device__ double bezier_phi(const double* __restrict__ ctrl, const double* c0, 
       const double nrm[3] = {e.Su[1] * e.Sv[2] - e.Su[2] * e.Sv[1], e.Su[2] * e.Sv[0] - e.Su[0] * e.Sv[2],
                              e.Su[0] * e.Sv[1] - e.Su[1] * e.Sv[0]};
       side = bz_dot(r, nrm);
+      hit.f = f;
+      hit.u = u;
+      hit.v = v;
+      for (int c = 0; c < 3; ++c) hit.r[c] = r[c];
     }
   }
   const double dist = sqrt(best);
+  hit.dist = dist;
+  hit.sgn = side > 0.0 ? -1.0 : 1.0;
   return side > 0.0 ? -dist : dist;
+}
+
+__device__ inline double bezier_phi(const double* __restrict__ ctrl, const double* c0, const double* a0, double x,
+                                    double y, double z) {
+  BzHit hit;
+  return bezier_closest(ctrl, c0, a0, x, y, z, hit);
 }
 
 // ---- level set -------------------------------------------------------------------------------
@@ -195,35 +215,28 @@ __global__ void k_osc_flags(const double* __restrict__ phi, int N, int M, int K,
   }
 }
 
-__device__ inline void check_osc(double* phi, long i, double& f, double lower, double upper) {
-  if ((f > EQ_DOUBLE) && (lower < 0) && (upper < 0)) {
-    f = 0.5 * (lower + upper);
-    phi[i] = f;
-  }
+__device__ inline void check_osc(double& f, double lower, double upper) {
+  if ((f > EQ_DOUBLE) && (lower < 0) && (upper < 0)) f = 0.5 * (lower + upper);
 }
 
-// One workgroup walks the flags in index order; thread 0 replays the checks of flagged nodes.
-__global__ void k_osc_replay(double* __restrict__ phi, const unsigned char* __restrict__ flag, int N, int M, int K) {
+// The reference visits nodes in index order and updates phi in place, so a flagged node reads
+// the final values of its lower neighbours (j-1, j-N, j-NM: visited before it) and the original
+// values of its upper ones.  One pass recomputes every flagged node from the current lower
+// values (`cur`) and the original upper values (`phi0`) into `nxt`; node j depends only on nodes
+// below it, so repeating the pass until nothing changes reproduces the ordered sweep exactly.
+__global__ void k_osc_iter(const double* __restrict__ phi0, const unsigned char* __restrict__ flag,
+                           const double* __restrict__ cur, double* __restrict__ nxt, int N, int M, int K,
+                           int* __restrict__ changed) {
   const long n = (long)N * M * K, NM = (long)N * M;
-  __shared__ int any;
-  for (long base = 0; base < n; base += blockDim.x) {
-    if (threadIdx.x == 0) any = 0;
-    __syncthreads();
-    const long i = base + threadIdx.x;
-    if (i < n && flag[i]) atomicOr(&any, 1);
-    __syncthreads();
-    if (any && threadIdx.x == 0) {
-      const long end = base + blockDim.x < n ? base + blockDim.x : n;
-      for (long j = base; j < end; ++j) {
-        if (!flag[j]) continue;
-        const int z = (int)(j / NM);
-        double f = phi[j];
-        check_osc(phi, j, f, phi[j - 1], phi[j + 1]);
-        check_osc(phi, j, f, phi[j - N], phi[j + N]);
-        if (z > 0 && z < K - 1) check_osc(phi, j, f, phi[j - NM], phi[j + NM]);
-      }
-    }
-    __syncthreads();
+  for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+    if (!flag[j]) continue;
+    const int z = (int)(j / NM);
+    double f = phi0[j];
+    check_osc(f, cur[j - 1], phi0[j + 1]);
+    check_osc(f, cur[j - N], phi0[j + N]);
+    if (z > 0 && z < K - 1) check_osc(f, cur[j - NM], phi0[j + NM]);
+    if (f != cur[j]) *changed = 1;
+    nxt[j] = f;
   }
 }
 
@@ -589,73 +602,84 @@ __device__ inline bool no_n(int t) { return t == N_Border || t == TN_Edge || t =
 __device__ inline bool no_w(int t) { return t == W_Border || t == TW_Edge || t == BW_Edge || t == NW_Edge || t == SW_Edge || t == TNW_Corner || t == TSW_Corner || t == BNW_Corner || t == BSW_Corner; }
 __device__ inline bool no_e(int t) { return t == E_Border || t == TE_Edge || t == BE_Edge || t == NE_Edge || t == SE_Edge || t == TNE_Corner || t == TSE_Corner || t == BNE_Corner || t == BSE_Corner; }
 
-// Pressure operator row of each cell as 7 coefficient planes [b,s,w,p,e,n,t] (absent = 0):
-// the values of LaplaceDiscreteOperator::calcLap (laplace_discrete_operator.h:107-686) with the
+// Pressure operator row of cell i as 7 coefficients c = [b,s,w,p,e,n,t] (absent = 0): the
+// values of LaplaceDiscreteOperator::calcLap (laplace_discrete_operator.h:107-686) with the
 // staggered masses M = (V_i + V_nb)/2 inlined, and identity rows at the two pinned cells.
+__device__ inline void laplace_row(long i, int x, int y, int z, const int* __restrict__ type,
+                                   const double* __restrict__ tx, const double* __restrict__ ty,
+                                   const double* __restrict__ tz, const double* __restrict__ vof,
+                                   const double* __restrict__ dxs, const double* __restrict__ dys,
+                                   const double* __restrict__ dzs, int N, int M, int pin, double* c) {
+  const long NM = (long)N * M;
+  const int t = type[i];
+  double a_b = 0, a_s = 0, a_w = 0, a_p = 0, a_e = 0, a_n = 0, a_t = 0;
+  if (t == SolidCell) {
+    a_p = PLUS_TINY;
+  } else {
+    const double v = vof[i];
+    const double ddx = dxs[x], ddy = dys[y], ddz = dzs[z];
+    const bool hb = !no_b(t), hs = !no_s(t), hw = !no_w(t), he = !no_e(t), hn = !no_n(t), ht = !no_t(t);
+    // masses of the six faces (only the present ones are read)
+    const double imw = hw ? inv_mass(0.5 * (vof[i - 1] + v)) : 0.0;
+    const double ime = he ? inv_mass(0.5 * (v + vof[i + 1])) : 0.0;
+    const double ims = hs ? inv_mass(0.5 * (vof[i - N] + v)) : 0.0;
+    const double imn = hn ? inv_mass(0.5 * (v + vof[i + N])) : 0.0;
+    const double imb = hb ? inv_mass(0.5 * (vof[i - NM] + v)) : 0.0;
+    const double imt = ht ? inv_mass(0.5 * (v + vof[i + NM])) : 0.0;
+    if (t == FluidCell) {
+      const double dyz = ddy * ddz, dxz = ddx * ddz, dxy = ddy * ddx;
+      double aux = dyz * dyz;
+      a_w = aux * imw;
+      a_e = aux * ime;
+      aux = dxz * dxz;
+      a_s = aux * ims;
+      a_n = aux * imn;
+      aux = dxy * dxy;
+      a_b = aux * imb;
+      a_t = aux * imt;
+      a_p = -a_b - a_s - a_w - a_e - a_n - a_t;
+    } else {
+      const double dyz = ddy * ddz, dxz = ddx * ddz, dxy = ddx * ddy;
+      double aux;
+      if (hw) { aux = tx[i - 1] * dyz; a_w = aux * aux * imw; }
+      if (he) { aux = tx[i] * dyz; a_e = aux * aux * ime; }
+      if (hs) { aux = ty[i - N] * dxz; a_s = aux * aux * ims; }
+      if (hn) { aux = ty[i] * dxz; a_n = aux * aux * imn; }
+      if (hb) { aux = tz[i - NM] * dxy; a_b = aux * aux * imb; }
+      if (ht) { aux = tz[i] * dxy; a_t = aux * aux * imt; }
+      bool first = true;
+      if (hb) { a_p = -a_b; first = false; }
+      if (hs) { a_p = first ? -a_s : a_p - a_s; first = false; }
+      if (hw) { a_p = first ? -a_w : a_p - a_w; first = false; }
+      if (he) { a_p = first ? -a_e : a_p - a_e; first = false; }
+      if (hn) { a_p = first ? -a_n : a_p - a_n; first = false; }
+      if (ht) { a_p = first ? -a_t : a_p - a_t; first = false; }
+    }
+    if (pin && (i == NM - 1 || i == N - 1)) {
+      a_b = a_s = a_w = a_e = a_n = a_t = 0.0;
+      a_p = 1.0;
+    }
+  }
+  c[0] = a_b;
+  c[1] = a_s;
+  c[2] = a_w;
+  c[3] = a_p;
+  c[4] = a_e;
+  c[5] = a_n;
+  c[6] = a_t;
+}
+
 __global__ void k_laplace7(const int* __restrict__ type, const double* __restrict__ tx, const double* __restrict__ ty,
                            const double* __restrict__ tz, const double* __restrict__ vof,
                            const double* __restrict__ dxs, const double* __restrict__ dys,
                            const double* __restrict__ dzs, int N, int M, int K, int pin, double* __restrict__ coef) {
-  const long n = (long)N * M * K, NM = (long)N * M;
+  const long n = (long)N * M * K;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     int x, y, z;
     decode(i, N, M, &x, &y, &z);
-    const int t = type[i];
-    double a_b = 0, a_s = 0, a_w = 0, a_p = 0, a_e = 0, a_n = 0, a_t = 0;
-    if (t == SolidCell) {
-      a_p = PLUS_TINY;
-    } else {
-      const double v = vof[i];
-      const double ddx = dxs[x], ddy = dys[y], ddz = dzs[z];
-      const bool hb = !no_b(t), hs = !no_s(t), hw = !no_w(t), he = !no_e(t), hn = !no_n(t), ht = !no_t(t);
-      // masses of the six faces (only the present ones are read)
-      const double imw = hw ? inv_mass(0.5 * (vof[i - 1] + v)) : 0.0;
-      const double ime = he ? inv_mass(0.5 * (v + vof[i + 1])) : 0.0;
-      const double ims = hs ? inv_mass(0.5 * (vof[i - N] + v)) : 0.0;
-      const double imn = hn ? inv_mass(0.5 * (v + vof[i + N])) : 0.0;
-      const double imb = hb ? inv_mass(0.5 * (vof[i - NM] + v)) : 0.0;
-      const double imt = ht ? inv_mass(0.5 * (v + vof[i + NM])) : 0.0;
-      if (t == FluidCell) {
-        const double dyz = ddy * ddz, dxz = ddx * ddz, dxy = ddy * ddx;
-        double aux = dyz * dyz;
-        a_w = aux * imw;
-        a_e = aux * ime;
-        aux = dxz * dxz;
-        a_s = aux * ims;
-        a_n = aux * imn;
-        aux = dxy * dxy;
-        a_b = aux * imb;
-        a_t = aux * imt;
-        a_p = -a_b - a_s - a_w - a_e - a_n - a_t;
-      } else {
-        const double dyz = ddy * ddz, dxz = ddx * ddz, dxy = ddx * ddy;
-        double aux;
-        if (hw) { aux = tx[i - 1] * dyz; a_w = aux * aux * imw; }
-        if (he) { aux = tx[i] * dyz; a_e = aux * aux * ime; }
-        if (hs) { aux = ty[i - N] * dxz; a_s = aux * aux * ims; }
-        if (hn) { aux = ty[i] * dxz; a_n = aux * aux * imn; }
-        if (hb) { aux = tz[i - NM] * dxy; a_b = aux * aux * imb; }
-        if (ht) { aux = tz[i] * dxy; a_t = aux * aux * imt; }
-        bool first = true;
-        if (hb) { a_p = -a_b; first = false; }
-        if (hs) { a_p = first ? -a_s : a_p - a_s; first = false; }
-        if (hw) { a_p = first ? -a_w : a_p - a_w; first = false; }
-        if (he) { a_p = first ? -a_e : a_p - a_e; first = false; }
-        if (hn) { a_p = first ? -a_n : a_p - a_n; first = false; }
-        if (ht) { a_p = first ? -a_t : a_p - a_t; first = false; }
-      }
-      if (pin && (i == NM - 1 || i == N - 1)) {
-        a_b = a_s = a_w = a_e = a_n = a_t = 0.0;
-        a_p = 1.0;
-      }
-    }
-    coef[0 * n + i] = a_b;
-    coef[1 * n + i] = a_s;
-    coef[2 * n + i] = a_w;
-    coef[3 * n + i] = a_p;
-    coef[4 * n + i] = a_e;
-    coef[5 * n + i] = a_n;
-    coef[6 * n + i] = a_t;
+    double c[7];
+    laplace_row(i, x, y, z, type, tx, ty, tz, vof, dxs, dys, dzs, N, M, pin, c);
+    for (int q = 0; q < 7; ++q) coef[q * n + i] = c[q];
   }
 }
 
@@ -674,9 +698,25 @@ void level_set(const double* cx, const double* cy, const double* cz, int N, int 
   k_level_set<<<grid_1d((long)N * M * K), 256, 0, s>>>(cx, cy, cz, N, M, K, box, body, phi);
 }
 
-void oscillation_fix(double* phi, unsigned char* flag, int N, int M, int K, Box box, hipStream_t s) {
-  k_osc_flags<<<grid_1d((long)N * M * K), 256, 0, s>>>(phi, N, M, K, box, flag);
-  k_osc_replay<<<1, 1024, 0, s>>>(phi, flag, N, M, K);
+int oscillation_fix(const double* phi0, double* phi, double* tmp, unsigned char* flag, int* changed, int N, int M,
+                    int K, Box box, hipStream_t s) {
+  const long n = (long)N * M * K;
+  k_osc_flags<<<grid_1d(n), 256, 0, s>>>(phi0, N, M, K, box, flag);
+  if (hipMemcpyAsync(phi, phi0, n * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess) return -1;
+  if (hipMemcpyAsync(tmp, phi0, n * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess) return -1;
+  double *cur = phi, *nxt = tmp;
+  for (int pass = 1;; ++pass) {
+    int h = 0;
+    if (hipMemsetAsync(changed, 0, sizeof(int), s) != hipSuccess) return -1;
+    k_osc_iter<<<grid_1d(n), 256, 0, s>>>(phi0, flag, cur, nxt, N, M, K, changed);
+    if (hipMemcpyAsync(&h, changed, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+    double* t = cur;
+    cur = nxt;
+    nxt = t;
+    // an unchanged pass leaves both buffers equal: phi holds the result
+    if (!h) return pass;
+  }
 }
 
 void alpha(const double* phi, int N, int M, int K, Box box, double* ax, double* ay, double* az, hipStream_t s) {
@@ -745,71 +785,87 @@ __device__ inline bool z_top(int t) {
          t == TNW_Corner || t == TSE_Corner || t == TSW_Corner;
 }
 
+struct RhsIn {
+  const double *u, *v, *w, *u_west, *u_out;
+  FlowBc bc;
+};
+
+__device__ inline double rhs_cell(long i, int x, int y, int z, const int* __restrict__ type,
+                                  const double* __restrict__ tx, const double* __restrict__ ty,
+                                  const double* __restrict__ tz, const double* __restrict__ dxa,
+                                  const double* __restrict__ dya, const double* __restrict__ dza, int N, int M, int K,
+                                  const RhsIn& in) {
+  const long NM = (long)N * M;
+  const double* __restrict__ u = in.u;
+  const double* __restrict__ v = in.v;
+  const double* __restrict__ w = in.w;
+  const FlowBc& bc = in.bc;
+  const int t = type[i];
+  const double dx = dxa[x], dy = dya[y], dz = dza[z];
+  const long row = y + (long)z * M;
+  // flux: initFlux, addFluxNSp, addFluxTBp, setBaseFlux (static body: ibFlux = 0)
+  double flux = 0.0;
+  if (x == 0) flux = -in.u_west[row] * (dy * dz);
+  if (x == N - 1) flux = bc.U_E * (dy * dz);
+  if (y == 0) flux -= (dx * dz) * bc.V_S;
+  if (y == M - 1) flux += (dx * dz) * bc.V_N;
+  if (z == 0) flux -= (dy * dx) * tz[i] * bc.W_B;
+  if (z == K - 1) flux += (dy * dx) * tz[i] * bc.W_T;
+  if (in.u_out && x == N - 1) flux = in.u_out[row * (N - 1) + N - 2] * dy * dz;
+  // Dx u
+  double aux = 0.0;
+  {
+    const long p = row * (N - 1) + x;
+    if (t == SolidCell) aux += 0.0 * u[x == 0 ? p : p - 1];
+    else if (x_east(t)) aux += (-tx[i - 1] * dy * dz) * u[p - 1];
+    else if (x_west(t)) aux += (tx[i] * dy * dz) * u[p];
+    else {
+      const double dyz = dy * dz;
+      aux += (-tx[i - 1] * dyz) * u[p - 1];
+      aux += (tx[i] * dyz) * u[p];
+    }
+  }
+  // (aux + flux) + Dy v
+  double r = aux + flux;
+  {
+    const long p = (long)z * N * (M - 1) + (long)y * N + x;
+    if (t == SolidCell) r += 0.0 * v[y == 0 ? p : p - N];
+    else if (y_north(t)) r += (-ty[i - N] * dx * dz) * v[p - N];
+    else if (y_south(t)) r += (ty[i] * dx * dz) * v[p];
+    else {
+      const double dxz = dx * dz;
+      r += (-ty[i - N] * dxz) * v[p - N];
+      r += (ty[i] * dxz) * v[p];
+    }
+  }
+  // + Dz w
+  double auxz = 0.0;
+  {
+    if (t == SolidCell) auxz += 0.0 * w[z == 0 ? i : i - NM];
+    else if (z_bottom(t)) auxz += (tz[i] * dx * dy) * w[i];
+    else if (z_top(t)) auxz += (-tz[i - NM] * dx * dy) * w[i - NM];
+    else {
+      const double dxy = dx * dy;
+      auxz += (-tz[i - NM] * dxy) * w[i - NM];
+      auxz += (tz[i] * dxy) * w[i];
+    }
+  }
+  r += auxz;
+  if (bc.pin && (i == N - 1 || i == NM - 1)) r = 0.0;
+  return r;
+}
+
 __global__ void k_pressure_rhs(const int* __restrict__ type, const double* __restrict__ tx,
                                const double* __restrict__ ty, const double* __restrict__ tz,
                                const double* __restrict__ dxa, const double* __restrict__ dya,
-                               const double* __restrict__ dza, int N, int M, int K, const double* __restrict__ u,
-                               const double* __restrict__ v, const double* __restrict__ w,
-                               const double* __restrict__ u_west, const double* __restrict__ u_out, FlowBc bc,
+                               const double* __restrict__ dza, int N, int M, int K, RhsIn in,
                                double* __restrict__ b) {
   const long n = (long)N * M * K, NM = (long)N * M;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const int z = (int)(i / NM);
     const long r2 = i - (long)z * NM;
     const int y = (int)(r2 / N), x = (int)(r2 - (long)y * N);
-    const int t = type[i];
-    const double dx = dxa[x], dy = dya[y], dz = dza[z];
-    const long row = y + (long)z * M;
-    // flux: initFlux, addFluxNSp, addFluxTBp, setBaseFlux (static body: ibFlux = 0)
-    double flux = 0.0;
-    if (x == 0) flux = -u_west[row] * (dy * dz);
-    if (x == N - 1) flux = bc.U_E * (dy * dz);
-    if (y == 0) flux -= (dx * dz) * bc.V_S;
-    if (y == M - 1) flux += (dx * dz) * bc.V_N;
-    if (z == 0) flux -= (dy * dx) * tz[i] * bc.W_B;
-    if (z == K - 1) flux += (dy * dx) * tz[i] * bc.W_T;
-    if (u_out && x == N - 1) flux = u_out[row * (N - 1) + N - 2] * dy * dz;
-    // Dx u
-    double aux = 0.0;
-    {
-      const long p = row * (N - 1) + x;
-      if (t == SolidCell) aux += 0.0 * u[x == 0 ? p : p - 1];
-      else if (x_east(t)) aux += (-tx[i - 1] * dy * dz) * u[p - 1];
-      else if (x_west(t)) aux += (tx[i] * dy * dz) * u[p];
-      else {
-        const double dyz = dy * dz;
-        aux += (-tx[i - 1] * dyz) * u[p - 1];
-        aux += (tx[i] * dyz) * u[p];
-      }
-    }
-    // (aux + flux) + Dy v
-    double r = aux + flux;
-    {
-      const long p = (long)z * N * (M - 1) + (long)y * N + x;
-      if (t == SolidCell) r += 0.0 * v[y == 0 ? p : p - N];
-      else if (y_north(t)) r += (-ty[i - N] * dx * dz) * v[p - N];
-      else if (y_south(t)) r += (ty[i] * dx * dz) * v[p];
-      else {
-        const double dxz = dx * dz;
-        r += (-ty[i - N] * dxz) * v[p - N];
-        r += (ty[i] * dxz) * v[p];
-      }
-    }
-    // + Dz w
-    double auxz = 0.0;
-    {
-      if (t == SolidCell) auxz += 0.0 * w[z == 0 ? i : i - NM];
-      else if (z_bottom(t)) auxz += (tz[i] * dx * dy) * w[i];
-      else if (z_top(t)) auxz += (-tz[i - NM] * dx * dy) * w[i - NM];
-      else {
-        const double dxy = dx * dy;
-        auxz += (-tz[i - NM] * dxy) * w[i - NM];
-        auxz += (tz[i] * dxy) * w[i];
-      }
-    }
-    r += auxz;
-    if (bc.pin && (i == N - 1 || i == NM - 1)) r = 0.0;
-    b[i] = r;
+    b[i] = rhs_cell(i, x, y, z, type, tx, ty, tz, dxa, dya, dza, N, M, K, in);
   }
 }
 
@@ -905,8 +961,222 @@ void correct(const int* type, const double* tx, const double* ty, const double* 
 void pressure_rhs(const int* type, const double* tx, const double* ty, const double* tz, const double* dx,
                   const double* dy, const double* dz, int N, int M, int K, const double* u, const double* v,
                   const double* w, const double* u_west, const double* u_out, FlowBc bc, double* b, hipStream_t s) {
-  k_pressure_rhs<<<grid_1d((long)N * M * K), 256, 0, s>>>(type, tx, ty, tz, dx, dy, dz, N, M, K, u, v, w, u_west,
-                                                          u_out, bc, b);
+  const RhsIn in{u, v, w, u_west, u_out, bc};
+  k_pressure_rhs<<<grid_1d((long)N * M * K), 256, 0, s>>>(type, tx, ty, tz, dx, dy, dz, N, M, K, in, b);
+}
+
+// ---- objective and shape gradient ---------------------------------------------------------------
+// Pressure drag J = sum over cut cells of (theta_x west - theta_x east) dy dz p: the pressure term
+// of LoadsCalculator::calc (loads_calculator.h:97-116, n_x dS = (theta^u_{i-1} - theta^u_i) dy dz).
+//
+// Shape gradient.  With A p = b solved and lambda = A^{-T} dJ/dp, the derivative of J along a
+// change of the geometry is that of the Lagrangian F = J(theta, p) + lambda . (b - A p) with p and
+// lambda held fixed.  F is a sum over cells of terms that read the level set at the corners of
+// the cell and of its six neighbours, after an oscillation fix that reads one node further:
+// corner c reaches the cells c + [-2, 3]^3.  Corners 6 apart in every direction therefore have
+// disjoint reach, so one pair of geometry rebuilds with all corners of one of the 216 classes
+// (x mod 6, y mod 6, z mod 6) moved by +h and -h gives the central difference dF/dphi_c of
+// every corner of the class at once, summed over its reach.  Differences are taken per
+// coefficient (c+ - c-) before they meet p, so the large cancelling products of A p never enter
+// them.  A Bezier body then chains dF/dphi to its control values through the closest point:
+// phi = sgn |X - S(u*, v*)| gives dphi/dP_ij = -sgn (X - S)/|X - S| B_i(u*) B_j(v*) (the
+// minimiser's own change does not enter at a minimum).
+namespace {
+
+constexpr int SG_SPACING = 6;
+
+__device__ inline bool cut_cell(int t) { return t > 0 && t != FluidCell; }
+
+__global__ void k_drag(const int* __restrict__ type, const double* __restrict__ tx, const double* __restrict__ dys,
+                       const double* __restrict__ dzs, int N, int M, int K, Box box, const double* __restrict__ p,
+                       double* __restrict__ dJdp, double* __restrict__ partial) {
+  __shared__ double lds[256];
+  const long n = (long)N * M * K;
+  const long per = (n + gridDim.x - 1) / gridDim.x;
+  const long lo = per * blockIdx.x, hi = lo + per < n ? lo + per : n;
+  double acc = 0.0;
+  for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    double g = 0.0;
+    if (in_box(x, y, box) && cut_cell(type[i])) g = (tx[i - 1] - tx[i]) * dys[y] * dzs[z];
+    if (dJdp) dJdp[i] = g;
+    acc += g * p[i];
+  }
+  lds[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) lds[threadIdx.x] += lds[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = lds[0];
+}
+
+__device__ inline int corner_class(int x, int y, int z) {
+  return x % SG_SPACING + SG_SPACING * (y % SG_SPACING) + SG_SPACING * SG_SPACING * (z % SG_SPACING);
+}
+
+__global__ void k_perturb(const double* __restrict__ phi_raw, int N, int M, int K, Box box, int cls, double h,
+                          double* __restrict__ phi_p, double* __restrict__ phi_m) {
+  const long n = (long)N * M * K;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    const double v = phi_raw[i];
+    const bool on = in_box(x, y, box) && corner_class(x, y, z) == cls;
+    phi_p[i] = on ? v + h : v;
+    phi_m[i] = on ? v - h : v;
+  }
+}
+
+__device__ inline double drag_cell(const GeoView& g, long i, int x, int y, int z, const double* dys,
+                                   const double* dzs, const Box& box, double p) {
+  if (!in_box(x, y, box) || !cut_cell(g.type[i])) return 0.0;
+  return (g.tx[i - 1] - g.tx[i]) * dys[y] * dzs[z] * p;
+}
+
+__global__ void k_lagr_diff(GeoView gp, GeoView gm, const double* __restrict__ dxs, const double* __restrict__ dys,
+                            const double* __restrict__ dzs, int N, int M, int K, Box box, int pin,
+                            const double* __restrict__ p, const double* __restrict__ lam, int objective, int has_rhs,
+                            RhsIn rin, double inv2h, double* __restrict__ dF) {
+  const long n = (long)N * M * K, NM = (long)N * M;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    double dJ = 0.0;
+    if (objective == OBJ_DRAG_X && in_box(x, y, box)) {
+      const bool cp = cut_cell(gp.type[i]), cm = cut_cell(gm.type[i]);
+      if (cp && cm) {
+        const double dw = gp.tx[i - 1] - gm.tx[i - 1], de = gp.tx[i] - gm.tx[i];
+        dJ = (dw - de) * dys[y] * dzs[z] * p[i];
+      } else {
+        dJ = drag_cell(gp, i, x, y, z, dys, dzs, box, p[i]) - drag_cell(gm, i, x, y, z, dys, dzs, box, p[i]);
+      }
+    }
+    double ca[7], cb[7];
+    laplace_row(i, x, y, z, gp.type, gp.tx, gp.ty, gp.tz, gp.vof, dxs, dys, dzs, N, M, pin, ca);
+    laplace_row(i, x, y, z, gm.type, gm.tx, gm.ty, gm.tz, gm.vof, dxs, dys, dzs, N, M, pin, cb);
+    const long nb[7] = {i - NM, i - N, i - 1, i, i + 1, i + N, i + NM};
+    const bool has[7] = {z > 0, y > 0, x > 0, true, x < N - 1, y < M - 1, z < K - 1};
+    double dAp = 0.0;
+    for (int q = 0; q < 7; ++q)
+      if (has[q]) dAp += (ca[q] - cb[q]) * p[nb[q]];
+    double db = 0.0;
+    if (has_rhs)
+      db = rhs_cell(i, x, y, z, gp.type, gp.tx, gp.ty, gp.tz, dxs, dys, dzs, N, M, K, rin) -
+           rhs_cell(i, x, y, z, gm.type, gm.tx, gm.ty, gm.tz, dxs, dys, dzs, N, M, K, rin);
+    dF[i] = (dJ + lam[i] * (db - dAp)) * inv2h;
+  }
+}
+
+// g[c] = sum of dF over the reach of corner c, for the corners of class `cls`
+__global__ void k_corner_gather(const double* __restrict__ dF, int N, int M, int K, Box box, int cls,
+                                double* __restrict__ g) {
+  const long n = (long)N * M * K, NM = (long)N * M;
+  for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(c, N, M, &x, &y, &z);
+    if (!in_box(x, y, box) || corner_class(x, y, z) != cls) continue;
+    double acc = 0.0;
+    for (int zz = z - 2; zz <= z + 3; ++zz) {
+      if (zz < 0 || zz >= K) continue;
+      for (int yy = y - 2; yy <= y + 3; ++yy) {
+        if (yy < 0 || yy >= M) continue;
+        for (int xx = x - 2; xx <= x + 3; ++xx) {
+          if (xx < 0 || xx >= N) continue;
+          acc += dF[xx + (long)N * yy + NM * zz];
+        }
+      }
+    }
+    g[c] = acc;
+  }
+}
+
+// per corner with g != 0: winning patch, its parameters and w = g dphi/dS (3 values)
+__global__ void k_bz_hits(const double* __restrict__ g, const double* __restrict__ cx, const double* __restrict__ cy,
+                          const double* __restrict__ cz, int N, int M, int K, Box box, Body body,
+                          int* __restrict__ hf, double* __restrict__ huv, double* __restrict__ hw) {
+  const long n = (long)N * M * K;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    int f = -1;
+    if (in_box(x, y, box) && g[i] != 0.0) {
+      BzHit hit;
+      bezier_closest(body.ctrl, body.c, body.a, cx[x], cy[y], cz[z], hit);
+      if (hit.dist > 0.0) {
+        f = hit.f;
+        huv[2 * i] = hit.u;
+        huv[2 * i + 1] = hit.v;
+        for (int c = 0; c < 3; ++c) hw[3 * i + c] = g[i] * (-hit.sgn * hit.r[c] / hit.dist);
+      }
+    }
+    hf[i] = f;
+  }
+}
+
+// one workgroup per control point (f, i, j): out[(f*16 + i*4 + j)*3 + c] = sum over its corners
+__global__ void k_bz_reduce(const int* __restrict__ hf, const double* __restrict__ huv, const double* __restrict__ hw,
+                            long n, double* __restrict__ out) {
+  __shared__ double lds[3][256];
+  const int cp = blockIdx.x, f = cp / 16, bi = (cp / 4) % 4, bj = cp % 4;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (long k = threadIdx.x; k < n; k += blockDim.x) {
+    if (hf[k] != f) continue;
+    double bu[4], dbu[4], ddbu[4], bv[4], dbv[4], ddbv[4];
+    bz_basis(huv[2 * k], bu, dbu, ddbu);
+    bz_basis(huv[2 * k + 1], bv, dbv, ddbv);
+    const double wgt = bu[bi] * bv[bj];
+    for (int c = 0; c < 3; ++c) acc[c] += wgt * hw[3 * k + c];
+  }
+  for (int c = 0; c < 3; ++c) lds[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int c = 0; c < 3; ++c) lds[c][threadIdx.x] += lds[c][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) out[cp * 3 + threadIdx.x] = lds[threadIdx.x][0];
+}
+
+}  // namespace
+
+constexpr int DRAG_BLOCKS = 256;
+
+double drag(const int* type, const double* tx, const double* dy, const double* dz, int N, int M, int K, Box box,
+            const double* p, double* dJdp, double* partial, hipStream_t s) {
+  k_drag<<<DRAG_BLOCKS, 256, 0, s>>>(type, tx, dy, dz, N, M, K, box, p, dJdp, partial);
+  double h[DRAG_BLOCKS];
+  if (hipMemcpyAsync(h, partial, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess) return NAN;
+  if (hipStreamSynchronize(s) != hipSuccess) return NAN;
+  double J = 0.0;
+  for (int b = 0; b < DRAG_BLOCKS; ++b) J += h[b];
+  return J;
+}
+
+void perturb_class(const double* phi_raw, int N, int M, int K, Box box, int cls, double h, double* phi_p,
+                   double* phi_m, hipStream_t s) {
+  k_perturb<<<grid_1d((long)N * M * K), 256, 0, s>>>(phi_raw, N, M, K, box, cls, h, phi_p, phi_m);
+}
+
+void lagrangian_diff(const GeoView& gp, const GeoView& gm, const double* dx, const double* dy, const double* dz, int N,
+                     int M, int K, Box box, int pin, const double* p, const double* lam, int objective,
+                     const RhsArgs* rhs, double h, double* dF, hipStream_t s) {
+  RhsIn in{nullptr, nullptr, nullptr, nullptr, nullptr, FlowBc{0, 0, 0, 0, 0, 0}};
+  if (rhs) in = RhsIn{rhs->u, rhs->v, rhs->w, rhs->u_west, rhs->u_out, rhs->bc};
+  k_lagr_diff<<<grid_1d((long)N * M * K), 256, 0, s>>>(gp, gm, dx, dy, dz, N, M, K, box, pin, p, lam, objective,
+                                                       rhs ? 1 : 0, in, 0.5 / h, dF);
+}
+
+void corner_gather(const double* dF, int N, int M, int K, Box box, int cls, double* g, hipStream_t s) {
+  k_corner_gather<<<grid_1d((long)N * M * K), 256, 0, s>>>(dF, N, M, K, box, cls, g);
+}
+
+void bezier_chain(const double* g, const double* cx, const double* cy, const double* cz, int N, int M, int K, Box box,
+                  Body body, int* hf, double* huv, double* hw, double* out, hipStream_t s) {
+  const long n = (long)N * M * K;
+  k_bz_hits<<<grid_1d(n), 256, 0, s>>>(g, cx, cy, cz, N, M, K, box, body, hf, huv, hw);
+  k_bz_reduce<<<96, 256, 0, s>>>(hf, huv, hw, n, out);
 }
 
 }  // namespace geo

@@ -36,7 +36,10 @@ struct Box {
 int rule_count();
 void level_set(const double* cx, const double* cy, const double* cz, int N, int M, int K, Box box, Body body,
                double* phi, hipStream_t s);
-void oscillation_fix(double* phi, unsigned char* flag, int N, int M, int K, Box box, hipStream_t s);
+// phi = phi0 after the oscillation fix (tmp: n doubles of scratch, changed: one device int);
+// synchronises the stream; returns the number of passes (-1 on a HIP error)
+int oscillation_fix(const double* phi0, double* phi, double* tmp, unsigned char* flag, int* changed, int N, int M,
+                    int K, Box box, hipStream_t s);
 void alpha(const double* phi, int N, int M, int K, Box box, double* ax, double* ay, double* az, hipStream_t s);
 void types(const double* phi, int N, int M, int K, Box box, int* type, int* sx, int* sy, int* sz, hipStream_t s);
 void theta(const double* ax, const double* ay, const double* az, const int* sx, const int* sy, const int* sz, int N,
@@ -63,6 +66,37 @@ void pressure_update(double* p, const double* dp, double coef, long n, hipStream
 void pressure_rhs(const int* type, const double* tx, const double* ty, const double* tz, const double* dx,
                   const double* dy, const double* dz, int N, int M, int K, const double* u, const double* v,
                   const double* w, const double* u_west, const double* u_out, FlowBc bc, double* b, hipStream_t s);
+
+// ---- objective and shape gradient ----------------------------------------------------------
+enum Objective : int { OBJ_NONE = 0, OBJ_DRAG_X = 1 };
+
+// read-only view of one geometry's fields
+struct GeoView {
+  const int* type;
+  const double *tx, *ty, *tz, *vof;
+};
+// the flow fields the pressure right-hand side is assembled from (device pointers)
+struct RhsArgs {
+  const double *u, *v, *w, *u_west, *u_out;
+  FlowBc bc;
+};
+
+// J = sum over cut cells of (theta_x[i-1] - theta_x[i]) dy dz p[i]; dJdp (may be null) gets the
+// coefficients; partial: 256 doubles of device scratch.  Synchronises the stream.
+double drag(const int* type, const double* tx, const double* dy, const double* dz, int N, int M, int K, Box box,
+            const double* p, double* dJdp, double* partial, hipStream_t s);
+// phi_p = phi_raw + h, phi_m = phi_raw - h at the in-box corners of class cls (x%6 + 6 (y%6) + 36 (z%6))
+void perturb_class(const double* phi_raw, int N, int M, int K, Box box, int cls, double h, double* phi_p,
+                   double* phi_m, hipStream_t s);
+// dF[i] = (F_i(+) - F_i(-)) / 2h with F_i = J_i + lambda_i (b_i - (A p)_i); rhs null: b fixed
+void lagrangian_diff(const GeoView& gp, const GeoView& gm, const double* dx, const double* dy, const double* dz, int N,
+                     int M, int K, Box box, int pin, const double* p, const double* lam, int objective,
+                     const RhsArgs* rhs, double h, double* dF, hipStream_t s);
+// g[c] = sum of dF over cells c + [-2, 3]^3 for the corners c of class cls
+void corner_gather(const double* dF, int N, int M, int K, Box box, int cls, double* g, hipStream_t s);
+// out[288] = sum over corners of g dphi/dctrl through the closest point (hf, huv, hw: n, 2n, 3n scratch)
+void bezier_chain(const double* g, const double* cx, const double* cy, const double* cz, int N, int M, int K, Box box,
+                  Body body, int* hf, double* huv, double* hw, double* out, hipStream_t s);
 
 }  // namespace geo
 }  // namespace lsx

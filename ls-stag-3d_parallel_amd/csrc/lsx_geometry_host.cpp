@@ -90,6 +90,9 @@ void Geometry::build(const geo::Body& body_in, geo::Box box, const double* host_
   cy.alloc(M);
   cz.alloc(K);
   phi.alloc(nn);
+  phi_raw.alloc(nn);
+  osc_tmp.alloc(nn);
+  dchanged.alloc(1);
   flag.alloc(nn);
   ax.alloc(nn);
   ay.alloc(nn);
@@ -105,15 +108,166 @@ void Geometry::build(const geo::Body& body_in, geo::Box box, const double* host_
   LSX_HIP(hipMemcpyAsync(cx.p, hcx.data(), N * sizeof(double), hipMemcpyHostToDevice, stream));
   LSX_HIP(hipMemcpyAsync(cy.p, hcy.data(), M * sizeof(double), hipMemcpyHostToDevice, stream));
   LSX_HIP(hipMemcpyAsync(cz.p, hcz.data(), K * sizeof(double), hipMemcpyHostToDevice, stream));
-  geo::level_set(cx.p, cy.p, cz.p, N, M, K, box, body, phi.p, stream);
-  geo::oscillation_fix(phi.p, flag.p, N, M, K, box, stream);
+  geo::level_set(cx.p, cy.p, cz.p, N, M, K, box, body, phi_raw.p, stream);
+  LSX_HIP(hipGetLastError());
+  osc_passes = geo::oscillation_fix(phi_raw.p, phi.p, osc_tmp.p, flag.p, dchanged.p, N, M, K, box, stream);
+  if (osc_passes < 0) throw Error(LSX_ERR_HIP, "level-set oscillation fix failed");
   geo::alpha(phi.p, N, M, K, box, ax.p, ay.p, az.p, stream);
   geo::types(phi.p, N, M, K, box, type.p, sx.p, sy.p, sz.p, stream);
   geo::theta(ax.p, ay.p, az.p, sx.p, sy.p, sz.p, N, M, K, box, tx.p, ty.p, tz.p, stream);
   geo::vof(type.p, ax.p, ay.p, az.p, tx.p, ty.p, tz.p, dx.p, dy.p, dz.p, N, M, K, vof.p, stream);
   LSX_HIP(hipGetLastError());
   LSX_HIP(hipStreamSynchronize(stream));
+  body_ = body;
+  box_ = box;
   built = true;
+  ++build_count;
+}
+
+void Geometry::stage_flow(const double* u, const double* v, const double* w, const double* u_west,
+                          const double* u_out, const lsx_flow_bc& bc, int mem, FlowStage& st) const {
+  if (!u || !v || !w || !u_west) throw Error(LSX_ERR_ARG, "null flow field");
+  const long nu = (long)(N - 1) * M * K, nv = (long)N * (M - 1) * K, nw = (long)N * M * (K - 1);
+  const long nrow = (long)M * K;
+  st.args.bc = geo::FlowBc{bc.U_E, bc.V_S, bc.V_N, bc.W_B, bc.W_T, bc.pin != 0};
+  if (mem == LSX_MEM_DEVICE) {
+    st.args.u = u;
+    st.args.v = v;
+    st.args.w = w;
+    st.args.u_west = u_west;
+    st.args.u_out = bc.outflow ? (u_out ? u_out : u) : nullptr;
+    return;
+  }
+  st.u.alloc(nu);
+  st.v.alloc(nv);
+  st.w.alloc(nw);
+  st.west.alloc(nrow);
+  LSX_HIP(hipMemcpyAsync(st.u.p, u, nu * sizeof(double), hipMemcpyHostToDevice, stream));
+  LSX_HIP(hipMemcpyAsync(st.v.p, v, nv * sizeof(double), hipMemcpyHostToDevice, stream));
+  LSX_HIP(hipMemcpyAsync(st.w.p, w, nw * sizeof(double), hipMemcpyHostToDevice, stream));
+  LSX_HIP(hipMemcpyAsync(st.west.p, u_west, nrow * sizeof(double), hipMemcpyHostToDevice, stream));
+  st.args.u = st.u.p;
+  st.args.v = st.v.p;
+  st.args.w = st.w.p;
+  st.args.u_west = st.west.p;
+  st.args.u_out = nullptr;
+  if (bc.outflow) {
+    if (u_out) {
+      st.out.alloc(nu);
+      LSX_HIP(hipMemcpyAsync(st.out.p, u_out, nu * sizeof(double), hipMemcpyHostToDevice, stream));
+      st.args.u_out = st.out.p;
+    } else {
+      st.args.u_out = st.u.p;
+    }
+  }
+}
+
+void Geometry::Work::alloc(long n) {
+  raw.alloc(n);
+  phi.alloc(n);
+  tmp.alloc(n);
+  ax.alloc(n);
+  ay.alloc(n);
+  az.alloc(n);
+  tx.alloc(n);
+  ty.alloc(n);
+  tz.alloc(n);
+  vof.alloc(n);
+  type.alloc(n);
+  sx.alloc(n);
+  sy.alloc(n);
+  sz.alloc(n);
+  flag.alloc(n);
+}
+
+void Geometry::rebuild_from(Work& w) {
+  if (geo::oscillation_fix(w.raw.p, w.phi.p, w.tmp.p, w.flag.p, dchanged.p, N, M, K, box_, stream) < 0)
+    throw Error(LSX_ERR_HIP, "level-set oscillation fix failed");
+  geo::alpha(w.phi.p, N, M, K, box_, w.ax.p, w.ay.p, w.az.p, stream);
+  geo::types(w.phi.p, N, M, K, box_, w.type.p, w.sx.p, w.sy.p, w.sz.p, stream);
+  geo::theta(w.ax.p, w.ay.p, w.az.p, w.sx.p, w.sy.p, w.sz.p, N, M, K, box_, w.tx.p, w.ty.p, w.tz.p, stream);
+  geo::vof(w.type.p, w.ax.p, w.ay.p, w.az.p, w.tx.p, w.ty.p, w.tz.p, dx.p, dy.p, dz.p, N, M, K, w.vof.p, stream);
+}
+
+double Geometry::drag(const double* p, double* dJdp, int mem) {
+  if (!built) throw Error(LSX_ERR_STATE, "geometry not built");
+  if (!p) throw Error(LSX_ERR_ARG, "null pressure");
+  const long nn = n();
+  sg_partial.alloc(256);
+  const double* dp = p;
+  double* dg = dJdp;
+  DevBuf<double> hp, hg;
+  if (mem == LSX_MEM_HOST) {
+    hp.alloc(nn);
+    LSX_HIP(hipMemcpyAsync(hp.p, p, nn * sizeof(double), hipMemcpyHostToDevice, stream));
+    dp = hp.p;
+    if (dJdp) {
+      hg.alloc(nn);
+      dg = hg.p;
+    }
+  }
+  const double J = geo::drag(type.p, tx.p, dy.p, dz.p, N, M, K, box_, dp, dg, sg_partial.p, stream);
+  LSX_HIP(hipGetLastError());
+  if (mem == LSX_MEM_HOST && dJdp) {
+    LSX_HIP(hipMemcpyAsync(dJdp, hg.p, nn * sizeof(double), hipMemcpyDeviceToHost, stream));
+    LSX_HIP(hipStreamSynchronize(stream));
+  }
+  return J;
+}
+
+void Geometry::shape_gradient(const double* p, const double* lam, int objective, const geo::RhsArgs* rhs, int pin,
+                              double h_rel, double* dJdphi, double* dJdctrl, int mem) {
+  if (!built) throw Error(LSX_ERR_STATE, "geometry not built");
+  if (!p || !lam) throw Error(LSX_ERR_ARG, "null pressure or adjoint");
+  if (objective != geo::OBJ_NONE && objective != geo::OBJ_DRAG_X) throw Error(LSX_ERR_ARG, "unknown objective");
+  if (dJdctrl && body_.kind != geo::BODY_BEZIER6)
+    throw Error(LSX_ERR_STATE, "control-point gradients need a Bezier body");
+  if (!(h_rel > 0.0) || !(h_rel < 1e-2)) throw Error(LSX_ERR_ARG, "h_rel must lie in (0, 1e-2)");
+  const long nn = n();
+  double wmin = hdx[0];
+  for (double v : hdx) wmin = v < wmin ? v : wmin;
+  for (double v : hdy) wmin = v < wmin ? v : wmin;
+  for (double v : hdz) wmin = v < wmin ? v : wmin;
+  const double h = h_rel * wmin;
+  const double* dp = p;
+  const double* dl = lam;
+  DevBuf<double> hp, hl;
+  if (mem == LSX_MEM_HOST) {
+    hp.alloc(nn);
+    hl.alloc(nn);
+    LSX_HIP(hipMemcpyAsync(hp.p, p, nn * sizeof(double), hipMemcpyHostToDevice, stream));
+    LSX_HIP(hipMemcpyAsync(hl.p, lam, nn * sizeof(double), hipMemcpyHostToDevice, stream));
+    dp = hp.p;
+    dl = hl.p;
+  }
+  wp.alloc(nn);
+  wm.alloc(nn);
+  sg_dF.alloc(nn);
+  sg_g.alloc(nn);
+  LSX_HIP(hipMemsetAsync(sg_g.p, 0, nn * sizeof(double), stream));
+  const int classes = 216;
+  for (int cls = 0; cls < classes; ++cls) {
+    geo::perturb_class(phi_raw.p, N, M, K, box_, cls, h, wp.raw.p, wm.raw.p, stream);
+    rebuild_from(wp);
+    rebuild_from(wm);
+    geo::lagrangian_diff(wp.view(), wm.view(), dx.p, dy.p, dz.p, N, M, K, box_, pin, dp, dl, objective, rhs, h,
+                         sg_dF.p, stream);
+    geo::corner_gather(sg_dF.p, N, M, K, box_, cls, sg_g.p, stream);
+  }
+  LSX_HIP(hipGetLastError());
+  if (dJdphi)
+    LSX_HIP(hipMemcpyAsync(dJdphi, sg_g.p, nn * sizeof(double),
+                           mem == LSX_MEM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, stream));
+  if (dJdctrl) {
+    sg_f.alloc(nn);
+    sg_uv.alloc(2 * nn);
+    sg_w.alloc(3 * nn);
+    sg_out.alloc(288);
+    geo::bezier_chain(sg_g.p, cx.p, cy.p, cz.p, N, M, K, box_, body_, sg_f.p, sg_uv.p, sg_w.p, sg_out.p, stream);
+    LSX_HIP(hipGetLastError());
+    LSX_HIP(hipMemcpyAsync(dJdctrl, sg_out.p, 288 * sizeof(double), hipMemcpyDeviceToHost, stream));
+  }
+  LSX_HIP(hipStreamSynchronize(stream));
 }
 
 void Geometry::fields(int* htype, double* hthx, double* hthy, double* hthz, double* hvof, double* hphi) const {

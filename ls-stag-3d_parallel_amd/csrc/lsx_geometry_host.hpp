@@ -11,8 +11,17 @@ namespace lsx {
 // six-patch bicubic Bezier fit of an ellipsoid into ctrl[288] (lsx_bezier.cpp)
 void bezier_ellipsoid(const double* center, const double* axes, double* ctrl);
 
+// Flow fields of a pressure right-hand side staged in device memory (copies when given on the host).
+struct FlowStage {
+  DevBuf<double> u, v, w, west, out;
+  geo::RhsArgs args{};
+};
+
 class Geometry {
  public:
+  // stage u*, v*, w*, u_west, u_out (`mem` space) for rhs-dependent gradients
+  void stage_flow(const double* u, const double* v, const double* w, const double* u_west, const double* u_out,
+                  const lsx_flow_bc& bc, int mem, FlowStage& st) const;
   Geometry(int N, int M, int K, const double* dx, const double* dy, const double* dz, double x0, double y0, double z0,
            hipStream_t stream);
   ~Geometry();
@@ -29,9 +38,22 @@ class Geometry {
   void correct(const double* dp, const double* us, const double* vs, const double* ws, double* u, double* v, double* w,
                double* p, double coef, int mem);
 
+  // pressure drag J and (dJdp non-null) its gradient; p and dJdp in `mem` space
+  double drag(const double* p, double* dJdp, int mem);
+  // Shape gradient of J at the solution p with the adjoint lambda = A^{-T} dJ/dp: dF/dphi per
+  // level-set corner (dJdphi, `mem` space, may be null) and, for a Bezier body, dJ/dctrl
+  // (288 host doubles, may be null).  `rhs` null: the right-hand side does not depend on the
+  // geometry; otherwise b = D u* + flux from these fields (device pointers).  h_rel: level-set
+  // step as a fraction of the smallest cell width.
+  void shape_gradient(const double* p, const double* lam, int objective, const geo::RhsArgs* rhs, int pin, double h_rel,
+                      double* dJdphi, double* dJdctrl, int mem);
+  hipStream_t get_stream() const { return stream; }
+  long version() const { return build_count; }
+
   int N, M, K;
   long n() const { return (long)N * M * K; }
   bool built = false;
+  long build_count = 0;
 
  private:
   hipStream_t stream = nullptr;
@@ -40,9 +62,26 @@ class Geometry {
   double x0, y0, z0;
   DevBuf<double> dx, dy, dz, cx, cy, cz;
   DevBuf<double> phi, ax, ay, az, tx, ty, tz, vof;
+  DevBuf<double> phi_raw, osc_tmp;  // level set before the oscillation fix; scratch of the fix
+  DevBuf<int> dchanged;
+  int osc_passes = 0;
   DevBuf<int> type, sx, sy, sz;
   DevBuf<unsigned char> flag;
   DevBuf<double> bez;  // Bezier control values (device)
+  geo::Body body_{};
+  geo::Box box_{};
+  // shape-gradient work: two perturbed geometries, dF per cell, g per corner, closest points
+  struct Work {
+    DevBuf<double> raw, phi, tmp, ax, ay, az, tx, ty, tz, vof;
+    DevBuf<int> type, sx, sy, sz;
+    DevBuf<unsigned char> flag;
+    void alloc(long n);
+    geo::GeoView view() const { return geo::GeoView{type.p, tx.p, ty.p, tz.p, vof.p}; }
+  };
+  Work wp, wm;
+  DevBuf<double> sg_dF, sg_g, sg_uv, sg_w, sg_partial, sg_out;
+  DevBuf<int> sg_f;
+  void rebuild_from(Work& w);  // fields of w from w.raw (oscillation fix onwards)
 };
 
 }  // namespace lsx

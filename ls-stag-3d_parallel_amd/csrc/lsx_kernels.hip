@@ -461,6 +461,68 @@ __device__ inline double row_apply_sym(const double* __restrict__ s4, long n, in
   return acc;
 }
 
+// Row i of the transposed operator: A^T(i, j) = A(j, i).  An identity-like row of A becomes a
+// full row of A^T (its own faces, which hold the neighbours' couplings to it), and every other
+// row drops its couplings to those cells, whose rows in A do not couple back.
+__device__ inline bool is_exc(const SymExc& ex, long j) {
+  bool e = false;
+#pragma unroll
+  for (int q = 0; q < SymExc::CAP; ++q) e |= (q < ex.count) && (ex.row[q] == j);
+  return e;
+}
+
+template <class XT>
+__device__ inline double row_apply_sym_t(const double* __restrict__ s4, long n, int i, int x, int y, int z, int N,
+                                         int M, int K, const XT* __restrict__ xv, double acc, bool subtract,
+                                         const SymExc& ex) {
+  const int NM = N * M;
+  const bool exc_row = is_exc(ex, i);
+  double t;
+  auto term = [&](double c, long j) {
+    if (!exc_row && is_exc(ex, j)) return;
+    t = c * (double)xv[j];
+    acc = subtract ? acc - t : acc + t;
+  };
+  if (z > 0) term(s4[3 * n + i - NM], i - NM);
+  if (y > 0) term(s4[2 * n + i - N], i - N);
+  if (x > 0) term(s4[1 * n + i - 1], i - 1);
+  t = s4[i] * (double)xv[i];
+  acc = subtract ? acc - t : acc + t;
+  if (x < N - 1) term(s4[1 * n + i], i + 1);
+  if (y < M - 1) term(s4[2 * n + i], i + N);
+  if (z < K - 1) term(s4[3 * n + i], i + NM);
+  return acc;
+}
+
+template <int MODE, int BS>
+__global__ void k_stencil_sym_t(const double* __restrict__ s4, int N, int M, int K, SymExc ex,
+                                const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ y,
+                                double* __restrict__ partial) {
+  const long n = (long)N * M * K;
+  long lo, hi;
+  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
+  double ss = 0.0;
+  for (long li = lo + threadIdx.x; li < hi; li += BS) {
+    const int i = (int)li;
+    int xx, yy, zz;
+    decode(i, N, M, &xx, &yy, &zz);
+    if (MODE == ST_PROD) {
+      y[i] = row_apply_sym_t(s4, n, i, xx, yy, zz, N, M, K, x, 0.0, false, ex);
+    } else if (MODE == ST_RESID) {
+      const double r = row_apply_sym_t(s4, n, i, xx, yy, zz, N, M, K, x, b[i], true, ex);
+      y[i] = r;
+      ss += r * r;
+    } else {
+      const double q = row_apply_sym_t(s4, n, i, xx, yy, zz, N, M, K, x, 0.0, false, ex);
+      y[i] = y[i] - q;
+    }
+  }
+  if (MODE == ST_RESID) {
+    ss = block_sum<BS>(ss);
+    if (threadIdx.x == 0) partial[blockIdx.x] = ss;
+  }
+}
+
 template <int MODE, int BS>
 __global__ void k_stencil_sym(const double* __restrict__ s4, int N, int M, int K, SymExc ex,
                               const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ y,
@@ -487,6 +549,18 @@ __global__ void k_stencil_sym(const double* __restrict__ s4, int N, int M, int K
   if (MODE == ST_RESID) {
     ss = block_sum<BS>(ss);
     if (threadIdx.x == 0) partial[blockIdx.x] = ss;
+  }
+}
+
+// y = A^T x with an FP32 x
+__global__ void k_stencil_sym_t_f32x(const double* __restrict__ s4, int N, int M, int K, SymExc ex,
+                                     const float* __restrict__ x, double* __restrict__ y) {
+  const long n = (long)N * M * K;
+  for (long li = blockIdx.x * (long)blockDim.x + threadIdx.x; li < n; li += (long)gridDim.x * blockDim.x) {
+    const int i = (int)li;
+    int xx, yy, zz;
+    decode(i, N, M, &xx, &yy, &zz);
+    y[i] = row_apply_sym_t(s4, n, i, xx, yy, zz, N, M, K, x, 0.0, false, ex);
   }
 }
 
@@ -1009,6 +1083,21 @@ void cgs_update(const double* V, long ldv, int cnt, const double* h, double* w, 
   const int nb = reduction_blocks(n);
   k_cgs_update<RED_THREADS><<<nb, RED_THREADS, 0, s>>>(V, ldv, cnt, h, w, n, norm_out != nullptr, partial);
   if (norm_out) k_finalize_multi<RED_THREADS><<<1, RED_THREADS, 0, s>>>(partial, nb, norm_out, 1);
+}
+
+void stencil_apply_sym_t(const double* s4, const SymExc& ex, Dims d, const double* x, const double* b, double* y,
+                         int mode, double* partial, hipStream_t s) {
+  const int nb = reduction_blocks(d.n());
+  if (mode == ST_PROD)
+    k_stencil_sym_t<ST_PROD, RED_THREADS><<<nb, RED_THREADS, 0, s>>>(s4, d.N, d.M, d.K, ex, x, b, y, partial);
+  else if (mode == ST_RESID)
+    k_stencil_sym_t<ST_RESID, RED_THREADS><<<nb, RED_THREADS, 0, s>>>(s4, d.N, d.M, d.K, ex, x, b, y, partial);
+  else
+    k_stencil_sym_t<ST_SUBR, RED_THREADS><<<nb, RED_THREADS, 0, s>>>(s4, d.N, d.M, d.K, ex, x, b, y, partial);
+}
+
+void stencil_prod_sym_t_f32x(const double* s4, const SymExc& ex, Dims d, const float* x, double* y, hipStream_t s) {
+  k_stencil_sym_t_f32x<<<grid_1d(d.n()), 256, 0, s>>>(s4, d.N, d.M, d.K, ex, x, y);
 }
 
 void stencil_prod_sym_f32x(const double* s4, const SymExc& ex, Dims d, const float* x, double* y, hipStream_t s) {

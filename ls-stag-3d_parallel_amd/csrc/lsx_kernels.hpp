@@ -32,6 +32,10 @@ void sym_build(const double* coef7, const unsigned char* exc, Dims d, double* s4
 void stencil_apply_sym(const double* s4, const SymExc& ex, Dims d, const double* x, const double* b, double* y,
                        int mode, double* partial, hipStream_t s);
 
+// the same with the transposed operator A^T (identity-like rows become full rows of A^T)
+void stencil_apply_sym_t(const double* s4, const SymExc& ex, Dims d, const double* x, const double* b, double* y,
+                         int mode, double* partial, hipStream_t s);
+
 // Deterministic reductions: partial sums per block into `partial`, then finalize sums them in
 // a fixed order into out[out_index] (device memory).
 constexpr int RED_THREADS = 256;
@@ -86,6 +90,7 @@ void cgs_update(const double* V, long ldv, int cnt, const double* h, double* w, 
 // mixed precision: y = A x with FP32 x (fine 7-point operator); x += sum_k y_k z_k with FP32 z
 void stencil7_prod_f32x(const double* coef, Dims d, const float* x, double* y, hipStream_t s);
 void stencil_prod_sym_f32x(const double* s4, const SymExc& ex, Dims d, const float* x, double* y, hipStream_t s);
+void stencil_prod_sym_t_f32x(const double* s4, const SymExc& ex, Dims d, const float* x, double* y, hipStream_t s);
 void sol_update_f32(double* x, const float* Z, long ldz, const double* yk, int count, long n, hipStream_t s);
 // BiCGStab fused updates (bicgstab_interface.h:39-56)
 void bicg_s_update(double* nu, const double* r, double* sv, double alpha, long n, hipStream_t s);

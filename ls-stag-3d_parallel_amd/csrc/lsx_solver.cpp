@@ -33,6 +33,7 @@ Solver::Solver(const lsx_options& o, hipStream_t s) : opt(o) {
   if (opt.ortho < LSX_ORTHO_AUTO || opt.ortho > LSX_ORTHO_CGS2) throw Error(LSX_ERR_ARG, "unknown ortho");
   if (opt.mg == LSX_MG_VCYCLE && (opt.n_pre < 1 || opt.n_post < 1))
     throw Error(LSX_ERR_ARG, "the V-cycle needs n_pre >= 1 and n_post >= 1");
+  if (opt.smoother != LSX_SMOOTH_RBGS && opt.smoother != LSX_SMOOTH_LINE) throw Error(LSX_ERR_ARG, "unknown smoother");
   if (s) {
     stream = s;
   } else {
@@ -47,6 +48,7 @@ Solver::Solver(const lsx_options& o, hipStream_t s) : opt(o) {
 }
 
 Solver::~Solver() {
+  clear_graphs();
   for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
@@ -63,10 +65,15 @@ hipEvent_t Solver::ev_take() {
   return ev_pool[ev_used++];
 }
 
+void Solver::clear_graphs() {
+  for (CycleGraph& g : cgraphs) (void)hipGraphExecDestroy(g.exec);
+  cgraphs.clear();
+}
+
 void Solver::tbeg(int k, hipEvent_t* e) {
   (void)k;
   *e = nullptr;
-  if (!timing) return;
+  if (!timing || capturing) return;
   *e = ev_take();
   LSX_HIP(hipEventRecord(*e, stream));
 }
@@ -116,7 +123,16 @@ double Solver::kernel_bytes(int k) const {
     case LSX_K_OPERATOR:                   // coefficients 32, w 8, z 8 (FP64) or 4 (FP32)
       return (mixed() ? 44.0 : 48.0) * n;
     case LSX_K_ORTHO: return 32.0 * n;     // w read + write 16, v_i 8, v_i+1 8
-    case LSX_K_SMOOTH:                     // n_pre red-black sweeps; per sweep x, three faces,
+    case LSX_K_SMOOTH:  // n_pre sweeps of the finest level
+      if (opt.smoother == LSX_SMOOTH_LINE) {
+        // x lines: b, 1/a_p, three faces, x_in, x_out; y and z lines also write and re-read the
+        // eliminated coefficients and re-read and rewrite x_out in the back substitution
+        const double vb = mixed() ? 4.0 : 8.0;
+        const double fb = opt.precision == LSX_PREC_MIXED16 ? 6.0 : (mixed() ? 12.0 : 24.0);
+        const double xl = 4.0 * vb + fb, yz = xl + 4.0 * vb;
+        return (xl + 2.0 * yz) * n * opt.n_pre;
+      }
+      // red-black: x, b, 1/a_p, three faces, x written
       return (opt.precision == LSX_PREC_MIXED16 ? 22.0 : (mixed() ? 28.0 : 56.0)) * n * opt.n_pre;
     default: return 0.0;
   }
@@ -205,15 +221,20 @@ void Solver::set_geometry(const Geometry& g, int pin) {
   check_dims(g.N, g.M, g.K);
   ready = false;
   setup_levels(g.N, g.M, g.K);
+  src_geo = nullptr;
   g.laplace7(pin, levels[0].coef.p, stream);
   from_csr = false;
   build_hierarchy();
   alloc_krylov();
   ready = true;
+  src_geo = &g;
+  src_version = g.version();
+  src_pin = pin;
 }
 
 void Solver::set_csr(int N, int M, int K, const int* rp, const int* cl, const double* vl, long nnz, int mem) {
   check_dims(N, M, K);
+  src_geo = nullptr;
   if (!rp || !cl || !vl || nnz <= 0) throw Error(LSX_ERR_ARG, "null CSR array or empty matrix");
   if (mem != LSX_MEM_HOST && mem != LSX_MEM_DEVICE) throw Error(LSX_ERR_ARG, "mem must be LSX_MEM_HOST or LSX_MEM_DEVICE");
   ready = false;
@@ -265,6 +286,7 @@ void Solver::rebuild() {
 }
 
 void Solver::build_hierarchy() {
+  clear_graphs();
   MGLevel& F = levels[0];
   if (from_csr) {
     LSX_HIP(hipMemsetAsync(err.p, 0, sizeof(int), stream));
@@ -361,6 +383,15 @@ void Solver::build_vcycle() {
       LSX_HIP(hipStreamSynchronize(stream));
       break;
     }
+    if (opt.smoother == LSX_SMOOTH_LINE) {
+      if (m32) {
+        S.fxt.alloc(nl);
+        if (l == 0) fcp.alloc(nl);
+      } else {
+        S.dxt.alloc(nl);
+        if (l == 0) dcp.alloc(nl);
+      }
+    }
     if (m32) {
       S.fs.alloc(nl);
       S.fb.alloc(nl);
@@ -425,8 +456,59 @@ float* Solver::vb<float>(VStore& S) { return S.fb.p; }
 template <>
 double* Solver::vb<double>(VStore& S) { return S.db.p; }
 
+template <>
+float* Solver::vxt<float>(VStore& S) { return S.fxt.p; }
+template <>
+double* Solver::vxt<double>(VStore& S) { return S.dxt.p; }
+template <>
+float* Solver::vcp<float>() { return fcp.p; }
+template <>
+double* Solver::vcp<double>() { return dcp.p; }
+template <>
+float* Solver::vr<float>(VStore& S) { return S.fr.p; }
+template <>
+double* Solver::vr<double>(VStore& S) { return S.dr.p; }
+
 template <class T>
 void Solver::vcycle_apply(const double* v, T* z) {
+  if (trans) {
+    vcycle_apply_tr<T>(v, z);
+    return;
+  }
+  if (!opt.graphs || capturing) {
+    vcycle_run<T>(v, z);
+    return;
+  }
+  // one captured graph per (v, z) pair: the Krylov loops apply the cycle to a fixed set of
+  // basis slots, so a graph replays every launch of a cycle with one submission
+  for (const CycleGraph& g : cgraphs)
+    if (g.v == v && g.z == (const void*)z) {
+      LSX_HIP(hipGraphLaunch(g.exec, stream));
+      return;
+    }
+  hipGraph_t graph = nullptr;
+  LSX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  capturing = true;
+  try {
+    vcycle_run<T>(v, z);
+  } catch (...) {
+    capturing = false;
+    (void)hipStreamEndCapture(stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  capturing = false;
+  LSX_HIP(hipStreamEndCapture(stream, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  LSX_HIP(ie);
+  cgraphs.push_back({v, (const void*)z, exec});
+  LSX_HIP(hipGraphLaunch(exec, stream));
+}
+
+template <class T>
+void Solver::vcycle_run(const double* v, T* z) {
   const int L = (int)vlev.size() - 1;
   // level 0 right-hand side: the input itself in FP64, an FP32 copy in mixed precision
   const T* b0;
@@ -438,7 +520,7 @@ void Solver::vcycle_apply(const double* v, T* z) {
   }
   // levels from `lc` down are small enough for one workgroup: a single launch does their cycle
   int lc = L + 1;
-  for (int l = 1; l <= L && opt.coarse_fuse; ++l)
+  for (int l = 1; l <= L && opt.coarse_fuse && opt.smoother == LSX_SMOOTH_RBGS; ++l)
     if (vlev[l].L.d.n() <= vc::COARSE_CELLS && L - l + 1 <= vc::COARSE_MAX) {
       lc = l;
       break;
@@ -451,10 +533,7 @@ void Solver::vcycle_apply(const double* v, T* z) {
     const T* b = l == 0 ? b0 : vb<T>(S);
     hipEvent_t e0 = nullptr;
     if (l == 0) tbeg(LSX_K_SMOOTH, &e0);
-    for (int s = 0; s < opt.n_pre; ++s) {
-      vc::rbgs<T>(S.L, x, b, 0, s == 0, stream);
-      vc::rbgs<T>(S.L, x, b, 1, 0, stream);
-    }
+    smooth<T>(l, x, b, opt.n_pre, true);
     if (l == 0) tend(LSX_K_SMOOTH, e0);
     vc::resid_restrict<T>(S.L, x, b, vb<T>(vlev[l + 1]), stream);
   }
@@ -483,9 +562,92 @@ void Solver::vcycle_apply(const double* v, T* z) {
     T* x = l == 0 ? z : vx<T>(S);
     const T* b = l == 0 ? b0 : vb<T>(S);
     vc::prolong_add<T>(vx<T>(vlev[l + 1]), vlev[l + 1].L.d, x, S.L.d, stream);
-    for (int s = 0; s < opt.n_post; ++s) {
-      vc::rbgs<T>(S.L, x, b, 0, 0, stream);
+    smooth<T>(l, x, b, opt.n_post, false);
+  }
+}
+
+// `sweeps` smoothing sweeps of level l on x (zero_guess: x is taken as 0 and not read).
+// Red-black Gauss-Seidel: red then black cells.  Line relaxation: x, y and z lines in turn,
+// each pass from one buffer into the other; the passes alternate between x and the level's
+// second buffer so that the last one writes x.
+template <class T>
+void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess) {
+  VStore& S = vlev[l];
+  if (opt.smoother == LSX_SMOOTH_RBGS) {
+    for (int s = 0; s < sweeps; ++s) {
+      vc::rbgs<T>(S.L, x, b, 0, zero_guess && s == 0, stream);
       vc::rbgs<T>(S.L, x, b, 1, 0, stream);
+    }
+    return;
+  }
+  T* xt = vxt<T>(S);
+  T* cpv = vcp<T>();
+  const int P = 3 * sweeps;
+  const long n = S.L.d.n();
+  T* src = x;
+  if (!zero_guess && (P - 1) % 2 == 0) {  // the first pass must not write the buffer it reads
+    vc::copy_vec<T>(x, xt, n, stream);
+    src = xt;
+  }
+  for (int p = 1; p <= P; ++p) {
+    T* dst = (P - p) % 2 == 0 ? x : xt;
+    vc::line_pass<T>(S.L, (p - 1) % 3, src, dst, cpv, b, zero_guess && p == 1, stream);
+    src = dst;
+  }
+}
+
+// The transpose of vcycle_apply: its steps in reverse order, each transposed (half-sweeps in the
+// opposite colour order on A^T, restriction by P^T, prolongation by the transposed sum
+// restriction, the symmetric coarsest LU unchanged).  Levels are launched one by one.
+template <class T>
+void Solver::vcycle_apply_tr(const double* v, T* z) {
+  const int L = (int)vlev.size() - 1;
+  const T* b0;
+  if constexpr (std::is_same<T, float>::value) {
+    vc::convert<double, float>(v, vlev[0].fb.p, d0.n(), stream);
+    b0 = vlev[0].fb.p;
+  } else {
+    b0 = v;
+  }
+  for (int l = 0; l < L; ++l) {
+    VStore& S = vlev[l];
+    if (!vr<T>(S)) {
+      if constexpr (std::is_same<T, float>::value)
+        S.fr.alloc(S.L.d.n());
+      else
+        S.dr.alloc(S.L.d.n());
+    }
+  }
+  for (int l = 0; l < L; ++l) {
+    VStore& S = vlev[l];
+    T* x = l == 0 ? z : vx<T>(S);
+    const T* b = l == 0 ? b0 : vb<T>(S);
+    for (int s = 0; s < opt.n_post; ++s) {  // transposed post-smoothing
+      vc::rbgs_tr<T>(S.L, x, b, 1, s == 0, stream);
+      vc::rbgs_tr<T>(S.L, x, b, 0, 0, stream);
+    }
+    vc::resid_tr<T>(S.L, x, b, vr<T>(S), stream);
+    vc::restrict_pt<T>(vr<T>(S), S.L.d, vb<T>(vlev[l + 1]), stream);
+  }
+  {
+    VStore& C = vlev[L];
+    const long nc = C.L.d.n();
+    if constexpr (std::is_same<T, float>::value) {
+      vc::convert<float, double>(C.fb.p, C.db.p, nc, stream);
+      lu_solve(lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, (int)nc, C.db.p, C.dx.p, stream);
+      vc::convert<double, float>(C.dx.p, C.fx.p, nc, stream);
+    } else {
+      lu_solve(lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, (int)nc, C.db.p, C.dx.p, stream);
+    }
+  }
+  for (int l = L - 1; l >= 0; --l) {
+    VStore& S = vlev[l];
+    T* x = l == 0 ? z : vx<T>(S);
+    const T* b = l == 0 ? b0 : vb<T>(S);
+    vc::prolong_inject_add<T>(vx<T>(vlev[l + 1]), vlev[l + 1].L.d, x, S.L.d, stream);
+    for (int s = 0; s < opt.n_pre; ++s) {  // transposed pre-smoothing
+      vc::rbgs_tr<T>(S.L, x, b, 1, 0, stream);
+      vc::rbgs_tr<T>(S.L, x, b, 0, 0, stream);
     }
   }
 }
@@ -502,6 +664,7 @@ void Solver::precond(const double* v, double* z64) {
 }
 
 void Solver::alloc_krylov() {
+  clear_graphs();
   const long n = d0.n();
   if (opt.krylov == LSX_KRYLOV_FGMRES) {
     V.alloc((size_t)opt.restart * n);
@@ -557,7 +720,9 @@ void Solver::sym_setup() {
 }
 
 void Solver::fine_apply(const double* x, const double* b, double* y, int mode) {
-  if (sym_ok)
+  if (trans)
+    stencil_apply_sym_t(s4.p, sym_ex, d0, x, b, y, mode, partial.p, stream);
+  else if (sym_ok)
     stencil_apply_sym(s4.p, sym_ex, d0, x, b, y, mode, partial.p, stream);
   else
     stencil_apply(levels[0].coef.p, 7, d0, x, b, y, mode, partial.p, stream);
@@ -670,7 +835,9 @@ int Solver::fgmres(const double* b, double* x) {
       hipEvent_t e0 = nullptr;
       tbeg(LSX_K_OPERATOR, &e0);
       if (m32) {
-        if (sym_ok)
+        if (trans)
+          stencil_prod_sym_t_f32x(s4.p, sym_ex, d0, zj32, w.p, stream);
+        else if (sym_ok)
           stencil_prod_sym_f32x(s4.p, sym_ex, d0, zj32, w.p, stream);
         else
           stencil7_prod_f32x(F.coef.p, d0, zj32, w.p, stream);
@@ -876,6 +1043,60 @@ void Solver::apply_operator(int level, const double* x, double* yv, int mem) {
   }
   LSX_HIP(hipGetLastError());
   LSX_HIP(hipStreamSynchronize(stream));
+}
+
+void Solver::require_transpose() const {
+  if (!ready) throw Error(LSX_ERR_STATE, "adjoint operation before the matrix is set");
+  if (!vcyc()) throw Error(LSX_ERR_STATE, "adjoint operations need the V-cycle preconditioner (LSX_MG_VCYCLE)");
+  if (!sym_ok)
+    throw Error(LSX_ERR_STATE, "adjoint operations need an operator symmetric off its pinned rows");
+}
+
+void Solver::solve_adjoint(const double* g, double* lambda, int mem, int* it_out, double* nr_out) {
+  require_transpose();
+  TransScope scope(*this);
+  solve(g, lambda, mem, it_out, nr_out);
+}
+
+void Solver::precondition_transpose(const double* v, double* z, int mem) {
+  require_transpose();
+  TransScope scope(*this);
+  precondition(v, z, mem);
+}
+
+void Solver::smooth_fine(const double* b, double* x, int sweeps, int zero_guess, int mem) {
+  if (!ready) throw Error(LSX_ERR_STATE, "smooth before the matrix is set");
+  if (!vcyc()) throw Error(LSX_ERR_STATE, "the smoother belongs to the V-cycle (LSX_MG_VCYCLE)");
+  if (sweeps < 1) throw Error(LSX_ERR_ARG, "sweeps must be >= 1");
+  if (!b || !x) throw Error(LSX_ERR_ARG, "null vector");
+  const long n = d0.n();
+  const double* db = b;
+  double* dx = x;
+  if (mem == LSX_MEM_HOST) {
+    hb.alloc(n);
+    hx.alloc(n);
+    LSX_HIP(hipMemcpyAsync(hb.p, b, n * sizeof(double), hipMemcpyHostToDevice, stream));
+    LSX_HIP(hipMemcpyAsync(hx.p, x, n * sizeof(double), hipMemcpyHostToDevice, stream));
+    db = hb.p;
+    dx = hx.p;
+  }
+  if (mixed()) {
+    vc::convert<double, float>(db, vlev[0].fb.p, n, stream);
+    vc::convert<double, float>(dx, ztmp32.p, n, stream);
+    smooth<float>(0, ztmp32.p, vlev[0].fb.p, sweeps, zero_guess != 0);
+    vc::convert<float, double>(ztmp32.p, dx, n, stream);
+  } else {
+    smooth<double>(0, dx, db, sweeps, zero_guess != 0);
+  }
+  if (mem == LSX_MEM_HOST) LSX_HIP(hipMemcpyAsync(x, dx, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+  LSX_HIP(hipGetLastError());
+  LSX_HIP(hipStreamSynchronize(stream));
+}
+
+void Solver::apply_operator_transpose(const double* x, double* yv, int mem) {
+  require_transpose();
+  TransScope scope(*this);
+  apply_operator(0, x, yv, mem);
 }
 
 long Solver::level_dims(int level, int* N, int* M, int* K) const {

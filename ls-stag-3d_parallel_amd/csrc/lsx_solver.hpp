@@ -78,10 +78,21 @@ class Solver {
 
   void set_csr(int N, int M, int K, const int* rowptr, const int* col, const double* val, long nnz, int mem);
   void set_geometry(const Geometry& g, int pin);
+  // the geometry (and its build and pin choice) the current operator was assembled from
+  bool operator_from(const Geometry* g, long version, int pin) const {
+    return ready && src_geo == g && src_version == version && src_pin == pin;
+  }
   void update_values(const double* val, int mem);
   void rebuild();
   void solve(const double* b, double* x, int mem, int* it, double* norm_r);
   void precondition(const double* v, double* z, int mem);
+  // adjoint: lambda = A^{-T} g by the configured Krylov method on A^T, right-preconditioned by
+  // the transposed V-cycle; transposed operator and preconditioner applications
+  void solve_adjoint(const double* g, double* lambda, int mem, int* it, double* norm_r);
+  void precondition_transpose(const double* v, double* z, int mem);
+  // `sweeps` sweeps of the V-cycle's smoother on the finest level, x updated in place
+  void smooth_fine(const double* b, double* x, int sweeps, int zero_guess, int mem);
+  void apply_operator_transpose(const double* x, double* y, int mem);
   void apply_operator(int level, const double* x, double* y, int mem);
   long level_dims(int level, int* N, int* M, int* K) const;
   int level_stencil(int level, double* host, long capacity) const;
@@ -104,6 +115,9 @@ class Solver {
   void setup_levels(int N, int M, int K);
   void build_hierarchy();
   bool from_csr = false;
+  const Geometry* src_geo = nullptr;
+  long src_version = -1;
+  int src_pin = -1;
   void alloc_krylov();
   void mg_apply(const double* v_dev, double* z_dev);
   int fgmres(const double* b, double* x);
@@ -138,6 +152,13 @@ class Solver {
   std::vector<int> pins;  // identity-like rows of the fine operator (pinned cells)
   void sym_setup();
   void fine_apply(const double* x, const double* b, double* y, int mode);  // StencilMode on level 0
+  bool trans = false;  // Krylov loops and preconditioner work on A^T (adjoint solves)
+  void require_transpose() const;
+  struct TransScope {  // sets `trans` for one adjoint operation
+    Solver& s;
+    explicit TransScope(Solver& sv) : s(sv) { s.trans = true; }
+    ~TransScope() { s.trans = false; }
+  };
 
   // V-cycle hierarchy (FP64, or FP32 values with FP32/FP16 faces in mixed precision)
   struct VStore {
@@ -145,7 +166,19 @@ class Solver {
     DevBuf<double> de, dn, dt, ds, dx, db;
     DevBuf<float> fe, fn, ft, fs, fx, fb;
     DevBuf<_Float16> he, hn, ht;
+    DevBuf<double> dr;  // level residual of the transposed cycle
+    DevBuf<float> fr;
+    DevBuf<double> dxt;  // second iterate buffer of the line relaxation
+    DevBuf<float> fxt;
   };
+  DevBuf<double> dcp;  // line-relaxation scratch (finest level's size)
+  DevBuf<float> fcp;
+  template <class T>
+  T* vxt(VStore& S);
+  template <class T>
+  T* vcp();
+  template <class T>
+  void smooth(int l, T* x, const T* b, int sweeps, bool zero_guess);
   std::vector<VStore> vlev;
   DevBuf<float> Z32, ztmp32;
   bool mixed() const { return opt.precision == LSX_PREC_MIXED || opt.precision == LSX_PREC_MIXED16; }
@@ -153,6 +186,21 @@ class Solver {
   void build_vcycle();
   template <class T>
   void vcycle_apply(const double* v, T* z);
+  template <class T>
+  void vcycle_apply_tr(const double* v, T* z);
+  template <class T>
+  void vcycle_run(const double* v, T* z);
+  // captured V-cycle applications, one per (input, output) pair (opt.graphs)
+  struct CycleGraph {
+    const double* v;
+    const void* z;
+    hipGraphExec_t exec;
+  };
+  std::vector<CycleGraph> cgraphs;
+  bool capturing = false;
+  void clear_graphs();
+  template <class T>
+  T* vr(VStore& S);
   template <class T>
   T* vx(VStore& S);
   template <class T>

@@ -273,6 +273,111 @@ __device__ inline void axis_w(int f, int nf, int* c, float* w, int* cnt) {
   }
 }
 
+// ---- transposed cycle ------------------------------------------------------------------------
+// The transpose of the cycle runs its steps in reverse order, each transposed: red-black
+// half-sweeps in the opposite colour order on A^T, restriction by P^T (trilinear weights
+// gathered from the 4 x 4 x 4 fine cells around a coarse cell), prolongation by the transposed
+// sum restriction (every child receives its parent's value).  The coarse operators are
+// symmetric; on the finest level the pinned rows of A^T are full rows and every other row
+// drops its coupling to a pinned cell.
+template <class T, class FT>
+__device__ inline T offdiag_tr(const T* __restrict__ x, const FT* __restrict__ ae, const FT* __restrict__ an,
+                               const FT* __restrict__ at, long i, int xx, int yy, int zz, int N, int M, int K,
+                               int pin0, int pin1) {
+  const long NM = (long)N * M;
+  const bool own = i == pin0 || i == pin1;
+  T acc = 0;
+  auto add = [&](T c, long j) {
+    if (own || (j != pin0 && j != pin1)) acc += c * x[j];
+  };
+  if (xx > 0) add(ld<T>(ae, i - 1), i - 1);
+  if (xx < N - 1) add(ld<T>(ae, i), i + 1);
+  if (yy > 0) add(ld<T>(an, i - N), i - N);
+  if (yy < M - 1) add(ld<T>(an, i), i + N);
+  if (zz > 0) add(ld<T>(at, i - NM), i - NM);
+  if (zz < K - 1) add(ld<T>(at, i), i + NM);
+  return acc;
+}
+
+template <class T, class FT>
+__global__ void k_rbgs_tr(T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
+                          const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s, T fs, int N,
+                          int M, int K, int color, int pin0, int pin1, int zero_guess) {
+  const long half_row = (N + 1) / 2;
+  const long total = half_row * M * K;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const long row = t / half_row;
+    const int yy = (int)(row % M), zz = (int)(row / M);
+    const int xx = 2 * (int)(t - row * half_row) + ((yy + zz + color) & 1);
+    if (xx >= N) continue;
+    const long i = (long)xx + (long)N * row;
+    x[i] = zero_guess ? b[i] * s[i]
+                      : (b[i] - fs * offdiag_tr<T, FT>(x, ae, an, at, i, xx, yy, zz, N, M, K, pin0, pin1)) * s[i];
+  }
+}
+
+template <class T, class FT>
+__global__ void k_resid_tr(const T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
+                           const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s, T fs, int N,
+                           int M, int K, int pin0, int pin1, T* __restrict__ r) {
+  const long n = (long)N * M * K;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int xx, yy, zz;
+    decode(i, N, M, &xx, &yy, &zz);
+    const T ap = s[i] != (T)0 ? (T)1 / s[i] : (T)0;
+    r[i] = b[i] - ap * x[i] - fs * offdiag_tr<T, FT>(x, ae, an, at, i, xx, yy, zz, N, M, K, pin0, pin1);
+  }
+}
+
+// weight of fine cell f in coarse cell C along one axis under the trilinear prolongation
+__device__ inline float axis_wt(int f, int nf, int C) {
+  int c[2], cnt;
+  float w[2];
+  axis_w(f, nf, c, w, &cnt);
+  float v = 0.0f;
+  for (int k = 0; k < cnt; ++k) v = c[k] == C ? w[k] : v;
+  return v;
+}
+
+template <class T>
+__global__ void k_restrict_pt(const T* __restrict__ r, int Nf, int Mf, int Kf, T* __restrict__ bc) {
+  const int Nc = Nf / 2, Mc = Mf / 2, Kc = Kf / 2;
+  const long nc = (long)Nc * Mc * Kc, NMf = (long)Nf * Mf;
+  for (long I = blockIdx.x * (long)blockDim.x + threadIdx.x; I < nc; I += (long)gridDim.x * blockDim.x) {
+    int X, Y, Z;
+    decode(I, Nc, Mc, &X, &Y, &Z);
+    T acc = 0;
+    for (int fz = 2 * Z - 1; fz <= 2 * Z + 2; ++fz) {
+      if (fz < 0 || fz >= Kf) continue;
+      const float wz = axis_wt(fz, Kf, Z);
+      if (wz == 0.0f) continue;
+      for (int fy = 2 * Y - 1; fy <= 2 * Y + 2; ++fy) {
+        if (fy < 0 || fy >= Mf) continue;
+        const float wy = axis_wt(fy, Mf, Y);
+        if (wy == 0.0f) continue;
+        for (int fx = 2 * X - 1; fx <= 2 * X + 2; ++fx) {
+          if (fx < 0 || fx >= Nf) continue;
+          const float wx = axis_wt(fx, Nf, X);
+          if (wx == 0.0f) continue;
+          acc += (T)(wx * wy * wz) * r[fx + (long)Nf * fy + NMf * fz];
+        }
+      }
+    }
+    bc[I] = acc;
+  }
+}
+
+template <class T>
+__global__ void k_prolong_inject_add(const T* __restrict__ xc, int Nc, int Mc, T* __restrict__ xf, int Nf, int Mf,
+                                     int Kf) {
+  const long nf = (long)Nf * Mf * Kf, NMc = (long)Nc * Mc;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nf; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, Nf, Mf, &x, &y, &z);
+    xf[i] += xc[(x >> 1) + (long)Nc * (y >> 1) + NMc * (z >> 1)];
+  }
+}
+
 template <class T>
 __device__ inline void prolong_add_body(const T* __restrict__ xc, int Nc, int Mc, T* __restrict__ xf, int Nf, int Mf,
                                         int Kf, long start, long stride) {
@@ -368,6 +473,206 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse_cycle(CoarseLevels<T>
 template <class A, class B>
 __global__ void k_convert(const A* __restrict__ a, B* __restrict__ b, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = (B)a[i];
+}
+
+// ---- line relaxation ---------------------------------------------------------------------------
+// One pass solves every line of direction d (0 x, 1 y, 2 z) exactly, with the couplings across
+// lines taken from x_in (line Jacobi): x_out = T_d^{-1} (b - (A - T_d) x_in), T_d the tridiagonal
+// part of A along d.  zero: x_in is 0 (first pass of a cycle).  A sweep is the three directions
+// in turn; on grids stretched towards the body, where cells are long in one or two directions
+// and point smoothers stall, every strong coupling is solved exactly in one of them.
+//
+// Row of cell i on its line: lower (coupling to the previous cell of the line), diag, upper,
+// rhs.  Pinned rows are identity rows; an empty row (1/a_p stored as 0) gives x = 0, as in the
+// point smoother.
+template <class T, class FT>
+__device__ inline void line_row(int d, const T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
+                                const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s, T fs,
+                                long i, int xx, int yy, int zz, int N, int M, int K, int pin0, int pin1, bool zero,
+                                T& lo, T& dg, T& up, T& rhs) {
+  const long NM = (long)N * M;
+  if (i == pin0 || i == pin1) {
+    lo = up = (T)0;
+    dg = (T)1;
+    rhs = b[i];
+    return;
+  }
+  const T si = s[i];
+  if (si == (T)0) {
+    lo = up = (T)0;
+    dg = (T)1;
+    rhs = (T)0;
+    return;
+  }
+  dg = (T)1 / si;
+  T off = (T)0;
+  if (d == 0) {
+    lo = xx > 0 ? fs * ld<T>(ae, i - 1) : (T)0;
+    up = xx < N - 1 ? fs * ld<T>(ae, i) : (T)0;
+    if (!zero) {
+      if (yy > 0) off += fs * ld<T>(an, i - N) * x[i - N];
+      if (yy < M - 1) off += fs * ld<T>(an, i) * x[i + N];
+      if (zz > 0) off += fs * ld<T>(at, i - NM) * x[i - NM];
+      if (zz < K - 1) off += fs * ld<T>(at, i) * x[i + NM];
+    }
+  } else if (d == 1) {
+    lo = yy > 0 ? fs * ld<T>(an, i - N) : (T)0;
+    up = yy < M - 1 ? fs * ld<T>(an, i) : (T)0;
+    if (!zero) {
+      if (xx > 0) off += fs * ld<T>(ae, i - 1) * x[i - 1];
+      if (xx < N - 1) off += fs * ld<T>(ae, i) * x[i + 1];
+      if (zz > 0) off += fs * ld<T>(at, i - NM) * x[i - NM];
+      if (zz < K - 1) off += fs * ld<T>(at, i) * x[i + NM];
+    }
+  } else {
+    lo = zz > 0 ? fs * ld<T>(at, i - NM) : (T)0;
+    up = zz < K - 1 ? fs * ld<T>(at, i) : (T)0;
+    if (!zero) {
+      if (xx > 0) off += fs * ld<T>(ae, i - 1) * x[i - 1];
+      if (xx < N - 1) off += fs * ld<T>(ae, i) * x[i + 1];
+      if (yy > 0) off += fs * ld<T>(an, i - N) * x[i - N];
+      if (yy < M - 1) off += fs * ld<T>(an, i) * x[i + N];
+    }
+  }
+  rhs = b[i] - off;
+}
+
+// x lines: one wavefront per line, CPL consecutive cells per lane (coalesced loads), solved by
+// parallel cyclic reduction over the 64 * CPL (padded with identity rows) equations.
+template <class T>
+__device__ inline T wave_get(T v, int src, bool ok) {
+  const T r = __shfl(v, ok ? src : 0, 64);
+  return r;
+}
+
+template <class T, class FT, int CPL>
+__global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* __restrict__ x_out,
+                                                const T* __restrict__ b, const FT* __restrict__ ae,
+                                                const FT* __restrict__ an, const FT* __restrict__ at,
+                                                const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
+                                                int pin1, int zero) {
+  const int lane = threadIdx.x & 63;
+  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= (long)M * K) return;  // whole wavefronts leave together
+  const int yy = (int)(row % M), zz = (int)(row / M);
+  constexpr int L = 64 * CPL;
+  T a[CPL], d[CPL], c[CPL], r[CPL];
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) {
+    const int k = lane * CPL + j;
+    if (k < N) {
+      line_row<T, FT>(0, x_in, b, ae, an, at, s, fs, row * N + k, k, yy, zz, N, M, K, pin0, pin1, zero != 0, a[j],
+                      d[j], c[j], r[j]);
+    } else {
+      a[j] = c[j] = r[j] = (T)0;
+      d[j] = (T)1;
+    }
+  }
+#pragma unroll
+  for (int st = 1; st < L; st <<= 1) {
+    T na[CPL], nd[CPL], nc[CPL], nr[CPL];
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      const int k = lane * CPL + j;
+      // equation k - st
+      T am, dm, cm, rm;
+      if (st < CPL && j - st >= 0) {
+        am = a[(j - st) & (CPL - 1)];
+        dm = d[(j - st) & (CPL - 1)];
+        cm = c[(j - st) & (CPL - 1)];
+        rm = r[(j - st) & (CPL - 1)];
+      } else {
+        const int jj = st < CPL ? ((j - st + CPL) & (CPL - 1)) : j;
+        const int src = st < CPL ? lane - 1 : lane - st / CPL;
+        const bool ok = k - st >= 0;
+        am = wave_get(a[jj], src, ok);
+        dm = wave_get(d[jj], src, ok);
+        cm = wave_get(c[jj], src, ok);
+        rm = wave_get(r[jj], src, ok);
+        if (!ok) {
+          am = cm = rm = (T)0;
+          dm = (T)1;
+        }
+      }
+      // equation k + st
+      T ap, dp, cp, rp;
+      if (st < CPL && j + st < CPL) {
+        ap = a[(j + st) & (CPL - 1)];
+        dp = d[(j + st) & (CPL - 1)];
+        cp = c[(j + st) & (CPL - 1)];
+        rp = r[(j + st) & (CPL - 1)];
+      } else {
+        const int jj = st < CPL ? ((j + st) & (CPL - 1)) : j;
+        const int src = st < CPL ? lane + 1 : lane + st / CPL;
+        const bool ok = k + st < L;
+        ap = wave_get(a[jj], src, ok);
+        dp = wave_get(d[jj], src, ok);
+        cp = wave_get(c[jj], src, ok);
+        rp = wave_get(r[jj], src, ok);
+        if (!ok) {
+          ap = cp = rp = (T)0;
+          dp = (T)1;
+        }
+      }
+      const T al = dm != (T)0 ? -a[j] / dm : (T)0;
+      const T ga = dp != (T)0 ? -c[j] / dp : (T)0;
+      na[j] = al * am;
+      nc[j] = ga * cp;
+      nd[j] = d[j] + al * cm + ga * ap;
+      nr[j] = r[j] + al * rm + ga * rp;
+    }
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      a[j] = na[j];
+      d[j] = nd[j];
+      c[j] = nc[j];
+      r[j] = nr[j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) {
+    const int k = lane * CPL + j;
+    if (k < N) x_out[row * N + k] = d[j] != (T)0 ? r[j] / d[j] : (T)0;
+  }
+}
+
+// y and z lines: one thread per line, Thomas elimination; the forward sweep keeps the
+// eliminated upper coefficients in `cpv` and the partial solution in x_out.
+template <class T, class FT>
+__global__ void k_line_t(int dir, const T* __restrict__ x_in, T* __restrict__ x_out, T* __restrict__ cpv,
+                         const T* __restrict__ b, const FT* __restrict__ ae, const FT* __restrict__ an,
+                         const FT* __restrict__ at, const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
+                         int pin1, int zero) {
+  const long NM = (long)N * M;
+  const long lines = dir == 1 ? (long)N * K : NM;
+  const int len = dir == 1 ? M : K;
+  const long stride = dir == 1 ? (long)N : NM;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < lines; t += (long)gridDim.x * blockDim.x) {
+    const int xx = (int)(t % N);
+    const int other = (int)(t / N);  // z for y lines, y for z lines
+    const long base = dir == 1 ? xx + NM * other : xx + (long)N * other;
+    T cprev = (T)0, gprev = (T)0;
+#pragma unroll 4
+    for (int k = 0; k < len; ++k) {
+      const long i = base + stride * k;
+      const int yy = dir == 1 ? k : other, zz = dir == 1 ? other : k;
+      T lo, dg, up, rhs;
+      line_row<T, FT>(dir, x_in, b, ae, an, at, s, fs, i, xx, yy, zz, N, M, K, pin0, pin1, zero != 0, lo, dg, up,
+                      rhs);
+      const T w = dg - lo * cprev;
+      const T winv = w != (T)0 ? (T)1 / w : (T)0;
+      cprev = up * winv;
+      gprev = (rhs - lo * gprev) * winv;
+      cpv[i] = cprev;
+      x_out[i] = gprev;
+    }
+    T xn = gprev;
+    for (int k = len - 2; k >= 0; --k) {
+      const long i = base + stride * k;
+      xn = x_out[i] - cpv[i] * xn;
+      x_out[i] = xn;
+    }
+  }
 }
 
 }  // namespace
@@ -468,6 +773,104 @@ void resid_restrict<float>(const Level& L, const float* x, const float* b, float
   else
     resid_restrict_t<float, float>(L, x, b, bc, s);
 }
+
+template <class T, class FT>
+static void rbgs_tr_t(const Level& L, T* x, const T* b, int color, int zero_guess, hipStream_t s) {
+  const long total = (long)((L.d.N + 1) / 2) * L.d.M * L.d.K;
+  k_rbgs_tr<T, FT><<<grid_1d(total), 256, 0, s>>>(x, b, (const FT*)L.f_e, (const FT*)L.f_n, (const FT*)L.f_t,
+                                                  (const T*)L.s, (T)L.fscale, L.d.N, L.d.M, L.d.K, color, L.pin0,
+                                                  L.pin1, zero_guess);
+}
+template <class T, class FT>
+static void resid_tr_t(const Level& L, const T* x, const T* b, T* r, hipStream_t s) {
+  k_resid_tr<T, FT><<<grid_1d(L.d.n()), 256, 0, s>>>(x, b, (const FT*)L.f_e, (const FT*)L.f_n, (const FT*)L.f_t,
+                                                     (const T*)L.s, (T)L.fscale, L.d.N, L.d.M, L.d.K, L.pin0,
+                                                     L.pin1, r);
+}
+
+template <>
+void rbgs_tr<double>(const Level& L, double* x, const double* b, int color, int zero_guess, hipStream_t s) {
+  rbgs_tr_t<double, double>(L, x, b, color, zero_guess, s);
+}
+template <>
+void rbgs_tr<float>(const Level& L, float* x, const float* b, int color, int zero_guess, hipStream_t s) {
+  if (L.ftype == FACE_F16)
+    rbgs_tr_t<float, _Float16>(L, x, b, color, zero_guess, s);
+  else
+    rbgs_tr_t<float, float>(L, x, b, color, zero_guess, s);
+}
+template <>
+void resid_tr<double>(const Level& L, const double* x, const double* b, double* r, hipStream_t s) {
+  resid_tr_t<double, double>(L, x, b, r, s);
+}
+template <>
+void resid_tr<float>(const Level& L, const float* x, const float* b, float* r, hipStream_t s) {
+  if (L.ftype == FACE_F16)
+    resid_tr_t<float, _Float16>(L, x, b, r, s);
+  else
+    resid_tr_t<float, float>(L, x, b, r, s);
+}
+
+template <class T>
+void restrict_pt(const T* r, Dims df, T* bc, hipStream_t s) {
+  const long nc = (long)(df.N / 2) * (df.M / 2) * (df.K / 2);
+  k_restrict_pt<T><<<grid_1d(nc), 256, 0, s>>>(r, df.N, df.M, df.K, bc);
+}
+template <class T>
+void prolong_inject_add(const T* xc, Dims dc, T* xf, Dims df, hipStream_t s) {
+  k_prolong_inject_add<T><<<grid_1d(df.n()), 256, 0, s>>>(xc, dc.N, dc.M, xf, df.N, df.M, df.K);
+}
+template void restrict_pt<float>(const float*, Dims, float*, hipStream_t);
+template void restrict_pt<double>(const double*, Dims, double*, hipStream_t);
+template void prolong_inject_add<float>(const float*, Dims, float*, Dims, hipStream_t);
+template void prolong_inject_add<double>(const double*, Dims, double*, Dims, hipStream_t);
+
+template <class T, class FT>
+static void line_pass_t(const Level& L, int dir, const T* x_in, T* x_out, T* cpv, const T* b, int zero,
+                        hipStream_t s) {
+  const Dims& d = L.d;
+  const FT* ae = (const FT*)L.f_e;
+  const FT* an = (const FT*)L.f_n;
+  const FT* at = (const FT*)L.f_t;
+  const T* sv = (const T*)L.s;
+  const T fs = (T)L.fscale;
+  if (dir == 0) {
+    const int blocks = (int)(((long)d.M * d.K + 3) / 4);
+    if (d.N <= 64)
+      k_line_x<T, FT, 1><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero);
+    else if (d.N <= 128)
+      k_line_x<T, FT, 2><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero);
+    else if (d.N <= 256)
+      k_line_x<T, FT, 4><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero);
+    else
+      k_line_x<T, FT, 8><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero);
+  } else {
+    const long lines = dir == 1 ? (long)d.N * d.K : (long)d.N * d.M;
+    k_line_t<T, FT><<<grid_1d(lines, 64), 64, 0, s>>>(dir, x_in, x_out, cpv, b, ae, an, at, sv, fs, d.N, d.M, d.K,
+                                                     L.pin0, L.pin1, zero);
+  }
+}
+
+template <>
+void line_pass<double>(const Level& L, int dir, const double* x_in, double* x_out, double* cpv, const double* b,
+                       int zero, hipStream_t s) {
+  line_pass_t<double, double>(L, dir, x_in, x_out, cpv, b, zero, s);
+}
+template <>
+void line_pass<float>(const Level& L, int dir, const float* x_in, float* x_out, float* cpv, const float* b,
+                      int zero, hipStream_t s) {
+  if (L.ftype == FACE_F16)
+    line_pass_t<float, _Float16>(L, dir, x_in, x_out, cpv, b, zero, s);
+  else
+    line_pass_t<float, float>(L, dir, x_in, x_out, cpv, b, zero, s);
+}
+
+template <class T>
+void copy_vec(const T* a, T* b, long n, hipStream_t s) {
+  k_convert<T, T><<<grid_1d(n), 256, 0, s>>>(a, b, n);
+}
+template void copy_vec<float>(const float*, float*, long, hipStream_t);
+template void copy_vec<double>(const double*, double*, long, hipStream_t);
 
 template <class T>
 void coarse_cycle(const CoarseLevels<T>& C, int n_pre, int n_post, const double* lu_scale, const int* lu_perm,

@@ -86,6 +86,40 @@ template <class T>
 void coarse_cycle(const CoarseLevels<T>& C, int n_pre, int n_post, const double* lu_scale, const int* lu_perm,
                   const double* L, const double* U, hipStream_t s);
 
+// transposed cycle: one colour of a half-sweep on A^T (the finest level's pinned rows are full
+// rows of A^T, other rows drop their couplings to them); r = b - A^T x on the whole level;
+// bc = P^T r (trilinear weights); x_f += R^T x_c (each child gets its parent's value)
+template <class T>
+void rbgs_tr(const Level& L, T* x, const T* b, int color, int zero_guess, hipStream_t s);
+template <class T>
+void resid_tr(const Level& L, const T* x, const T* b, T* r, hipStream_t s);
+template <class T>
+void restrict_pt(const T* r, Dims df, T* bc, hipStream_t s);
+template <class T>
+void prolong_inject_add(const T* xc, Dims dc, T* xf, Dims df, hipStream_t s);
+
+template <>
+void rbgs_tr<double>(const Level& L, double* x, const double* b, int color, int zero_guess, hipStream_t s);
+template <>
+void rbgs_tr<float>(const Level& L, float* x, const float* b, int color, int zero_guess, hipStream_t s);
+template <>
+void resid_tr<double>(const Level& L, const double* x, const double* b, double* r, hipStream_t s);
+template <>
+void resid_tr<float>(const Level& L, const float* x, const float* b, float* r, hipStream_t s);
+
+// line relaxation: every line of direction dir (0 x, 1 y, 2 z) solved exactly against x_in's
+// values across lines (x_in unread when zero); cpv: scratch of the level's size
+template <class T>
+void line_pass(const Level& L, int dir, const T* x_in, T* x_out, T* cpv, const T* b, int zero, hipStream_t s);
+template <>
+void line_pass<double>(const Level& L, int dir, const double* x_in, double* x_out, double* cpv, const double* b,
+                       int zero, hipStream_t s);
+template <>
+void line_pass<float>(const Level& L, int dir, const float* x_in, float* x_out, float* cpv, const float* b,
+                      int zero, hipStream_t s);
+template <class T>
+void copy_vec(const T* a, T* b, long n, hipStream_t s);
+
 template <>
 void rbgs<double>(const Level& L, double* x, const double* b, int color, int zero_guess, hipStream_t s);
 template <>

@@ -67,7 +67,7 @@ class Options(C.Structure):
         ("krylov", C.c_int), ("restart", C.c_int), ("max_it", C.c_int), ("eps", C.c_double),
         ("mg", C.c_int), ("n_post", C.c_int), ("n_pre", C.c_int), ("omega", C.c_double),
         ("precision", C.c_int), ("levels", C.c_int), ("z_head_fix", C.c_int), ("ortho", C.c_int),
-        ("coarse_fuse", C.c_int),
+        ("coarse_fuse", C.c_int), ("graphs", C.c_int), ("smoother", C.c_int),
     ]
 
 
@@ -96,6 +96,18 @@ def lib():
     L.lsx_get_stats.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.lsx_precondition.argtypes = [vp, vp, vp, C.c_int]
     L.lsx_apply_operator.argtypes = [vp, C.c_int, vp, vp, C.c_int]
+    L.lsx_solve_adjoint.argtypes = [vp, vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.lsx_geometry_drag.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_double), vp]
+    L.lsx_geometry_shape_gradient.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, C.POINTER(FlowBc), C.c_int,
+                                              C.c_double, C.c_int, vp, vp]
+    L.lsx_adjoint_forward.argtypes = [vp, vp, C.c_int, vp, vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_double)]
+    L.lsx_adjoint_backward.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp, vp, C.POINTER(FlowBc), C.c_double, C.c_int,
+                                       vp, vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.lsx_tape_destroy.argtypes = [vp]
+    L.lsx_precondition_transpose.argtypes = [vp, vp, vp, C.c_int]
+    L.lsx_smooth.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int]
+    L.lsx_apply_operator_transpose.argtypes = [vp, vp, vp, C.c_int]
     L.lsx_level_dims.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.lsx_level_dims.restype = C.c_long
     L.lsx_level_stencil.argtypes = [vp, C.c_int, vp, C.c_long]
@@ -155,6 +167,10 @@ def bezier_perturb(ctrl, amplitude, seed):
         for (f, i, j) in copies[key]:
             ctrl[f, i, j] += d
     return ctrl
+
+
+OBJ_NONE = 0
+OBJ_DRAG_X = 1
 
 
 class FlowBc(C.Structure):
@@ -228,6 +244,49 @@ class Geometry:
                                       C.byref(bc), _ptr(b), MEM_HOST))
         return b
 
+    def flow_args(self, u, v, w, U_W=1.0, U_E=1.0, V_S=0.0, V_N=0.0, W_B=0.0, W_T=0.0, outflow=True, pin=True,
+                  u_out=None):
+        """The flow fields of a pressure right-hand side as C arguments (arrays kept alive)."""
+        M, K = self.dims[1], self.dims[2]
+        nu, nv, nw = self.face_shapes()
+        arr = [np.ascontiguousarray(a, np.float64) for a in (u, v, w)]
+        if arr[0].size != nu or arr[1].size != nv or arr[2].size != nw:
+            raise LsxError("face arrays have the wrong sizes")
+        arr.append(np.ascontiguousarray(np.broadcast_to(np.asarray(U_W, np.float64), (M * K,))))
+        arr.append(None if u_out is None else np.ascontiguousarray(u_out, np.float64))
+        bc = FlowBc(U_E, V_S, V_N, W_B, W_T, 1 if outflow else 0, 1 if pin else 0)
+        return arr, bc
+
+    def drag(self, p):
+        """Pressure drag J = sum over cut cells of (theta_x[i-1] - theta_x[i]) dy dz p[i]
+        (pressure term of the reference LoadsCalculator::calc); returns (J, dJ/dp)."""
+        p = np.ascontiguousarray(p, np.float64)
+        if p.size != self.n:
+            raise LsxError("p has the wrong size")
+        J = C.c_double(0.0)
+        g = np.zeros(self.n)
+        _check(lib().lsx_geometry_drag(self.h, _ptr(p), MEM_HOST, C.byref(J), _ptr(g)))
+        return J.value, g
+
+    def shape_gradient(self, p, lam, objective=OBJ_NONE, flow=None, pin=True, h_rel=1e-7, ctrl=True):
+        """Shape gradient at the solution p with adjoint lam = A^{-T} dJ/dp: returns
+        (dJ/dphi per level-set corner, dJ/dctrl[288] or None).  flow: None for a right-hand side
+        that does not depend on the geometry, else a dict of pressure_rhs arguments."""
+        p = np.ascontiguousarray(p, np.float64)
+        lam = np.ascontiguousarray(lam, np.float64)
+        gphi = np.zeros(self.n)
+        gctrl = np.zeros(288) if ctrl else None
+        if flow is None:
+            fa, bc = [None] * 5, None
+        else:
+            fa, bc = self.flow_args(**flow)
+        ptrs = [None if a is None else _ptr(a) for a in fa]
+        _check(lib().lsx_geometry_shape_gradient(self.h, _ptr(p), _ptr(lam), int(objective), *ptrs,
+                                                 None if bc is None else C.byref(bc), 1 if pin else 0,
+                                                 float(h_rel), MEM_HOST, _ptr(gphi),
+                                                 None if gctrl is None else _ptr(gctrl)))
+        return gphi, gctrl
+
     def correct(self, dp, us, vs, ws, p=None, coef=1.0):
         """Projection correction on the device (reference SetSolutions::correction): returns the new
         face velocities (u, v, w); p (host array) is updated in place by p += dp * coef."""
@@ -281,9 +340,14 @@ class IterativeSolver:
 
     krylov = KRYLOV_FGMRES
 
-    def __init__(self, precision=PREC_FP64, mg=MG_CASCADE, z_head_fix=False, stream=None, ortho=0, coarse_fuse=True):
+    def __init__(self, precision=PREC_FP64, mg=MG_CASCADE, z_head_fix=False, stream=None, ortho=0, coarse_fuse=True,
+                 graphs=False, smoother="rbgs"):
         self.ortho = ortho
         self.coarse_fuse = coarse_fuse
+        self.graphs = graphs
+        if smoother not in ("rbgs", "line"):
+            raise ValueError(f"unknown smoother {smoother!r}")
+        self.smoother = smoother
         self.precision = precision
         self.mg = mg
         self.z_head_fix = z_head_fix
@@ -313,6 +377,9 @@ class IterativeSolver:
         opt.z_head_fix = 1 if self.z_head_fix else 0
         opt.ortho = int(self.ortho)
         opt.coarse_fuse = 1 if self.coarse_fuse else 0
+        opt.graphs = 1 if self.graphs else 0
+        opt.levels = int(getattr(self, "levels_cap", 0))
+        opt.smoother = 1 if self.smoother == "line" else 0
         h = C.c_void_p()
         _check(lib().lsx_create(C.byref(h), C.byref(opt), self.stream))
         self.h = h
@@ -343,6 +410,9 @@ class IterativeSolver:
         opt.z_head_fix = 1 if self.z_head_fix else 0
         opt.ortho = int(self.ortho)
         opt.coarse_fuse = 1 if self.coarse_fuse else 0
+        opt.graphs = 1 if self.graphs else 0
+        opt.levels = int(getattr(self, "levels_cap", 0))
+        opt.smoother = 1 if self.smoother == "line" else 0
         h = C.c_void_p()
         _check(lib().lsx_create(C.byref(h), C.byref(opt), self.stream))
         self.h = h
@@ -376,6 +446,53 @@ class IterativeSolver:
         z = np.zeros_like(v)
         _check(lib().lsx_precondition(self.h, _ptr(v), _ptr(z), MEM_HOST))
         return z
+
+    def solve_adjoint(self, g, lam=None):
+        """lambda = A^{-T} g (FGMRES / BiCGStab on A^T with the transposed V-cycle); returns
+        (lambda, iterations, |g - A^T lambda|).  `lam` is the initial guess (0 by default)."""
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        lam = np.zeros_like(g) if lam is None else np.array(lam, dtype=np.float64, copy=True)
+        _check(lib().lsx_set_eps(self.h, self.eps))
+        it = C.c_int(0)
+        nr = C.c_double(0.0)
+        _check(lib().lsx_solve_adjoint(self.h, _ptr(g), _ptr(lam), MEM_HOST, C.byref(it), C.byref(nr)))
+        return lam, it.value, nr.value
+
+    def adjoint_forward(self, geometry, b, pin=True, p0=None):
+        """Solve A(geometry) p = b and record a tape for adjoint_backward; returns (p, it, tape)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        p = np.zeros_like(b) if p0 is None else np.array(p0, dtype=np.float64, copy=True)
+        _check(lib().lsx_set_eps(self.h, self.eps))
+        tape = C.c_void_p()
+        it = C.c_int(0)
+        nr = C.c_double(0.0)
+        _check(lib().lsx_adjoint_forward(self.h, geometry.h, 1 if pin else 0, _ptr(b), _ptr(p), MEM_HOST,
+                                         C.byref(tape), C.byref(it), C.byref(nr)))
+        self.it, self.norm_r = it.value, nr.value
+        return p, it.value, Tape(tape, geometry, self)
+
+    def use_geometry(self, geometry, pin=True):
+        """Replace the operator by that of `geometry` on this handle (options unchanged)."""
+        _check(lib().lsx_set_geometry(self.h, geometry.h, 1 if pin else 0))
+
+    def smooth(self, b, x, sweeps=1, zero_guess=False):
+        """`sweeps` sweeps of the V-cycle smoother on the finest level; returns the new x."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.array(x, dtype=np.float64, copy=True)
+        _check(lib().lsx_smooth(self.h, _ptr(b), _ptr(x), int(sweeps), 1 if zero_guess else 0, MEM_HOST))
+        return x
+
+    def precondition_transpose(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        z = np.zeros_like(v)
+        _check(lib().lsx_precondition_transpose(self.h, _ptr(v), _ptr(z), MEM_HOST))
+        return z
+
+    def apply_operator_transpose(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        _check(lib().lsx_apply_operator_transpose(self.h, _ptr(x), _ptr(y), MEM_HOST))
+        return y
 
     def apply_operator(self, x, level=0):
         x = np.ascontiguousarray(x, dtype=np.float64)
@@ -447,6 +564,49 @@ class MGFGMRES(IterativeSolver):
 class MGBiCGStab(IterativeSolver):
     """Reference MGBiCGStab (src/Base/Solvers/BiCGStab/mg_bicgstab.h:33-52)."""
     krylov = KRYLOV_BICGSTAB
+
+
+class Tape:
+    """Record of a forward pressure solve (lsx_adjoint_forward).  It refers to the solver handle
+    and the geometry of the forward call, which must stay alive (a solver re-created by set() or
+    set_geometry() invalidates the tape)."""
+
+    def __init__(self, h, geometry, solver):
+        self.h = h
+        self.geometry = geometry
+        self.solver = solver
+        self.solver_h = solver.h.value
+
+    def backward(self, dJdp, objective=OBJ_NONE, flow=None, h_rel=1e-7, ctrl=True):
+        """dJ/db = A^{-T} dJ/dp and (ctrl) dJ/dctrl; returns (dJdb, dJdctrl or None, adjoint iterations)."""
+        if self.solver.h is None or self.solver.h.value != self.solver_h:
+            raise LsxError("the tape's solver handle was closed or re-created")
+        _check(lib().lsx_set_eps(self.solver.h, self.solver.eps))
+        g = np.ascontiguousarray(dJdp, np.float64)
+        dJdb = np.zeros_like(g)
+        gctrl = np.zeros(288) if ctrl else None
+        if flow is None:
+            fa, bc = [None] * 5, None
+        else:
+            fa, bc = self.geometry.flow_args(**flow)
+        ptrs = [None if a is None else _ptr(a) for a in fa]
+        it = C.c_int(0)
+        nr = C.c_double(0.0)
+        _check(lib().lsx_adjoint_backward(self.h, _ptr(g), int(objective), *ptrs, None if bc is None else C.byref(bc),
+                                          float(h_rel), MEM_HOST, _ptr(dJdb), None if gctrl is None else _ptr(gctrl),
+                                          C.byref(it), C.byref(nr)))
+        return dJdb, gctrl, it.value
+
+    def close(self):
+        if self.h:
+            lib().lsx_tape_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def mixed_task(eps, restart=30, max_it=1000, n_pre=2, n_post=2):
