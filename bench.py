@@ -130,9 +130,11 @@ def cpu_baseline(sample_n, gpu_n, restart, krylov, body="bezier"):
     }
 
 
-def run_solver(lsx, hip, geom, precision, krylov, restart, steps, warmup, max_it=1000, smooth=(2, 2), graphs=False):
+def run_solver(lsx, hip, geom, precision, krylov, restart, steps, warmup, max_it=1000, smooth=(2, 2), graphs=False,
+               smoother="rbgs"):
     """Set up one solver on the device geometry and time `steps` solves (x0 = 0 each)."""
-    S = lsx.solver(krylov, precision, graphs=graphs) if precision != "fp64" else lsx.solver(krylov, precision)
+    S = lsx.solver(krylov, precision, graphs=graphs, smoother=smoother) if precision != "fp64" else \
+        lsx.solver(krylov, precision)
     task = lsx.Task(eps=1.0, MAX_ITER=max_it, FGMRES_RESTART_NUM=restart) if precision == "fp64" else \
         lsx.mixed_task(1.0, restart=restart, max_it=max_it, n_pre=smooth[0], n_post=smooth[1])
     t_setup = time.perf_counter()
@@ -231,6 +233,8 @@ def main():
     ap.add_argument("--cpu-sample-grid", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graphs", action="store_true", help="replay the V-cycles from captured HIP graphs")
+    ap.add_argument("--smoother", choices=["rbgs", "line"], default="rbgs",
+                    help="V-cycle smoother: red-black Gauss-Seidel or x/y/z line relaxation")
     ap.add_argument("--projection-steps", type=int, default=0,
                     help="instead of the solve benchmark: this many time steps of the 384x256x256 sphere "
                          "projection run (one JSON line of per-step results)")
@@ -259,13 +263,13 @@ def main():
 
     smooth = tuple(int(q) for q in args.smooth.split(","))
     main_run = run_solver(lsx, hip, geom, args.precision, args.krylov, args.restart, args.steps, args.warmup,
-                          smooth=smooth, graphs=args.graphs)
+                          smooth=smooth, graphs=args.graphs, smoother=args.smoother)
     print(f"[bench] {args.precision}: {args.steps} solves in {main_run['elapsed']:.3f} s, its {main_run['its']}",
           file=sys.stderr, flush=True)
     fp64 = None
     if args.precision in ("mixed", "mixed16") and not args.no_fp64_compare:
         fp64 = run_solver(lsx, hip, geom, "fp64-vcycle", args.krylov, args.restart, max(1, args.steps // 2), 1,
-                          smooth=smooth, graphs=args.graphs)
+                          smooth=smooth, graphs=args.graphs, smoother=args.smoother)
         print(f"[bench] fp64-vcycle: {len(fp64['its'])} solves in {fp64['elapsed']:.3f} s, its {fp64['its']}",
               file=sys.stderr, flush=True)
     cascade = None
@@ -291,17 +295,19 @@ def main():
         except (ValueError, OSError):
             traffic = None
     value = args.steps / elapsed
+    sm_name = "red-black GS" if args.smoother == "rbgs" else "x/y/z line-relaxation"
     cfg = {
         "workload": (f"{n}^3 stretched grid (ratio 1.02 to centre), "
                      + ("six-patch bicubic Bezier body fitted to the ellipsoid 0.25/0.15/0.15, control points "
                         "+ U(-0.01, 0.01) (seed 0)" if args.body == "bezier" else "ellipsoid body 0.25/0.15/0.15")
                      + f", {args.krylov.upper()}"
                      f"({args.restart}), rel tol 1e-8, "
-                     + {"mixed": f"mixed: FP32 V{smooth} red-black GS multigrid, FP16 coarse faces, FP64 Krylov",
-                        "mixed16": f"mixed: FP32 V{smooth} red-black GS multigrid, FP16 faces, FP64 Krylov",
-                        "fp64-vcycle": f"FP64 V{smooth} red-black GS multigrid",
+                     + {"mixed": f"mixed: FP32 V{smooth} {sm_name} multigrid, FP16 coarse faces, FP64 Krylov",
+                        "mixed16": f"mixed: FP32 V{smooth} {sm_name} multigrid, FP16 faces, FP64 Krylov",
+                        "fp64-vcycle": f"FP64 V{smooth} {sm_name} multigrid",
                         "fp64": "FP64 reference multigrid cascade (ADLJ)"}[args.precision]),
-        "precision": args.precision, "krylov": args.krylov, "cells": geom.n,
+        "precision": args.precision, "krylov": args.krylov, "smoother": args.smoother, "graphs": args.graphs,
+        "cells": geom.n,
         "iterations": main_run["its"], "true_rel_residual": main_run["rel_res"],
         "geometry_setup_s": geo_s, "solver_setup_s": main_run["setup_s"], "mg_levels": main_run["levels"],
     }

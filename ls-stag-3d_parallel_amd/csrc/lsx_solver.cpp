@@ -575,8 +575,14 @@ void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess) {
   VStore& S = vlev[l];
   if (opt.smoother == LSX_SMOOTH_RBGS) {
     for (int s = 0; s < sweeps; ++s) {
-      vc::rbgs<T>(S.L, x, b, 0, zero_guess && s == 0, stream);
-      vc::rbgs<T>(S.L, x, b, 1, 0, stream);
+      const bool first = zero_guess && s == 0;
+      if (trans) {  // transposed sweeps: black then red, on A^T
+        vc::rbgs_tr<T>(S.L, x, b, 1, first ? 1 : 0, stream);
+        vc::rbgs_tr<T>(S.L, x, b, 0, first ? 2 : 0, stream);
+      } else {
+        vc::rbgs<T>(S.L, x, b, 0, first ? 1 : 0, stream);
+        vc::rbgs<T>(S.L, x, b, 1, first ? 2 : 0, stream);
+      }
     }
     return;
   }
@@ -591,7 +597,9 @@ void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess) {
   }
   for (int p = 1; p <= P; ++p) {
     T* dst = (P - p) % 2 == 0 ? x : xt;
-    vc::line_pass<T>(S.L, (p - 1) % 3, src, dst, cpv, b, zero_guess && p == 1, stream);
+    // x, y, z lines in turn; the transposed cycle runs them in the opposite order on A^T
+    const int dir = trans ? 2 - (p - 1) % 3 : (p - 1) % 3;
+    vc::line_pass<T>(S.L, dir, src, dst, cpv, b, zero_guess && p == 1, trans ? 1 : 0, stream);
     src = dst;
   }
 }
@@ -622,10 +630,7 @@ void Solver::vcycle_apply_tr(const double* v, T* z) {
     VStore& S = vlev[l];
     T* x = l == 0 ? z : vx<T>(S);
     const T* b = l == 0 ? b0 : vb<T>(S);
-    for (int s = 0; s < opt.n_post; ++s) {  // transposed post-smoothing
-      vc::rbgs_tr<T>(S.L, x, b, 1, s == 0, stream);
-      vc::rbgs_tr<T>(S.L, x, b, 0, 0, stream);
-    }
+    smooth<T>(l, x, b, opt.n_post, true);  // transposed post-smoothing
     vc::resid_tr<T>(S.L, x, b, vr<T>(S), stream);
     vc::restrict_pt<T>(vr<T>(S), S.L.d, vb<T>(vlev[l + 1]), stream);
   }
@@ -645,10 +650,7 @@ void Solver::vcycle_apply_tr(const double* v, T* z) {
     T* x = l == 0 ? z : vx<T>(S);
     const T* b = l == 0 ? b0 : vb<T>(S);
     vc::prolong_inject_add<T>(vx<T>(vlev[l + 1]), vlev[l + 1].L.d, x, S.L.d, stream);
-    for (int s = 0; s < opt.n_pre; ++s) {  // transposed pre-smoothing
-      vc::rbgs_tr<T>(S.L, x, b, 1, 0, stream);
-      vc::rbgs_tr<T>(S.L, x, b, 0, 0, stream);
-    }
+    smooth<T>(l, x, b, opt.n_pre, false);  // transposed pre-smoothing
   }
 }
 
@@ -998,6 +1000,14 @@ void Solver::solve(const double* b, double* x, int mem, int* it_out, double* nr_
     dx = hx.p;
   }
   it = opt.krylov == LSX_KRYLOV_FGMRES ? fgmres(db, dx) : bicgstab(db, dx);
+  // empty rows (closed cut cells) carry no equation; the V-cycle leaves them whatever its
+  // corrections put there, the solution reports 0
+  if (vcyc()) {
+    if (mixed())
+      vc::zero_empty<float>(vlev[0].L, dx, stream);
+    else
+      vc::zero_empty<double>(vlev[0].L, dx, stream);
+  }
   if (mem == LSX_MEM_HOST) LSX_HIP(hipMemcpyAsync(x, dx, n * sizeof(double), hipMemcpyDeviceToHost, stream));
   LSX_HIP(hipGetLastError());
   LSX_HIP(hipStreamSynchronize(stream));

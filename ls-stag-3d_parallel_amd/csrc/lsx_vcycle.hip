@@ -195,7 +195,10 @@ __device__ inline T offdiag(const T* __restrict__ x, const FT* __restrict__ ae, 
   return acc;
 }
 
-// one colour of a red-black sweep; work item t (from start, by stride) owns the t-th cell of it
+// one colour of a red-black sweep; work item t (from start, by stride) owns the t-th cell of it.
+// zero_guess: 0 normal; 1 x is zero before this half-sweep; 2 first update of this colour after
+// a zero start (its values are not read).  An empty row (1/a_p stored as 0) keeps its value, so
+// that every step has the form x + W (b - A x) and the cycle's transpose is the transposed cycle.
 template <class T, class FT>
 __device__ inline void rbgs_body(T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
                                  const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s, T fs,
@@ -213,7 +216,9 @@ __device__ inline void rbgs_body(T* __restrict__ x, const T* __restrict__ b, con
     T v;
     if (i == pin0 || i == pin1) {
       v = b[i] * s[i];
-    } else if (zero_guess) {
+    } else if (s[i] == (T)0) {
+      v = zero_guess ? (T)0 : x[i];  // empty row: keeps its value (0 from a zero start)
+    } else if (zero_guess == 1) {
       v = b[i] * s[i];
     } else {
       v = (b[i] - fs * offdiag<T>(x, ae, an, at, i, xx, yy, zz, N, M, K)) * s[i];
@@ -311,8 +316,12 @@ __global__ void k_rbgs_tr(T* __restrict__ x, const T* __restrict__ b, const FT* 
     const int xx = 2 * (int)(t - row * half_row) + ((yy + zz + color) & 1);
     if (xx >= N) continue;
     const long i = (long)xx + (long)N * row;
-    x[i] = zero_guess ? b[i] * s[i]
-                      : (b[i] - fs * offdiag_tr<T, FT>(x, ae, an, at, i, xx, yy, zz, N, M, K, pin0, pin1)) * s[i];
+    if (s[i] == (T)0)
+      x[i] = zero_guess ? (T)0 : x[i];
+    else
+      x[i] = zero_guess == 1
+                 ? b[i] * s[i]
+                 : (b[i] - fs * offdiag_tr<T, FT>(x, ae, an, at, i, xx, yy, zz, N, M, K, pin0, pin1)) * s[i];
   }
 }
 
@@ -425,10 +434,11 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse_cycle(CoarseLevels<T>
     const FT* t = (const FT*)L.f_t;
     const T* s = (const T*)L.s;
     for (int sw = 0; sw < n_pre; ++sw) {
-      rbgs_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, 0, L.pin0, L.pin1, sw == 0,
-                       tid, nth);
+      rbgs_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, 0, L.pin0, L.pin1,
+                       sw == 0 ? 1 : 0, tid, nth);
       __syncthreads();
-      rbgs_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, 1, L.pin0, L.pin1, 0, tid, nth);
+      rbgs_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, 1, L.pin0, L.pin1,
+                       sw == 0 ? 2 : 0, tid, nth);
       __syncthreads();
     }
     resid_restrict_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, L.pin0, L.pin1,
@@ -475,6 +485,12 @@ __global__ void k_convert(const A* __restrict__ a, B* __restrict__ b, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = (B)a[i];
 }
 
+template <class T>
+__global__ void k_zero_empty(const T* __restrict__ s, double* __restrict__ x, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    if (s[i] == (T)0) x[i] = 0.0;
+}
+
 // ---- line relaxation ---------------------------------------------------------------------------
 // One pass solves every line of direction d (0 x, 1 y, 2 z) exactly, with the couplings across
 // lines taken from x_in (line Jacobi): x_out = T_d^{-1} (b - (A - T_d) x_in), T_d the tridiagonal
@@ -483,55 +499,61 @@ __global__ void k_convert(const A* __restrict__ a, B* __restrict__ b, long n) {
 // and point smoothers stall, every strong coupling is solved exactly in one of them.
 //
 // Row of cell i on its line: lower (coupling to the previous cell of the line), diag, upper,
-// rhs.  Pinned rows are identity rows; an empty row (1/a_p stored as 0) gives x = 0, as in the
-// point smoother.
+// rhs.  Pinned rows are identity rows; an empty row (1/a_p stored as 0) keeps its value, as in
+// the point smoother.
 template <class T, class FT>
 __device__ inline void line_row(int d, const T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
                                 const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s, T fs,
                                 long i, int xx, int yy, int zz, int N, int M, int K, int pin0, int pin1, bool zero,
-                                T& lo, T& dg, T& up, T& rhs) {
+                                bool trans, T& lo, T& dg, T& up, T& rhs) {
   const long NM = (long)N * M;
-  if (i == pin0 || i == pin1) {
+  const bool own = i == pin0 || i == pin1;
+  if (own && !trans) {
     lo = up = (T)0;
     dg = (T)1;
     rhs = b[i];
     return;
   }
   const T si = s[i];
-  if (si == (T)0) {
+  if (si == (T)0) {  // empty row: keeps its value
     lo = up = (T)0;
     dg = (T)1;
-    rhs = (T)0;
+    rhs = zero ? (T)0 : x[i];
     return;
   }
   dg = (T)1 / si;
+  // A^T (trans): a pinned row is a full row, other rows drop their couplings to pinned cells
+  auto keep = [&](long j) { return !trans || own || (j != pin0 && j != pin1); };
+  const bool hw = xx > 0 && keep(i - 1), he = xx < N - 1 && keep(i + 1);
+  const bool hs = yy > 0 && keep(i - N), hn = yy < M - 1 && keep(i + N);
+  const bool hb = zz > 0 && keep(i - NM), ht = zz < K - 1 && keep(i + NM);
   T off = (T)0;
   if (d == 0) {
-    lo = xx > 0 ? fs * ld<T>(ae, i - 1) : (T)0;
-    up = xx < N - 1 ? fs * ld<T>(ae, i) : (T)0;
+    lo = hw ? fs * ld<T>(ae, i - 1) : (T)0;
+    up = he ? fs * ld<T>(ae, i) : (T)0;
     if (!zero) {
-      if (yy > 0) off += fs * ld<T>(an, i - N) * x[i - N];
-      if (yy < M - 1) off += fs * ld<T>(an, i) * x[i + N];
-      if (zz > 0) off += fs * ld<T>(at, i - NM) * x[i - NM];
-      if (zz < K - 1) off += fs * ld<T>(at, i) * x[i + NM];
+      if (hs) off += fs * ld<T>(an, i - N) * x[i - N];
+      if (hn) off += fs * ld<T>(an, i) * x[i + N];
+      if (hb) off += fs * ld<T>(at, i - NM) * x[i - NM];
+      if (ht) off += fs * ld<T>(at, i) * x[i + NM];
     }
   } else if (d == 1) {
-    lo = yy > 0 ? fs * ld<T>(an, i - N) : (T)0;
-    up = yy < M - 1 ? fs * ld<T>(an, i) : (T)0;
+    lo = hs ? fs * ld<T>(an, i - N) : (T)0;
+    up = hn ? fs * ld<T>(an, i) : (T)0;
     if (!zero) {
-      if (xx > 0) off += fs * ld<T>(ae, i - 1) * x[i - 1];
-      if (xx < N - 1) off += fs * ld<T>(ae, i) * x[i + 1];
-      if (zz > 0) off += fs * ld<T>(at, i - NM) * x[i - NM];
-      if (zz < K - 1) off += fs * ld<T>(at, i) * x[i + NM];
+      if (hw) off += fs * ld<T>(ae, i - 1) * x[i - 1];
+      if (he) off += fs * ld<T>(ae, i) * x[i + 1];
+      if (hb) off += fs * ld<T>(at, i - NM) * x[i - NM];
+      if (ht) off += fs * ld<T>(at, i) * x[i + NM];
     }
   } else {
-    lo = zz > 0 ? fs * ld<T>(at, i - NM) : (T)0;
-    up = zz < K - 1 ? fs * ld<T>(at, i) : (T)0;
+    lo = hb ? fs * ld<T>(at, i - NM) : (T)0;
+    up = ht ? fs * ld<T>(at, i) : (T)0;
     if (!zero) {
-      if (xx > 0) off += fs * ld<T>(ae, i - 1) * x[i - 1];
-      if (xx < N - 1) off += fs * ld<T>(ae, i) * x[i + 1];
-      if (yy > 0) off += fs * ld<T>(an, i - N) * x[i - N];
-      if (yy < M - 1) off += fs * ld<T>(an, i) * x[i + N];
+      if (hw) off += fs * ld<T>(ae, i - 1) * x[i - 1];
+      if (he) off += fs * ld<T>(ae, i) * x[i + 1];
+      if (hs) off += fs * ld<T>(an, i - N) * x[i - N];
+      if (hn) off += fs * ld<T>(an, i) * x[i + N];
     }
   }
   rhs = b[i] - off;
@@ -550,7 +572,7 @@ __global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* _
                                                 const T* __restrict__ b, const FT* __restrict__ ae,
                                                 const FT* __restrict__ an, const FT* __restrict__ at,
                                                 const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
-                                                int pin1, int zero) {
+                                                int pin1, int zero, int trans) {
   const int lane = threadIdx.x & 63;
   const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= (long)M * K) return;  // whole wavefronts leave together
@@ -561,8 +583,8 @@ __global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* _
   for (int j = 0; j < CPL; ++j) {
     const int k = lane * CPL + j;
     if (k < N) {
-      line_row<T, FT>(0, x_in, b, ae, an, at, s, fs, row * N + k, k, yy, zz, N, M, K, pin0, pin1, zero != 0, a[j],
-                      d[j], c[j], r[j]);
+      line_row<T, FT>(0, x_in, b, ae, an, at, s, fs, row * N + k, k, yy, zz, N, M, K, pin0, pin1, zero != 0,
+                      trans != 0, a[j], d[j], c[j], r[j]);
     } else {
       a[j] = c[j] = r[j] = (T)0;
       d[j] = (T)1;
@@ -642,7 +664,7 @@ template <class T, class FT>
 __global__ void k_line_t(int dir, const T* __restrict__ x_in, T* __restrict__ x_out, T* __restrict__ cpv,
                          const T* __restrict__ b, const FT* __restrict__ ae, const FT* __restrict__ an,
                          const FT* __restrict__ at, const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
-                         int pin1, int zero) {
+                         int pin1, int zero, int trans) {
   const long NM = (long)N * M;
   const long lines = dir == 1 ? (long)N * K : NM;
   const int len = dir == 1 ? M : K;
@@ -657,8 +679,8 @@ __global__ void k_line_t(int dir, const T* __restrict__ x_in, T* __restrict__ x_
       const long i = base + stride * k;
       const int yy = dir == 1 ? k : other, zz = dir == 1 ? other : k;
       T lo, dg, up, rhs;
-      line_row<T, FT>(dir, x_in, b, ae, an, at, s, fs, i, xx, yy, zz, N, M, K, pin0, pin1, zero != 0, lo, dg, up,
-                      rhs);
+      line_row<T, FT>(dir, x_in, b, ae, an, at, s, fs, i, xx, yy, zz, N, M, K, pin0, pin1, zero != 0, trans != 0, lo,
+                      dg, up, rhs);
       const T w = dg - lo * cprev;
       const T winv = w != (T)0 ? (T)1 / w : (T)0;
       cprev = up * winv;
@@ -826,7 +848,7 @@ template void prolong_inject_add<float>(const float*, Dims, float*, Dims, hipStr
 template void prolong_inject_add<double>(const double*, Dims, double*, Dims, hipStream_t);
 
 template <class T, class FT>
-static void line_pass_t(const Level& L, int dir, const T* x_in, T* x_out, T* cpv, const T* b, int zero,
+static void line_pass_t(const Level& L, int dir, const T* x_in, T* x_out, T* cpv, const T* b, int zero, int trans,
                         hipStream_t s) {
   const Dims& d = L.d;
   const FT* ae = (const FT*)L.f_e;
@@ -837,33 +859,44 @@ static void line_pass_t(const Level& L, int dir, const T* x_in, T* x_out, T* cpv
   if (dir == 0) {
     const int blocks = (int)(((long)d.M * d.K + 3) / 4);
     if (d.N <= 64)
-      k_line_x<T, FT, 1><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero);
+      k_line_x<T, FT, 1><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero,
+                                                   trans);
     else if (d.N <= 128)
-      k_line_x<T, FT, 2><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero);
+      k_line_x<T, FT, 2><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero,
+                                                   trans);
     else if (d.N <= 256)
-      k_line_x<T, FT, 4><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero);
+      k_line_x<T, FT, 4><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero,
+                                                   trans);
     else
-      k_line_x<T, FT, 8><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero);
+      k_line_x<T, FT, 8><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero,
+                                                   trans);
   } else {
     const long lines = dir == 1 ? (long)d.N * d.K : (long)d.N * d.M;
     k_line_t<T, FT><<<grid_1d(lines, 64), 64, 0, s>>>(dir, x_in, x_out, cpv, b, ae, an, at, sv, fs, d.N, d.M, d.K,
-                                                     L.pin0, L.pin1, zero);
+                                                     L.pin0, L.pin1, zero, trans);
   }
 }
 
 template <>
 void line_pass<double>(const Level& L, int dir, const double* x_in, double* x_out, double* cpv, const double* b,
-                       int zero, hipStream_t s) {
-  line_pass_t<double, double>(L, dir, x_in, x_out, cpv, b, zero, s);
+                       int zero, int trans, hipStream_t s) {
+  line_pass_t<double, double>(L, dir, x_in, x_out, cpv, b, zero, trans, s);
 }
 template <>
 void line_pass<float>(const Level& L, int dir, const float* x_in, float* x_out, float* cpv, const float* b,
-                      int zero, hipStream_t s) {
+                      int zero, int trans, hipStream_t s) {
   if (L.ftype == FACE_F16)
-    line_pass_t<float, _Float16>(L, dir, x_in, x_out, cpv, b, zero, s);
+    line_pass_t<float, _Float16>(L, dir, x_in, x_out, cpv, b, zero, trans, s);
   else
-    line_pass_t<float, float>(L, dir, x_in, x_out, cpv, b, zero, s);
+    line_pass_t<float, float>(L, dir, x_in, x_out, cpv, b, zero, trans, s);
 }
+
+template <class T>
+void zero_empty(const Level& L, double* x, hipStream_t s) {
+  k_zero_empty<T><<<grid_1d(L.d.n()), 256, 0, s>>>((const T*)L.s, x, L.d.n());
+}
+template void zero_empty<float>(const Level&, double*, hipStream_t);
+template void zero_empty<double>(const Level&, double*, hipStream_t);
 
 template <class T>
 void copy_vec(const T* a, T* b, long n, hipStream_t s) {

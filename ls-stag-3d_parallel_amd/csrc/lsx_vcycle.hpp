@@ -69,7 +69,8 @@ void to27(const double* fe, const double* fn, const double* ft, const double* dg
           double* c27, hipStream_t s);
 
 // ---- cycle (T = double with FACE_F64 levels, T = float with FACE_F32 / FACE_F16 levels) -------
-// one colour of a red-black Gauss-Seidel sweep (zero_guess: x = 0 before this half-sweep)
+// one colour of a red-black Gauss-Seidel sweep (zero_guess: 1 x = 0 before this half-sweep,
+// 2 first update of this colour after such a start, 0 otherwise)
 template <class T>
 void rbgs(const Level& L, T* x, const T* b, int color, int zero_guess, hipStream_t s);
 // bc = sum over the 8 children of (b - A x)
@@ -108,17 +109,22 @@ template <>
 void resid_tr<float>(const Level& L, const float* x, const float* b, float* r, hipStream_t s);
 
 // line relaxation: every line of direction dir (0 x, 1 y, 2 z) solved exactly against x_in's
-// values across lines (x_in unread when zero); cpv: scratch of the level's size
+// values across lines (x_in unread when zero); trans: the same on A^T (the finest level's pinned
+// rows are full rows, other rows drop their couplings to them); cpv: scratch of the level's size
 template <class T>
-void line_pass(const Level& L, int dir, const T* x_in, T* x_out, T* cpv, const T* b, int zero, hipStream_t s);
+void line_pass(const Level& L, int dir, const T* x_in, T* x_out, T* cpv, const T* b, int zero, int trans,
+               hipStream_t s);
 template <>
 void line_pass<double>(const Level& L, int dir, const double* x_in, double* x_out, double* cpv, const double* b,
-                       int zero, hipStream_t s);
+                       int zero, int trans, hipStream_t s);
 template <>
 void line_pass<float>(const Level& L, int dir, const float* x_in, float* x_out, float* cpv, const float* b,
-                      int zero, hipStream_t s);
+                      int zero, int trans, hipStream_t s);
 template <class T>
 void copy_vec(const T* a, T* b, long n, hipStream_t s);
+// x = 0 on the empty rows of level L (whose 1/a_p, of type T, is stored as 0)
+template <class T>
+void zero_empty(const Level& L, double* x, hipStream_t s);
 
 template <>
 void rbgs<double>(const Level& L, double* x, const double* b, int color, int zero_guess, hipStream_t s);

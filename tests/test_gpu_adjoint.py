@@ -13,8 +13,8 @@ import problems as P
 pytestmark = pytest.mark.gpu
 
 
-def make_solver(lsx, A, kind="fgmres", precision="fp64-vcycle", eps=1.0, max_it=500):
-    S = lsx.solver(kind, precision)
+def make_solver(lsx, A, kind="fgmres", precision="fp64-vcycle", eps=1.0, max_it=500, smoother="rbgs"):
+    S = lsx.solver(kind, precision, smoother=smoother)
     S.set(A, np.zeros(A.n), np.zeros(A.n), lsx.mixed_task(eps, restart=30, max_it=max_it))
     return S
 
@@ -45,13 +45,14 @@ def test_transposed_operator(lsx, name, mk):
     assert np.linalg.norm(S.apply_operator(y) - ATy) > 1e-6 * np.linalg.norm(ATy)
 
 
+@pytest.mark.parametrize("smoother", ["rbgs", "line"])
 @pytest.mark.parametrize("precision,tol", [("fp64-vcycle", 1e-12), ("mixed", 2e-5), ("mixed16", 2e-3)])
 @pytest.mark.parametrize("name,mk", CASES, ids=[c[0] for c in CASES])
-def test_transposed_vcycle(lsx, name, mk, precision, tol):
+def test_transposed_vcycle(lsx, name, mk, precision, tol, smoother):
     """The transposed cycle is the transpose of the cycle as a linear map (exact in FP64; in
-    mixed precision up to its FP32/FP16 rounding)."""
+    mixed precision up to its FP32/FP16 rounding), with either smoother."""
     g, A = mk()
-    S = make_solver(lsx, A, precision=precision)
+    S = make_solver(lsx, A, precision=precision, smoother=smoother)
     rng = np.random.default_rng(7)
     # vectors on the active rows (solid and closed cells carry no equation)
     act = A.to_scipy().diagonal() < 0
@@ -59,7 +60,7 @@ def test_transposed_vcycle(lsx, name, mk, precision, tol):
     Bx, BTy = S.precondition(x), S.precondition_transpose(y)
     lhs, rhs = Bx @ y, x @ BTy
     scale = np.linalg.norm(Bx) * np.linalg.norm(y)
-    print(f"case={name} precision={precision} <Bx,y>={lhs:.15e} <x,B^T y>={rhs:.15e} "
+    print(f"case={name} precision={precision} smoother={smoother} <Bx,y>={lhs:.15e} <x,B^T y>={rhs:.15e} "
           f"rel={abs(lhs - rhs) / scale:.3e}", flush=True)
     assert abs(lhs - rhs) <= tol * scale
     # B is not symmetric (trilinear prolongation against sum restriction): B^T y != B y
