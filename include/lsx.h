@@ -63,6 +63,8 @@ typedef struct lsx_options {
   int coarse_fuse;   /* V-cycle: 1 = levels of <= 4096 cells in one launch (default), 0 = per level */
   int graphs;        /* V-cycle: 1 = replay each cycle from a captured HIP graph (one per basis slot) */
   int smoother;      /* V-cycle: LSX_SMOOTH_RBGS (default) or LSX_SMOOTH_LINE                  */
+  int fused_sweep;   /* V-cycle red-black: 1 = both colours in one pass on a finest level of
+                        >= 2^23 cells (default), 2 = on every level, 0 = never (half-sweeps)    */
 } lsx_options;
 
 #define LSX_SMOOTH_RBGS 0 /* red-black Gauss-Seidel                                             */

@@ -34,6 +34,7 @@ Solver::Solver(const lsx_options& o, hipStream_t s) : opt(o) {
   if (opt.mg == LSX_MG_VCYCLE && (opt.n_pre < 1 || opt.n_post < 1))
     throw Error(LSX_ERR_ARG, "the V-cycle needs n_pre >= 1 and n_post >= 1");
   if (opt.smoother != LSX_SMOOTH_RBGS && opt.smoother != LSX_SMOOTH_LINE) throw Error(LSX_ERR_ARG, "unknown smoother");
+  if (opt.fused_sweep < 0 || opt.fused_sweep > 2) throw Error(LSX_ERR_ARG, "fused_sweep must be 0, 1 or 2");
   if (s) {
     stream = s;
   } else {
@@ -391,6 +392,11 @@ void Solver::build_vcycle() {
         S.dxt.alloc(nl);
         if (l == 0) dcp.alloc(nl);
       }
+    } else if (fused_level(l)) {  // second iterate buffer of the one-pass sweeps
+      if (m32)
+        S.fxt.alloc(nl);
+      else
+        S.dxt.alloc(nl);
     }
     if (m32) {
       S.fs.alloc(nl);
@@ -573,7 +579,7 @@ void Solver::vcycle_run(const double* v, T* z) {
 template <class T>
 void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess) {
   VStore& S = vlev[l];
-  if (opt.smoother == LSX_SMOOTH_RBGS) {
+  if (opt.smoother == LSX_SMOOTH_RBGS && (trans || !fused_level(l))) {
     for (int s = 0; s < sweeps; ++s) {
       const bool first = zero_guess && s == 0;
       if (trans) {  // transposed sweeps: black then red, on A^T
@@ -587,9 +593,22 @@ void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess) {
     return;
   }
   T* xt = vxt<T>(S);
+  const long n = S.L.d.n();
+  if (opt.smoother == LSX_SMOOTH_RBGS) {  // one pass per fused red-black sweep
+    T* src = x;
+    if (!zero_guess && (sweeps - 1) % 2 == 0) {  // the first pass must not write the buffer it reads
+      vc::copy_vec<T>(x, xt, n, stream);
+      src = xt;
+    }
+    for (int p = 1; p <= sweeps; ++p) {
+      T* dst = (sweeps - p) % 2 == 0 ? x : xt;
+      vc::rb_sweep<T>(S.L, src, dst, b, zero_guess && p == 1, stream);
+      src = dst;
+    }
+    return;
+  }
   T* cpv = vcp<T>();
   const int P = 3 * sweeps;
-  const long n = S.L.d.n();
   T* src = x;
   if (!zero_guess && (P - 1) % 2 == 0) {  // the first pass must not write the buffer it reads
     vc::copy_vec<T>(x, xt, n, stream);

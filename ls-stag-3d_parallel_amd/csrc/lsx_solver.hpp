@@ -179,6 +179,11 @@ class Solver {
   T* vcp();
   template <class T>
   void smooth(int l, T* x, const T* b, int sweeps, bool zero_guess);
+  // levels whose red-black sweeps run as one fused pass: the finest level when it has at least
+  // 2^23 cells, enough 64 x 16 x 32 tiles to fill the device (opt.fused_sweep 1), or every level (2)
+  bool fused_level(int l) const {
+    return opt.fused_sweep == 2 || (opt.fused_sweep == 1 && l == 0 && levels[l].d.n() >= (1L << 23));
+  }
   std::vector<VStore> vlev;
   DevBuf<float> Z32, ztmp32;
   bool mixed() const { return opt.precision == LSX_PREC_MIXED || opt.precision == LSX_PREC_MIXED16; }

@@ -485,6 +485,183 @@ __global__ void k_convert(const A* __restrict__ a, B* __restrict__ b, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = (B)a[i];
 }
 
+// ---- fused red-black sweep ---------------------------------------------------------------------
+// One full red-black sweep in one pass, x_in -> x_out.  A workgroup owns a 64 x 16 tile of
+// (x, y) columns and a run of FZ z planes and marches up in z: at plane p it updates the red
+// cells of p over the tile and a one-cell ring around it (from x_in, the old values), then the
+// black cells of plane p - 1 over the tile (from the red values of planes p - 2, p - 1, p, all
+// new).  The ring's red values are recomputed by the neighbouring workgroups too, so no
+// workgroup reads another's results and x_out is the same bit for bit as red and black
+// half-sweeps in place: the same formula, the same operand order.  Each array is read about
+// once instead of once per colour.
+constexpr int FX = 64, FY = 16, FZ = 32;
+constexpr int HX = FX + 4, HY = FY + 4;  // x_in staging: tile + 2-cell ring
+constexpr int RX = FX + 2, RY = FY + 2;  // red values: tile + 1-cell ring
+
+// per-cell data of one plane: right-hand side, 1/a_p and the six faces (absent ones read the cell
+// itself and are never used)
+template <class T>
+struct RbCell {
+  T b, s, fw, fe, fs, fn, fb, ft;
+};
+
+template <class T, class FT>
+__device__ inline RbCell<T> rb_load(const T* __restrict__ b, const FT* __restrict__ ae, const FT* __restrict__ an,
+                                    const FT* __restrict__ at, const T* __restrict__ s, bool ok, int gx, int gy,
+                                    int p, int N, int M) {
+  RbCell<T> d;
+  if (!ok) {
+    d.b = d.s = d.fw = d.fe = d.fs = d.fn = d.fb = d.ft = (T)0;
+    return d;
+  }
+  const long NM = (long)N * M;
+  const long i = gx + (long)N * gy + NM * p;
+  d.b = b[i];
+  d.s = s[i];
+  d.fw = ld<T>(ae, gx > 0 ? i - 1 : i);
+  d.fe = ld<T>(ae, i);
+  d.fs = ld<T>(an, gy > 0 ? i - N : i);
+  d.fn = ld<T>(an, i);
+  d.fb = ld<T>(at, p > 0 ? i - NM : i);
+  d.ft = ld<T>(at, i);
+  return d;
+}
+
+// the red-black update of one cell from its neighbours' values (operand order of offdiag)
+template <class T>
+__device__ inline T rb_apply(const RbCell<T>& d, T fs, int gx, int gy, int p, int N, int M, int K, T xw, T xe, T xs,
+                             T xn, T xb, T xt) {
+  T acc = 0;
+  if (gx > 0) acc += d.fw * xw;
+  if (gx < N - 1) acc += d.fe * xe;
+  if (gy > 0) acc += d.fs * xs;
+  if (gy < M - 1) acc += d.fn * xn;
+  if (p > 0) acc += d.fb * xb;
+  if (p < K - 1) acc += d.ft * xt;
+  return (d.b - fs * acc) * d.s;
+}
+
+// workgroup barrier for LDS traffic only: global loads in flight (the next plane's prefetch)
+// are not waited for
+__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <class T, class FT>
+__global__ void __launch_bounds__(FX* FY) k_rb_fused(const T* __restrict__ x_in, T* __restrict__ x_out,
+                                                      const T* __restrict__ b, const FT* __restrict__ ae,
+                                                      const FT* __restrict__ an, const FT* __restrict__ at,
+                                                      const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
+                                                      int pin1, int zero) {
+  __shared__ T xs_[3][HY][HX];
+  __shared__ T rs_[3][RY][RX];
+  constexpr int NT = FX * FY;
+  const int tid = threadIdx.x;
+  const int X0 = blockIdx.x * FX, Y0 = blockIdx.y * FY, Z0 = blockIdx.z * FZ;
+  const int Z1 = Z0 + FZ < K ? Z0 + FZ : K;
+  const long NM = (long)N * M;
+  auto slot = [](int p) { return ((p % 3) + 3) % 3; };
+  // this thread's ring cells (tile + 1-cell ring, RX x RY) and staging cells (HX x HY)
+  int cgx[2], cgy[2];
+  bool cin[2], cint[2];
+  for (int k = 0; k < 2; ++k) {
+    const int c = tid + k * NT;
+    const int rx = c % RX, ry = c / RX;
+    cgx[k] = X0 - 1 + rx;
+    cgy[k] = Y0 - 1 + ry;
+    cin[k] = c < RX * RY && cgx[k] >= 0 && cgx[k] < N && cgy[k] >= 0 && cgy[k] < M;
+    cint[k] = cin[k] && rx >= 1 && rx <= FX && ry >= 1 && ry <= FY;
+  }
+  int sgx[2], sgy[2];
+  bool sin_[2];
+  for (int k = 0; k < 2; ++k) {
+    const int c = tid + k * NT;
+    sgx[k] = X0 - 2 + c % HX;
+    sgy[k] = Y0 - 2 + c / HX;
+    sin_[k] = c < HX * HY && sgx[k] >= 0 && sgx[k] < N && sgy[k] >= 0 && sgy[k] < M;
+  }
+  auto xload = [&](int k, int q) -> T {
+    return (!zero && sin_[k] && q >= 0 && q < K) ? x_in[sgx[k] + (long)N * sgy[k] + NM * q] : (T)0;
+  };
+  auto xstore = [&](int k, int q, T v) {
+    if (sin_[k] && q >= 0 && q < K) xs_[slot(q)][sgy[k] - Y0 + 2][sgx[k] - X0 + 2] = v;
+  };
+  auto xin = [&](int q, int gx, int gy) -> T { return xs_[slot(q)][gy - Y0 + 2][gx - X0 + 2]; };
+  auto red = [&](int q, int gx, int gy) -> T { return rs_[slot(q)][gy - Y0 + 1][gx - X0 + 1]; };
+  auto dload = [&](int k, int q) {
+    return rb_load<T, FT>(b, ae, an, at, s, cin[k] && q >= 0 && q < K, cgx[k], cgy[k], q, N, M);
+  };
+
+  // prologue: x_in planes Z0-2 .. Z0 staged, the data of planes Z0-1 (current) and Z0 (next)
+  for (int k = 0; k < 2; ++k) {
+    xstore(k, Z0 - 2, xload(k, Z0 - 2));
+    xstore(k, Z0 - 1, xload(k, Z0 - 1));
+  }
+  T xr[2] = {xload(0, Z0), xload(1, Z0)};
+  RbCell<T> dp[2], dc[2] = {dload(0, Z0 - 1), dload(1, Z0 - 1)};
+  for (int p = Z0 - 1; p <= Z1; ++p) {
+    // loads of the next step, in flight while this plane is computed
+    const RbCell<T> dn[2] = {dload(0, p + 1), dload(1, p + 1)};
+    const T xn2[2] = {xload(0, p + 2), xload(1, p + 2)};
+    xstore(0, p + 1, xr[0]);
+    xstore(1, p + 1, xr[1]);
+    lds_barrier();
+    // red cells of plane p over the tile and its ring, from the old values
+    if (p >= 0 && p < K) {
+      for (int k = 0; k < 2; ++k) {
+        const int gx = cgx[k], gy = cgy[k];
+        if (!cin[k] || ((gx + gy + p) & 1)) continue;
+        const long i = gx + (long)N * gy + NM * p;
+        T v;
+        if (i == pin0 || i == pin1) {
+          v = dc[k].b * dc[k].s;
+        } else if (dc[k].s == (T)0) {
+          v = zero ? (T)0 : xin(p, gx, gy);
+        } else if (zero) {
+          v = dc[k].b * dc[k].s;
+        } else {
+          const T xw = gx > 0 ? xin(p, gx - 1, gy) : (T)0, xe = gx < N - 1 ? xin(p, gx + 1, gy) : (T)0;
+          const T xs = gy > 0 ? xin(p, gx, gy - 1) : (T)0, xn = gy < M - 1 ? xin(p, gx, gy + 1) : (T)0;
+          const T xb = p > 0 ? xin(p - 1, gx, gy) : (T)0, xt = p < K - 1 ? xin(p + 1, gx, gy) : (T)0;
+          v = rb_apply<T>(dc[k], fs, gx, gy, p, N, M, K, xw, xe, xs, xn, xb, xt);
+        }
+        rs_[slot(p)][gy - Y0 + 1][gx - X0 + 1] = v;
+      }
+    }
+    lds_barrier();
+    // plane p - 1 of the tile: black cells from the new red values, red cells as computed
+    const int q = p - 1;
+    if (q >= Z0 && q < Z1) {
+      for (int k = 0; k < 2; ++k) {
+        if (!cint[k]) continue;
+        const int gx = cgx[k], gy = cgy[k];
+        const long i = gx + (long)N * gy + NM * q;
+        T v;
+        if ((gx + gy + q) & 1) {
+          if (i == pin0 || i == pin1) {
+            v = dp[k].b * dp[k].s;
+          } else if (dp[k].s == (T)0) {
+            v = zero ? (T)0 : xin(q, gx, gy);
+          } else {
+            const T xw = gx > 0 ? red(q, gx - 1, gy) : (T)0, xe = gx < N - 1 ? red(q, gx + 1, gy) : (T)0;
+            const T xs = gy > 0 ? red(q, gx, gy - 1) : (T)0, xn = gy < M - 1 ? red(q, gx, gy + 1) : (T)0;
+            const T xb = q > 0 ? red(q - 1, gx, gy) : (T)0, xt = q < K - 1 ? red(q + 1, gx, gy) : (T)0;
+            v = rb_apply<T>(dp[k], fs, gx, gy, q, N, M, K, xw, xe, xs, xn, xb, xt);
+          }
+        } else {
+          v = red(q, gx, gy);
+        }
+        x_out[i] = v;
+      }
+    }
+    // the next step overwrites slot (p + 2) % 3 of both rings
+    lds_barrier();
+    for (int k = 0; k < 2; ++k) {
+      dp[k] = dc[k];
+      dc[k] = dn[k];
+      xr[k] = xn2[k];
+    }
+  }
+}
+
 template <class T>
 __global__ void k_zero_empty(const T* __restrict__ s, double* __restrict__ x, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -506,57 +683,43 @@ __device__ inline void line_row(int d, const T* __restrict__ x, const T* __restr
                                 const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s, T fs,
                                 long i, int xx, int yy, int zz, int N, int M, int K, int pin0, int pin1, bool zero,
                                 bool trans, T& lo, T& dg, T& up, T& rhs) {
+  // Every load is issued unconditionally (absent neighbours read the cell itself and are masked
+  // out), so that a caller evaluating many rows can have all their loads in flight at once.
   const long NM = (long)N * M;
   const bool own = i == pin0 || i == pin1;
-  if (own && !trans) {
-    lo = up = (T)0;
-    dg = (T)1;
-    rhs = b[i];
-    return;
-  }
-  const T si = s[i];
-  if (si == (T)0) {  // empty row: keeps its value
-    lo = up = (T)0;
-    dg = (T)1;
-    rhs = zero ? (T)0 : x[i];
-    return;
-  }
-  dg = (T)1 / si;
-  // A^T (trans): a pinned row is a full row, other rows drop their couplings to pinned cells
   auto keep = [&](long j) { return !trans || own || (j != pin0 && j != pin1); };
   const bool hw = xx > 0 && keep(i - 1), he = xx < N - 1 && keep(i + 1);
   const bool hs = yy > 0 && keep(i - N), hn = yy < M - 1 && keep(i + N);
   const bool hb = zz > 0 && keep(i - NM), ht = zz < K - 1 && keep(i + NM);
-  T off = (T)0;
-  if (d == 0) {
-    lo = hw ? fs * ld<T>(ae, i - 1) : (T)0;
-    up = he ? fs * ld<T>(ae, i) : (T)0;
-    if (!zero) {
-      if (hs) off += fs * ld<T>(an, i - N) * x[i - N];
-      if (hn) off += fs * ld<T>(an, i) * x[i + N];
-      if (hb) off += fs * ld<T>(at, i - NM) * x[i - NM];
-      if (ht) off += fs * ld<T>(at, i) * x[i + NM];
-    }
-  } else if (d == 1) {
-    lo = hs ? fs * ld<T>(an, i - N) : (T)0;
-    up = hn ? fs * ld<T>(an, i) : (T)0;
-    if (!zero) {
-      if (hw) off += fs * ld<T>(ae, i - 1) * x[i - 1];
-      if (he) off += fs * ld<T>(ae, i) * x[i + 1];
-      if (hb) off += fs * ld<T>(at, i - NM) * x[i - NM];
-      if (ht) off += fs * ld<T>(at, i) * x[i + NM];
-    }
-  } else {
-    lo = hb ? fs * ld<T>(at, i - NM) : (T)0;
-    up = ht ? fs * ld<T>(at, i) : (T)0;
-    if (!zero) {
-      if (hw) off += fs * ld<T>(ae, i - 1) * x[i - 1];
-      if (he) off += fs * ld<T>(ae, i) * x[i + 1];
-      if (hs) off += fs * ld<T>(an, i - N) * x[i - N];
-      if (hn) off += fs * ld<T>(an, i) * x[i + N];
-    }
+  const long iw = xx > 0 ? i - 1 : i, ie = xx < N - 1 ? i + 1 : i;
+  const long is = yy > 0 ? i - N : i, in = yy < M - 1 ? i + N : i;
+  const long ib = zz > 0 ? i - NM : i, it = zz < K - 1 ? i + NM : i;
+  const T si = s[i], bi = b[i];
+  const T fw = fs * ld<T>(ae, iw), fe = fs * ld<T>(ae, i);
+  const T fsf = fs * ld<T>(an, is), fn = fs * ld<T>(an, i);
+  const T fb = fs * ld<T>(at, ib), ft = fs * ld<T>(at, i);
+  const T cw = hw ? fw : (T)0, ce = he ? fe : (T)0, cs = hs ? fsf : (T)0, cn = hn ? fn : (T)0;
+  const T cb = hb ? fb : (T)0, ct = ht ? ft : (T)0;
+  T off = (T)0, xi = (T)0;
+  if (!zero) {  // uniform over the launch
+    xi = x[i];
+    if (d == 0) off = cs * x[is] + cn * x[in] + cb * x[ib] + ct * x[it];
+    else if (d == 1) off = cw * x[iw] + ce * x[ie] + cb * x[ib] + ct * x[it];
+    else off = cw * x[iw] + ce * x[ie] + cs * x[is] + cn * x[in];
   }
-  rhs = b[i] - off;
+  lo = d == 0 ? cw : (d == 1 ? cs : cb);
+  up = d == 0 ? ce : (d == 1 ? cn : ct);
+  dg = si != (T)0 ? (T)1 / si : (T)1;
+  rhs = bi - off;
+  if (own && !trans) {  // identity row
+    lo = up = (T)0;
+    dg = (T)1;
+    rhs = bi;
+  } else if (si == (T)0) {  // empty row: keeps its value
+    lo = up = (T)0;
+    dg = (T)1;
+    rhs = xi;
+  }
 }
 
 // x lines: one wavefront per line, CPL consecutive cells per lane (coalesced loads), solved by
@@ -659,12 +822,18 @@ __global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* _
 }
 
 // y and z lines: one thread per line, Thomas elimination; the forward sweep keeps the
-// eliminated upper coefficients in `cpv` and the partial solution in x_out.
+// eliminated upper coefficients in `cpv` and the partial solution in x_out.  Rows are taken in
+// blocks of LB: the loads of a whole block are issued before its recurrence runs, so a thread
+// has LB rows of memory traffic in flight instead of one (a line is a dependent chain, and the
+// lines alone do not fill the device).
+constexpr int LB = 16;
+
 template <class T, class FT>
-__global__ void k_line_t(int dir, const T* __restrict__ x_in, T* __restrict__ x_out, T* __restrict__ cpv,
-                         const T* __restrict__ b, const FT* __restrict__ ae, const FT* __restrict__ an,
-                         const FT* __restrict__ at, const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
-                         int pin1, int zero, int trans) {
+__global__ void __launch_bounds__(64) k_line_t(int dir, const T* __restrict__ x_in, T* __restrict__ x_out,
+                                               T* __restrict__ cpv, const T* __restrict__ b,
+                                               const FT* __restrict__ ae, const FT* __restrict__ an,
+                                               const FT* __restrict__ at, const T* __restrict__ s, T fs, int N,
+                                               int M, int K, int pin0, int pin1, int zero, int trans) {
   const long NM = (long)N * M;
   const long lines = dir == 1 ? (long)N * K : NM;
   const int len = dir == 1 ? M : K;
@@ -674,25 +843,46 @@ __global__ void k_line_t(int dir, const T* __restrict__ x_in, T* __restrict__ x_
     const int other = (int)(t / N);  // z for y lines, y for z lines
     const long base = dir == 1 ? xx + NM * other : xx + (long)N * other;
     T cprev = (T)0, gprev = (T)0;
-#pragma unroll 4
-    for (int k = 0; k < len; ++k) {
-      const long i = base + stride * k;
-      const int yy = dir == 1 ? k : other, zz = dir == 1 ? other : k;
-      T lo, dg, up, rhs;
-      line_row<T, FT>(dir, x_in, b, ae, an, at, s, fs, i, xx, yy, zz, N, M, K, pin0, pin1, zero != 0, trans != 0, lo,
-                      dg, up, rhs);
-      const T w = dg - lo * cprev;
-      const T winv = w != (T)0 ? (T)1 / w : (T)0;
-      cprev = up * winv;
-      gprev = (rhs - lo * gprev) * winv;
-      cpv[i] = cprev;
-      x_out[i] = gprev;
+    for (int k0 = 0; k0 < len; k0 += LB) {
+      T lo[LB], dg[LB], up[LB], rh[LB];
+#pragma unroll
+      for (int q = 0; q < LB; ++q) {
+        const int k = k0 + q < len ? k0 + q : len - 1;
+        const long i = base + stride * k;
+        const int yy = dir == 1 ? k : other, zz = dir == 1 ? other : k;
+        line_row<T, FT>(dir, x_in, b, ae, an, at, s, fs, i, xx, yy, zz, N, M, K, pin0, pin1, zero != 0, trans != 0,
+                        lo[q], dg[q], up[q], rh[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < LB; ++q) {
+        if (k0 + q < len) {
+          const long i = base + stride * (k0 + q);
+          const T w = dg[q] - lo[q] * cprev;
+          const T winv = w != (T)0 ? (T)1 / w : (T)0;
+          cprev = up[q] * winv;
+          gprev = (rh[q] - lo[q] * gprev) * winv;
+          cpv[i] = cprev;
+          x_out[i] = gprev;
+        }
+      }
     }
     T xn = gprev;
-    for (int k = len - 2; k >= 0; --k) {
-      const long i = base + stride * k;
-      xn = x_out[i] - cpv[i] * xn;
-      x_out[i] = xn;
+    for (int k1 = len - 2; k1 >= 0; k1 -= LB) {
+      T cv[LB], gv[LB];
+#pragma unroll
+      for (int q = 0; q < LB; ++q) {
+        const int k = k1 - q >= 0 ? k1 - q : 0;
+        const long i = base + stride * k;
+        cv[q] = cpv[i];
+        gv[q] = x_out[i];
+      }
+#pragma unroll
+      for (int q = 0; q < LB; ++q) {
+        if (k1 - q >= 0) {
+          xn = gv[q] - cv[q] * xn;
+          x_out[base + stride * (k1 - q)] = xn;
+        }
+      }
     }
   }
 }
@@ -889,6 +1079,25 @@ void line_pass<float>(const Level& L, int dir, const float* x_in, float* x_out, 
     line_pass_t<float, _Float16>(L, dir, x_in, x_out, cpv, b, zero, trans, s);
   else
     line_pass_t<float, float>(L, dir, x_in, x_out, cpv, b, zero, trans, s);
+}
+
+template <class T, class FT>
+static void rb_sweep_t(const Level& L, const T* x_in, T* x_out, const T* b, int zero, hipStream_t s) {
+  const Dims& d = L.d;
+  const dim3 grid((d.N + FX - 1) / FX, (d.M + FY - 1) / FY, (d.K + FZ - 1) / FZ);
+  k_rb_fused<T, FT><<<grid, FX * FY, 0, s>>>(x_in, x_out, b, (const FT*)L.f_e, (const FT*)L.f_n, (const FT*)L.f_t,
+                                              (const T*)L.s, (T)L.fscale, d.N, d.M, d.K, L.pin0, L.pin1, zero);
+}
+template <>
+void rb_sweep<double>(const Level& L, const double* x_in, double* x_out, const double* b, int zero, hipStream_t s) {
+  rb_sweep_t<double, double>(L, x_in, x_out, b, zero, s);
+}
+template <>
+void rb_sweep<float>(const Level& L, const float* x_in, float* x_out, const float* b, int zero, hipStream_t s) {
+  if (L.ftype == FACE_F16)
+    rb_sweep_t<float, _Float16>(L, x_in, x_out, b, zero, s);
+  else
+    rb_sweep_t<float, float>(L, x_in, x_out, b, zero, s);
 }
 
 template <class T>

@@ -122,6 +122,14 @@ void line_pass<float>(const Level& L, int dir, const float* x_in, float* x_out, 
                       int zero, int trans, hipStream_t s);
 template <class T>
 void copy_vec(const T* a, T* b, long n, hipStream_t s);
+// one full red-black sweep x_in -> x_out in a single pass (zero: x_in taken as 0, not read);
+// bit-identical to the red and black half-sweeps in place
+template <class T>
+void rb_sweep(const Level& L, const T* x_in, T* x_out, const T* b, int zero, hipStream_t s);
+template <>
+void rb_sweep<double>(const Level& L, const double* x_in, double* x_out, const double* b, int zero, hipStream_t s);
+template <>
+void rb_sweep<float>(const Level& L, const float* x_in, float* x_out, const float* b, int zero, hipStream_t s);
 // x = 0 on the empty rows of level L (whose 1/a_p, of type T, is stored as 0)
 template <class T>
 void zero_empty(const Level& L, double* x, hipStream_t s);

@@ -67,7 +67,7 @@ class Options(C.Structure):
         ("krylov", C.c_int), ("restart", C.c_int), ("max_it", C.c_int), ("eps", C.c_double),
         ("mg", C.c_int), ("n_post", C.c_int), ("n_pre", C.c_int), ("omega", C.c_double),
         ("precision", C.c_int), ("levels", C.c_int), ("z_head_fix", C.c_int), ("ortho", C.c_int),
-        ("coarse_fuse", C.c_int), ("graphs", C.c_int), ("smoother", C.c_int),
+        ("coarse_fuse", C.c_int), ("graphs", C.c_int), ("smoother", C.c_int), ("fused_sweep", C.c_int),
     ]
 
 
@@ -341,13 +341,14 @@ class IterativeSolver:
     krylov = KRYLOV_FGMRES
 
     def __init__(self, precision=PREC_FP64, mg=MG_CASCADE, z_head_fix=False, stream=None, ortho=0, coarse_fuse=True,
-                 graphs=False, smoother="rbgs"):
+                 graphs=False, smoother="rbgs", fused_sweep=True):
         self.ortho = ortho
         self.coarse_fuse = coarse_fuse
         self.graphs = graphs
         if smoother not in ("rbgs", "line"):
             raise ValueError(f"unknown smoother {smoother!r}")
         self.smoother = smoother
+        self.fused_sweep = fused_sweep
         self.precision = precision
         self.mg = mg
         self.z_head_fix = z_head_fix
@@ -380,6 +381,7 @@ class IterativeSolver:
         opt.graphs = 1 if self.graphs else 0
         opt.levels = int(getattr(self, "levels_cap", 0))
         opt.smoother = 1 if self.smoother == "line" else 0
+        opt.fused_sweep = int(self.fused_sweep)
         h = C.c_void_p()
         _check(lib().lsx_create(C.byref(h), C.byref(opt), self.stream))
         self.h = h
@@ -413,6 +415,7 @@ class IterativeSolver:
         opt.graphs = 1 if self.graphs else 0
         opt.levels = int(getattr(self, "levels_cap", 0))
         opt.smoother = 1 if self.smoother == "line" else 0
+        opt.fused_sweep = int(self.fused_sweep)
         h = C.c_void_p()
         _check(lib().lsx_create(C.byref(h), C.byref(opt), self.stream))
         self.h = h
@@ -457,6 +460,15 @@ class IterativeSolver:
         nr = C.c_double(0.0)
         _check(lib().lsx_solve_adjoint(self.h, _ptr(g), _ptr(lam), MEM_HOST, C.byref(it), C.byref(nr)))
         return lam, it.value, nr.value
+
+    def solve_device(self, b, x):
+        """Solve with b and x (initial guess in, solution out) as DeviceVectors; returns it."""
+        _check(lib().lsx_set_eps(self.h, self.eps))
+        it = C.c_int(0)
+        nr = C.c_double(0.0)
+        _check(lib().lsx_solve(self.h, b.p, x.p, MEM_DEVICE, C.byref(it), C.byref(nr)))
+        self.it, self.norm_r = it.value, nr.value
+        return it.value
 
     def adjoint_forward(self, geometry, b, pin=True, p0=None):
         """Solve A(geometry) p = b and record a tape for adjoint_backward; returns (p, it, tape)."""
@@ -564,6 +576,55 @@ class MGFGMRES(IterativeSolver):
 class MGBiCGStab(IterativeSolver):
     """Reference MGBiCGStab (src/Base/Solvers/BiCGStab/mg_bicgstab.h:33-52)."""
     krylov = KRYLOV_BICGSTAB
+
+
+class DeviceVector:
+    """A float64 vector in device memory (hipMalloc through the HIP runtime), for solves whose
+    inputs and outputs stay on the GPU."""
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            h = C.CDLL("libamdhip64.so")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, n):
+        self.n = int(n)
+        self.p = C.c_void_p()
+        if self.hip().hipMalloc(C.byref(self.p), self.n * 8) != 0:
+            raise LsxError("hipMalloc failed")
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, np.float64)
+        if a.size != self.n or self.hip().hipMemcpy(self.p, a.ctypes.data, self.n * 8, 1) != 0:
+            raise LsxError("upload failed")
+
+    def download(self, out=None):
+        out = np.empty(self.n) if out is None else out
+        if self.hip().hipMemcpy(out.ctypes.data, self.p, self.n * 8, 2) != 0:
+            raise LsxError("download failed")
+        return out
+
+    def zero(self):
+        if self.hip().hipMemset(self.p, 0, self.n * 8) != 0:
+            raise LsxError("hipMemset failed")
+
+    def close(self):
+        if self.p:
+            self.hip().hipFree(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Tape:

@@ -105,3 +105,34 @@ def test_line_vcycle_on_stretched_grid(lsx, precision):
               f"true_rel_res={true_res / np.linalg.norm(b):.3e}", flush=True)
         assert true_res <= 1.05 * eps
     assert its["line"] < its["rbgs"]
+
+
+FUSED_CASES = [("sphere64", lambda: P.sphere(64)), ("cut32x16x24", lambda: P.synthetic_cut(32, 16, 24, seed=3)),
+               ("sphere96x64x32", lambda: P.sphere(96, 64, 32))]
+
+
+@pytest.mark.parametrize("precision", ["mixed", "mixed16", "fp64-vcycle"])
+@pytest.mark.parametrize("name,mk", FUSED_CASES, ids=[c[0] for c in FUSED_CASES])
+def test_fused_sweep_is_bitwise(lsx, name, mk, precision):
+    """The one-pass red-black sweep (tiles of 64 x 16 columns, 32 planes, red values of a
+    one-cell ring recomputed per tile) gives the same bits as red and black half-sweeps in
+    place: one and two sweeps, from zero and from a given x, and whole solves."""
+    g, A = mk()
+    act = A.to_scipy().diagonal() < 0
+    rng = np.random.default_rng(5)
+    b = P.random_rhs(g, seed=3, A=A)
+    x0 = rng.standard_normal(A.n) * act
+    out = {}
+    for fused in (0, 2):
+        S = lsx.solver("fgmres", precision, fused_sweep=fused)
+        x = np.zeros(A.n)
+        S.set(A, b, x, lsx.mixed_task(1e-8 * np.linalg.norm(b), restart=30, max_it=500))
+        sm = [S.smooth(b, x0, sweeps=k, zero_guess=z) for k in (1, 2) for z in (False, True)]
+        it = S.solve()
+        out[fused] = (sm, x.copy(), it)
+    for a, c in zip(out[0][0], out[2][0]):
+        assert np.array_equal(a, c), np.abs(a - c).max()
+    print(f"case={name} precision={precision} it={out[0][2]}/{out[2][2]} "
+          f"max|dx|={np.abs(out[0][1] - out[2][1]).max():.3e}", flush=True)
+    assert out[0][2] == out[2][2]
+    assert np.array_equal(out[0][1], out[2][1])
