@@ -324,6 +324,8 @@ void Solver::build_hierarchy() {
 // -----------------------------------------------------------------------------------------------
 void Solver::build_vcycle() {
   const int L = (int)levels.size() - 1;
+  if (opt.smoother == LSX_SMOOTH_LINE && (d0.M > 256 || d0.K > 256))
+    throw Error(LSX_ERR_ARG, "line relaxation handles y and z lines of at most 256 cells");
   vlev.clear();
   vlev.resize(L + 1);
   std::vector<DevBuf<double>> fe(2), fn(2), ft(2), dg(2);

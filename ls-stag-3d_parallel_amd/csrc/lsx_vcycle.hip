@@ -821,69 +821,131 @@ __global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* _
   }
 }
 
-// y and z lines: one thread per line, Thomas elimination; the forward sweep keeps the
-// eliminated upper coefficients in `cpv` and the partial solution in x_out.  Rows are taken in
-// blocks of LB: the loads of a whole block are issued before its recurrence runs, so a thread
-// has LB rows of memory traffic in flight instead of one (a line is a dependent chain, and the
-// lines alone do not fill the device).
-constexpr int LB = 16;
+// y and z lines: a workgroup holds 64 lines (adjacent x, so every load coalesces) split into S
+// segments of SEG rows, one wavefront per segment (S = ceil(len / SEG) <= 16: lines of at most
+// 256 cells).  Each thread eliminates its segment with the
+// two couplings to the neighbouring segments kept as unknowns, x_k = G_k + HL_k x_L + UR_k x_R
+// (x_L the last row of the segment before, x_R the first of the one after; all in registers);
+// one thread per line then solves the 2S interface values in order through LDS, and every
+// segment writes x_k.  Inputs are read once and x written once, with 64 S threads per 64 lines
+// instead of 64.
+constexpr int SEG = 16;
 
 template <class T, class FT>
-__global__ void __launch_bounds__(64) k_line_t(int dir, const T* __restrict__ x_in, T* __restrict__ x_out,
-                                               T* __restrict__ cpv, const T* __restrict__ b,
-                                               const FT* __restrict__ ae, const FT* __restrict__ an,
-                                               const FT* __restrict__ at, const T* __restrict__ s, T fs, int N,
-                                               int M, int K, int pin0, int pin1, int zero, int trans) {
+__global__ void __launch_bounds__(1024) k_line_seg(int dir, const T* __restrict__ x_in, T* __restrict__ x_out,
+                                                   const T* __restrict__ b, const FT* __restrict__ ae,
+                                                   const FT* __restrict__ an, const FT* __restrict__ at,
+                                                   const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
+                                                   int pin1, int zero, int trans, int S) {
+  extern __shared__ unsigned char lds_raw[];
+  T* ifc = reinterpret_cast<T*>(lds_raw);  // [8][S][64]: Gs HLs URs Ge HLe URe, then L R
+  auto IF = [&](int q, int j, int l) -> T& { return ifc[((long)q * S + j) * 64 + l]; };
   const long NM = (long)N * M;
-  const long lines = dir == 1 ? (long)N * K : NM;
-  const int len = dir == 1 ? M : K;
-  const long stride = dir == 1 ? (long)N : NM;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < lines; t += (long)gridDim.x * blockDim.x) {
-    const int xx = (int)(t % N);
-    const int other = (int)(t / N);  // z for y lines, y for z lines
-    const long base = dir == 1 ? xx + NM * other : xx + (long)N * other;
-    T cprev = (T)0, gprev = (T)0;
-    for (int k0 = 0; k0 < len; k0 += LB) {
-      T lo[LB], dg[LB], up[LB], rh[LB];
+  const int lane = threadIdx.x & 63, seg = threadIdx.x >> 6;
+  // x lines: the 64 lanes take 64 adjacent rows (y); y and z lines: 64 adjacent x
+  const int cross = blockIdx.x * 64 + lane, other = blockIdx.y;
+  const int len = dir == 0 ? N : (dir == 1 ? M : K);
+  const long stride = dir == 0 ? 1L : (dir == 1 ? (long)N : NM);
+  const bool live = cross < (dir == 0 ? M : N);
+  const long base = dir == 0 ? (long)N * cross + NM * other
+                             : (dir == 1 ? cross + NM * other : cross + (long)N * other);
+  const int k0 = seg * SEG;
+  const int cnt = live ? (len - k0 < SEG ? len - k0 : SEG) : 0;
+  T G[SEG], HL[SEG], UR[SEG];
+  {
+    T cprev = (T)0, gprev = (T)0, hprev = (T)0;
 #pragma unroll
-      for (int q = 0; q < LB; ++q) {
-        const int k = k0 + q < len ? k0 + q : len - 1;
-        const long i = base + stride * k;
-        const int yy = dir == 1 ? k : other, zz = dir == 1 ? other : k;
+    for (int k = 0; k < SEG; ++k) {
+      if (k < cnt) {
+        const int kk = k0 + k;
+        const long i = base + stride * kk;
+        const int xx = dir == 0 ? kk : cross;
+        const int yy = dir == 0 ? cross : (dir == 1 ? kk : other), zz = dir == 2 ? kk : other;
+        T lo, dg, up, rhs;
         line_row<T, FT>(dir, x_in, b, ae, an, at, s, fs, i, xx, yy, zz, N, M, K, pin0, pin1, zero != 0, trans != 0,
-                        lo[q], dg[q], up[q], rh[q]);
-      }
-#pragma unroll
-      for (int q = 0; q < LB; ++q) {
-        if (k0 + q < len) {
-          const long i = base + stride * (k0 + q);
-          const T w = dg[q] - lo[q] * cprev;
-          const T winv = w != (T)0 ? (T)1 / w : (T)0;
-          cprev = up[q] * winv;
-          gprev = (rh[q] - lo[q] * gprev) * winv;
-          cpv[i] = cprev;
-          x_out[i] = gprev;
-        }
+                        lo, dg, up, rhs);
+        const T w = dg - lo * cprev;
+        const T winv = w != (T)0 ? (T)1 / w : (T)0;
+        gprev = (rhs - lo * gprev) * winv;
+        hprev = (k == 0 ? -lo : -lo * hprev) * winv;
+        cprev = up * winv;
+        G[k] = gprev;
+        HL[k] = hprev;
+        UR[k] = cprev;  // eliminated upper coefficient until the backward pass
       }
     }
-    T xn = gprev;
-    for (int k1 = len - 2; k1 >= 0; k1 -= LB) {
-      T cv[LB], gv[LB];
+    // backward: x_k = G_k + HL_k x_L + UR_k x_R
+    T Gn = (T)0, Hn = (T)0, Un = (T)0;
 #pragma unroll
-      for (int q = 0; q < LB; ++q) {
-        const int k = k1 - q >= 0 ? k1 - q : 0;
-        const long i = base + stride * k;
-        cv[q] = cpv[i];
-        gv[q] = x_out[i];
-      }
-#pragma unroll
-      for (int q = 0; q < LB; ++q) {
-        if (k1 - q >= 0) {
-          xn = gv[q] - cv[q] * xn;
-          x_out[base + stride * (k1 - q)] = xn;
+    for (int k = SEG - 1; k >= 0; --k) {
+      if (k < cnt) {
+        const T cp = UR[k];
+        if (k == cnt - 1) {
+          Gn = G[k];
+          Hn = HL[k];
+          Un = -cp;
+        } else {
+          Gn = G[k] - cp * Gn;
+          Hn = HL[k] - cp * Hn;
+          Un = -cp * Un;
         }
+        G[k] = Gn;
+        HL[k] = Hn;
+        UR[k] = Un;
       }
     }
+  }
+  if (cnt > 0) {
+    IF(0, seg, lane) = G[0];
+    IF(1, seg, lane) = HL[0];
+    IF(2, seg, lane) = UR[0];
+    T ge = G[0], he = HL[0], ue = UR[0];
+#pragma unroll
+    for (int k = 1; k < SEG; ++k)
+      if (k == cnt - 1) {
+        ge = G[k];
+        he = HL[k];
+        ue = UR[k];
+      }
+    IF(3, seg, lane) = ge;
+    IF(4, seg, lane) = he;
+    IF(5, seg, lane) = ue;
+  }
+  __syncthreads();
+  if (seg == 0 && live) {
+    // x_s(j) = gam + del x_s(j+1), x_e(j) = alp + bet x_s(j+1), left to right
+    T alp = (T)0, bet = (T)0;
+    for (int j = 0; j < S; ++j) {
+      const T gs = IF(0, j, lane), hs = IF(1, j, lane), us = IF(2, j, lane);
+      const T ge = IF(3, j, lane), he = IF(4, j, lane), ue = IF(5, j, lane);
+      const T den = (T)1 - hs * bet;
+      const T inv = den != (T)0 ? (T)1 / den : (T)0;
+      const T gam = (gs + hs * alp) * inv, del = us * inv;
+      const T alp_n = ge + he * (alp + bet * gam);
+      const T bet_n = he * bet * del + ue;
+      IF(0, j, lane) = gam;
+      IF(1, j, lane) = del;
+      IF(3, j, lane) = alp_n;
+      IF(4, j, lane) = bet_n;
+      alp = alp_n;
+      bet = bet_n;
+    }
+    T xs_next = (T)0;  // x_s of the segment after the last one: none
+    for (int j = S - 1; j >= 0; --j) {
+      const T xe = IF(3, j, lane) + IF(4, j, lane) * xs_next;
+      const T xsj = IF(0, j, lane) + IF(1, j, lane) * xs_next;
+      IF(7, j, lane) = xs_next;  // R of segment j
+      if (j + 1 < S) IF(6, j + 1, lane) = xe;  // L of segment j + 1
+      xs_next = xsj;
+    }
+    IF(6, 0, lane) = (T)0;
+  }
+  __syncthreads();
+  if (cnt > 0) {
+    const T xl = IF(6, seg, lane), xr = IF(7, seg, lane);
+#pragma unroll
+    for (int k = 0; k < SEG; ++k)
+      if (k < cnt) x_out[base + stride * (k0 + k)] = G[k] + HL[k] * xl + UR[k] * xr;
   }
 }
 
@@ -1061,9 +1123,12 @@ static void line_pass_t(const Level& L, int dir, const T* x_in, T* x_out, T* cpv
       k_line_x<T, FT, 8><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero,
                                                    trans);
   } else {
-    const long lines = dir == 1 ? (long)d.N * d.K : (long)d.N * d.M;
-    k_line_t<T, FT><<<grid_1d(lines, 64), 64, 0, s>>>(dir, x_in, x_out, cpv, b, ae, an, at, sv, fs, d.N, d.M, d.K,
-                                                     L.pin0, L.pin1, zero, trans);
+    const int len = dir == 0 ? d.N : (dir == 1 ? d.M : d.K);
+    const int S = (len + SEG - 1) / SEG;
+    const dim3 grid(((dir == 0 ? d.M : d.N) + 63) / 64, dir == 1 ? d.K : (dir == 0 ? d.K : d.M));
+    (void)cpv;
+    k_line_seg<T, FT><<<grid, 64 * S, 8 * S * 64 * sizeof(T), s>>>(dir, x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M,
+                                                                   d.K, L.pin0, L.pin1, zero, trans, S);
   }
 }
 
