@@ -675,16 +675,21 @@ __global__ void k_zero_empty(const T* __restrict__ s, double* __restrict__ x, lo
 // in turn; on grids stretched towards the body, where cells are long in one or two directions
 // and point smoothers stall, every strong coupling is solved exactly in one of them.
 //
-// Row of cell i on its line: lower (coupling to the previous cell of the line), diag, upper,
-// rhs.  Pinned rows are identity rows; an empty row (1/a_p stored as 0) keeps its value, as in
-// the point smoother.
-template <class T, class FT>
-__device__ inline void line_row(int d, const T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
+// 1 / v (0 for v = 0): the FP32 hardware reciprocal (1 ulp), FP64 division
+__device__ inline float recip0(float v) { return v != 0.f ? __builtin_amdgcn_rcpf(v) : 0.f; }
+__device__ inline double recip0(double v) { return v != 0.0 ? 1.0 / v : 0.0; }
+
+// Row of cell i on its line of direction DIR: lower (coupling to the previous cell of the line),
+// diag, upper, rhs.  Pinned rows are identity rows; an empty row (1/a_p stored as 0) keeps its
+// value, as in the point smoother.  ZERO: x taken as 0 (not read).
+template <class T, class FT, int DIR, bool ZERO>
+__device__ inline void line_row(const T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
                                 const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s, T fs,
-                                long i, int xx, int yy, int zz, int N, int M, int K, int pin0, int pin1, bool zero,
-                                bool trans, T& lo, T& dg, T& up, T& rhs) {
-  // Every load is issued unconditionally (absent neighbours read the cell itself and are masked
-  // out), so that a caller evaluating many rows can have all their loads in flight at once.
+                                long i, int xx, int yy, int zz, int N, int M, int K, int pin0, int pin1, bool trans,
+                                T& lo, T& dg, T& up, T& rhs) {
+  // Straight-line code: every load is issued unconditionally (absent neighbours read the cell
+  // itself and are masked out), so that a caller evaluating many rows can have all their loads
+  // in flight at once.
   const long NM = (long)N * M;
   const bool own = i == pin0 || i == pin1;
   auto keep = [&](long j) { return !trans || own || (j != pin0 && j != pin1); };
@@ -701,251 +706,329 @@ __device__ inline void line_row(int d, const T* __restrict__ x, const T* __restr
   const T cw = hw ? fw : (T)0, ce = he ? fe : (T)0, cs = hs ? fsf : (T)0, cn = hn ? fn : (T)0;
   const T cb = hb ? fb : (T)0, ct = ht ? ft : (T)0;
   T off = (T)0, xi = (T)0;
-  if (!zero) {  // uniform over the launch
+  if constexpr (!ZERO) {
     xi = x[i];
-    if (d == 0) off = cs * x[is] + cn * x[in] + cb * x[ib] + ct * x[it];
-    else if (d == 1) off = cw * x[iw] + ce * x[ie] + cb * x[ib] + ct * x[it];
+    if constexpr (DIR == 0) off = cs * x[is] + cn * x[in] + cb * x[ib] + ct * x[it];
+    else if constexpr (DIR == 1) off = cw * x[iw] + ce * x[ie] + cb * x[ib] + ct * x[it];
     else off = cw * x[iw] + ce * x[ie] + cs * x[is] + cn * x[in];
   }
-  lo = d == 0 ? cw : (d == 1 ? cs : cb);
-  up = d == 0 ? ce : (d == 1 ? cn : ct);
-  dg = si != (T)0 ? (T)1 / si : (T)1;
-  rhs = bi - off;
-  if (own && !trans) {  // identity row
-    lo = up = (T)0;
-    dg = (T)1;
-    rhs = bi;
-  } else if (si == (T)0) {  // empty row: keeps its value
-    lo = up = (T)0;
-    dg = (T)1;
-    rhs = xi;
-  }
+  const bool ident = own && !trans, empty = si == (T)0;
+  const bool cut = ident || empty;  // identity row (pinned) or kept value (empty)
+  const T l0 = DIR == 0 ? cw : (DIR == 1 ? cs : cb);
+  const T u0 = DIR == 0 ? ce : (DIR == 1 ? cn : ct);
+  lo = cut ? (T)0 : l0;
+  up = cut ? (T)0 : u0;
+  dg = cut ? (T)1 : recip0(si);
+  rhs = ident ? bi : (empty ? xi : bi - off);
 }
 
-// x lines: one wavefront per line, CPL consecutive cells per lane (coalesced loads), solved by
-// parallel cyclic reduction over the 64 * CPL (padded with identity rows) equations.
+// x lines: one wavefront per line, CPL consecutive cells per lane (coalesced loads).  Partition
+// method: each lane eliminates its first CPL - 1 cells with two unknowns kept, the last cell of
+// the lane before (x_L) and its own last cell (x_E): x_k = G_k + H_k x_L + Q_k x_E.  The rows of
+// the 64 last cells then form a tridiagonal system in x_E, one unknown per lane, solved by
+// parallel cyclic reduction with cross-lane shuffles (6 steps); every lane substitutes back.
 template <class T>
 __device__ inline T wave_get(T v, int src, bool ok) {
   const T r = __shfl(v, ok ? src : 0, 64);
   return r;
 }
 
-template <class T, class FT, int CPL>
+// CPL consecutive elements of p from index i (i a multiple of CPL) as one wide load
+template <class E, int C>
+__device__ inline void ld_pack(const E* __restrict__ p, long i, E (&out)[C]) {
+  struct alignas(sizeof(E) * C) Pack {
+    E v[C];
+  };
+  const Pack pk = *reinterpret_cast<const Pack*>(p + i);
+#pragma unroll
+  for (int j = 0; j < C; ++j) out[j] = pk.v[j];
+}
+
+// The rows of cells i0 .. i0 + CPL - 1 of one x line (i0 = row start + CPL * lane, N a multiple
+// of CPL): the same rows as line_row, each array read with one wide load per lane.  The lines'
+// neighbours in y and z exist or not for the whole row; pinned cells are checked per cell.
+template <class T, class FT, int CPL, bool ZERO>
+__device__ inline void line_rows_x(const T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
+                                   const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s,
+                                   T fs, long i0, int k0, int yy, int zz, int N, int M, int K, int pin0, int pin1,
+                                   bool trans, T (&lo)[CPL], T (&dg)[CPL], T (&up)[CPL], T (&rh)[CPL]) {
+  const long NM = (long)N * M;
+  const bool ys = yy > 0, yn = yy < M - 1, zb = zz > 0, zt = zz < K - 1;
+  const long is0 = ys ? i0 - N : i0, in0 = yn ? i0 + N : i0, ib0 = zb ? i0 - NM : i0, it0 = zt ? i0 + NM : i0;
+  T sv[CPL], bv[CPL];
+  FT fev[CPL], fnv[CPL], fsv[CPL], ftv[CPL], fbv[CPL];
+  ld_pack<T, CPL>(s, i0, sv);
+  ld_pack<T, CPL>(b, i0, bv);
+  ld_pack<FT, CPL>(ae, i0, fev);
+  ld_pack<FT, CPL>(an, i0, fnv);
+  ld_pack<FT, CPL>(an, is0, fsv);
+  ld_pack<FT, CPL>(at, i0, ftv);
+  ld_pack<FT, CPL>(at, ib0, fbv);
+  const FT fw0 = ae[k0 > 0 ? i0 - 1 : i0];
+  T xv[CPL], xs[CPL], xn[CPL], xb[CPL], xt[CPL];
+  if constexpr (!ZERO) {
+    ld_pack<T, CPL>(x, i0, xv);
+    ld_pack<T, CPL>(x, is0, xs);
+    ld_pack<T, CPL>(x, in0, xn);
+    ld_pack<T, CPL>(x, ib0, xb);
+    ld_pack<T, CPL>(x, it0, xt);
+  }
+  auto pinned = [&](long j) { return j == pin0 || j == pin1; };
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) {
+    const long i = i0 + j;
+    const int k = k0 + j;
+    const bool own = pinned(i);
+    auto keep = [&](long q) { return !trans || own || !pinned(q); };
+    const bool hw = k > 0 && keep(i - 1), he = k < N - 1 && keep(i + 1);
+    const bool hs = ys && keep(i - N), hn = yn && keep(i + N);
+    const bool hb = zb && keep(i - NM), ht = zt && keep(i + NM);
+    const T fw = fs * (T)(j == 0 ? fw0 : fev[j > 0 ? j - 1 : 0]), fe = fs * (T)fev[j];
+    const T fsf = fs * (T)fsv[j], fn = fs * (T)fnv[j];
+    const T fb = fs * (T)fbv[j], ft = fs * (T)ftv[j];
+    const T cw = hw ? fw : (T)0, ce = he ? fe : (T)0, cs = hs ? fsf : (T)0, cn = hn ? fn : (T)0;
+    const T cb = hb ? fb : (T)0, ct = ht ? ft : (T)0;
+    T off = (T)0, xi = (T)0;
+    if constexpr (!ZERO) {
+      xi = xv[j];
+      off = cs * xs[j] + cn * xn[j] + cb * xb[j] + ct * xt[j];
+    }
+    const T si = sv[j], bi = bv[j];
+    const bool ident = own && !trans, empty = si == (T)0, cut = ident || empty;
+    lo[j] = cut ? (T)0 : cw;
+    up[j] = cut ? (T)0 : ce;
+    dg[j] = cut ? (T)1 : recip0(si);
+    rh[j] = ident ? bi : (empty ? xi : bi - off);
+  }
+}
+
+template <class T, class FT, int CPL, bool ZERO>
 __global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* __restrict__ x_out,
                                                 const T* __restrict__ b, const FT* __restrict__ ae,
                                                 const FT* __restrict__ an, const FT* __restrict__ at,
                                                 const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
-                                                int pin1, int zero, int trans) {
+                                                int pin1, int trans) {
   const int lane = threadIdx.x & 63;
   const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= (long)M * K) return;  // whole wavefronts leave together
   const int yy = (int)(row % M), zz = (int)(row / M);
-  constexpr int L = 64 * CPL;
-  T a[CPL], d[CPL], c[CPL], r[CPL];
-#pragma unroll
-  for (int j = 0; j < CPL; ++j) {
-    const int k = lane * CPL + j;
-    if (k < N) {
-      line_row<T, FT>(0, x_in, b, ae, an, at, s, fs, row * N + k, k, yy, zz, N, M, K, pin0, pin1, zero != 0,
-                      trans != 0, a[j], d[j], c[j], r[j]);
-    } else {
-      a[j] = c[j] = r[j] = (T)0;
-      d[j] = (T)1;
-    }
-  }
-#pragma unroll
-  for (int st = 1; st < L; st <<= 1) {
-    T na[CPL], nd[CPL], nc[CPL], nr[CPL];
+  T lo[CPL], dg[CPL], up[CPL], rh[CPL];
+  if (N % CPL == 0 && lane * CPL < N) {
+    line_rows_x<T, FT, CPL, ZERO>(x_in, b, ae, an, at, s, fs, row * N + lane * CPL, lane * CPL, yy, zz, N, M, K,
+                                  pin0, pin1, trans != 0, lo, dg, up, rh);
+  } else {
 #pragma unroll
     for (int j = 0; j < CPL; ++j) {
       const int k = lane * CPL + j;
-      // equation k - st
-      T am, dm, cm, rm;
-      if (st < CPL && j - st >= 0) {
-        am = a[(j - st) & (CPL - 1)];
-        dm = d[(j - st) & (CPL - 1)];
-        cm = c[(j - st) & (CPL - 1)];
-        rm = r[(j - st) & (CPL - 1)];
-      } else {
-        const int jj = st < CPL ? ((j - st + CPL) & (CPL - 1)) : j;
-        const int src = st < CPL ? lane - 1 : lane - st / CPL;
-        const bool ok = k - st >= 0;
-        am = wave_get(a[jj], src, ok);
-        dm = wave_get(d[jj], src, ok);
-        cm = wave_get(c[jj], src, ok);
-        rm = wave_get(r[jj], src, ok);
-        if (!ok) {
-          am = cm = rm = (T)0;
-          dm = (T)1;
-        }
+      if (k < N) {
+        line_row<T, FT, 0, ZERO>(x_in, b, ae, an, at, s, fs, row * N + k, k, yy, zz, N, M, K, pin0, pin1,
+                                 trans != 0, lo[j], dg[j], up[j], rh[j]);
+      } else {  // padding: decoupled identity rows
+        lo[j] = up[j] = rh[j] = (T)0;
+        dg[j] = (T)1;
       }
-      // equation k + st
-      T ap, dp, cp, rp;
-      if (st < CPL && j + st < CPL) {
-        ap = a[(j + st) & (CPL - 1)];
-        dp = d[(j + st) & (CPL - 1)];
-        cp = c[(j + st) & (CPL - 1)];
-        rp = r[(j + st) & (CPL - 1)];
-      } else {
-        const int jj = st < CPL ? ((j + st) & (CPL - 1)) : j;
-        const int src = st < CPL ? lane + 1 : lane + st / CPL;
-        const bool ok = k + st < L;
-        ap = wave_get(a[jj], src, ok);
-        dp = wave_get(d[jj], src, ok);
-        cp = wave_get(c[jj], src, ok);
-        rp = wave_get(r[jj], src, ok);
-        if (!ok) {
-          ap = cp = rp = (T)0;
-          dp = (T)1;
-        }
-      }
-      const T al = dm != (T)0 ? -a[j] / dm : (T)0;
-      const T ga = dp != (T)0 ? -c[j] / dp : (T)0;
-      na[j] = al * am;
-      nc[j] = ga * cp;
-      nd[j] = d[j] + al * cm + ga * ap;
-      nr[j] = r[j] + al * rm + ga * rp;
-    }
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      a[j] = na[j];
-      d[j] = nd[j];
-      c[j] = nc[j];
-      r[j] = nr[j];
     }
   }
+  // local elimination: forward x_j = g_j + h_j x_L - c_j x_{j+1}, backward with x_{CPL-1} = x_E
+  T G[CPL], H[CPL], Q[CPL];
+  {
+    T cp = (T)0, gp = (T)0, hp = (T)1;  // x_{-1} = x_L
+#pragma unroll
+    for (int j = 0; j < CPL - 1; ++j) {
+      const T winv = recip0(dg[j] - lo[j] * cp);
+      gp = (rh[j] - lo[j] * gp) * winv;
+      hp = -lo[j] * hp * winv;
+      cp = up[j] * winv;
+      G[j] = gp;
+      H[j] = hp;
+      Q[j] = cp;
+    }
+    T Gn = (T)0, Hn = (T)0, Qn = (T)1;  // cell CPL-1 is x_E
+#pragma unroll
+    for (int j = CPL - 2; j >= 0; --j) {
+      const T c = Q[j];
+      Gn = G[j] - c * Gn;
+      Hn = H[j] - c * Hn;
+      Qn = -c * Qn;
+      G[j] = Gn;
+      H[j] = Hn;
+      Q[j] = Qn;
+    }
+  }
+  // the cell before the last in terms of (x_L, x_E), and the first cell, which the lane before needs
+  const T Gb = CPL > 1 ? G[CPL > 1 ? CPL - 2 : 0] : (T)0;
+  const T Hb = CPL > 1 ? H[CPL > 1 ? CPL - 2 : 0] : (T)1;
+  const T Qb = CPL > 1 ? Q[CPL > 1 ? CPL - 2 : 0] : (T)0;
+  const T Gf = CPL > 1 ? G[0] : (T)0, Hf = CPL > 1 ? H[0] : (T)0, Qf = CPL > 1 ? Q[0] : (T)1;
+  const bool nxt = lane < 63;
+  // every lane takes part in the shuffles (a lane outside a branch supplies no value)
+  const T G1v = wave_get(Gf, lane + 1, nxt), H1v = wave_get(Hf, lane + 1, nxt), Q1v = wave_get(Qf, lane + 1, nxt);
+  const T G1 = nxt ? G1v : (T)0, H1 = nxt ? H1v : (T)0, Q1 = nxt ? Q1v : (T)0;
+  // row of the last cell: lo x_{CPL-2} + dg x_E + up x_0(next lane)
+  const T le = lo[CPL - 1], ue = up[CPL - 1];
+  T a = le * Hb;
+  T d = dg[CPL - 1] + le * Qb + ue * H1;
+  T c = ue * Q1;
+  T r = rh[CPL - 1] - le * Gb - ue * G1;
+#pragma unroll
+  for (int st = 1; st < 64; st <<= 1) {
+    const bool okm = lane - st >= 0, okp = lane + st < 64;
+    T am = wave_get(a, lane - st, okm), dm = wave_get(d, lane - st, okm);
+    T cm = wave_get(c, lane - st, okm), rm = wave_get(r, lane - st, okm);
+    T ap = wave_get(a, lane + st, okp), dp = wave_get(d, lane + st, okp);
+    T cp = wave_get(c, lane + st, okp), rp = wave_get(r, lane + st, okp);
+    if (!okm) {
+      am = cm = rm = (T)0;
+      dm = (T)1;
+    }
+    if (!okp) {
+      ap = cp = rp = (T)0;
+      dp = (T)1;
+    }
+    const T al = -a * recip0(dm), ga = -c * recip0(dp);
+    a = al * am;
+    c = ga * cp;
+    d = d + al * cm + ga * ap;
+    r = r + al * rm + ga * rp;
+  }
+  const T xe = r * recip0(d);
+  const T xl0 = wave_get(xe, lane - 1, lane > 0);
+  const T xl = lane > 0 ? xl0 : (T)0;
 #pragma unroll
   for (int j = 0; j < CPL; ++j) {
     const int k = lane * CPL + j;
-    if (k < N) x_out[row * N + k] = d[j] != (T)0 ? r[j] / d[j] : (T)0;
+    const T v = j == CPL - 1 ? xe : G[j] + H[j] * xl + Q[j] * xe;
+    if (k < N) x_out[row * N + k] = v;
   }
 }
 
-// y and z lines: a workgroup holds 64 lines (adjacent x, so every load coalesces) split into S
-// segments of SEG rows, one wavefront per segment (S = ceil(len / SEG) <= 16: lines of at most
-// 256 cells).  Each thread eliminates its segment with the
-// two couplings to the neighbouring segments kept as unknowns, x_k = G_k + HL_k x_L + UR_k x_R
-// (x_L the last row of the segment before, x_R the first of the one after; all in registers);
-// one thread per line then solves the 2S interface values in order through LDS, and every
-// segment writes x_k.  Inputs are read once and x written once, with 64 S threads per 64 lines
-// instead of 64.
-constexpr int SEG = 16;
+// y and z lines: a workgroup holds LPW = 32 lines (adjacent x: every load is a full 128-byte
+// line for FP32) split into S segments of SEG = 8 rows; thread t takes line t % 32 and segment
+// t / 32 (S = ceil(len / SEG) <= 32: lines of at most 256 cells).  The partition method of the
+// x lines with segments in place of lanes: each thread eliminates its segment's first cnt - 1
+// rows keeping x_L (last cell of the segment before) and x_E (its own last cell) as unknowns,
+// x_k = G_k + H_k x_L + Q_k x_E; the S last-cell rows of a line form a tridiagonal system,
+// solved by cyclic reduction across the wavefronts through LDS (log2 S steps); every segment
+// then writes x_k.  Inputs are read once and x written once.
+constexpr int SEG = 8;
+constexpr int LPW = 32;
 
-template <class T, class FT>
-__global__ void __launch_bounds__(1024) k_line_seg(int dir, const T* __restrict__ x_in, T* __restrict__ x_out,
+template <class T, class FT, int DIR, bool ZERO>
+__global__ void __launch_bounds__(1024) k_line_seg(const T* __restrict__ x_in, T* __restrict__ x_out,
                                                    const T* __restrict__ b, const FT* __restrict__ ae,
                                                    const FT* __restrict__ an, const FT* __restrict__ at,
                                                    const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
-                                                   int pin1, int zero, int trans, int S) {
+                                                   int pin1, int trans, int S) {
   extern __shared__ unsigned char lds_raw[];
-  T* ifc = reinterpret_cast<T*>(lds_raw);  // [8][S][64]: Gs HLs URs Ge HLe URe, then L R
-  auto IF = [&](int q, int j, int l) -> T& { return ifc[((long)q * S + j) * 64 + l]; };
+  T* ifc = reinterpret_cast<T*>(lds_raw);  // [8][S][LPW]: a d c r of the reduced rows, G H Q of first cells, x_E
+  auto IF = [&](int q, int j, int l) -> T& { return ifc[((long)q * S + j) * LPW + l]; };
   const long NM = (long)N * M;
-  const int lane = threadIdx.x & 63, seg = threadIdx.x >> 6;
-  // x lines: the 64 lanes take 64 adjacent rows (y); y and z lines: 64 adjacent x
-  const int cross = blockIdx.x * 64 + lane, other = blockIdx.y;
-  const int len = dir == 0 ? N : (dir == 1 ? M : K);
-  const long stride = dir == 0 ? 1L : (dir == 1 ? (long)N : NM);
-  const bool live = cross < (dir == 0 ? M : N);
-  const long base = dir == 0 ? (long)N * cross + NM * other
-                             : (dir == 1 ? cross + NM * other : cross + (long)N * other);
+  const int lane = threadIdx.x % LPW, seg = threadIdx.x / LPW;
+  const int xr = blockIdx.x * LPW + lane, other = blockIdx.y;
+  const bool live = xr < N;
+  const int xx = live ? xr : N - 1;  // lanes past the last line repeat it and store nothing
+  const int len = DIR == 1 ? M : K;
+  const long stride = DIR == 1 ? (long)N : NM;
+  const long base = DIR == 1 ? xx + NM * other : xx + (long)N * other;
   const int k0 = seg * SEG;
-  const int cnt = live ? (len - k0 < SEG ? len - k0 : SEG) : 0;
-  T G[SEG], HL[SEG], UR[SEG];
+  const int cnt = len - k0 < SEG ? len - k0 : SEG;  // >= 1
+  // Straight-line code over the SEG rows (rows past the segment's end re-read its last row and
+  // are masked), so that the loads of all of them are in flight together.
+  T G[SEG], H[SEG], Q[SEG];
+  T le = (T)0, de = (T)1, ue = (T)0, re = (T)0;  // row of the last cell
+  T Gb = (T)0, Hb = (T)1, Qb = (T)0;              // the cell before the last, in (x_L, x_E)
+  T Gf = (T)0, Hf = (T)0, Qf = (T)1;              // the first cell, in (x_L, x_E)
   {
-    T cprev = (T)0, gprev = (T)0, hprev = (T)0;
+    T lo[SEG], dg[SEG], up[SEG], rh[SEG];
 #pragma unroll
     for (int k = 0; k < SEG; ++k) {
-      if (k < cnt) {
-        const int kk = k0 + k;
-        const long i = base + stride * kk;
-        const int xx = dir == 0 ? kk : cross;
-        const int yy = dir == 0 ? cross : (dir == 1 ? kk : other), zz = dir == 2 ? kk : other;
-        T lo, dg, up, rhs;
-        line_row<T, FT>(dir, x_in, b, ae, an, at, s, fs, i, xx, yy, zz, N, M, K, pin0, pin1, zero != 0, trans != 0,
-                        lo, dg, up, rhs);
-        const T w = dg - lo * cprev;
-        const T winv = w != (T)0 ? (T)1 / w : (T)0;
-        gprev = (rhs - lo * gprev) * winv;
-        hprev = (k == 0 ? -lo : -lo * hprev) * winv;
-        cprev = up * winv;
-        G[k] = gprev;
-        HL[k] = hprev;
-        UR[k] = cprev;  // eliminated upper coefficient until the backward pass
-      }
+      const int kk = k0 + (k < cnt ? k : cnt - 1);
+      const int yy = DIR == 1 ? kk : other, zz = DIR == 1 ? other : kk;
+      line_row<T, FT, DIR, ZERO>(x_in, b, ae, an, at, s, fs, base + stride * kk, xx, yy, zz, N, M, K, pin0, pin1,
+                                 trans != 0, lo[k], dg[k], up[k], rh[k]);
     }
-    // backward: x_k = G_k + HL_k x_L + UR_k x_R
-    T Gn = (T)0, Hn = (T)0, Un = (T)0;
+    T cp = (T)0, gp = (T)0, hp = (T)1;  // x_{-1} = x_L
 #pragma unroll
-    for (int k = SEG - 1; k >= 0; --k) {
-      if (k < cnt) {
-        const T cp = UR[k];
-        if (k == cnt - 1) {
-          Gn = G[k];
-          Hn = HL[k];
-          Un = -cp;
-        } else {
-          Gn = G[k] - cp * Gn;
-          Hn = HL[k] - cp * Hn;
-          Un = -cp * Un;
-        }
-        G[k] = Gn;
-        HL[k] = Hn;
-        UR[k] = Un;
-      }
+    for (int k = 0; k < SEG; ++k) {
+      // x_k = g_k + h_k x_L - c_k x_{k+1} for the rows before the last
+      const bool elim = k < cnt - 1, last = k == cnt - 1;
+      const T winv = recip0(dg[k] - lo[k] * cp);
+      const T gn = (rh[k] - lo[k] * gp) * winv, hn = -lo[k] * hp * winv, cn = up[k] * winv;
+      gp = elim ? gn : gp;
+      hp = elim ? hn : hp;
+      cp = elim ? cn : cp;
+      G[k] = gp;
+      H[k] = hp;
+      Q[k] = cp;
+      le = last ? lo[k] : le;
+      de = last ? dg[k] : de;
+      ue = last ? up[k] : ue;
+      re = last ? rh[k] : re;
     }
   }
-  if (cnt > 0) {
-    IF(0, seg, lane) = G[0];
-    IF(1, seg, lane) = HL[0];
-    IF(2, seg, lane) = UR[0];
-    T ge = G[0], he = HL[0], ue = UR[0];
+  {
+    T Gn = (T)0, Hn = (T)0, Qn = (T)1;  // x_{cnt-1} = x_E
 #pragma unroll
-    for (int k = 1; k < SEG; ++k)
-      if (k == cnt - 1) {
-        ge = G[k];
-        he = HL[k];
-        ue = UR[k];
-      }
-    IF(3, seg, lane) = ge;
-    IF(4, seg, lane) = he;
-    IF(5, seg, lane) = ue;
-  }
-  __syncthreads();
-  if (seg == 0 && live) {
-    // x_s(j) = gam + del x_s(j+1), x_e(j) = alp + bet x_s(j+1), left to right
-    T alp = (T)0, bet = (T)0;
-    for (int j = 0; j < S; ++j) {
-      const T gs = IF(0, j, lane), hs = IF(1, j, lane), us = IF(2, j, lane);
-      const T ge = IF(3, j, lane), he = IF(4, j, lane), ue = IF(5, j, lane);
-      const T den = (T)1 - hs * bet;
-      const T inv = den != (T)0 ? (T)1 / den : (T)0;
-      const T gam = (gs + hs * alp) * inv, del = us * inv;
-      const T alp_n = ge + he * (alp + bet * gam);
-      const T bet_n = he * bet * del + ue;
-      IF(0, j, lane) = gam;
-      IF(1, j, lane) = del;
-      IF(3, j, lane) = alp_n;
-      IF(4, j, lane) = bet_n;
-      alp = alp_n;
-      bet = bet_n;
+    for (int k = SEG - 2; k >= 0; --k) {
+      const bool act = k < cnt - 1;
+      const T c = Q[k];
+      const T gn = G[k] - c * Gn, hn = H[k] - c * Hn, qn = -c * Qn;
+      Gn = act ? gn : Gn;
+      Hn = act ? hn : Hn;
+      Qn = act ? qn : Qn;
+      G[k] = Gn;
+      H[k] = Hn;
+      Q[k] = Qn;
+      const bool bl = k == cnt - 2;
+      Gb = bl ? Gn : Gb;
+      Hb = bl ? Hn : Hb;
+      Qb = bl ? Qn : Qb;
     }
-    T xs_next = (T)0;  // x_s of the segment after the last one: none
-    for (int j = S - 1; j >= 0; --j) {
-      const T xe = IF(3, j, lane) + IF(4, j, lane) * xs_next;
-      const T xsj = IF(0, j, lane) + IF(1, j, lane) * xs_next;
-      IF(7, j, lane) = xs_next;  // R of segment j
-      if (j + 1 < S) IF(6, j + 1, lane) = xe;  // L of segment j + 1
-      xs_next = xsj;
+    if (cnt > 1) {
+      Gf = Gn;
+      Hf = Hn;
+      Qf = Qn;
     }
-    IF(6, 0, lane) = (T)0;
   }
+  IF(4, seg, lane) = Gf;
+  IF(5, seg, lane) = Hf;
+  IF(6, seg, lane) = Qf;
   __syncthreads();
-  if (cnt > 0) {
-    const T xl = IF(6, seg, lane), xr = IF(7, seg, lane);
+  const bool nxt = seg + 1 < S;
+  const T G1 = nxt ? IF(4, seg + 1, lane) : (T)0;
+  const T H1 = nxt ? IF(5, seg + 1, lane) : (T)0;
+  const T Q1 = nxt ? IF(6, seg + 1, lane) : (T)0;
+  // row of the last cell: lo x_before + dg x_E + up x_first(next segment)
+  T a = le * Hb;
+  T d = de + le * Qb + ue * H1;
+  T c = ue * Q1;
+  T r = re - le * Gb - ue * G1;
+  for (int st = 1; st < S; st <<= 1) {  // S is uniform over the workgroup
+    IF(0, seg, lane) = a;
+    IF(1, seg, lane) = d;
+    IF(2, seg, lane) = c;
+    IF(3, seg, lane) = r;
+    __syncthreads();
+    const bool okm = seg - st >= 0, okp = seg + st < S;
+    const T am = okm ? IF(0, seg - st, lane) : (T)0, dm = okm ? IF(1, seg - st, lane) : (T)1;
+    const T cm = okm ? IF(2, seg - st, lane) : (T)0, rm = okm ? IF(3, seg - st, lane) : (T)0;
+    const T ap = okp ? IF(0, seg + st, lane) : (T)0, dp = okp ? IF(1, seg + st, lane) : (T)1;
+    const T cq = okp ? IF(2, seg + st, lane) : (T)0, rp = okp ? IF(3, seg + st, lane) : (T)0;
+    const T al = -a * recip0(dm), ga = -c * recip0(dp);
+    a = al * am;
+    c = ga * cq;
+    d = d + al * cm + ga * ap;
+    r = r + al * rm + ga * rp;
+    __syncthreads();
+  }
+  const T xe = r * recip0(d);
+  IF(7, seg, lane) = xe;
+  __syncthreads();
+  const T xl = seg > 0 ? IF(7, seg - 1, lane) : (T)0;
+  if (live) {
 #pragma unroll
     for (int k = 0; k < SEG; ++k)
-      if (k < cnt) x_out[base + stride * (k0 + k)] = G[k] + HL[k] * xl + UR[k] * xr;
+      if (k < cnt) x_out[base + stride * (k0 + k)] = k == cnt - 1 ? xe : G[k] + H[k] * xl + Q[k] * xe;
   }
 }
 
@@ -1099,9 +1182,8 @@ template void restrict_pt<double>(const double*, Dims, double*, hipStream_t);
 template void prolong_inject_add<float>(const float*, Dims, float*, Dims, hipStream_t);
 template void prolong_inject_add<double>(const double*, Dims, double*, Dims, hipStream_t);
 
-template <class T, class FT>
-static void line_pass_t(const Level& L, int dir, const T* x_in, T* x_out, T* cpv, const T* b, int zero, int trans,
-                        hipStream_t s) {
+template <class T, class FT, bool ZERO>
+static void line_pass_z(const Level& L, int dir, const T* x_in, T* x_out, const T* b, int trans, hipStream_t s) {
   const Dims& d = L.d;
   const FT* ae = (const FT*)L.f_e;
   const FT* an = (const FT*)L.f_n;
@@ -1110,26 +1192,40 @@ static void line_pass_t(const Level& L, int dir, const T* x_in, T* x_out, T* cpv
   const T fs = (T)L.fscale;
   if (dir == 0) {
     const int blocks = (int)(((long)d.M * d.K + 3) / 4);
+#define LSX_LINE_X(CPL)                                                                                        \
+  k_line_x<T, FT, CPL, ZERO><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, \
+                                                    L.pin1, trans)
     if (d.N <= 64)
-      k_line_x<T, FT, 1><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero,
-                                                   trans);
+      LSX_LINE_X(1);
     else if (d.N <= 128)
-      k_line_x<T, FT, 2><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero,
-                                                   trans);
+      LSX_LINE_X(2);
     else if (d.N <= 256)
-      k_line_x<T, FT, 4><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero,
-                                                   trans);
+      LSX_LINE_X(4);
     else
-      k_line_x<T, FT, 8><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, L.pin1, zero,
-                                                   trans);
+      LSX_LINE_X(8);
+#undef LSX_LINE_X
   } else {
-    const int len = dir == 0 ? d.N : (dir == 1 ? d.M : d.K);
+    const int len = dir == 1 ? d.M : d.K;
     const int S = (len + SEG - 1) / SEG;
-    const dim3 grid(((dir == 0 ? d.M : d.N) + 63) / 64, dir == 1 ? d.K : (dir == 0 ? d.K : d.M));
-    (void)cpv;
-    k_line_seg<T, FT><<<grid, 64 * S, 8 * S * 64 * sizeof(T), s>>>(dir, x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M,
-                                                                   d.K, L.pin0, L.pin1, zero, trans, S);
+    const dim3 grid((d.N + LPW - 1) / LPW, dir == 1 ? d.K : d.M);
+    const size_t lds = 8 * (size_t)S * LPW * sizeof(T);
+    if (dir == 1)
+      k_line_seg<T, FT, 1, ZERO><<<grid, LPW * S, lds, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0,
+                                                            L.pin1, trans, S);
+    else
+      k_line_seg<T, FT, 2, ZERO><<<grid, LPW * S, lds, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0,
+                                                            L.pin1, trans, S);
   }
+}
+
+template <class T, class FT>
+static void line_pass_t(const Level& L, int dir, const T* x_in, T* x_out, T* cpv, const T* b, int zero, int trans,
+                        hipStream_t s) {
+  (void)cpv;
+  if (zero)
+    line_pass_z<T, FT, true>(L, dir, x_in, x_out, b, trans, s);
+  else
+    line_pass_z<T, FT, false>(L, dir, x_in, x_out, b, trans, s);
 }
 
 template <>
