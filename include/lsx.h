@@ -161,12 +161,13 @@ int lsx_synchronize(lsx_solver* s);
  * the summed device time and the kernel's algorithmic HBM bytes per launch, averaged over the
  * launches (compulsory traffic: each array read or written once). */
 #define LSX_K_RESIDUAL 0   /* r -= A c inside the smoother                 */
-#define LSX_K_LINE_X 1     /* x-line solve, c = w T^-1 r, ch += c           */
-#define LSX_K_LINE_Y 2
-#define LSX_K_LINE_Z 3
+#define LSX_K_LINE_X 1     /* x-line solve: cascade c = w T^-1 r, ch += c;  */
+#define LSX_K_LINE_Y 2     /*   V-cycle line relaxation, one pass of x, y  */
+#define LSX_K_LINE_Z 3     /*   or z lines                                  */
 #define LSX_K_OPERATOR 4   /* w = A z in the Krylov loop                    */
 #define LSX_K_ORTHO 5      /* MGS step: fused w -= h v_i ; h' = w . v_{i+1}  */
-#define LSX_K_SMOOTH 6     /* smoother sweeps of the fine level (V-cycle)   */
+#define LSX_K_SMOOTH 6     /* red-black launch of the fine level: one fused */
+                           /*   sweep, or one colour (half a sweep)         */
 #define LSX_K_CGS_DOT 7    /* block Gram-Schmidt h = V^T w (V-cycle loops)  */
 #define LSX_K_CGS_UPDATE 8 /* w -= V h fused with V^T w or ||w|| (V-cycle)  */
 #define LSX_K_COUNT 16

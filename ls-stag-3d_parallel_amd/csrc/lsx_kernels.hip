@@ -436,7 +436,9 @@ __global__ void k_sym_build(const double* __restrict__ c7, const unsigned char* 
 }
 
 // Row i of the symmetric fine operator, terms in the reference CSR order [b,s,w,p,e,n,t]
-// (identical values and order to the 7-plane row, hence bit-identical results).
+// (identical values and order to the 7-plane row, hence bit-identical results).  Every load is
+// issued unconditionally (absent neighbours read the cell itself) and absent terms leave the sum
+// unchanged, so the seven loads of a row are in flight together.
 template <class XT>
 __device__ inline double row_apply_sym(const double* __restrict__ s4, long n, int i, int x, int y, int z, int N,
                                        int M, int K, const XT* __restrict__ xv, double acc, bool subtract,
@@ -445,19 +447,26 @@ __device__ inline double row_apply_sym(const double* __restrict__ s4, long n, in
   bool exc_row = false;
 #pragma unroll
   for (int q = 0; q < SymExc::CAP; ++q) exc_row |= (q < ex.count) && (ex.row[q] == i);
-  double t;
-  if (!exc_row) {
-    if (z > 0) { t = s4[3 * n + i - NM] * (double)xv[i - NM]; acc = subtract ? acc - t : acc + t; }
-    if (y > 0) { t = s4[2 * n + i - N] * (double)xv[i - N]; acc = subtract ? acc - t : acc + t; }
-    if (x > 0) { t = s4[1 * n + i - 1] * (double)xv[i - 1]; acc = subtract ? acc - t : acc + t; }
-  }
-  t = s4[i] * (double)xv[i];
-  acc = subtract ? acc - t : acc + t;
-  if (!exc_row) {
-    if (x < N - 1) { t = s4[1 * n + i] * (double)xv[i + 1]; acc = subtract ? acc - t : acc + t; }
-    if (y < M - 1) { t = s4[2 * n + i] * (double)xv[i + N]; acc = subtract ? acc - t : acc + t; }
-    if (z < K - 1) { t = s4[3 * n + i] * (double)xv[i + NM]; acc = subtract ? acc - t : acc + t; }
-  }
+  const bool hb = !exc_row && z > 0, hs = !exc_row && y > 0, hw = !exc_row && x > 0;
+  const bool he = !exc_row && x < N - 1, hn = !exc_row && y < M - 1, ht = !exc_row && z < K - 1;
+  const int ib = z > 0 ? i - NM : i, is = y > 0 ? i - N : i, iw = x > 0 ? i - 1 : i;
+  const int ie = x < N - 1 ? i + 1 : i, in = y < M - 1 ? i + N : i, it = z < K - 1 ? i + NM : i;
+  const double cb = s4[3 * n + ib], cs = s4[2 * n + is], cw = s4[1 * n + iw], cp = s4[i];
+  const double ce = s4[1 * n + i], cn = s4[2 * n + i], ct = s4[3 * n + i];
+  const double xb = (double)xv[ib], xs = (double)xv[is], xw = (double)xv[iw], xp = (double)xv[i];
+  const double xe = (double)xv[ie], xn = (double)xv[in], xt = (double)xv[it];
+  auto add = [&](bool h, double c, double xj) {
+    const double t = c * xj;
+    const double a = subtract ? acc - t : acc + t;
+    acc = h ? a : acc;
+  };
+  add(hb, cb, xb);
+  add(hs, cs, xs);
+  add(hw, cw, xw);
+  add(true, cp, xp);
+  add(he, ce, xe);
+  add(hn, cn, xn);
+  add(ht, ct, xt);
   return acc;
 }
 
@@ -770,6 +779,33 @@ __global__ void k_axpy_dot(double* __restrict__ w, const double* __restrict__ v,
 // pass keeps the coefficients in LDS and streams the vectors row by row.
 constexpr int CGS_GROUP = 8;
 
+// acc[k] += sum over the thread's rows i of V_k[i] w[i], rows in increasing order, two rows
+// (i and i + BS) per step so that 2 (KC + 1) loads are in flight
+template <int BS, int KC>
+__device__ inline void cgs_dot_rows(const double* __restrict__ Vg, long ldv, const double* __restrict__ w, long lo,
+                                    long hi, double (&acc)[CGS_GROUP]) {
+  long i = lo + threadIdx.x;
+  for (; i + BS < hi; i += 2 * BS) {
+    const double w0 = w[i], w1 = w[i + BS];
+    double v0[KC], v1[KC];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      v0[k] = Vg[k * ldv + i];
+      v1[k] = Vg[k * ldv + i + BS];
+    }
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      acc[k] += v0[k] * w0;
+      acc[k] += v1[k] * w1;
+    }
+  }
+  if (i < hi) {
+    const double w0 = w[i];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) acc[k] += Vg[k * ldv + i] * w0;
+  }
+}
+
 template <int BS>
 __global__ void __launch_bounds__(BS) k_cgs_dot(const double* __restrict__ V, long ldv, int cnt,
                                                 const double* __restrict__ w, long n, double* __restrict__ partial) {
@@ -783,11 +819,15 @@ __global__ void __launch_bounds__(BS) k_cgs_dot(const double* __restrict__ V, lo
   double acc[CGS_GROUP];
 #pragma unroll
   for (int k = 0; k < CGS_GROUP; ++k) acc[k] = 0.0;
-  for (long i = lo + threadIdx.x; i < hi; i += BS) {
-    const double wi = w[i];
-#pragma unroll
-    for (int k = 0; k < CGS_GROUP; ++k)
-      if (k < kc) acc[k] += Vg[k * ldv + i] * wi;
+  switch (kc) {  // uniform over the block: every load of a step is issued unconditionally
+    case 1: cgs_dot_rows<BS, 1>(Vg, ldv, w, lo, hi, acc); break;
+    case 2: cgs_dot_rows<BS, 2>(Vg, ldv, w, lo, hi, acc); break;
+    case 3: cgs_dot_rows<BS, 3>(Vg, ldv, w, lo, hi, acc); break;
+    case 4: cgs_dot_rows<BS, 4>(Vg, ldv, w, lo, hi, acc); break;
+    case 5: cgs_dot_rows<BS, 5>(Vg, ldv, w, lo, hi, acc); break;
+    case 6: cgs_dot_rows<BS, 6>(Vg, ldv, w, lo, hi, acc); break;
+    case 7: cgs_dot_rows<BS, 7>(Vg, ldv, w, lo, hi, acc); break;
+    default: cgs_dot_rows<BS, CGS_GROUP>(Vg, ldv, w, lo, hi, acc); break;
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -815,7 +855,21 @@ __global__ void __launch_bounds__(BS) k_cgs_update(const double* __restrict__ V,
   long lo, hi;
   chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
   double ss = 0.0;
-  for (long i = lo + threadIdx.x; i < hi; i += BS) {
+  long i = lo + threadIdx.x;
+  for (; i + BS < hi; i += 2 * BS) {  // two rows per step, each updated in the order k = 0, 1, ...
+    double w0 = w[i], w1 = w[i + BS];
+    const double* vp = V + i;
+#pragma unroll 8
+    for (int k = 0; k < cnt; ++k) {
+      w0 -= hk[k] * vp[k * ldv];
+      w1 -= hk[k] * vp[k * ldv + BS];
+    }
+    w[i] = w0;
+    w[i + BS] = w1;
+    ss += w0 * w0;
+    ss += w1 * w1;
+  }
+  if (i < hi) {
     double wn = w[i];
     const double* vp = V + i;
 #pragma unroll 8

@@ -124,17 +124,8 @@ double Solver::kernel_bytes(int k) const {
     case LSX_K_OPERATOR:                   // coefficients 32, w 8, z 8 (FP64) or 4 (FP32)
       return (mixed() ? 44.0 : 48.0) * n;
     case LSX_K_ORTHO: return 32.0 * n;     // w read + write 16, v_i 8, v_i+1 8
-    case LSX_K_SMOOTH:  // n_pre sweeps of the finest level
-      if (opt.smoother == LSX_SMOOTH_LINE) {
-        // x lines: b, 1/a_p, three faces, x_in, x_out; y and z lines also write and re-read the
-        // eliminated coefficients and re-read and rewrite x_out in the back substitution
-        const double vb = mixed() ? 4.0 : 8.0;
-        const double fb = opt.precision == LSX_PREC_MIXED16 ? 6.0 : (mixed() ? 12.0 : 24.0);
-        const double xl = 4.0 * vb + fb, yz = xl + 4.0 * vb;
-        return (xl + 2.0 * yz) * n * opt.n_pre;
-      }
-      // red-black: x, b, 1/a_p, three faces, x written
-      return (opt.precision == LSX_PREC_MIXED16 ? 22.0 : (mixed() ? 28.0 : 56.0)) * n * opt.n_pre;
+    case LSX_K_SMOOTH:  // one red-black sweep of the finest level: x, b, 1/a_p, three faces, x written
+      return (opt.precision == LSX_PREC_MIXED16 ? 22.0 : (mixed() ? 28.0 : 56.0)) * n;
     default: return 0.0;
   }
 }
@@ -539,10 +530,7 @@ void Solver::vcycle_run(const double* v, T* z) {
     VStore& S = vlev[l];
     T* x = l == 0 ? z : vx<T>(S);
     const T* b = l == 0 ? b0 : vb<T>(S);
-    hipEvent_t e0 = nullptr;
-    if (l == 0) tbeg(LSX_K_SMOOTH, &e0);
     smooth<T>(l, x, b, opt.n_pre, true);
-    if (l == 0) tend(LSX_K_SMOOTH, e0);
     vc::resid_restrict<T>(S.L, x, b, vb<T>(vlev[l + 1]), stream);
   }
   if (lc <= L) {
@@ -581,15 +569,27 @@ void Solver::vcycle_run(const double* v, T* z) {
 template <class T>
 void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess) {
   VStore& S = vlev[l];
+  // finest-level launches are timed one by one: compulsory bytes per cell of a full red-black
+  // sweep or of one line pass (x in and out, b, 1/a_p, three faces)
+  const double vbytes = sizeof(T), fbytes = S.L.ftype == vc::FACE_F16 ? 2.0 : (S.L.ftype == vc::FACE_F32 ? 4.0 : 8.0);
+  const double pass_bytes = (4.0 * vbytes + 3.0 * fbytes) * (double)S.L.d.n();
+  hipEvent_t e0 = nullptr;
+  auto t0 = [&](int k) {
+    if (l == 0) tbeg(k, &e0);
+  };
+  auto t1 = [&](int k, double bytes) {
+    if (l == 0) tend(k, e0, bytes);
+  };
   if (opt.smoother == LSX_SMOOTH_RBGS && (trans || !fused_level(l))) {
     for (int s = 0; s < sweeps; ++s) {
       const bool first = zero_guess && s == 0;
-      if (trans) {  // transposed sweeps: black then red, on A^T
-        vc::rbgs_tr<T>(S.L, x, b, 1, first ? 1 : 0, stream);
-        vc::rbgs_tr<T>(S.L, x, b, 0, first ? 2 : 0, stream);
-      } else {
-        vc::rbgs<T>(S.L, x, b, 0, first ? 1 : 0, stream);
-        vc::rbgs<T>(S.L, x, b, 1, first ? 2 : 0, stream);
+      for (int h = 0; h < 2; ++h) {
+        t0(LSX_K_SMOOTH);
+        if (trans)  // transposed sweeps: black then red, on A^T
+          vc::rbgs_tr<T>(S.L, x, b, 1 - h, first ? 1 + h : 0, stream);
+        else
+          vc::rbgs<T>(S.L, x, b, h, first ? 1 + h : 0, stream);
+        t1(LSX_K_SMOOTH, 0.5 * pass_bytes);
       }
     }
     return;
@@ -604,7 +604,9 @@ void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess) {
     }
     for (int p = 1; p <= sweeps; ++p) {
       T* dst = (sweeps - p) % 2 == 0 ? x : xt;
+      t0(LSX_K_SMOOTH);
       vc::rb_sweep<T>(S.L, src, dst, b, zero_guess && p == 1, stream);
+      t1(LSX_K_SMOOTH, pass_bytes);
       src = dst;
     }
     return;
@@ -620,7 +622,9 @@ void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess) {
     T* dst = (P - p) % 2 == 0 ? x : xt;
     // x, y, z lines in turn; the transposed cycle runs them in the opposite order on A^T
     const int dir = trans ? 2 - (p - 1) % 3 : (p - 1) % 3;
+    t0(LSX_K_LINE_X + dir);
     vc::line_pass<T>(S.L, dir, src, dst, cpv, b, zero_guess && p == 1, trans ? 1 : 0, stream);
+    t1(LSX_K_LINE_X + dir, pass_bytes);
     src = dst;
   }
 }

@@ -195,6 +195,28 @@ __device__ inline T offdiag(const T* __restrict__ x, const FT* __restrict__ ae, 
   return acc;
 }
 
+// offdiag with every load issued unconditionally (absent neighbours read the cell itself) and
+// absent terms leaving the sum unchanged: the same value, with the row's loads in flight together
+template <class T, class FT>
+__device__ inline T offdiag_u(const T* __restrict__ x, const FT* __restrict__ ae, const FT* __restrict__ an,
+                              const FT* __restrict__ at, long i, int xx, int yy, int zz, int N, int M, int K) {
+  const long NM = (long)N * M;
+  const bool hw = xx > 0, he = xx < N - 1, hs = yy > 0, hn = yy < M - 1, hb = zz > 0, ht = zz < K - 1;
+  const long iw = hw ? i - 1 : i, ie = he ? i + 1 : i, is = hs ? i - N : i, in = hn ? i + N : i;
+  const long ib = hb ? i - NM : i, it = ht ? i + NM : i;
+  const T fw = ld<T>(ae, iw), fe = ld<T>(ae, i), fs_ = ld<T>(an, is), fn = ld<T>(an, i);
+  const T fb = ld<T>(at, ib), ft = ld<T>(at, i);
+  const T xw = x[iw], xe = x[ie], xs = x[is], xn = x[in], xb = x[ib], xt = x[it];
+  T acc = 0;
+  acc = hw ? acc + fw * xw : acc;
+  acc = he ? acc + fe * xe : acc;
+  acc = hs ? acc + fs_ * xs : acc;
+  acc = hn ? acc + fn * xn : acc;
+  acc = hb ? acc + fb * xb : acc;
+  acc = ht ? acc + ft * xt : acc;
+  return acc;
+}
+
 // one colour of a red-black sweep; work item t (from start, by stride) owns the t-th cell of it.
 // zero_guess: 0 normal; 1 x is zero before this half-sweep; 2 first update of this colour after
 // a zero start (its values are not read).  An empty row (1/a_p stored as 0) keeps its value, so
@@ -259,13 +281,40 @@ __device__ inline void resid_restrict_body(const T* __restrict__ x, const T* __r
   }
 }
 
+// The same restriction over the whole device: one thread per fine cell (64 adjacent x by 2 y by
+// 2 z per workgroup, so loads coalesce and each row's loads are in flight together), residuals
+// through LDS, and the 8 children of each of the 32 coarse cells summed by one thread in the
+// order of resid_restrict_body (bit-identical results).
 template <class T, class FT>
-__global__ void k_resid_restrict(const T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
-                                 const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s, T fs,
-                                 int N, int M, int K, int pin0, int pin1, T* __restrict__ bc) {
-  resid_restrict_body<T, FT>(x, b, ae, an, at, s, fs, N, M, K, pin0, pin1, bc,
-                             blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x);
+__global__ void __launch_bounds__(256) k_resid_restrict_cells(const T* __restrict__ x, const T* __restrict__ b,
+                                                              const FT* __restrict__ ae, const FT* __restrict__ an,
+                                                              const FT* __restrict__ at, const T* __restrict__ s,
+                                                              T fs, int N, int M, int K, int pin0, int pin1,
+                                                              T* __restrict__ bc) {
+  __shared__ T rr[2][2][64];
+  const int tx = threadIdx.x & 63, ty = (threadIdx.x >> 6) & 1, tz = threadIdx.x >> 7;
+  const int Nc = N / 2, Mc = M / 2;
+  const int xx = blockIdx.x * 64 + tx, yy = 2 * (int)blockIdx.y + ty, zz = 2 * (int)blockIdx.z + tz;
+  const bool live = xx < N;
+  const int xc = live ? xx : N - 1;
+  const long i = xc + (long)N * yy + (long)N * M * zz;
+  const T si = s[i];
+  const T ap = si != (T)0 ? (T)1 / si : (T)0;
+  const T r0 = b[i] - ap * x[i];
+  const T od = offdiag_u<T, FT>(x, ae, an, at, i, xc, yy, zz, N, M, K);
+  rr[tz][ty][tx] = (i != pin0 && i != pin1) ? r0 - fs * od : r0;
+  __syncthreads();
+  if (threadIdx.x < 32) {
+    const int X = blockIdx.x * 32 + threadIdx.x;
+    if (X < Nc) {
+      T acc = 0;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc += rr[c >> 2][(c >> 1) & 1][2 * threadIdx.x + (c & 1)];
+      bc[X + (long)Nc * blockIdx.y + (long)Nc * Mc * blockIdx.z] = acc;
+    }
+  }
 }
+
 
 __device__ inline void axis_w(int f, int nf, int* c, float* w, int* cnt) {
   const int p = f >> 1;
@@ -802,12 +851,12 @@ __device__ inline void line_rows_x(const T* __restrict__ x, const T* __restrict_
   }
 }
 
-template <class T, class FT, int CPL, bool ZERO>
+template <class T, class FT, int CPL, bool ZERO, bool TRANS>
 __global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* __restrict__ x_out,
                                                 const T* __restrict__ b, const FT* __restrict__ ae,
                                                 const FT* __restrict__ an, const FT* __restrict__ at,
                                                 const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
-                                                int pin1, int trans) {
+                                                int pin1) {
   const int lane = threadIdx.x & 63;
   const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= (long)M * K) return;  // whole wavefronts leave together
@@ -815,14 +864,14 @@ __global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* _
   T lo[CPL], dg[CPL], up[CPL], rh[CPL];
   if (N % CPL == 0 && lane * CPL < N) {
     line_rows_x<T, FT, CPL, ZERO>(x_in, b, ae, an, at, s, fs, row * N + lane * CPL, lane * CPL, yy, zz, N, M, K,
-                                  pin0, pin1, trans != 0, lo, dg, up, rh);
+                                  pin0, pin1, TRANS, lo, dg, up, rh);
   } else {
 #pragma unroll
     for (int j = 0; j < CPL; ++j) {
       const int k = lane * CPL + j;
       if (k < N) {
         line_row<T, FT, 0, ZERO>(x_in, b, ae, an, at, s, fs, row * N + k, k, yy, zz, N, M, K, pin0, pin1,
-                                 trans != 0, lo[j], dg[j], up[j], rh[j]);
+                                 TRANS, lo[j], dg[j], up[j], rh[j]);
       } else {  // padding: decoupled identity rows
         lo[j] = up[j] = rh[j] = (T)0;
         dg[j] = (T)1;
@@ -913,12 +962,58 @@ __global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* _
 constexpr int SEG = 8;
 constexpr int LPW = 32;
 
+// element i of p addressed by a 32-bit byte offset (i * sizeof(E) < 2^32), so that loads take a
+// uniform base and one 32-bit offset register
+template <class E>
+__device__ inline E ldo(const E* __restrict__ p, unsigned i) {
+  return *reinterpret_cast<const E*>(reinterpret_cast<const char*>(p) + i * (unsigned)sizeof(E));
+}
+
+// Row of cell i of a y (DIR 1) or z (DIR 2) line of the forward operator, with the line's
+// per-thread constants: the x neighbours (A) and the other cross direction (B) exist or not for
+// the whole line (offsets of absent ones are 0: the cell itself, masked).  Same values and
+// operand order as line_row.
 template <class T, class FT, int DIR, bool ZERO>
+struct SegLine {
+  const T* x;
+  const T* b;
+  const T* s;
+  const FT* ae;
+  const FT* fB;  // faces of the B direction (z faces for y lines, y faces for z lines)
+  const FT* fL;  // faces along the line
+  T fs;
+  int strideL, oAl, oAu, oBl, oBu, len, pin0, pin1;
+  bool hAl, hAu, hBl, hBu;
+  __device__ inline void row(unsigned i, int kk, bool first, T& fl_carry, T& lo, T& dg, T& up, T& rh) const {
+    const bool hl = kk > 0, hu = kk < len - 1;
+    const T sv = ldo(s, i), bv = ldo(b, i);
+    const T fal = (T)ldo(ae, i + oAl), fau = (T)ldo(ae, i);
+    const T fbl = (T)ldo(fB, i + oBl), fbu = (T)ldo(fB, i);
+    const T flu = (T)ldo(fL, i);
+    const T fll = first ? (T)ldo(fL, hl ? i - strideL : i) : fl_carry;  // upper face of the row before
+    fl_carry = flu;
+    T off = (T)0, xi = (T)0;
+    if constexpr (!ZERO) {
+      xi = ldo(x, i);
+      const T xal = ldo(x, i + oAl), xau = ldo(x, i + oAu), xbl = ldo(x, i + oBl), xbu = ldo(x, i + oBu);
+      const T cAl = hAl ? fs * fal : (T)0, cAu = hAu ? fs * fau : (T)0;
+      const T cBl = hBl ? fs * fbl : (T)0, cBu = hBu ? fs * fbu : (T)0;
+      off = cAl * xal + cAu * xau + cBl * xbl + cBu * xbu;
+    }
+    const bool own = (int)i == pin0 || (int)i == pin1, empty = sv == (T)0, cut = own || empty;
+    lo = cut || !hl ? (T)0 : fs * fll;
+    up = cut || !hu ? (T)0 : fs * flu;
+    dg = cut ? (T)1 : recip0(sv);
+    rh = own ? bv : (empty ? xi : bv - off);
+  }
+};
+
+template <class T, class FT, int DIR, bool ZERO, bool TRANS>
 __global__ void __launch_bounds__(1024) k_line_seg(const T* __restrict__ x_in, T* __restrict__ x_out,
                                                    const T* __restrict__ b, const FT* __restrict__ ae,
                                                    const FT* __restrict__ an, const FT* __restrict__ at,
                                                    const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
-                                                   int pin1, int trans, int S) {
+                                                   int pin1, int S) {
   extern __shared__ unsigned char lds_raw[];
   T* ifc = reinterpret_cast<T*>(lds_raw);  // [8][S][LPW]: a d c r of the reduced rows, G H Q of first cells, x_E
   auto IF = [&](int q, int j, int l) -> T& { return ifc[((long)q * S + j) * LPW + l]; };
@@ -939,31 +1034,64 @@ __global__ void __launch_bounds__(1024) k_line_seg(const T* __restrict__ x_in, T
   T Gb = (T)0, Hb = (T)1, Qb = (T)0;              // the cell before the last, in (x_L, x_E)
   T Gf = (T)0, Hf = (T)0, Qf = (T)1;              // the first cell, in (x_L, x_E)
   {
-    T lo[SEG], dg[SEG], up[SEG], rh[SEG];
-#pragma unroll
-    for (int k = 0; k < SEG; ++k) {
-      const int kk = k0 + (k < cnt ? k : cnt - 1);
-      const int yy = DIR == 1 ? kk : other, zz = DIR == 1 ? other : kk;
-      line_row<T, FT, DIR, ZERO>(x_in, b, ae, an, at, s, fs, base + stride * kk, xx, yy, zz, N, M, K, pin0, pin1,
-                                 trans != 0, lo[k], dg[k], up[k], rh[k]);
-    }
     T cp = (T)0, gp = (T)0, hp = (T)1;  // x_{-1} = x_L
-#pragma unroll
-    for (int k = 0; k < SEG; ++k) {
-      // x_k = g_k + h_k x_L - c_k x_{k+1} for the rows before the last
-      const bool elim = k < cnt - 1, last = k == cnt - 1;
-      const T winv = recip0(dg[k] - lo[k] * cp);
-      const T gn = (rh[k] - lo[k] * gp) * winv, hn = -lo[k] * hp * winv, cn = up[k] * winv;
-      gp = elim ? gn : gp;
-      hp = elim ? hn : hp;
-      cp = elim ? cn : cp;
+    // x_k = g_k + h_k x_L - c_k x_{k+1} for the rows before the last, each row eliminated as soon
+    // as it is built
+    auto elim = [&](int k, T lo, T dg, T up, T rh) {
+      const bool el = k < cnt - 1, last = k == cnt - 1;
+      const T winv = recip0(dg - lo * cp);
+      const T gn = (rh - lo * gp) * winv, hn = -lo * hp * winv, cn = up * winv;
+      gp = el ? gn : gp;
+      hp = el ? hn : hp;
+      cp = el ? cn : cp;
       G[k] = gp;
       H[k] = hp;
       Q[k] = cp;
-      le = last ? lo[k] : le;
-      de = last ? dg[k] : de;
-      ue = last ? up[k] : ue;
-      re = last ? rh[k] : re;
+      le = last ? lo : le;
+      de = last ? dg : de;
+      ue = last ? up : ue;
+      re = last ? rh : re;
+    };
+    if constexpr (!TRANS) {
+      SegLine<T, FT, DIR, ZERO> Ln;
+      const int nB = DIR == 1 ? K : M, dB = DIR == 1 ? N * M : N;
+      Ln.x = x_in;
+      Ln.b = b;
+      Ln.s = s;
+      Ln.ae = ae;
+      Ln.fB = DIR == 1 ? at : an;
+      Ln.fL = DIR == 1 ? an : at;
+      Ln.fs = fs;
+      Ln.strideL = (int)stride;
+      Ln.len = len;
+      Ln.pin0 = pin0;
+      Ln.pin1 = pin1;
+      Ln.hAl = xx > 0;
+      Ln.hAu = xx < N - 1;
+      Ln.hBl = other > 0;
+      Ln.hBu = other < nB - 1;
+      Ln.oAl = Ln.hAl ? -1 : 0;
+      Ln.oAu = Ln.hAu ? 1 : 0;
+      Ln.oBl = Ln.hBl ? -dB : 0;
+      Ln.oBu = Ln.hBu ? dB : 0;
+      T carry = (T)0;
+#pragma unroll
+      for (int k = 0; k < SEG; ++k) {
+        const int kk = k0 + (k < cnt ? k : cnt - 1);
+        T lo, dg, up, rh;
+        Ln.row((unsigned)(base + stride * kk), kk, k == 0, carry, lo, dg, up, rh);
+        elim(k, lo, dg, up, rh);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < SEG; ++k) {
+        const int kk = k0 + (k < cnt ? k : cnt - 1);
+        const int yy = DIR == 1 ? kk : other, zz = DIR == 1 ? other : kk;
+        T lo, dg, up, rh;
+        line_row<T, FT, DIR, ZERO>(x_in, b, ae, an, at, s, fs, base + stride * kk, xx, yy, zz, N, M, K, pin0, pin1,
+                                   true, lo, dg, up, rh);
+        elim(k, lo, dg, up, rh);
+      }
     }
   }
   {
@@ -1113,10 +1241,10 @@ void rbgs<float>(const Level& L, float* x, const float* b, int color, int zero_g
 
 template <class T, class FT>
 static void resid_restrict_t(const Level& L, const T* x, const T* b, T* bc, hipStream_t s) {
-  const long nc = (long)(L.d.N / 2) * (L.d.M / 2) * (L.d.K / 2);
-  k_resid_restrict<T, FT><<<grid_1d(nc), 256, 0, s>>>(x, b, (const FT*)L.f_e, (const FT*)L.f_n, (const FT*)L.f_t,
-                                                      (const T*)L.s, (T)L.fscale, L.d.N, L.d.M, L.d.K, L.pin0,
-                                                      L.pin1, bc);
+  const dim3 grid((L.d.N + 63) / 64, L.d.M / 2, L.d.K / 2);
+  k_resid_restrict_cells<T, FT><<<grid, 256, 0, s>>>(x, b, (const FT*)L.f_e, (const FT*)L.f_n, (const FT*)L.f_t,
+                                                     (const T*)L.s, (T)L.fscale, L.d.N, L.d.M, L.d.K, L.pin0,
+                                                     L.pin1, bc);
 }
 
 template <>
@@ -1193,8 +1321,14 @@ static void line_pass_z(const Level& L, int dir, const T* x_in, T* x_out, const 
   if (dir == 0) {
     const int blocks = (int)(((long)d.M * d.K + 3) / 4);
 #define LSX_LINE_X(CPL)                                                                                        \
-  k_line_x<T, FT, CPL, ZERO><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0, \
-                                                    L.pin1, trans)
+  do {                                                                                                         \
+    if (trans)                                                                                                 \
+      k_line_x<T, FT, CPL, ZERO, true><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, \
+                                                              L.pin0, L.pin1);                                 \
+    else                                                                                                       \
+      k_line_x<T, FT, CPL, ZERO, false><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, \
+                                                               L.pin0, L.pin1);                                \
+  } while (0)
     if (d.N <= 64)
       LSX_LINE_X(1);
     else if (d.N <= 128)
@@ -1209,12 +1343,21 @@ static void line_pass_z(const Level& L, int dir, const T* x_in, T* x_out, const 
     const int S = (len + SEG - 1) / SEG;
     const dim3 grid((d.N + LPW - 1) / LPW, dir == 1 ? d.K : d.M);
     const size_t lds = 8 * (size_t)S * LPW * sizeof(T);
-    if (dir == 1)
-      k_line_seg<T, FT, 1, ZERO><<<grid, LPW * S, lds, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0,
-                                                            L.pin1, trans, S);
-    else
-      k_line_seg<T, FT, 2, ZERO><<<grid, LPW * S, lds, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, L.pin0,
-                                                            L.pin1, trans, S);
+#define LSX_LINE_SEG(DIR, TR)                                                                                  \
+  k_line_seg<T, FT, DIR, ZERO, TR><<<grid, LPW * S, lds, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, \
+                                                              L.pin0, L.pin1, S)
+    if (dir == 1) {
+      if (trans)
+        LSX_LINE_SEG(1, true);
+      else
+        LSX_LINE_SEG(1, false);
+    } else {
+      if (trans)
+        LSX_LINE_SEG(2, true);
+      else
+        LSX_LINE_SEG(2, false);
+    }
+#undef LSX_LINE_SEG
   }
 }
 
