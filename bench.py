@@ -3,8 +3,9 @@
 
 A step is one pressure Poisson solve of the cut-cell operator to relative residual 1e-8
 (absolute eps = 1e-8 ||b||, x0 = 0), FGMRES(30) (or BiCGStab) preconditioned by a geometric
-multigrid V-cycle: mixed precision (FP32 red-black Gauss-Seidel, FP16 coarse faces) by
-default, compared with the same V-cycle in FP64 (the plain FP64 GPU solve) and with the
+multigrid V-cycle: mixed precision by default (FP32 values, FP16 coarse faces; x/y/z line
+relaxation on the two finest levels and red-black Gauss-Seidel below, or red-black on every
+level with --smoother rbgs), compared with the same V-cycle in FP64 (the plain FP64 GPU solve) and with the
 reference's FP64 multigrid cascade, which is reported at an iteration cap because it does not
 reach 1e-8 on this stretched grid.  Workload: 256^3 cells on the unit cube, per-axis geometric
 stretching ratio 1.02 towards the body centre, a six-patch bicubic Bezier body fitted to the
@@ -131,10 +132,10 @@ def cpu_baseline(sample_n, gpu_n, restart, krylov, body="bezier"):
 
 
 def run_solver(lsx, hip, geom, precision, krylov, restart, steps, warmup, max_it=1000, smooth=(2, 2), graphs=False,
-               smoother="rbgs"):
+               smoother="rbgs", line_levels=0):
     """Set up one solver on the device geometry and time `steps` solves (x0 = 0 each)."""
-    S = lsx.solver(krylov, precision, graphs=graphs, smoother=smoother) if precision != "fp64" else \
-        lsx.solver(krylov, precision)
+    S = lsx.solver(krylov, precision, graphs=graphs, smoother=smoother, line_levels=line_levels) \
+        if precision != "fp64" else lsx.solver(krylov, precision)
     task = lsx.Task(eps=1.0, MAX_ITER=max_it, FGMRES_RESTART_NUM=restart) if precision == "fp64" else \
         lsx.mixed_task(1.0, restart=restart, max_it=max_it, n_pre=smooth[0], n_post=smooth[1])
     t_setup = time.perf_counter()
@@ -233,8 +234,11 @@ def main():
     ap.add_argument("--cpu-sample-grid", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graphs", action="store_true", help="replay the V-cycles from captured HIP graphs")
-    ap.add_argument("--smoother", choices=["rbgs", "line"], default="rbgs",
+    ap.add_argument("--smoother", choices=["rbgs", "line"], default="line",
                     help="V-cycle smoother: red-black Gauss-Seidel or x/y/z line relaxation")
+    ap.add_argument("--line-levels", type=int, default=2,
+                    help="with --smoother line: line relaxation on the finest this many levels, red-black "
+                         "below (0 = every level)")
     ap.add_argument("--projection-steps", type=int, default=0,
                     help="instead of the solve benchmark: this many time steps of the 384x256x256 sphere "
                          "projection run (one JSON line of per-step results)")
@@ -263,13 +267,13 @@ def main():
 
     smooth = tuple(int(q) for q in args.smooth.split(","))
     main_run = run_solver(lsx, hip, geom, args.precision, args.krylov, args.restart, args.steps, args.warmup,
-                          smooth=smooth, graphs=args.graphs, smoother=args.smoother)
+                          smooth=smooth, graphs=args.graphs, smoother=args.smoother, line_levels=args.line_levels)
     print(f"[bench] {args.precision}: {args.steps} solves in {main_run['elapsed']:.3f} s, its {main_run['its']}",
           file=sys.stderr, flush=True)
     fp64 = None
     if args.precision in ("mixed", "mixed16") and not args.no_fp64_compare:
         fp64 = run_solver(lsx, hip, geom, "fp64-vcycle", args.krylov, args.restart, max(1, args.steps // 2), 1,
-                          smooth=smooth, graphs=args.graphs, smoother=args.smoother)
+                          smooth=smooth, graphs=args.graphs, smoother=args.smoother, line_levels=args.line_levels)
         print(f"[bench] fp64-vcycle: {len(fp64['its'])} solves in {fp64['elapsed']:.3f} s, its {fp64['its']}",
               file=sys.stderr, flush=True)
     cascade = None
@@ -295,7 +299,9 @@ def main():
         except (ValueError, OSError):
             traffic = None
     value = args.steps / elapsed
-    sm_name = "red-black GS" if args.smoother == "rbgs" else "x/y/z line-relaxation"
+    sm_name = "red-black GS" if args.smoother == "rbgs" else (
+        "x/y/z line-relaxation" if args.line_levels <= 0 else
+        f"x/y/z line-relaxation on the finest {args.line_levels} levels, red-black GS below")
     cfg = {
         "workload": (f"{n}^3 stretched grid (ratio 1.02 to centre), "
                      + ("six-patch bicubic Bezier body fitted to the ellipsoid 0.25/0.15/0.15, control points "
@@ -307,6 +313,7 @@ def main():
                         "fp64-vcycle": f"FP64 V{smooth} {sm_name} multigrid",
                         "fp64": "FP64 reference multigrid cascade (ADLJ)"}[args.precision]),
         "precision": args.precision, "krylov": args.krylov, "smoother": args.smoother, "graphs": args.graphs,
+        "line_levels": args.line_levels if args.smoother == "line" else None,
         "cells": geom.n,
         "iterations": main_run["its"], "true_rel_residual": main_run["rel_res"],
         "geometry_setup_s": geo_s, "solver_setup_s": main_run["setup_s"], "mg_levels": main_run["levels"],

@@ -65,6 +65,8 @@ typedef struct lsx_options {
   int smoother;      /* V-cycle: LSX_SMOOTH_RBGS (default) or LSX_SMOOTH_LINE                  */
   int fused_sweep;   /* V-cycle red-black: 1 = both colours in one pass on a finest level of
                         >= 2^23 cells (default), 2 = on every level, 0 = never (half-sweeps)    */
+  int line_levels;   /* V-cycle with LSX_SMOOTH_LINE: the finest line_levels levels are smoothed
+                        with lines, coarser ones with red-black sweeps; 0 = every level (default) */
 } lsx_options;
 
 #define LSX_SMOOTH_RBGS 0 /* red-black Gauss-Seidel                                             */

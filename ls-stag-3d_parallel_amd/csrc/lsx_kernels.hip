@@ -509,7 +509,8 @@ __global__ void k_stencil_sym_t(const double* __restrict__ s4, int N, int M, int
                                 double* __restrict__ partial) {
   const long n = (long)N * M * K;
   long lo, hi;
-  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
+  const int cb = blockIdx.x;
+  chunk_of(n, cb, gridDim.x, &lo, &hi);
   double ss = 0.0;
   for (long li = lo + threadIdx.x; li < hi; li += BS) {
     const int i = (int)li;
@@ -528,7 +529,7 @@ __global__ void k_stencil_sym_t(const double* __restrict__ s4, int N, int M, int
   }
   if (MODE == ST_RESID) {
     ss = block_sum<BS>(ss);
-    if (threadIdx.x == 0) partial[blockIdx.x] = ss;
+    if (threadIdx.x == 0) partial[cb] = ss;
   }
 }
 
@@ -538,7 +539,8 @@ __global__ void k_stencil_sym(const double* __restrict__ s4, int N, int M, int K
                               double* __restrict__ partial) {
   const long n = (long)N * M * K;
   long lo, hi;
-  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
+  const int cb = blockIdx.x;
+  chunk_of(n, cb, gridDim.x, &lo, &hi);
   double ss = 0.0;
   for (long li = lo + threadIdx.x; li < hi; li += BS) {
     const int i = (int)li;
@@ -557,7 +559,7 @@ __global__ void k_stencil_sym(const double* __restrict__ s4, int N, int M, int K
   }
   if (MODE == ST_RESID) {
     ss = block_sum<BS>(ss);
-    if (threadIdx.x == 0) partial[blockIdx.x] = ss;
+    if (threadIdx.x == 0) partial[cb] = ss;
   }
 }
 
@@ -565,7 +567,9 @@ __global__ void k_stencil_sym(const double* __restrict__ s4, int N, int M, int K
 __global__ void k_stencil_sym_t_f32x(const double* __restrict__ s4, int N, int M, int K, SymExc ex,
                                      const float* __restrict__ x, double* __restrict__ y) {
   const long n = (long)N * M * K;
-  for (long li = blockIdx.x * (long)blockDim.x + threadIdx.x; li < n; li += (long)gridDim.x * blockDim.x) {
+  long lo, hi;
+  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
+  for (long li = lo + threadIdx.x; li < hi; li += blockDim.x) {
     const int i = (int)li;
     int xx, yy, zz;
     decode(i, N, M, &xx, &yy, &zz);
@@ -577,7 +581,9 @@ __global__ void k_stencil_sym_t_f32x(const double* __restrict__ s4, int N, int M
 __global__ void k_stencil_sym_f32x(const double* __restrict__ s4, int N, int M, int K, SymExc ex,
                                    const float* __restrict__ x, double* __restrict__ y) {
   const long n = (long)N * M * K;
-  for (long li = blockIdx.x * (long)blockDim.x + threadIdx.x; li < n; li += (long)gridDim.x * blockDim.x) {
+  long lo, hi;
+  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
+  for (long li = lo + threadIdx.x; li < hi; li += blockDim.x) {
     const int i = (int)li;
     int xx, yy, zz;
     decode(i, N, M, &xx, &yy, &zz);
@@ -995,6 +1001,14 @@ int grid_1d(long n, int bs = 256) {
 // ================================================================================================
 // launchers
 // ================================================================================================
+// blocks of contiguous 4096-cell chunks (at most 4096 blocks) for the XCD-ordered stencil products
+static int chunk_blocks(long n) {
+  long b = (n + 4095) / 4096;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
 int reduction_blocks(long n) {
   long b = (n + RED_THREADS * 4 - 1) / (RED_THREADS * 4);
   if (b > 1024) b = 1024;
@@ -1124,9 +1138,16 @@ void sol_update(double* x, const double* Z, long ldz, const double* yk, int coun
   }
 }
 
+int cgs_blocks(long n) {
+  long b = (n + 1023) / 1024;
+  if (b > CGS_BLOCKS_MAX) b = CGS_BLOCKS_MAX;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
 void cgs_dot(const double* V, long ldv, int cnt, const double* w, long n, double* partial, double* out,
              hipStream_t s) {
-  const int nb = reduction_blocks(n);
+  const int nb = cgs_blocks(n);
   const dim3 grid(nb, (cnt + CGS_GROUP - 1) / CGS_GROUP);
   k_cgs_dot<RED_THREADS><<<grid, RED_THREADS, 0, s>>>(V, ldv, cnt, w, n, partial);
   k_finalize_multi<RED_THREADS><<<cnt, RED_THREADS, 0, s>>>(partial, nb, out, 0);
@@ -1134,7 +1155,7 @@ void cgs_dot(const double* V, long ldv, int cnt, const double* w, long n, double
 
 void cgs_update(const double* V, long ldv, int cnt, const double* h, double* w, long n, double* partial,
                 double* norm_out, hipStream_t s) {
-  const int nb = reduction_blocks(n);
+  const int nb = cgs_blocks(n);
   k_cgs_update<RED_THREADS><<<nb, RED_THREADS, 0, s>>>(V, ldv, cnt, h, w, n, norm_out != nullptr, partial);
   if (norm_out) k_finalize_multi<RED_THREADS><<<1, RED_THREADS, 0, s>>>(partial, nb, norm_out, 1);
 }
@@ -1151,11 +1172,11 @@ void stencil_apply_sym_t(const double* s4, const SymExc& ex, Dims d, const doubl
 }
 
 void stencil_prod_sym_t_f32x(const double* s4, const SymExc& ex, Dims d, const float* x, double* y, hipStream_t s) {
-  k_stencil_sym_t_f32x<<<grid_1d(d.n()), 256, 0, s>>>(s4, d.N, d.M, d.K, ex, x, y);
+  k_stencil_sym_t_f32x<<<chunk_blocks(d.n()), 256, 0, s>>>(s4, d.N, d.M, d.K, ex, x, y);
 }
 
 void stencil_prod_sym_f32x(const double* s4, const SymExc& ex, Dims d, const float* x, double* y, hipStream_t s) {
-  k_stencil_sym_f32x<<<grid_1d(d.n()), 256, 0, s>>>(s4, d.N, d.M, d.K, ex, x, y);
+  k_stencil_sym_f32x<<<chunk_blocks(d.n()), 256, 0, s>>>(s4, d.N, d.M, d.K, ex, x, y);
 }
 
 void stencil7_prod_f32x(const double* coef, Dims d, const float* x, double* y, hipStream_t s) {

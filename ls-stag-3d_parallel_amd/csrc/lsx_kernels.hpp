@@ -82,7 +82,10 @@ void axpy_dot(double* w, const double* v, const double* h, const double* vnext, 
 void sol_update(double* x, const double* Z, long ldz, const double* yk, int count, long n, hipStream_t s);
 // classical Gram-Schmidt passes over cnt <= CGS_MAX basis vectors (vector k at V + k*ldv):
 // cgs_dot: out[k] = v_k . w (k < cnt);  cgs_update: w -= sum_k h_k v_k, and norm_out[0] = ||w||
-// when norm_out is given.  partial needs CGS_MAX * reduction_blocks(n) doubles.
+// when norm_out is given.  Row chunks of cgs_blocks(n) blocks (at most CGS_BLOCKS_MAX); partial
+// needs CGS_MAX * cgs_blocks(n) doubles.
+constexpr int CGS_BLOCKS_MAX = 4096;
+int cgs_blocks(long n);
 void cgs_dot(const double* V, long ldv, int cnt, const double* w, long n, double* partial, double* out,
              hipStream_t s);
 void cgs_update(const double* V, long ldv, int cnt, const double* h, double* w, long n, double* partial,

@@ -44,7 +44,7 @@ Solver::Solver(const lsx_options& o, hipStream_t s) : opt(o) {
   err.alloc(1);
   scalars.alloc(SC_H + 256 * 257 / 2 + 8);
   partial.alloc(2048);
-  mpartial.alloc((size_t)CGS_MAX * 1024);
+  mpartial.alloc((size_t)CGS_MAX * CGS_BLOCKS_MAX);
   cgs_h.alloc(2 * CGS_MAX);
 }
 
@@ -315,8 +315,10 @@ void Solver::build_hierarchy() {
 // -----------------------------------------------------------------------------------------------
 void Solver::build_vcycle() {
   const int L = (int)levels.size() - 1;
-  if (opt.smoother == LSX_SMOOTH_LINE && (d0.M > 256 || d0.K > 256))
-    throw Error(LSX_ERR_ARG, "line relaxation handles y and z lines of at most 256 cells");
+  if (opt.smoother == LSX_SMOOTH_LINE && (d0.N > 512 || d0.M > 256 || d0.K > 256))
+    throw Error(LSX_ERR_ARG, "line relaxation handles x lines of at most 512 cells and y and z lines of at most 256");
+  if (opt.smoother == LSX_SMOOTH_LINE && d0.n() * (mixed() ? 4L : 8L) >= (1L << 32))
+    throw Error(LSX_ERR_ARG, "line relaxation addresses vectors with 32-bit byte offsets: grid too large");
   vlev.clear();
   vlev.resize(L + 1);
   std::vector<DevBuf<double>> fe(2), fn(2), ft(2), dg(2);
@@ -377,7 +379,7 @@ void Solver::build_vcycle() {
       LSX_HIP(hipStreamSynchronize(stream));
       break;
     }
-    if (opt.smoother == LSX_SMOOTH_LINE) {
+    if (line_level(l)) {
       if (m32) {
         S.fxt.alloc(nl);
         if (l == 0) fcp.alloc(nl);
@@ -519,8 +521,8 @@ void Solver::vcycle_run(const double* v, T* z) {
   }
   // levels from `lc` down are small enough for one workgroup: a single launch does their cycle
   int lc = L + 1;
-  for (int l = 1; l <= L && opt.coarse_fuse && opt.smoother == LSX_SMOOTH_RBGS; ++l)
-    if (vlev[l].L.d.n() <= vc::COARSE_CELLS && L - l + 1 <= vc::COARSE_MAX) {
+  for (int l = 1; l <= L && opt.coarse_fuse; ++l)
+    if (!line_level(l) && vlev[l].L.d.n() <= vc::COARSE_CELLS && L - l + 1 <= vc::COARSE_MAX) {
       lc = l;
       break;
     }
@@ -580,7 +582,7 @@ void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess) {
   auto t1 = [&](int k, double bytes) {
     if (l == 0) tend(k, e0, bytes);
   };
-  if (opt.smoother == LSX_SMOOTH_RBGS && (trans || !fused_level(l))) {
+  if (!line_level(l) && (trans || !fused_level(l))) {
     for (int s = 0; s < sweeps; ++s) {
       const bool first = zero_guess && s == 0;
       for (int h = 0; h < 2; ++h) {
@@ -596,7 +598,7 @@ void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess) {
   }
   T* xt = vxt<T>(S);
   const long n = S.L.d.n();
-  if (opt.smoother == LSX_SMOOTH_RBGS) {  // one pass per fused red-black sweep
+  if (!line_level(l)) {  // one pass per fused red-black sweep
     T* src = x;
     if (!zero_guess && (sweeps - 1) % 2 == 0) {  // the first pass must not write the buffer it reads
       vc::copy_vec<T>(x, xt, n, stream);

@@ -184,6 +184,10 @@ class Solver {
   bool fused_level(int l) const {
     return opt.fused_sweep == 2 || (opt.fused_sweep == 1 && l == 0 && levels[l].d.n() >= (1L << 23));
   }
+  // levels smoothed by line relaxation: every level, or the finest opt.line_levels of them
+  bool line_level(int l) const {
+    return opt.smoother == LSX_SMOOTH_LINE && (opt.line_levels <= 0 || l < opt.line_levels);
+  }
   std::vector<VStore> vlev;
   DevBuf<float> Z32, ztmp32;
   bool mixed() const { return opt.precision == LSX_PREC_MIXED || opt.precision == LSX_PREC_MIXED16; }

@@ -294,7 +294,11 @@ __global__ void __launch_bounds__(256) k_resid_restrict_cells(const T* __restric
   __shared__ T rr[2][2][64];
   const int tx = threadIdx.x & 63, ty = (threadIdx.x >> 6) & 1, tz = threadIdx.x >> 7;
   const int Nc = N / 2, Mc = M / 2;
-  const int xx = blockIdx.x * 64 + tx, yy = 2 * (int)blockIdx.y + ty, zz = 2 * (int)blockIdx.z + tz;
+  // block (bx, by, bz), bx fastest
+  const int gx = (N + 63) / 64;
+  const int lb = blockIdx.x;
+  const int bx = lb % gx, by = (lb / gx) % Mc, bz = lb / (gx * Mc);
+  const int xx = bx * 64 + tx, yy = 2 * by + ty, zz = 2 * bz + tz;
   const bool live = xx < N;
   const int xc = live ? xx : N - 1;
   const long i = xc + (long)N * yy + (long)N * M * zz;
@@ -305,12 +309,12 @@ __global__ void __launch_bounds__(256) k_resid_restrict_cells(const T* __restric
   rr[tz][ty][tx] = (i != pin0 && i != pin1) ? r0 - fs * od : r0;
   __syncthreads();
   if (threadIdx.x < 32) {
-    const int X = blockIdx.x * 32 + threadIdx.x;
+    const int X = bx * 32 + threadIdx.x;
     if (X < Nc) {
       T acc = 0;
 #pragma unroll
       for (int c = 0; c < 8; ++c) acc += rr[c >> 2][(c >> 1) & 1][2 * threadIdx.x + (c & 1)];
-      bc[X + (long)Nc * blockIdx.y + (long)Nc * Mc * blockIdx.z] = acc;
+      bc[X + (long)Nc * by + (long)Nc * Mc * bz] = acc;
     }
   }
 }
@@ -1241,8 +1245,8 @@ void rbgs<float>(const Level& L, float* x, const float* b, int color, int zero_g
 
 template <class T, class FT>
 static void resid_restrict_t(const Level& L, const T* x, const T* b, T* bc, hipStream_t s) {
-  const dim3 grid((L.d.N + 63) / 64, L.d.M / 2, L.d.K / 2);
-  k_resid_restrict_cells<T, FT><<<grid, 256, 0, s>>>(x, b, (const FT*)L.f_e, (const FT*)L.f_n, (const FT*)L.f_t,
+  const long blocks = (long)((L.d.N + 63) / 64) * (L.d.M / 2) * (L.d.K / 2);
+  k_resid_restrict_cells<T, FT><<<(unsigned)blocks, 256, 0, s>>>(x, b, (const FT*)L.f_e, (const FT*)L.f_n, (const FT*)L.f_t,
                                                      (const T*)L.s, (T)L.fscale, L.d.N, L.d.M, L.d.K, L.pin0,
                                                      L.pin1, bc);
 }

@@ -68,6 +68,7 @@ class Options(C.Structure):
         ("mg", C.c_int), ("n_post", C.c_int), ("n_pre", C.c_int), ("omega", C.c_double),
         ("precision", C.c_int), ("levels", C.c_int), ("z_head_fix", C.c_int), ("ortho", C.c_int),
         ("coarse_fuse", C.c_int), ("graphs", C.c_int), ("smoother", C.c_int), ("fused_sweep", C.c_int),
+        ("line_levels", C.c_int),
     ]
 
 
@@ -341,7 +342,7 @@ class IterativeSolver:
     krylov = KRYLOV_FGMRES
 
     def __init__(self, precision=PREC_FP64, mg=MG_CASCADE, z_head_fix=False, stream=None, ortho=0, coarse_fuse=True,
-                 graphs=False, smoother="rbgs", fused_sweep=True):
+                 graphs=False, smoother="rbgs", fused_sweep=True, line_levels=0):
         self.ortho = ortho
         self.coarse_fuse = coarse_fuse
         self.graphs = graphs
@@ -349,6 +350,7 @@ class IterativeSolver:
             raise ValueError(f"unknown smoother {smoother!r}")
         self.smoother = smoother
         self.fused_sweep = fused_sweep
+        self.line_levels = line_levels
         self.precision = precision
         self.mg = mg
         self.z_head_fix = z_head_fix
@@ -382,6 +384,7 @@ class IterativeSolver:
         opt.levels = int(getattr(self, "levels_cap", 0))
         opt.smoother = 1 if self.smoother == "line" else 0
         opt.fused_sweep = int(self.fused_sweep)
+        opt.line_levels = int(self.line_levels)
         h = C.c_void_p()
         _check(lib().lsx_create(C.byref(h), C.byref(opt), self.stream))
         self.h = h
@@ -416,6 +419,7 @@ class IterativeSolver:
         opt.levels = int(getattr(self, "levels_cap", 0))
         opt.smoother = 1 if self.smoother == "line" else 0
         opt.fused_sweep = int(self.fused_sweep)
+        opt.line_levels = int(self.line_levels)
         h = C.c_void_p()
         _check(lib().lsx_create(C.byref(h), C.byref(opt), self.stream))
         self.h = h

@@ -14,7 +14,9 @@ pytestmark = pytest.mark.gpu
 
 
 def make_solver(lsx, A, kind="fgmres", precision="fp64-vcycle", eps=1.0, max_it=500, smoother="rbgs"):
-    S = lsx.solver(kind, precision, smoother=smoother)
+    # "line1": line relaxation on the finest level, red-black below
+    S = lsx.solver(kind, precision, smoother="line" if smoother.startswith("line") else smoother,
+                   line_levels=1 if smoother == "line1" else 0)
     S.set(A, np.zeros(A.n), np.zeros(A.n), lsx.mixed_task(eps, restart=30, max_it=max_it))
     return S
 
@@ -45,7 +47,7 @@ def test_transposed_operator(lsx, name, mk):
     assert np.linalg.norm(S.apply_operator(y) - ATy) > 1e-6 * np.linalg.norm(ATy)
 
 
-@pytest.mark.parametrize("smoother", ["rbgs", "line"])
+@pytest.mark.parametrize("smoother", ["rbgs", "line", "line1"])
 @pytest.mark.parametrize("precision,tol", [("fp64-vcycle", 1e-12), ("mixed", 2e-5), ("mixed16", 2e-3)])
 @pytest.mark.parametrize("name,mk", CASES, ids=[c[0] for c in CASES])
 def test_transposed_vcycle(lsx, name, mk, precision, tol, smoother):
