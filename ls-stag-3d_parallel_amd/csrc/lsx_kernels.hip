@@ -509,8 +509,7 @@ __global__ void k_stencil_sym_t(const double* __restrict__ s4, int N, int M, int
                                 double* __restrict__ partial) {
   const long n = (long)N * M * K;
   long lo, hi;
-  const int cb = blockIdx.x;
-  chunk_of(n, cb, gridDim.x, &lo, &hi);
+  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
   double ss = 0.0;
   for (long li = lo + threadIdx.x; li < hi; li += BS) {
     const int i = (int)li;
@@ -529,7 +528,7 @@ __global__ void k_stencil_sym_t(const double* __restrict__ s4, int N, int M, int
   }
   if (MODE == ST_RESID) {
     ss = block_sum<BS>(ss);
-    if (threadIdx.x == 0) partial[cb] = ss;
+    if (threadIdx.x == 0) partial[blockIdx.x] = ss;
   }
 }
 
@@ -539,8 +538,7 @@ __global__ void k_stencil_sym(const double* __restrict__ s4, int N, int M, int K
                               double* __restrict__ partial) {
   const long n = (long)N * M * K;
   long lo, hi;
-  const int cb = blockIdx.x;
-  chunk_of(n, cb, gridDim.x, &lo, &hi);
+  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
   double ss = 0.0;
   for (long li = lo + threadIdx.x; li < hi; li += BS) {
     const int i = (int)li;
@@ -559,7 +557,7 @@ __global__ void k_stencil_sym(const double* __restrict__ s4, int N, int M, int K
   }
   if (MODE == ST_RESID) {
     ss = block_sum<BS>(ss);
-    if (threadIdx.x == 0) partial[cb] = ss;
+    if (threadIdx.x == 0) partial[blockIdx.x] = ss;
   }
 }
 
@@ -1001,7 +999,8 @@ int grid_1d(long n, int bs = 256) {
 // ================================================================================================
 // launchers
 // ================================================================================================
-// blocks of contiguous 4096-cell chunks (at most 4096 blocks) for the XCD-ordered stencil products
+// blocks of contiguous 4096-cell chunks (at most 4096 blocks) for the stencil products: each
+// thread walks its rows of the chunk, so a block reads neighbouring rows it has just read
 static int chunk_blocks(long n) {
   long b = (n + 4095) / 4096;
   if (b > 4096) b = 4096;

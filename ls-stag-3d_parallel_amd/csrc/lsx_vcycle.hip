@@ -294,10 +294,9 @@ __global__ void __launch_bounds__(256) k_resid_restrict_cells(const T* __restric
   __shared__ T rr[2][2][64];
   const int tx = threadIdx.x & 63, ty = (threadIdx.x >> 6) & 1, tz = threadIdx.x >> 7;
   const int Nc = N / 2, Mc = M / 2;
-  // block (bx, by, bz), bx fastest
+  // block (bx, by, bz) of a 1-D grid, bx fastest
   const int gx = (N + 63) / 64;
-  const int lb = blockIdx.x;
-  const int bx = lb % gx, by = (lb / gx) % Mc, bz = lb / (gx * Mc);
+  const int bx = (int)(blockIdx.x % gx), by = (int)(blockIdx.x / gx) % Mc, bz = (int)(blockIdx.x / gx) / Mc;
   const int xx = bx * 64 + tx, yy = 2 * by + ty, zz = 2 * bz + tz;
   const bool live = xx < N;
   const int xc = live ? xx : N - 1;
