@@ -739,6 +739,7 @@ __device__ inline void line_row(const T* __restrict__ x, const T* __restrict__ b
                                 const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s, T fs,
                                 long i, int xx, int yy, int zz, int N, int M, int K, int pin0, int pin1, bool trans,
                                 T& lo, T& dg, T& up, T& rhs) {
+  #pragma clang fp contract(fast)  // line solves: no bitwise twin to match
   // Straight-line code: every load is issued unconditionally (absent neighbours read the cell
   // itself and are masked out), so that a caller evaluating many rows can have all their loads
   // in flight at once.
@@ -804,6 +805,7 @@ __device__ inline void line_rows_x(const T* __restrict__ x, const T* __restrict_
                                    const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s,
                                    T fs, long i0, int k0, int yy, int zz, int N, int M, int K, int pin0, int pin1,
                                    bool trans, T (&lo)[CPL], T (&dg)[CPL], T (&up)[CPL], T (&rh)[CPL]) {
+  #pragma clang fp contract(fast)  // line solves: no bitwise twin to match
   const long NM = (long)N * M;
   const bool ys = yy > 0, yn = yy < M - 1, zb = zz > 0, zt = zz < K - 1;
   const long is0 = ys ? i0 - N : i0, in0 = yn ? i0 + N : i0, ib0 = zb ? i0 - NM : i0, it0 = zt ? i0 + NM : i0;
@@ -860,6 +862,7 @@ __global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* _
                                                 const FT* __restrict__ an, const FT* __restrict__ at,
                                                 const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
                                                 int pin1) {
+  #pragma clang fp contract(fast)  // line solves: no bitwise twin to match
   const int lane = threadIdx.x & 63;
   const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= (long)M * K) return;  // whole wavefronts leave together
@@ -988,6 +991,7 @@ struct SegLine {
   int strideL, oAl, oAu, oBl, oBu, len, pin0, pin1;
   bool hAl, hAu, hBl, hBu;
   __device__ inline void row(unsigned i, int kk, bool first, T& fl_carry, T& lo, T& dg, T& up, T& rh) const {
+  #pragma clang fp contract(fast)  // line solves: no bitwise twin to match
     const bool hl = kk > 0, hu = kk < len - 1;
     const T sv = ldo(s, i), bv = ldo(b, i);
     const T fal = (T)ldo(ae, i + oAl), fau = (T)ldo(ae, i);
@@ -1017,6 +1021,7 @@ __global__ void __launch_bounds__(1024) k_line_seg(const T* __restrict__ x_in, T
                                                    const FT* __restrict__ an, const FT* __restrict__ at,
                                                    const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
                                                    int pin1, int S) {
+  #pragma clang fp contract(fast)  // line solves: no bitwise twin to match
   extern __shared__ unsigned char lds_raw[];
   T* ifc = reinterpret_cast<T*>(lds_raw);  // [8][S][LPW]: a d c r of the reduced rows, G H Q of first cells, x_E
   auto IF = [&](int q, int j, int l) -> T& { return ifc[((long)q * S + j) * LPW + l]; };
