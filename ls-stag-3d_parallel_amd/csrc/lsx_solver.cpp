@@ -560,7 +560,7 @@ void Solver::vcycle_run(const double* v, T* z) {
     T* x = l == 0 ? z : vx<T>(S);
     const T* b = l == 0 ? b0 : vb<T>(S);
     vc::prolong_add<T>(vx<T>(vlev[l + 1]), vlev[l + 1].L.d, x, S.L.d, stream);
-    smooth<T>(l, x, b, opt.n_post, false);
+    smooth<T>(l, x, b, opt.n_post, false, true);  // lines z, y, x: the cycle is symmetric in its line order
   }
 }
 
@@ -569,7 +569,7 @@ void Solver::vcycle_run(const double* v, T* z) {
 // each pass from one buffer into the other; the passes alternate between x and the level's
 // second buffer so that the last one writes x.
 template <class T>
-void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess) {
+void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess, bool rev_lines) {
   VStore& S = vlev[l];
   // finest-level launches are timed one by one: compulsory bytes per cell of a full red-black
   // sweep or of one line pass (x in and out, b, 1/a_p, three faces)
@@ -622,8 +622,8 @@ void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess) {
   }
   for (int p = 1; p <= P; ++p) {
     T* dst = (P - p) % 2 == 0 ? x : xt;
-    // x, y, z lines in turn; the transposed cycle runs them in the opposite order on A^T
-    const int dir = trans ? 2 - (p - 1) % 3 : (p - 1) % 3;
+    // x, y, z lines in turn, or z, y, x (rev_lines)
+    const int dir = rev_lines ? 2 - (p - 1) % 3 : (p - 1) % 3;
     t0(LSX_K_LINE_X + dir);
     vc::line_pass<T>(S.L, dir, src, dst, cpv, b, zero_guess && p == 1, trans ? 1 : 0, stream);
     t1(LSX_K_LINE_X + dir, pass_bytes);
@@ -657,7 +657,7 @@ void Solver::vcycle_apply_tr(const double* v, T* z) {
     VStore& S = vlev[l];
     T* x = l == 0 ? z : vx<T>(S);
     const T* b = l == 0 ? b0 : vb<T>(S);
-    smooth<T>(l, x, b, opt.n_post, true);  // transposed post-smoothing
+    smooth<T>(l, x, b, opt.n_post, true, false);  // transposed post-smoothing (lines x, y, z)
     vc::resid_tr<T>(S.L, x, b, vr<T>(S), stream);
     vc::restrict_pt<T>(vr<T>(S), S.L.d, vb<T>(vlev[l + 1]), stream);
   }
@@ -677,7 +677,7 @@ void Solver::vcycle_apply_tr(const double* v, T* z) {
     T* x = l == 0 ? z : vx<T>(S);
     const T* b = l == 0 ? b0 : vb<T>(S);
     vc::prolong_inject_add<T>(vx<T>(vlev[l + 1]), vlev[l + 1].L.d, x, S.L.d, stream);
-    smooth<T>(l, x, b, opt.n_pre, false);  // transposed pre-smoothing
+    smooth<T>(l, x, b, opt.n_pre, false, true);  // transposed pre-smoothing (lines z, y, x)
   }
 }
 

@@ -178,7 +178,7 @@ class Solver {
   template <class T>
   T* vcp();
   template <class T>
-  void smooth(int l, T* x, const T* b, int sweeps, bool zero_guess);
+  void smooth(int l, T* x, const T* b, int sweeps, bool zero_guess, bool rev_lines = false);
   // levels whose red-black sweeps run as one fused pass: the finest level when it has at least
   // 2^23 cells, enough 64 x 16 x 32 tiles to fill the device (opt.fused_sweep 1), or every level (2)
   bool fused_level(int l) const {
