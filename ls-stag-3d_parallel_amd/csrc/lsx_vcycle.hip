@@ -1164,7 +1164,11 @@ __global__ void __launch_bounds__(1024) k_line_seg(const T* __restrict__ x_in, T
   if (live) {
 #pragma unroll
     for (int k = 0; k < SEG; ++k)
-      if (k < cnt) x_out[base + stride * (k0 + k)] = k == cnt - 1 ? xe : G[k] + H[k] * xl + Q[k] * xe;
+      if (k < cnt) {
+        const unsigned i = (unsigned)(base + stride * (k0 + k));
+        *reinterpret_cast<T*>(reinterpret_cast<char*>(x_out) + i * (unsigned)sizeof(T)) =
+            k == cnt - 1 ? xe : G[k] + H[k] * xl + Q[k] * xe;
+      }
   }
 }
 
