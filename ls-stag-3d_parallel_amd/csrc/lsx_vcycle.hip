@@ -36,6 +36,16 @@ __device__ inline T ld(const FT* p, long i) {
   return (T)p[i];
 }
 
+// Workgroup b of a 1-D grid of nb runs on XCD b % 8 (the 8 XCDs take workgroups round-robin,
+// each XCD with its own L2).  xcd_order(b, nb) renumbers them so that XCD x takes a contiguous
+// run of logical blocks: a bijection of [0, nb).  It pays where a block reads halo data that
+// its logical neighbours read too (the fused sweep and the y/z line passes); for the plain
+// stencil products and the x-line pass it measured slower and is not used.
+__device__ inline int xcd_order(int b, int nb) {
+  const int q = nb / 8, r = nb % 8, x = b % 8, k = b / 8;
+  return x * q + (x < r ? x : r) + k;
+}
+
 // ---- setup -----------------------------------------------------------------------------------
 __global__ void k_faces_from7(const double* __restrict__ c7, int N, int M, int K, int pin0, int pin1,
                               double* __restrict__ fe, double* __restrict__ fn, double* __restrict__ ft,
@@ -607,7 +617,12 @@ __global__ void __launch_bounds__(FX* FY) k_rb_fused(const T* __restrict__ x_in,
   __shared__ T rs_[3][RY][RX];
   constexpr int NT = FX * FY;
   const int tid = threadIdx.x;
-  const int X0 = blockIdx.x * FX, Y0 = blockIdx.y * FY, Z0 = blockIdx.z * FZ;
+  // tiles in XCD-contiguous order (x fastest, then y, then z): neighbouring tiles, whose ring
+  // cells a tile reads, run on the same XCD and share its L2 (measured fetch 26 instead of
+  // 50 B/cell)
+  const int gxt = (N + FX - 1) / FX, gyt = (M + FY - 1) / FY;
+  const int lb = xcd_order(blockIdx.x, gridDim.x);
+  const int X0 = (lb % gxt) * FX, Y0 = ((lb / gxt) % gyt) * FY, Z0 = (lb / (gxt * gyt)) * FZ;
   const int Z1 = Z0 + FZ < K ? Z0 + FZ : K;
   const long NM = (long)N * M;
   auto slot = [](int p) { return ((p % 3) + 3) % 3; };
@@ -1027,7 +1042,12 @@ __global__ void __launch_bounds__(1024) k_line_seg(const T* __restrict__ x_in, T
   auto IF = [&](int q, int j, int l) -> T& { return ifc[((long)q * S + j) * LPW + l]; };
   const long NM = (long)N * M;
   const int lane = threadIdx.x % LPW, seg = threadIdx.x / LPW;
-  const int xr = blockIdx.x * LPW + lane, other = blockIdx.y;
+  // (column, plane) pairs in XCD-contiguous order, columns fastest: the halo columns and
+  // neighbouring planes a workgroup reads are read by workgroups of the same XCD at about the
+  // same time (measured fetch 26 instead of 35 B/cell, 24 B/cell compulsory reads; 9 % less time)
+  const int gxb = (N + LPW - 1) / LPW;
+  const int lb = xcd_order(blockIdx.x, gridDim.x);
+  const int xr = (lb % gxb) * LPW + lane, other = lb / gxb;
   const bool live = xr < N;
   const int xx = live ? xr : N - 1;  // lanes past the last line repeat it and store nothing
   const int len = DIR == 1 ? M : K;
@@ -1353,7 +1373,7 @@ static void line_pass_z(const Level& L, int dir, const T* x_in, T* x_out, const 
   } else {
     const int len = dir == 1 ? d.M : d.K;
     const int S = (len + SEG - 1) / SEG;
-    const dim3 grid((d.N + LPW - 1) / LPW, dir == 1 ? d.K : d.M);
+    const dim3 grid(((d.N + LPW - 1) / LPW) * (dir == 1 ? d.K : d.M));
     const size_t lds = 8 * (size_t)S * LPW * sizeof(T);
 #define LSX_LINE_SEG(DIR, TR)                                                                                  \
   k_line_seg<T, FT, DIR, ZERO, TR><<<grid, LPW * S, lds, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, \
@@ -1400,7 +1420,7 @@ void line_pass<float>(const Level& L, int dir, const float* x_in, float* x_out, 
 template <class T, class FT>
 static void rb_sweep_t(const Level& L, const T* x_in, T* x_out, const T* b, int zero, hipStream_t s) {
   const Dims& d = L.d;
-  const dim3 grid((d.N + FX - 1) / FX, (d.M + FY - 1) / FY, (d.K + FZ - 1) / FZ);
+  const dim3 grid(((d.N + FX - 1) / FX) * ((d.M + FY - 1) / FY) * ((d.K + FZ - 1) / FZ));
   k_rb_fused<T, FT><<<grid, FX * FY, 0, s>>>(x_in, x_out, b, (const FT*)L.f_e, (const FT*)L.f_n, (const FT*)L.f_t,
                                               (const T*)L.s, (T)L.fscale, d.N, d.M, d.K, L.pin0, L.pin1, zero);
 }

@@ -138,3 +138,13 @@ def test_fused_sweep_is_bitwise(lsx, name, mk, precision):
           f"max|dx|={np.abs(out[0][1] - out[2][1]).max():.3e}", flush=True)
     assert out[0][2] == out[2][2]
     assert np.array_equal(out[0][1], out[2][1])
+
+
+def test_line_relaxation_limits(lsx):
+    """Line relaxation takes x lines of at most 512 cells and y and z lines of at most 256: a
+    longer line is refused when the hierarchy is built, with either limit."""
+    for dims in ((520, 4, 4), (8, 264, 4)):
+        g, A = P.body_free(*dims)
+        S = lsx.solver("fgmres", "mixed", smoother="line")
+        with pytest.raises(lsx.LsxError):
+            S.set(A, np.zeros(A.n), np.zeros(A.n), lsx.mixed_task(1.0))
