@@ -581,7 +581,18 @@ __global__ void k_stencil_sym_f32x(const double* __restrict__ s4, int N, int M, 
   const long n = (long)N * M * K;
   long lo, hi;
   chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
-  for (long li = lo + threadIdx.x; li < hi; li += blockDim.x) {
+  long li = lo + threadIdx.x;
+  for (; li + blockDim.x < hi; li += 2 * blockDim.x) {  // two rows per step, their loads in flight together
+    const int i0 = (int)li, i1 = (int)(li + blockDim.x);
+    int x0, y0, z0, x1, y1, z1;
+    decode(i0, N, M, &x0, &y0, &z0);
+    decode(i1, N, M, &x1, &y1, &z1);
+    const double r0 = row_apply_sym(s4, n, i0, x0, y0, z0, N, M, K, x, 0.0, false, ex);
+    const double r1 = row_apply_sym(s4, n, i1, x1, y1, z1, N, M, K, x, 0.0, false, ex);
+    y[i0] = r0;
+    y[i1] = r1;
+  }
+  if (li < hi) {
     const int i = (int)li;
     int xx, yy, zz;
     decode(i, N, M, &xx, &yy, &zz);
@@ -783,27 +794,30 @@ __global__ void k_axpy_dot(double* __restrict__ w, const double* __restrict__ v,
 // pass keeps the coefficients in LDS and streams the vectors row by row.
 constexpr int CGS_GROUP = 8;
 
-// acc[k] += sum over the thread's rows i of V_k[i] w[i], rows in increasing order, two rows
-// (i and i + BS) per step so that 2 (KC + 1) loads are in flight
+// rows per step of the Gram-Schmidt passes: their loads are in flight together
+constexpr int CGS_ROWS = 2;
+
+// acc[k] += sum over the thread's rows i of V_k[i] w[i], rows in increasing order, CGS_ROWS
+// rows (i, i + BS, ...) per step so that CGS_ROWS (KC + 1) loads are in flight
 template <int BS, int KC>
 __device__ inline void cgs_dot_rows(const double* __restrict__ Vg, long ldv, const double* __restrict__ w, long lo,
                                     long hi, double (&acc)[CGS_GROUP]) {
+  constexpr int R = CGS_ROWS;
   long i = lo + threadIdx.x;
-  for (; i + BS < hi; i += 2 * BS) {
-    const double w0 = w[i], w1 = w[i + BS];
-    double v0[KC], v1[KC];
+  for (; i + (R - 1) * BS < hi; i += R * BS) {
+    double wr[R], v[R][KC];
 #pragma unroll
-    for (int k = 0; k < KC; ++k) {
-      v0[k] = Vg[k * ldv + i];
-      v1[k] = Vg[k * ldv + i + BS];
+    for (int r = 0; r < R; ++r) {
+      wr[r] = w[i + r * BS];
+#pragma unroll
+      for (int k = 0; k < KC; ++k) v[r][k] = Vg[k * ldv + i + r * BS];
     }
 #pragma unroll
-    for (int k = 0; k < KC; ++k) {
-      acc[k] += v0[k] * w0;
-      acc[k] += v1[k] * w1;
-    }
+    for (int k = 0; k < KC; ++k)
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[k] += v[r][k] * wr[r];
   }
-  if (i < hi) {
+  for (; i < hi; i += BS) {
     const double w0 = w[i];
 #pragma unroll
     for (int k = 0; k < KC; ++k) acc[k] += Vg[k * ldv + i] * w0;
@@ -859,21 +873,24 @@ __global__ void __launch_bounds__(BS) k_cgs_update(const double* __restrict__ V,
   long lo, hi;
   chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
   double ss = 0.0;
+  constexpr int R = CGS_ROWS;
   long i = lo + threadIdx.x;
-  for (; i + BS < hi; i += 2 * BS) {  // two rows per step, each updated in the order k = 0, 1, ...
-    double w0 = w[i], w1 = w[i + BS];
+  for (; i + (R - 1) * BS < hi; i += R * BS) {  // R rows per step, each updated in the order k = 0, 1, ...
+    double wr[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) wr[r] = w[i + r * BS];
     const double* vp = V + i;
 #pragma unroll 8
-    for (int k = 0; k < cnt; ++k) {
-      w0 -= hk[k] * vp[k * ldv];
-      w1 -= hk[k] * vp[k * ldv + BS];
+    for (int k = 0; k < cnt; ++k)
+#pragma unroll
+      for (int r = 0; r < R; ++r) wr[r] -= hk[k] * vp[k * ldv + r * BS];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      w[i + r * BS] = wr[r];
+      ss += wr[r] * wr[r];
     }
-    w[i] = w0;
-    w[i + BS] = w1;
-    ss += w0 * w0;
-    ss += w1 * w1;
   }
-  if (i < hi) {
+  for (; i < hi; i += BS) {
     double wn = w[i];
     const double* vp = V + i;
 #pragma unroll 8

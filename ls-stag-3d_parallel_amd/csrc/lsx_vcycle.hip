@@ -39,8 +39,9 @@ __device__ inline T ld(const FT* p, long i) {
 // Workgroup b of a 1-D grid of nb runs on XCD b % 8 (the 8 XCDs take workgroups round-robin,
 // each XCD with its own L2).  xcd_order(b, nb) renumbers them so that XCD x takes a contiguous
 // run of logical blocks: a bijection of [0, nb).  It pays where a block reads halo data that
-// its logical neighbours read too (the fused sweep and the y/z line passes); for the plain
-// stencil products and the x-line pass it measured slower and is not used.
+// its logical neighbours read too (the fused sweep, the y/z line passes, the residual
+// restriction); for the plain stencil products and the x-line pass it measured slower and is
+// not used.
 __device__ inline int xcd_order(int b, int nb) {
   const int q = nb / 8, r = nb % 8, x = b % 8, k = b / 8;
   return x * q + (x < r ? x : r) + k;
@@ -304,9 +305,11 @@ __global__ void __launch_bounds__(256) k_resid_restrict_cells(const T* __restric
   __shared__ T rr[2][2][64];
   const int tx = threadIdx.x & 63, ty = (threadIdx.x >> 6) & 1, tz = threadIdx.x >> 7;
   const int Nc = N / 2, Mc = M / 2;
-  // block (bx, by, bz) of a 1-D grid, bx fastest
+  // block (bx, by, bz) of a 1-D grid, bx fastest, in XCD-contiguous order (neighbouring rows and
+  // planes read by the same XCD)
   const int gx = (N + 63) / 64;
-  const int bx = (int)(blockIdx.x % gx), by = (int)(blockIdx.x / gx) % Mc, bz = (int)(blockIdx.x / gx) / Mc;
+  const int lb = xcd_order(blockIdx.x, gridDim.x);
+  const int bx = lb % gx, by = (lb / gx) % Mc, bz = (lb / gx) / Mc;
   const int xx = bx * 64 + tx, yy = 2 * by + ty, zz = 2 * bz + tz;
   const bool live = xx < N;
   const int xc = live ? xx : N - 1;
@@ -462,11 +465,17 @@ __device__ inline void prolong_add_body(const T* __restrict__ xc, int Nc, int Mc
     axis_w(x, Nf, cx, wx, &nx);
     axis_w(y, Mf, cy, wy, &ny);
     axis_w(z, Kf, cz, wz, &nz);
+    // always two taps per axis (an absent second tap reads the first with weight 0), so that the
+    // eight loads are straight-line code
+    if (nx < 2) { cx[1] = cx[0]; wx[1] = 0.f; }
+    if (ny < 2) { cy[1] = cy[0]; wy[1] = 0.f; }
+    if (nz < 2) { cz[1] = cz[0]; wz[1] = 0.f; }
+    T v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = xc[cx[c & 1] + Nc * cy[(c >> 1) & 1] + NMc * cz[c >> 2]];
     T acc = 0;
-    for (int c3 = 0; c3 < nz; ++c3)
-      for (int c2 = 0; c2 < ny; ++c2)
-        for (int c1 = 0; c1 < nx; ++c1)
-          acc += (T)(wx[c1] * wy[c2] * wz[c3]) * xc[cx[c1] + Nc * cy[c2] + NMc * cz[c3]];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc += (T)(wx[c & 1] * wy[(c >> 1) & 1] * wz[c >> 2]) * v[c];
     xf[i] += acc;
   }
 }
