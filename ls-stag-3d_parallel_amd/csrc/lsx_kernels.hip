@@ -581,18 +581,7 @@ __global__ void k_stencil_sym_f32x(const double* __restrict__ s4, int N, int M, 
   const long n = (long)N * M * K;
   long lo, hi;
   chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
-  long li = lo + threadIdx.x;
-  for (; li + blockDim.x < hi; li += 2 * blockDim.x) {  // two rows per step, their loads in flight together
-    const int i0 = (int)li, i1 = (int)(li + blockDim.x);
-    int x0, y0, z0, x1, y1, z1;
-    decode(i0, N, M, &x0, &y0, &z0);
-    decode(i1, N, M, &x1, &y1, &z1);
-    const double r0 = row_apply_sym(s4, n, i0, x0, y0, z0, N, M, K, x, 0.0, false, ex);
-    const double r1 = row_apply_sym(s4, n, i1, x1, y1, z1, N, M, K, x, 0.0, false, ex);
-    y[i0] = r0;
-    y[i1] = r1;
-  }
-  if (li < hi) {
+  for (long li = lo + threadIdx.x; li < hi; li += blockDim.x) {
     const int i = (int)li;
     int xx, yy, zz;
     decode(i, N, M, &xx, &yy, &zz);
@@ -795,7 +784,7 @@ __global__ void k_axpy_dot(double* __restrict__ w, const double* __restrict__ v,
 constexpr int CGS_GROUP = 8;
 
 // rows per step of the Gram-Schmidt passes: their loads are in flight together
-constexpr int CGS_ROWS = 2;
+constexpr int CGS_ROWS = 4;
 
 // acc[k] += sum over the thread's rows i of V_k[i] w[i], rows in increasing order, CGS_ROWS
 // rows (i, i + BS, ...) per step so that CGS_ROWS (KC + 1) loads are in flight

@@ -10,6 +10,8 @@
 
 #include <hip/hip_fp16.h>
 
+#include <type_traits>
+
 #include "lsx_vcycle.hpp"
 
 namespace lsx {
@@ -1014,6 +1016,14 @@ struct SegLine {
   T fs;
   int strideL, oAl, oAu, oBl, oBu, len, pin0, pin1;
   bool hAl, hAu, hBl, hBu;
+  // a face in operator units: FP16 faces are stored divided by fs; FP32 and FP64 faces are
+  // stored as they are (fs = 1)
+  __device__ inline T scl(T f) const {
+    if constexpr (std::is_same<FT, _Float16>::value)
+      return fs * f;
+    else
+      return f;
+  }
   __device__ inline void row(unsigned i, int kk, bool first, T& fl_carry, T& lo, T& dg, T& up, T& rh) const {
   #pragma clang fp contract(fast)  // line solves: no bitwise twin to match
     const bool hl = kk > 0, hu = kk < len - 1;
@@ -1027,13 +1037,13 @@ struct SegLine {
     if constexpr (!ZERO) {
       xi = ldo(x, i);
       const T xal = ldo(x, i + oAl), xau = ldo(x, i + oAu), xbl = ldo(x, i + oBl), xbu = ldo(x, i + oBu);
-      const T cAl = hAl ? fs * fal : (T)0, cAu = hAu ? fs * fau : (T)0;
-      const T cBl = hBl ? fs * fbl : (T)0, cBu = hBu ? fs * fbu : (T)0;
+      const T cAl = hAl ? scl(fal) : (T)0, cAu = hAu ? scl(fau) : (T)0;
+      const T cBl = hBl ? scl(fbl) : (T)0, cBu = hBu ? scl(fbu) : (T)0;
       off = cAl * xal + cAu * xau + cBl * xbl + cBu * xbu;
     }
     const bool own = (int)i == pin0 || (int)i == pin1, empty = sv == (T)0, cut = own || empty;
-    lo = cut || !hl ? (T)0 : fs * fll;
-    up = cut || !hu ? (T)0 : fs * flu;
+    lo = cut || !hl ? (T)0 : scl(fll);
+    up = cut || !hu ? (T)0 : scl(flu);
     dg = cut ? (T)1 : recip0(sv);
     rh = own ? bv : (empty ? xi : bv - off);
   }
@@ -1070,91 +1080,101 @@ __global__ void __launch_bounds__(1024) k_line_seg(const T* __restrict__ x_in, T
   T le = (T)0, de = (T)1, ue = (T)0, re = (T)0;  // row of the last cell
   T Gb = (T)0, Hb = (T)1, Qb = (T)0;              // the cell before the last, in (x_L, x_E)
   T Gf = (T)0, Hf = (T)0, Qf = (T)1;              // the first cell, in (x_L, x_E)
-  {
-    T cp = (T)0, gp = (T)0, hp = (T)1;  // x_{-1} = x_L
-    // x_k = g_k + h_k x_L - c_k x_{k+1} for the rows before the last, each row eliminated as soon
-    // as it is built
-    auto elim = [&](int k, T lo, T dg, T up, T rh) {
-      const bool el = k < cnt - 1, last = k == cnt - 1;
-      const T winv = recip0(dg - lo * cp);
-      const T gn = (rh - lo * gp) * winv, hn = -lo * hp * winv, cn = up * winv;
-      gp = el ? gn : gp;
-      hp = el ? hn : hp;
-      cp = el ? cn : cp;
-      G[k] = gp;
-      H[k] = hp;
-      Q[k] = cp;
-      le = last ? lo : le;
-      de = last ? dg : de;
-      ue = last ? up : ue;
-      re = last ? rh : re;
-    };
-    if constexpr (!TRANS) {
-      SegLine<T, FT, DIR, ZERO> Ln;
-      const int nB = DIR == 1 ? K : M, dB = DIR == 1 ? N * M : N;
-      Ln.x = x_in;
-      Ln.b = b;
-      Ln.s = s;
-      Ln.ae = ae;
-      Ln.fB = DIR == 1 ? at : an;
-      Ln.fL = DIR == 1 ? an : at;
-      Ln.fs = fs;
-      Ln.strideL = (int)stride;
-      Ln.len = len;
-      Ln.pin0 = pin0;
-      Ln.pin1 = pin1;
-      Ln.hAl = xx > 0;
-      Ln.hAu = xx < N - 1;
-      Ln.hBl = other > 0;
-      Ln.hBu = other < nB - 1;
-      Ln.oAl = Ln.hAl ? -1 : 0;
-      Ln.oAu = Ln.hAu ? 1 : 0;
-      Ln.oBl = Ln.hBl ? -dB : 0;
-      Ln.oBu = Ln.hBu ? dB : 0;
-      T carry = (T)0;
-#pragma unroll
-      for (int k = 0; k < SEG; ++k) {
-        const int kk = k0 + (k < cnt ? k : cnt - 1);
-        T lo, dg, up, rh;
-        Ln.row((unsigned)(base + stride * kk), kk, k == 0, carry, lo, dg, up, rh);
-        elim(k, lo, dg, up, rh);
+  // the elimination of the segment; FULL: a segment of SEG rows (every segment of a line whose
+  // length is a multiple of SEG), whose masks are then known at compile time
+  auto eliminate = [&](auto full_tag) {
+    constexpr bool FULL = decltype(full_tag)::value;
+    const int nrow = FULL ? SEG : cnt;
+    {
+      T cp = (T)0, gp = (T)0, hp = (T)1;  // x_{-1} = x_L
+      // x_k = g_k + h_k x_L - c_k x_{k+1} for the rows before the last, each row eliminated as soon
+      // as it is built
+      auto elim = [&](int k, T lo, T dg, T up, T rh) {
+        const bool el = k < nrow - 1, last = k == nrow - 1;
+        const T winv = recip0(dg - lo * cp);
+        const T gn = (rh - lo * gp) * winv, hn = -lo * hp * winv, cn = up * winv;
+        gp = el ? gn : gp;
+        hp = el ? hn : hp;
+        cp = el ? cn : cp;
+        G[k] = gp;
+        H[k] = hp;
+        Q[k] = cp;
+        le = last ? lo : le;
+        de = last ? dg : de;
+        ue = last ? up : ue;
+        re = last ? rh : re;
+      };
+      if constexpr (!TRANS) {
+        SegLine<T, FT, DIR, ZERO> Ln;
+        const int nB = DIR == 1 ? K : M, dB = DIR == 1 ? N * M : N;
+        Ln.x = x_in;
+        Ln.b = b;
+        Ln.s = s;
+        Ln.ae = ae;
+        Ln.fB = DIR == 1 ? at : an;
+        Ln.fL = DIR == 1 ? an : at;
+        Ln.fs = fs;
+        Ln.strideL = (int)stride;
+        Ln.len = len;
+        Ln.pin0 = pin0;
+        Ln.pin1 = pin1;
+        Ln.hAl = xx > 0;
+        Ln.hAu = xx < N - 1;
+        Ln.hBl = other > 0;
+        Ln.hBu = other < nB - 1;
+        Ln.oAl = Ln.hAl ? -1 : 0;
+        Ln.oAu = Ln.hAu ? 1 : 0;
+        Ln.oBl = Ln.hBl ? -dB : 0;
+        Ln.oBu = Ln.hBu ? dB : 0;
+        T carry = (T)0;
+  #pragma unroll
+        for (int k = 0; k < SEG; ++k) {
+          const int kk = k0 + (k < nrow ? k : nrow - 1);
+          T lo, dg, up, rh;
+          Ln.row((unsigned)(base + stride * kk), kk, k == 0, carry, lo, dg, up, rh);
+          elim(k, lo, dg, up, rh);
+        }
+      } else {
+  #pragma unroll
+        for (int k = 0; k < SEG; ++k) {
+          const int kk = k0 + (k < nrow ? k : nrow - 1);
+          const int yy = DIR == 1 ? kk : other, zz = DIR == 1 ? other : kk;
+          T lo, dg, up, rh;
+          line_row<T, FT, DIR, ZERO>(x_in, b, ae, an, at, s, fs, base + stride * kk, xx, yy, zz, N, M, K, pin0, pin1,
+                                     true, lo, dg, up, rh);
+          elim(k, lo, dg, up, rh);
+        }
       }
-    } else {
-#pragma unroll
-      for (int k = 0; k < SEG; ++k) {
-        const int kk = k0 + (k < cnt ? k : cnt - 1);
-        const int yy = DIR == 1 ? kk : other, zz = DIR == 1 ? other : kk;
-        T lo, dg, up, rh;
-        line_row<T, FT, DIR, ZERO>(x_in, b, ae, an, at, s, fs, base + stride * kk, xx, yy, zz, N, M, K, pin0, pin1,
-                                   true, lo, dg, up, rh);
-        elim(k, lo, dg, up, rh);
+    }
+    {
+      T Gn = (T)0, Hn = (T)0, Qn = (T)1;  // x_{nrow-1} = x_E
+  #pragma unroll
+      for (int k = SEG - 2; k >= 0; --k) {
+        const bool act = k < nrow - 1;
+        const T c = Q[k];
+        const T gn = G[k] - c * Gn, hn = H[k] - c * Hn, qn = -c * Qn;
+        Gn = act ? gn : Gn;
+        Hn = act ? hn : Hn;
+        Qn = act ? qn : Qn;
+        G[k] = Gn;
+        H[k] = Hn;
+        Q[k] = Qn;
+        const bool bl = k == nrow - 2;
+        Gb = bl ? Gn : Gb;
+        Hb = bl ? Hn : Hb;
+        Qb = bl ? Qn : Qb;
+      }
+      if (nrow > 1) {
+        Gf = Gn;
+        Hf = Hn;
+        Qf = Qn;
       }
     }
-  }
-  {
-    T Gn = (T)0, Hn = (T)0, Qn = (T)1;  // x_{cnt-1} = x_E
-#pragma unroll
-    for (int k = SEG - 2; k >= 0; --k) {
-      const bool act = k < cnt - 1;
-      const T c = Q[k];
-      const T gn = G[k] - c * Gn, hn = H[k] - c * Hn, qn = -c * Qn;
-      Gn = act ? gn : Gn;
-      Hn = act ? hn : Hn;
-      Qn = act ? qn : Qn;
-      G[k] = Gn;
-      H[k] = Hn;
-      Q[k] = Qn;
-      const bool bl = k == cnt - 2;
-      Gb = bl ? Gn : Gb;
-      Hb = bl ? Hn : Hb;
-      Qb = bl ? Qn : Qb;
-    }
-    if (cnt > 1) {
-      Gf = Gn;
-      Hf = Hn;
-      Qf = Qn;
-    }
-  }
+  };
+  if (cnt == SEG)
+    eliminate(std::true_type{});
+  else
+    eliminate(std::false_type{});
   IF(4, seg, lane) = Gf;
   IF(5, seg, lane) = Hf;
   IF(6, seg, lane) = Qf;
