@@ -234,8 +234,8 @@ def main():
     ap.add_argument("--cpu-sample-grid", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graphs", action="store_true", help="replay the V-cycles from captured HIP graphs")
-    ap.add_argument("--smoother", choices=["rbgs", "line"], default="line",
-                    help="V-cycle smoother: red-black Gauss-Seidel or x/y/z line relaxation")
+    ap.add_argument("--smoother", choices=["rbgs", "line", "jacobi"], default="line",
+                    help="V-cycle smoother: red-black Gauss-Seidel, x/y/z line relaxation or damped Jacobi")
     ap.add_argument("--line-levels", type=int, default=2,
                     help="with --smoother line: line relaxation on the finest this many levels, red-black "
                          "below (0 = every level)")
@@ -299,7 +299,7 @@ def main():
         except (ValueError, OSError):
             traffic = None
     value = args.steps / elapsed
-    sm_name = "red-black GS" if args.smoother == "rbgs" else (
+    sm_name = "red-black GS" if args.smoother == "rbgs" else "damped Jacobi" if args.smoother == "jacobi" else (
         "x/y/z line-relaxation" if args.line_levels <= 0 else
         f"x/y/z line-relaxation on the finest {args.line_levels} levels, red-black GS below")
     cfg = {

@@ -72,6 +72,8 @@ typedef struct lsx_options {
 #define LSX_SMOOTH_RBGS 0 /* red-black Gauss-Seidel                                             */
 #define LSX_SMOOTH_LINE 1 /* line relaxation: x, y and z lines solved exactly in turn (line     \
                              Jacobi), for grids whose cells are long in some direction          */
+#define LSX_SMOOTH_JACOBI 2 /* damped point Jacobi x + omega D^-1 (b - A x) on every level, with \
+                               the omega of the options (in (0, 1])                           */
 
 #define LSX_ORTHO_AUTO 0 /* MGS with the cascade (reference), CGS with the V-cycle           */
 #define LSX_ORTHO_MGS 1  /* modified Gram-Schmidt (fgmres_interface.h:97-104)               */

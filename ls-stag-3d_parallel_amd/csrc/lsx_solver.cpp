@@ -33,7 +33,10 @@ Solver::Solver(const lsx_options& o, hipStream_t s) : opt(o) {
   if (opt.ortho < LSX_ORTHO_AUTO || opt.ortho > LSX_ORTHO_CGS2) throw Error(LSX_ERR_ARG, "unknown ortho");
   if (opt.mg == LSX_MG_VCYCLE && (opt.n_pre < 1 || opt.n_post < 1))
     throw Error(LSX_ERR_ARG, "the V-cycle needs n_pre >= 1 and n_post >= 1");
-  if (opt.smoother != LSX_SMOOTH_RBGS && opt.smoother != LSX_SMOOTH_LINE) throw Error(LSX_ERR_ARG, "unknown smoother");
+  if (opt.smoother != LSX_SMOOTH_RBGS && opt.smoother != LSX_SMOOTH_LINE && opt.smoother != LSX_SMOOTH_JACOBI)
+    throw Error(LSX_ERR_ARG, "unknown smoother");
+  if (opt.smoother == LSX_SMOOTH_JACOBI && !(opt.omega > 0.0 && opt.omega <= 1.0))
+    throw Error(LSX_ERR_ARG, "Jacobi damping omega must be in (0, 1]");
   if (opt.fused_sweep < 0 || opt.fused_sweep > 2) throw Error(LSX_ERR_ARG, "fused_sweep must be 0, 1 or 2");
   if (s) {
     stream = s;
@@ -379,14 +382,11 @@ void Solver::build_vcycle() {
       LSX_HIP(hipStreamSynchronize(stream));
       break;
     }
-    if (line_level(l)) {
-      if (m32) {
+    if (!rb_level(l)) {  // ping-pong buffer of the line and Jacobi passes
+      if (m32)
         S.fxt.alloc(nl);
-        if (l == 0) fcp.alloc(nl);
-      } else {
+      else
         S.dxt.alloc(nl);
-        if (l == 0) dcp.alloc(nl);
-      }
     } else if (fused_level(l)) {  // second iterate buffer of the one-pass sweeps
       if (m32)
         S.fxt.alloc(nl);
@@ -462,10 +462,6 @@ float* Solver::vxt<float>(VStore& S) { return S.fxt.p; }
 template <>
 double* Solver::vxt<double>(VStore& S) { return S.dxt.p; }
 template <>
-float* Solver::vcp<float>() { return fcp.p; }
-template <>
-double* Solver::vcp<double>() { return dcp.p; }
-template <>
 float* Solver::vr<float>(VStore& S) { return S.fr.p; }
 template <>
 double* Solver::vr<double>(VStore& S) { return S.dr.p; }
@@ -522,7 +518,7 @@ void Solver::vcycle_run(const double* v, T* z) {
   // levels from `lc` down are small enough for one workgroup: a single launch does their cycle
   int lc = L + 1;
   for (int l = 1; l <= L && opt.coarse_fuse; ++l)
-    if (!line_level(l) && vlev[l].L.d.n() <= vc::COARSE_CELLS && L - l + 1 <= vc::COARSE_MAX) {
+    if (rb_level(l) && vlev[l].L.d.n() <= vc::COARSE_CELLS && L - l + 1 <= vc::COARSE_MAX) {
       lc = l;
       break;
     }
@@ -582,7 +578,7 @@ void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess, bool r
   auto t1 = [&](int k, double bytes) {
     if (l == 0) tend(k, e0, bytes);
   };
-  if (!line_level(l) && (trans || !fused_level(l))) {
+  if (rb_level(l) && (trans || !fused_level(l))) {
     for (int s = 0; s < sweeps; ++s) {
       const bool first = zero_guess && s == 0;
       for (int h = 0; h < 2; ++h) {
@@ -598,7 +594,7 @@ void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess, bool r
   }
   T* xt = vxt<T>(S);
   const long n = S.L.d.n();
-  if (!line_level(l)) {  // one pass per fused red-black sweep
+  if (rb_level(l)) {  // one pass per fused red-black sweep
     T* src = x;
     if (!zero_guess && (sweeps - 1) % 2 == 0) {  // the first pass must not write the buffer it reads
       vc::copy_vec<T>(x, xt, n, stream);
@@ -613,7 +609,21 @@ void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess, bool r
     }
     return;
   }
-  T* cpv = vcp<T>();
+  if (!line_level(l)) {  // damped Jacobi, one pass per sweep (the transposed sweep: the same on A^T)
+    T* src = x;
+    if (!zero_guess && (sweeps - 1) % 2 == 0) {  // the first pass must not write the buffer it reads
+      vc::copy_vec<T>(x, xt, n, stream);
+      src = xt;
+    }
+    for (int p = 1; p <= sweeps; ++p) {
+      T* dst = (sweeps - p) % 2 == 0 ? x : xt;
+      t0(LSX_K_SMOOTH);
+      vc::jacobi<T>(S.L, src, dst, b, (T)opt.omega, zero_guess && p == 1, trans ? 1 : 0, stream);
+      t1(LSX_K_SMOOTH, pass_bytes);
+      src = dst;
+    }
+    return;
+  }
   const int P = 3 * sweeps;
   T* src = x;
   if (!zero_guess && (P - 1) % 2 == 0) {  // the first pass must not write the buffer it reads
@@ -625,7 +635,7 @@ void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess, bool r
     // x, y, z lines in turn, or z, y, x (rev_lines)
     const int dir = rev_lines ? 2 - (p - 1) % 3 : (p - 1) % 3;
     t0(LSX_K_LINE_X + dir);
-    vc::line_pass<T>(S.L, dir, src, dst, cpv, b, zero_guess && p == 1, trans ? 1 : 0, stream);
+    vc::line_pass<T>(S.L, dir, src, dst, b, zero_guess && p == 1, trans ? 1 : 0, stream);
     t1(LSX_K_LINE_X + dir, pass_bytes);
     src = dst;
   }

@@ -171,12 +171,8 @@ class Solver {
     DevBuf<double> dxt;  // second iterate buffer of the line relaxation
     DevBuf<float> fxt;
   };
-  DevBuf<double> dcp;  // line-relaxation scratch (finest level's size)
-  DevBuf<float> fcp;
   template <class T>
   T* vxt(VStore& S);
-  template <class T>
-  T* vcp();
   template <class T>
   void smooth(int l, T* x, const T* b, int sweeps, bool zero_guess, bool rev_lines = false);
   // levels whose red-black sweeps run as one fused pass: the finest level when it has at least
@@ -188,6 +184,8 @@ class Solver {
   bool line_level(int l) const {
     return opt.smoother == LSX_SMOOTH_LINE && (opt.line_levels <= 0 || l < opt.line_levels);
   }
+  // levels smoothed by red-black Gauss-Seidel (the others: lines, or damped Jacobi on every level)
+  bool rb_level(int l) const { return opt.smoother != LSX_SMOOTH_JACOBI && !line_level(l); }
   std::vector<VStore> vlev;
   DevBuf<float> Z32, ztmp32;
   bool mixed() const { return opt.precision == LSX_PREC_MIXED || opt.precision == LSX_PREC_MIXED16; }

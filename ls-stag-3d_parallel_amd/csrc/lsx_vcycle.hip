@@ -740,6 +740,37 @@ __global__ void __launch_bounds__(FX* FY) k_rb_fused(const T* __restrict__ x_in,
   }
 }
 
+// ---- damped Jacobi ---------------------------------------------------------------------------
+// One damped Jacobi sweep x_out = x_in + omega D^-1 (b - A x_in): every row of the form
+// x + W (b - A x) with W = omega D^-1, so the transposed sweep is the same with A^T (TRANS: the
+// finest level's pinned rows are full rows of A^T, other rows drop their couplings to them).
+// Empty rows (1/a_p stored as 0) keep their value; ZERO: x_in is 0 and not read.  One thread per
+// cell, every load of a row issued before the update.
+template <class T, class FT, bool ZERO, bool TRANS>
+__global__ void __launch_bounds__(256) k_jacobi(const T* __restrict__ x_in, T* __restrict__ x_out,
+                                                const T* __restrict__ b, const FT* __restrict__ ae,
+                                                const FT* __restrict__ an, const FT* __restrict__ at,
+                                                const T* __restrict__ s, T fs, T omega, int N, int M, int K,
+                                                int pin0, int pin1) {
+  const long n = (long)N * M * K;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int xx, yy, zz;
+    decode(i, N, M, &xx, &yy, &zz);
+    const T si = s[i], bi = b[i];
+    T xi = (T)0, od = (T)0;
+    if constexpr (!ZERO) {
+      xi = x_in[i];
+      if constexpr (TRANS) {
+        od = offdiag_tr<T, FT>(x_in, ae, an, at, i, xx, yy, zz, N, M, K, pin0, pin1);
+      } else {
+        const T o = offdiag_u<T, FT>(x_in, ae, an, at, i, xx, yy, zz, N, M, K);
+        od = (i == pin0 || i == pin1) ? (T)0 : o;  // pinned rows of A are identity rows
+      }
+    }
+    x_out[i] = si == (T)0 ? xi : xi + omega * ((bi - fs * od) * si - xi);
+  }
+}
+
 template <class T>
 __global__ void k_zero_empty(const T* __restrict__ s, double* __restrict__ x, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -1281,6 +1312,41 @@ void to27(const double* fe, const double* fn, const double* ft, const double* dg
 }
 
 template <class T, class FT>
+static void jacobi_t(const Level& L, const T* x_in, T* x_out, const T* b, T omega, int zero, int trans,
+                     hipStream_t s) {
+  const FT* ae = (const FT*)L.f_e;
+  const FT* an = (const FT*)L.f_n;
+  const FT* at = (const FT*)L.f_t;
+  const T* sv = (const T*)L.s;
+  const T fs = (T)L.fscale;
+  const Dims& d = L.d;
+#define LSX_JACOBI(Z, TR)                                                                                     \
+  k_jacobi<T, FT, Z, TR><<<grid_1d(d.n()), 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, omega, d.N, d.M, d.K, \
+                                                        L.pin0, L.pin1)
+  if (zero)
+    LSX_JACOBI(true, false);  // zero start: the transposed first sweep is the same
+  else if (trans)
+    LSX_JACOBI(false, true);
+  else
+    LSX_JACOBI(false, false);
+#undef LSX_JACOBI
+}
+
+template <>
+void jacobi<double>(const Level& L, const double* x_in, double* x_out, const double* b, double omega, int zero,
+                    int trans, hipStream_t s) {
+  jacobi_t<double, double>(L, x_in, x_out, b, omega, zero, trans, s);
+}
+template <>
+void jacobi<float>(const Level& L, const float* x_in, float* x_out, const float* b, float omega, int zero, int trans,
+                   hipStream_t s) {
+  if (L.ftype == FACE_F16)
+    jacobi_t<float, _Float16>(L, x_in, x_out, b, omega, zero, trans, s);
+  else
+    jacobi_t<float, float>(L, x_in, x_out, b, omega, zero, trans, s);
+}
+
+template <class T, class FT>
 static void rbgs_t(const Level& L, T* x, const T* b, int color, int zero_guess, hipStream_t s) {
   const long total = (long)((L.d.N + 1) / 2) * L.d.M * L.d.K;
   k_rbgs<T, FT><<<grid_1d(total), 256, 0, s>>>(x, b, (const FT*)L.f_e, (const FT*)L.f_n, (const FT*)L.f_t,
@@ -1423,9 +1489,8 @@ static void line_pass_z(const Level& L, int dir, const T* x_in, T* x_out, const 
 }
 
 template <class T, class FT>
-static void line_pass_t(const Level& L, int dir, const T* x_in, T* x_out, T* cpv, const T* b, int zero, int trans,
+static void line_pass_t(const Level& L, int dir, const T* x_in, T* x_out, const T* b, int zero, int trans,
                         hipStream_t s) {
-  (void)cpv;
   if (zero)
     line_pass_z<T, FT, true>(L, dir, x_in, x_out, b, trans, s);
   else
@@ -1433,17 +1498,17 @@ static void line_pass_t(const Level& L, int dir, const T* x_in, T* x_out, T* cpv
 }
 
 template <>
-void line_pass<double>(const Level& L, int dir, const double* x_in, double* x_out, double* cpv, const double* b,
+void line_pass<double>(const Level& L, int dir, const double* x_in, double* x_out, const double* b,
                        int zero, int trans, hipStream_t s) {
-  line_pass_t<double, double>(L, dir, x_in, x_out, cpv, b, zero, trans, s);
+  line_pass_t<double, double>(L, dir, x_in, x_out, b, zero, trans, s);
 }
 template <>
-void line_pass<float>(const Level& L, int dir, const float* x_in, float* x_out, float* cpv, const float* b,
+void line_pass<float>(const Level& L, int dir, const float* x_in, float* x_out, const float* b,
                       int zero, int trans, hipStream_t s) {
   if (L.ftype == FACE_F16)
-    line_pass_t<float, _Float16>(L, dir, x_in, x_out, cpv, b, zero, trans, s);
+    line_pass_t<float, _Float16>(L, dir, x_in, x_out, b, zero, trans, s);
   else
-    line_pass_t<float, float>(L, dir, x_in, x_out, cpv, b, zero, trans, s);
+    line_pass_t<float, float>(L, dir, x_in, x_out, b, zero, trans, s);
 }
 
 template <class T, class FT>

@@ -110,18 +110,27 @@ void resid_tr<float>(const Level& L, const float* x, const float* b, float* r, h
 
 // line relaxation: every line of direction dir (0 x, 1 y, 2 z) solved exactly against x_in's
 // values across lines (x_in unread when zero); trans: the same on A^T (the finest level's pinned
-// rows are full rows, other rows drop their couplings to them); cpv: scratch of the level's size
+// rows are full rows, other rows drop their couplings to them)
 template <class T>
-void line_pass(const Level& L, int dir, const T* x_in, T* x_out, T* cpv, const T* b, int zero, int trans,
-               hipStream_t s);
+void line_pass(const Level& L, int dir, const T* x_in, T* x_out, const T* b, int zero, int trans, hipStream_t s);
 template <>
-void line_pass<double>(const Level& L, int dir, const double* x_in, double* x_out, double* cpv, const double* b,
+void line_pass<double>(const Level& L, int dir, const double* x_in, double* x_out, const double* b,
                        int zero, int trans, hipStream_t s);
 template <>
-void line_pass<float>(const Level& L, int dir, const float* x_in, float* x_out, float* cpv, const float* b,
+void line_pass<float>(const Level& L, int dir, const float* x_in, float* x_out, const float* b,
                       int zero, int trans, hipStream_t s);
 template <class T>
 void copy_vec(const T* a, T* b, long n, hipStream_t s);
+// one damped Jacobi sweep x_out = x_in + omega D^-1 (b - A x_in) (zero: x_in taken as 0, not
+// read; trans: the same with A^T); empty rows keep their value
+template <class T>
+void jacobi(const Level& L, const T* x_in, T* x_out, const T* b, T omega, int zero, int trans, hipStream_t s);
+template <>
+void jacobi<double>(const Level& L, const double* x_in, double* x_out, const double* b, double omega, int zero,
+                    int trans, hipStream_t s);
+template <>
+void jacobi<float>(const Level& L, const float* x_in, float* x_out, const float* b, float omega, int zero, int trans,
+                   hipStream_t s);
 // one full red-black sweep x_in -> x_out in a single pass (zero: x_in taken as 0, not read);
 // bit-identical to the red and black half-sweeps in place
 template <class T>

@@ -72,6 +72,10 @@ class Options(C.Structure):
     ]
 
 
+# V-cycle smoothers (lsx_options.smoother): red-black Gauss-Seidel, x/y/z line relaxation,
+# damped point Jacobi (damping = the task's omega)
+SMOOTHERS = {"rbgs": 0, "line": 1, "jacobi": 2}
+
 _lib = None
 
 
@@ -346,7 +350,7 @@ class IterativeSolver:
         self.ortho = ortho
         self.coarse_fuse = coarse_fuse
         self.graphs = graphs
-        if smoother not in ("rbgs", "line"):
+        if smoother not in SMOOTHERS:
             raise ValueError(f"unknown smoother {smoother!r}")
         self.smoother = smoother
         self.fused_sweep = fused_sweep
@@ -382,7 +386,7 @@ class IterativeSolver:
         opt.coarse_fuse = 1 if self.coarse_fuse else 0
         opt.graphs = 1 if self.graphs else 0
         opt.levels = int(getattr(self, "levels_cap", 0))
-        opt.smoother = 1 if self.smoother == "line" else 0
+        opt.smoother = SMOOTHERS[self.smoother]
         opt.fused_sweep = int(self.fused_sweep)
         opt.line_levels = int(self.line_levels)
         h = C.c_void_p()
@@ -417,7 +421,7 @@ class IterativeSolver:
         opt.coarse_fuse = 1 if self.coarse_fuse else 0
         opt.graphs = 1 if self.graphs else 0
         opt.levels = int(getattr(self, "levels_cap", 0))
-        opt.smoother = 1 if self.smoother == "line" else 0
+        opt.smoother = SMOOTHERS[self.smoother]
         opt.fused_sweep = int(self.fused_sweep)
         opt.line_levels = int(self.line_levels)
         h = C.c_void_p()

@@ -47,7 +47,7 @@ def test_transposed_operator(lsx, name, mk):
     assert np.linalg.norm(S.apply_operator(y) - ATy) > 1e-6 * np.linalg.norm(ATy)
 
 
-@pytest.mark.parametrize("smoother", ["rbgs", "line", "line1"])
+@pytest.mark.parametrize("smoother", ["rbgs", "line", "line1", "jacobi"])
 @pytest.mark.parametrize("precision,tol", [("fp64-vcycle", 1e-12), ("mixed", 2e-5), ("mixed16", 2e-3)])
 @pytest.mark.parametrize("name,mk", CASES, ids=[c[0] for c in CASES])
 def test_transposed_vcycle(lsx, name, mk, precision, tol, smoother):

@@ -2,7 +2,8 @@
 operator A (scipy):
   red-black Gauss-Seidel: red cells (x+y+z even) then black cells, x_i = (b_i - sum_j a_ij x_j) / a_ii;
   line relaxation: for d = x, y, z in turn, x <- T_d^{-1} (b - (A - T_d) x), T_d the part of A
-  coupling cells along d (the diagonal included), solved exactly per line.
+  coupling cells along d (the diagonal included), solved exactly per line;
+  damped Jacobi: x <- x + omega D^{-1} (b - A x), D the diagonal of A.
 Rows of A that are identities (pinned cells) stay identities, empty rows give 0.  FP64 results
 agree to rounding; the mixed-precision smoother (FP32 values and faces) to FP32 rounding.  Then
 the line-relaxation V-cycle against the red-black one on a strongly stretched grid."""
@@ -45,6 +46,16 @@ def ref_line_sweep(A, dims, b, x):
     return x
 
 
+JACOBI_OMEGA = 0.805  # the damping of lsx.mixed_task (the Task default omega)
+
+
+def ref_jacobi_sweep(A, dims, b, x):
+    S = A.to_scipy().tocsr()
+    d = S.diagonal()
+    upd = x + JACOBI_OMEGA * (b - S @ x) / np.where(d != 0, d, 1.0)
+    return np.where(d != 0, upd, x)
+
+
 def ref_rb_sweep(A, dims, b, x):
     N, M, K = dims
     S = A.to_scipy().tocsr()
@@ -63,7 +74,7 @@ CASES = [("sphere32", lambda: P.sphere(32)), ("ellipsoid32_stretched", lambda: P
          ("cut32x16x24", lambda: P.synthetic_cut(32, 16, 24, seed=3))]
 
 
-@pytest.mark.parametrize("smoother", ["line", "rbgs"])
+@pytest.mark.parametrize("smoother", ["line", "rbgs", "jacobi"])
 @pytest.mark.parametrize("precision,tol", [("fp64-vcycle", 1e-12), ("mixed", 2e-5)])
 @pytest.mark.parametrize("name,mk", CASES, ids=[c[0] for c in CASES])
 def test_smoother_sweep(lsx, name, mk, precision, tol, smoother):
@@ -75,7 +86,7 @@ def test_smoother_sweep(lsx, name, mk, precision, tol, smoother):
     rng = np.random.default_rng(3)
     b = rng.standard_normal(A.n) * act
     x0 = rng.standard_normal(A.n) * act
-    ref = ref_line_sweep if smoother == "line" else ref_rb_sweep
+    ref = {"line": ref_line_sweep, "rbgs": ref_rb_sweep, "jacobi": ref_jacobi_sweep}[smoother]
     x1 = S.smooth(b, x0, sweeps=1)
     x1r = ref(A, dims, b, x0)
     x0z = S.smooth(b, x0, sweeps=1, zero_guess=True)
@@ -148,3 +159,23 @@ def test_line_relaxation_limits(lsx):
         S = lsx.solver("fgmres", "mixed", smoother="line")
         with pytest.raises(lsx.LsxError):
             S.set(A, np.zeros(A.n), np.zeros(A.n), lsx.mixed_task(1.0))
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64-vcycle"])
+def test_jacobi_vcycle_converges(lsx, precision):
+    """The damped-Jacobi V-cycle preconditions FGMRES to the tolerance on the true residual (more
+    iterations than red-black, which converges in fewer)."""
+    g, A = P.sphere(64)
+    b = P.random_rhs(g, seed=5, A=A)
+    eps = 1e-8 * np.linalg.norm(b)
+    its = {}
+    for smoother in ("rbgs", "jacobi"):
+        S = lsx.solver("fgmres", precision, smoother=smoother)
+        x = np.zeros(A.n)
+        S.set(A, b, x, lsx.mixed_task(eps, restart=30, max_it=500))
+        its[smoother] = S.solve()
+        true_res = np.linalg.norm(b - A.matvec(x))
+        print(f"precision={precision} smoother={smoother} it={its[smoother]} "
+              f"true_rel_res={true_res / np.linalg.norm(b):.3e}", flush=True)
+        assert true_res <= 1.05 * eps
+    assert its["rbgs"] <= its["jacobi"] < 500
