@@ -62,6 +62,19 @@ __device__ inline void chunk_of(long n, int b, int nb, long* lo, long* hi) {
 }
 
 // ---- stencil geometry helpers -------------------------------------------------------------
+// One thread per cell for the plain stencil products: 256-thread blocks of 64 x by 4 y cells,
+// blocks x fastest, then y, then z (tile_blocks of them).  The cell of this thread, false past the
+// grid's edge.  No per-cell division, and each row's loads in flight together (measured 139 us
+// against 156 us for 4096-cell chunks walked by 256 threads at 256^3).
+__device__ inline bool tile_cell(int N, int M, int* x, int* y, int* z) {
+  const int gx = (N + 63) / 64, gy = (M + 3) / 4;
+  const int bx = (int)(blockIdx.x % gx), r = (int)(blockIdx.x / gx);
+  *x = bx * 64 + (int)(threadIdx.x & 63);
+  *y = (r % gy) * 4 + (int)(threadIdx.x >> 6);
+  *z = r / gy;
+  return *x < N && *y < M;
+}
+
 __device__ inline void decode(int i, int N, int M, int* x, int* y, int* z) {
   const int NM = N * M;
   *z = i / NM;
@@ -565,28 +578,20 @@ __global__ void k_stencil_sym(const double* __restrict__ s4, int N, int M, int K
 __global__ void k_stencil_sym_t_f32x(const double* __restrict__ s4, int N, int M, int K, SymExc ex,
                                      const float* __restrict__ x, double* __restrict__ y) {
   const long n = (long)N * M * K;
-  long lo, hi;
-  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
-  for (long li = lo + threadIdx.x; li < hi; li += blockDim.x) {
-    const int i = (int)li;
-    int xx, yy, zz;
-    decode(i, N, M, &xx, &yy, &zz);
-    y[i] = row_apply_sym_t(s4, n, i, xx, yy, zz, N, M, K, x, 0.0, false, ex);
-  }
+  int xx, yy, zz;
+  if (!tile_cell(N, M, &xx, &yy, &zz)) return;
+  const int i = xx + N * (yy + M * zz);
+  y[i] = row_apply_sym_t(s4, n, i, xx, yy, zz, N, M, K, x, 0.0, false, ex);
 }
 
 // y = A x with an FP32 x (mixed-precision Krylov products)
 __global__ void k_stencil_sym_f32x(const double* __restrict__ s4, int N, int M, int K, SymExc ex,
                                    const float* __restrict__ x, double* __restrict__ y) {
   const long n = (long)N * M * K;
-  long lo, hi;
-  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
-  for (long li = lo + threadIdx.x; li < hi; li += blockDim.x) {
-    const int i = (int)li;
-    int xx, yy, zz;
-    decode(i, N, M, &xx, &yy, &zz);
-    y[i] = row_apply_sym(s4, n, i, xx, yy, zz, N, M, K, x, 0.0, false, ex);
-  }
+  int xx, yy, zz;
+  if (!tile_cell(N, M, &xx, &yy, &zz)) return;
+  const int i = xx + N * (yy + M * zz);
+  y[i] = row_apply_sym(s4, n, i, xx, yy, zz, N, M, K, x, 0.0, false, ex);
 }
 
 // ---- kernels: solve phase ---------------------------------------------------------------------
@@ -1005,14 +1010,8 @@ int grid_1d(long n, int bs = 256) {
 // ================================================================================================
 // launchers
 // ================================================================================================
-// blocks of contiguous 4096-cell chunks (at most 4096 blocks) for the stencil products: each
-// thread walks its rows of the chunk, so a block reads neighbouring rows it has just read
-static int chunk_blocks(long n) {
-  long b = (n + 4095) / 4096;
-  if (b > 4096) b = 4096;
-  if (b < 1) b = 1;
-  return (int)b;
-}
+// blocks of tile_cell
+static unsigned tile_blocks(Dims d) { return (unsigned)((long)((d.N + 63) / 64) * ((d.M + 3) / 4) * d.K); }
 
 int reduction_blocks(long n) {
   long b = (n + RED_THREADS * 4 - 1) / (RED_THREADS * 4);
@@ -1177,11 +1176,11 @@ void stencil_apply_sym_t(const double* s4, const SymExc& ex, Dims d, const doubl
 }
 
 void stencil_prod_sym_t_f32x(const double* s4, const SymExc& ex, Dims d, const float* x, double* y, hipStream_t s) {
-  k_stencil_sym_t_f32x<<<chunk_blocks(d.n()), 256, 0, s>>>(s4, d.N, d.M, d.K, ex, x, y);
+  k_stencil_sym_t_f32x<<<tile_blocks(d), 256, 0, s>>>(s4, d.N, d.M, d.K, ex, x, y);
 }
 
 void stencil_prod_sym_f32x(const double* s4, const SymExc& ex, Dims d, const float* x, double* y, hipStream_t s) {
-  k_stencil_sym_f32x<<<chunk_blocks(d.n()), 256, 0, s>>>(s4, d.N, d.M, d.K, ex, x, y);
+  k_stencil_sym_f32x<<<tile_blocks(d), 256, 0, s>>>(s4, d.N, d.M, d.K, ex, x, y);
 }
 
 void stencil7_prod_f32x(const double* coef, Dims d, const float* x, double* y, hipStream_t s) {

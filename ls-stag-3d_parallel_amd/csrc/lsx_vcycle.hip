@@ -752,10 +752,13 @@ __global__ void __launch_bounds__(256) k_jacobi(const T* __restrict__ x_in, T* _
                                                 const FT* __restrict__ an, const FT* __restrict__ at,
                                                 const T* __restrict__ s, T fs, T omega, int N, int M, int K,
                                                 int pin0, int pin1) {
-  const long n = (long)N * M * K;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    int xx, yy, zz;
-    decode(i, N, M, &xx, &yy, &zz);
+  // one thread per cell: 64 x by 4 y cells per block, blocks x fastest, then y, then z
+  const int gx = (N + 63) / 64, gy = (M + 3) / 4;
+  const int r = (int)(blockIdx.x / gx);
+  const int xx = (int)(blockIdx.x % gx) * 64 + (int)(threadIdx.x & 63);
+  const int yy = (r % gy) * 4 + (int)(threadIdx.x >> 6), zz = r / gy;
+  if (xx < N && yy < M) {
+    const long i = xx + (long)N * (yy + (long)M * zz);
     const T si = s[i], bi = b[i];
     T xi = (T)0, od = (T)0;
     if constexpr (!ZERO) {
@@ -1321,7 +1324,8 @@ static void jacobi_t(const Level& L, const T* x_in, T* x_out, const T* b, T omeg
   const T fs = (T)L.fscale;
   const Dims& d = L.d;
 #define LSX_JACOBI(Z, TR)                                                                                     \
-  k_jacobi<T, FT, Z, TR><<<grid_1d(d.n()), 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, omega, d.N, d.M, d.K, \
+  k_jacobi<T, FT, Z, TR><<<(unsigned)((long)((d.N + 63) / 64) * ((d.M + 3) / 4) * d.K), 256, 0, s>>>(             \
+      x_in, x_out, b, ae, an, at, sv, fs, omega, d.N, d.M, d.K,                                                \
                                                         L.pin0, L.pin1)
   if (zero)
     LSX_JACOBI(true, false);  // zero start: the transposed first sweep is the same
