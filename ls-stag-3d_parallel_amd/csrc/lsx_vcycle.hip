@@ -455,13 +455,10 @@ __global__ void k_prolong_inject_add(const T* __restrict__ xc, int Nc, int Mc, T
 }
 
 template <class T>
-__device__ inline void prolong_add_body(const T* __restrict__ xc, int Nc, int Mc, T* __restrict__ xf, int Nf, int Mf,
-                                        int Kf, long start, long stride) {
-  const long nf = (long)Nf * Mf * Kf;
+__device__ inline void prolong_cell(const T* __restrict__ xc, int Nc, int Mc, T* __restrict__ xf, int Nf, int Mf,
+                                    int Kf, long i, int x, int y, int z) {
   const long NMc = (long)Nc * Mc;
-  for (long i = start; i < nf; i += stride) {
-    int x, y, z;
-    decode(i, Nf, Mf, &x, &y, &z);
+  {
     int cx[2], cy[2], cz[2], nx, ny, nz;
     float wx[2], wy[2], wz[2];
     axis_w(x, Nf, cx, wx, &nx);
@@ -483,9 +480,25 @@ __device__ inline void prolong_add_body(const T* __restrict__ xc, int Nc, int Mc
 }
 
 template <class T>
-__global__ void k_prolong_add(const T* __restrict__ xc, int Nc, int Mc, T* __restrict__ xf, int Nf, int Mf, int Kf) {
-  prolong_add_body<T>(xc, Nc, Mc, xf, Nf, Mf, Kf, blockIdx.x * (long)blockDim.x + threadIdx.x,
-                      (long)gridDim.x * blockDim.x);
+__device__ inline void prolong_add_body(const T* __restrict__ xc, int Nc, int Mc, T* __restrict__ xf, int Nf, int Mf,
+                                        int Kf, long start, long stride) {
+  const long nf = (long)Nf * Mf * Kf;
+  for (long i = start; i < nf; i += stride) {
+    int x, y, z;
+    decode(i, Nf, Mf, &x, &y, &z);
+    prolong_cell<T>(xc, Nc, Mc, xf, Nf, Mf, Kf, i, x, y, z);
+  }
+}
+
+// one thread per fine cell: 64 x by 4 y cells per block, blocks x fastest, then y, then z
+template <class T>
+__global__ void __launch_bounds__(256) k_prolong_add(const T* __restrict__ xc, int Nc, int Mc, T* __restrict__ xf,
+                                                     int Nf, int Mf, int Kf) {
+  const int gx = (Nf + 63) / 64, gy = (Mf + 3) / 4;
+  const int r = (int)(blockIdx.x / gx);
+  const int x = (int)(blockIdx.x % gx) * 64 + (int)(threadIdx.x & 63);
+  const int y = (r % gy) * 4 + (int)(threadIdx.x >> 6), z = r / gy;
+  if (x < Nf && y < Mf) prolong_cell<T>(xc, Nc, Mc, xf, Nf, Mf, Kf, x + (long)Nf * (y + (long)Mf * z), x, y, z);
 }
 
 // The small levels of the cycle in one workgroup: pre-smoothing and restriction down to the
@@ -1562,7 +1575,8 @@ void coarse_cycle(const CoarseLevels<T>& C, int n_pre, int n_post, const double*
 
 template <class T>
 void prolong_add(const T* xc, Dims dc, T* xf, Dims df, hipStream_t s) {
-  k_prolong_add<T><<<grid_1d(df.n()), 256, 0, s>>>(xc, dc.N, dc.M, xf, df.N, df.M, df.K);
+  k_prolong_add<T><<<(unsigned)((long)((df.N + 63) / 64) * ((df.M + 3) / 4) * df.K), 256, 0, s>>>(xc, dc.N, dc.M, xf,
+                                                                                             df.N, df.M, df.K);
 }
 template void prolong_add<float>(const float*, Dims, float*, Dims, hipStream_t);
 template void prolong_add<double>(const double*, Dims, double*, Dims, hipStream_t);
