@@ -1062,6 +1062,10 @@ struct SegLine {
   const FT* fL;  // faces along the line
   T fs;
   int strideL, oAl, oAu, oBl, oBu, len, pin0, pin1;
+  // x neighbours come from the neighbouring lanes of the 32-lane group (lane l32 holds cell
+  // x0 + l32 of the row); only the group's west and east halo cells are loaded, at offsets
+  // oW0 / oE0 from the cell (the same address for the whole group)
+  int l32, oW0, oE0;
   bool hAl, hAu, hBl, hBu;
   // a face in operator units: FP16 faces are stored divided by fs; FP32 and FP64 faces are
   // stored as they are (fs = 1)
@@ -1075,7 +1079,7 @@ struct SegLine {
   #pragma clang fp contract(fast)  // line solves: no bitwise twin to match
     const bool hl = kk > 0, hu = kk < len - 1;
     const T sv = ldo(s, i), bv = ldo(b, i);
-    const T fal = (T)ldo(ae, i + oAl), fau = (T)ldo(ae, i);
+    const T fau = (T)ldo(ae, i);
     const T fbl = (T)ldo(fB, i + oBl), fbu = (T)ldo(fB, i);
     const T flu = (T)ldo(fL, i);
     const T fll = first ? (T)ldo(fL, hl ? i - strideL : i) : fl_carry;  // upper face of the row before
@@ -1083,7 +1087,10 @@ struct SegLine {
     T off = (T)0, xi = (T)0;
     if constexpr (!ZERO) {
       xi = ldo(x, i);
-      const T xal = ldo(x, i + oAl), xau = ldo(x, i + oAu), xbl = ldo(x, i + oBl), xbu = ldo(x, i + oBu);
+      const T xbl = ldo(x, i + oBl), xbu = ldo(x, i + oBu);
+      const T xw0 = ldo(x, i + oW0), xe0 = ldo(x, i + oE0), fw0 = (T)ldo(ae, i + oW0);
+      const T xws = __shfl_up(xi, 1, 32), xes = __shfl_down(xi, 1, 32), fws = __shfl_up(fau, 1, 32);
+      const T xal = l32 == 0 ? xw0 : xws, xau = l32 == 31 ? xe0 : xes, fal = l32 == 0 ? fw0 : fws;
       const T cAl = hAl ? scl(fal) : (T)0, cAu = hAu ? scl(fau) : (T)0;
       const T cBl = hBl ? scl(fbl) : (T)0, cBu = hBu ? scl(fbu) : (T)0;
       off = cAl * xal + cAu * xau + cBl * xbl + cBu * xbu;
@@ -1173,6 +1180,12 @@ __global__ void __launch_bounds__(1024) k_line_seg(const T* __restrict__ x_in, T
         Ln.oAu = Ln.hAu ? 1 : 0;
         Ln.oBl = Ln.hBl ? -dB : 0;
         Ln.oBu = Ln.hBu ? dB : 0;
+        {
+          const int x0 = (lb % gxb) * LPW, xl = x0 + LPW - 1 < N - 1 ? x0 + LPW - 1 : N - 1;
+          Ln.l32 = lane;
+          Ln.oW0 = (x0 - xx) + (x0 > 0 ? -1 : 0);
+          Ln.oE0 = (xl - xx) + (xl < N - 1 ? 1 : 0);
+        }
         T carry = (T)0;
   #pragma unroll
         for (int k = 0; k < SEG; ++k) {
