@@ -1,0 +1,64 @@
+"""The 256^3 stretched-grid Bezier workload (the benchmark's solve) at full size, checked through
+properties that do not depend on a CPU solve: the mixed and FP64 solves reach relative residual
+1e-8 on the true residual, their pressures agree to the mixed-precision level, and the
+transposed cycle is the transpose of the cycle at this size.  Geometry, operator and hierarchy
+are built on the GPU; about 3 GB of device memory."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = 256
+RATIO = 1.02
+CENTER = (0.5, 0.5, 0.5)
+AXES = (0.25, 0.15, 0.15)
+
+
+@pytest.fixture(scope="module")
+def workload(lsx):
+    widths = [lsx.stretched_axis(N, 1.0, CENTER[q], RATIO) for q in range(3)]
+    ctrl = lsx.bezier_perturb(lsx.bezier_ellipsoid(CENTER, AXES), 0.01, 0)
+    geom = lsx.Geometry(*widths).build("bezier", CENTER, AXES, ctrl=ctrl)
+    return geom
+
+
+@pytest.mark.parametrize("smoother,line_levels", [("line", 2), ("rbgs", 0)])
+def test_full_size_mixed_and_fp64_solves(lsx, workload, smoother, line_levels):
+    geom = workload
+    out = {}
+    for precision in ("mixed", "fp64-vcycle"):
+        S = lsx.solver("fgmres", precision, smoother=smoother, line_levels=line_levels)
+        S.set_geometry(geom, None, None, lsx.mixed_task(1.0, restart=30, max_it=200))
+        diag = S.operator_diagonal()
+        b = lsx.random_divergence_rhs(diag, geom.dims)
+        nb = float(np.linalg.norm(b))
+        S.eps = 1e-8 * nb
+        db, dx = lsx.DeviceVector(geom.n), lsx.DeviceVector(geom.n)
+        db.upload(b)
+        dx.zero()
+        it = S.solve_device(db, dx)
+        x = dx.download()
+        rel = float(np.linalg.norm(b - S.apply_operator(x)) / nb)
+        print(f"smoother={smoother} precision={precision} it={it} true_rel_res={rel:.3e}", flush=True)
+        assert rel <= 1.05e-8 and it < 200
+        out[precision] = (x, diag < 0)
+        S.close()
+    (xm, act), (xd, _) = out["mixed"], out["fp64-vcycle"]
+    diff = float(np.linalg.norm((xm - xd)[act]) / np.linalg.norm(xd[act]))
+    print(f"smoother={smoother} pressure rel diff mixed vs fp64 = {diff:.3e}", flush=True)
+    assert diff <= 1e-6
+
+
+def test_full_size_transposed_cycle(lsx, workload):
+    geom = workload
+    S = lsx.solver("fgmres", "mixed", smoother="line", line_levels=2)
+    S.set_geometry(geom, None, None, lsx.mixed_task(1.0, restart=30, max_it=200))
+    act = S.operator_diagonal() < 0
+    rng = np.random.default_rng(9)
+    x, y = rng.standard_normal(geom.n) * act, rng.standard_normal(geom.n) * act
+    Bx, BTy = S.precondition(x), S.precondition_transpose(y)
+    lhs, rhs = Bx @ y, x @ BTy
+    rel = abs(lhs - rhs) / (np.linalg.norm(Bx) * np.linalg.norm(y))
+    print(f"<Bx,y>={lhs:.12e} <x,B^T y>={rhs:.12e} rel={rel:.3e}", flush=True)
+    assert rel <= 2e-5
+    S.close()
