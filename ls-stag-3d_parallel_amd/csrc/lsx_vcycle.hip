@@ -581,8 +581,8 @@ __global__ void k_convert(const A* __restrict__ a, B* __restrict__ b, long n) {
 // half-sweeps in place: the same formula, the same operand order.  Each array is read about
 // once instead of once per colour.
 constexpr int FX = 64, FY = 16, FZ = 32;
-constexpr int HX = FX + 4, HY = FY + 4;  // x_in staging: tile + 2-cell ring
-constexpr int RX = FX + 2, RY = FY + 2;  // red values: tile + 1-cell ring
+constexpr int HX = FX + 4;  // x_in staging: tile + 2-cell ring (rows: tile rows + 4)
+constexpr int RX = FX + 2;  // red values: tile + 1-cell ring (rows: tile rows + 2)
 
 // per-cell data of one plane: right-hand side, 1/a_p and the six faces (absent ones read the cell
 // itself and are never used)
@@ -631,22 +631,23 @@ __device__ inline T rb_apply(const RbCell<T>& d, T fs, int gx, int gy, int p, in
 // are not waited for
 __device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-template <class T, class FT>
-__global__ void __launch_bounds__(FX* FY) k_rb_fused(const T* __restrict__ x_in, T* __restrict__ x_out,
+template <class T, class FT, int TY>
+__global__ void __launch_bounds__(FX* TY) k_rb_fused(const T* __restrict__ x_in, T* __restrict__ x_out,
                                                       const T* __restrict__ b, const FT* __restrict__ ae,
                                                       const FT* __restrict__ an, const FT* __restrict__ at,
                                                       const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
                                                       int pin1, int zero) {
+  constexpr int HY = TY + 4, RY = TY + 2;
   __shared__ T xs_[3][HY][HX];
   __shared__ T rs_[3][RY][RX];
-  constexpr int NT = FX * FY;
+  constexpr int NT = FX * TY;
   const int tid = threadIdx.x;
   // tiles in XCD-contiguous order (x fastest, then y, then z): neighbouring tiles, whose ring
   // cells a tile reads, run on the same XCD and share its L2 (measured fetch 26 instead of
   // 50 B/cell)
-  const int gxt = (N + FX - 1) / FX, gyt = (M + FY - 1) / FY;
+  const int gxt = (N + FX - 1) / FX, gyt = (M + TY - 1) / TY;
   const int lb = xcd_order(blockIdx.x, gridDim.x);
-  const int X0 = (lb % gxt) * FX, Y0 = ((lb / gxt) % gyt) * FY, Z0 = (lb / (gxt * gyt)) * FZ;
+  const int X0 = (lb % gxt) * FX, Y0 = ((lb / gxt) % gyt) * TY, Z0 = (lb / (gxt * gyt)) * FZ;
   const int Z1 = Z0 + FZ < K ? Z0 + FZ : K;
   const long NM = (long)N * M;
   auto slot = [](int p) { return ((p % 3) + 3) % 3; };
@@ -659,7 +660,7 @@ __global__ void __launch_bounds__(FX* FY) k_rb_fused(const T* __restrict__ x_in,
     cgx[k] = X0 - 1 + rx;
     cgy[k] = Y0 - 1 + ry;
     cin[k] = c < RX * RY && cgx[k] >= 0 && cgx[k] < N && cgy[k] >= 0 && cgy[k] < M;
-    cint[k] = cin[k] && rx >= 1 && rx <= FX && ry >= 1 && ry <= FY;
+    cint[k] = cin[k] && rx >= 1 && rx <= FX && ry >= 1 && ry <= TY;
   }
   int sgx[2], sgy[2];
   bool sin_[2];
@@ -1544,8 +1545,11 @@ void line_pass<float>(const Level& L, int dir, const float* x_in, float* x_out, 
 template <class T, class FT>
 static void rb_sweep_t(const Level& L, const T* x_in, T* x_out, const T* b, int zero, hipStream_t s) {
   const Dims& d = L.d;
-  const dim3 grid(((d.N + FX - 1) / FX) * ((d.M + FY - 1) / FY) * ((d.K + FZ - 1) / FZ));
-  k_rb_fused<T, FT><<<grid, FX * FY, 0, s>>>(x_in, x_out, b, (const FT*)L.f_e, (const FT*)L.f_n, (const FT*)L.f_t,
+  // FP64 cells run 512-thread workgroups (8 rows): at 1024 threads a thread has 128 VGPRs and
+  // the three planes of FP64 cell data spilled
+  constexpr int TY = sizeof(T) == 4 ? FY : FY / 2;
+  const dim3 grid(((d.N + FX - 1) / FX) * ((d.M + TY - 1) / TY) * ((d.K + FZ - 1) / FZ));
+  k_rb_fused<T, FT, TY><<<grid, FX * TY, 0, s>>>(x_in, x_out, b, (const FT*)L.f_e, (const FT*)L.f_n, (const FT*)L.f_t,
                                               (const T*)L.s, (T)L.fscale, d.N, d.M, d.K, L.pin0, L.pin1, zero);
 }
 template <>
