@@ -638,6 +638,8 @@ __global__ void __launch_bounds__(FX* TY) k_rb_fused(const T* __restrict__ x_in,
                                                       const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
                                                       int pin1, int zero) {
   constexpr int HY = TY + 4, RY = TY + 2;
+  // two ring cells and two staging cells per thread cover the tile's rings
+  static_assert(2 * FX * TY >= RX * RY && 2 * FX * TY >= HX * HY, "tile rings exceed two cells per thread");
   __shared__ T xs_[3][HY][HX];
   __shared__ T rs_[3][RY][RX];
   constexpr int NT = FX * TY;
