@@ -132,9 +132,10 @@ def cpu_baseline(sample_n, gpu_n, restart, krylov, body="bezier"):
 
 
 def run_solver(lsx, hip, geom, precision, krylov, restart, steps, warmup, max_it=1000, smooth=(2, 2), graphs=False,
-               smoother="rbgs", line_levels=0):
-    """Set up one solver on the device geometry and time `steps` solves (x0 = 0 each)."""
-    S = lsx.solver(krylov, precision, graphs=graphs, smoother=smoother, line_levels=line_levels) \
+               smoother="rbgs", line_levels=0, levels=0, profile_steps=0):
+    """Set up one solver on the device geometry and time `steps` solves (x0 = 0 each); then
+    `profile_steps` more solves with per-kernel event timing."""
+    S = lsx.solver(krylov, precision, graphs=graphs, smoother=smoother, line_levels=line_levels, levels=levels) \
         if precision != "fp64" else lsx.solver(krylov, precision)
     task = lsx.Task(eps=1.0, MAX_ITER=max_it, FGMRES_RESTART_NUM=restart) if precision == "fp64" else \
         lsx.mixed_task(1.0, restart=restart, max_it=max_it, n_pre=smooth[0], n_post=smooth[1])
@@ -163,8 +164,6 @@ def run_solver(lsx, hip, geom, precision, krylov, restart, steps, warmup, max_it
     for _ in range(warmup):
         step()
     hip.sync()
-    S.kernel_timing(True)
-    hip.sync()
     t0 = time.perf_counter()
     its = []
     for _ in range(steps):
@@ -172,10 +171,18 @@ def run_solver(lsx, hip, geom, precision, krylov, restart, steps, warmup, max_it
         its.append(it.value)
     hip.sync()
     elapsed = time.perf_counter() - t0
-    stats = {k: S.kernel_stats(k) for k in lsx.KERNEL_NAMES}
-    S.kernel_timing(False)
     x = np.zeros(ncell)
     hip.d2h(x, dx)
+    # per-kernel durations from HIP events on the solver stream, in separate solves after the timed
+    # region (the events and their synchronisation stay out of the headline time)
+    stats = {}
+    if profile_steps > 0:
+        S.kernel_timing(True)
+        for _ in range(profile_steps):
+            step()
+        hip.sync()
+        stats = {k: S.kernel_stats(k) for k in lsx.KERNEL_NAMES}
+        S.kernel_timing(False)
     r = b - S.apply_operator(x, 0)
     hip.free(db)
     hip.free(dx)
@@ -228,6 +235,8 @@ def main():
     ap.add_argument("--restart", type=int, default=30)
     ap.add_argument("--body", choices=["bezier", "ellipsoid"], default="bezier")
     ap.add_argument("--smooth", default="2,2", help="V-cycle pre,post red-black sweeps")
+    ap.add_argument("--levels", type=int, default=0,
+                    help="V-cycle levels, the finest included (0 = automatic: halve to 2 cells)")
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--cascade-max-it", type=int, default=200,
                     help="iteration cap of the reference-cascade comparison solve (0 = skip it)")
@@ -267,13 +276,15 @@ def main():
 
     smooth = tuple(int(q) for q in args.smooth.split(","))
     main_run = run_solver(lsx, hip, geom, args.precision, args.krylov, args.restart, args.steps, args.warmup,
-                          smooth=smooth, graphs=args.graphs, smoother=args.smoother, line_levels=args.line_levels)
+                          smooth=smooth, graphs=args.graphs, smoother=args.smoother, line_levels=args.line_levels,
+                          levels=args.levels, profile_steps=max(1, min(3, args.steps)))
     print(f"[bench] {args.precision}: {args.steps} solves in {main_run['elapsed']:.3f} s, its {main_run['its']}",
           file=sys.stderr, flush=True)
     fp64 = None
     if args.precision in ("mixed", "mixed16") and not args.no_fp64_compare:
         fp64 = run_solver(lsx, hip, geom, "fp64-vcycle", args.krylov, args.restart, max(1, args.steps // 2), 1,
-                          smooth=smooth, graphs=args.graphs, smoother=args.smoother, line_levels=args.line_levels)
+                          smooth=smooth, graphs=args.graphs, smoother=args.smoother, line_levels=args.line_levels,
+                          levels=args.levels)
         print(f"[bench] fp64-vcycle: {len(fp64['its'])} solves in {fp64['elapsed']:.3f} s, its {fp64['its']}",
               file=sys.stderr, flush=True)
     fp64_rb = None

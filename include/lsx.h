@@ -57,7 +57,10 @@ typedef struct lsx_options {
   int n_pre;         /* pre-smoothing sweeps (V-cycle only; the cascade has none) */
   double omega;      /* ADLJ damping (DEFAULT_OMEGA_3D = 0.805)               */
   int precision;     /* LSX_PREC_*                                            */
-  int levels;        /* 0 = automatic level count; >0 = cap (V-cycle)        */
+  int levels;        /* 0 = the reference's automatic level count; >= 2 = at most this many
+                        levels, the finest included (V-cycle only: the cascade rejects a cap);
+                        a capped coarsest level of 65..4096 unknowns is solved by its dense
+                        FP64 inverse                                            */
   int z_head_fix;    /* 0 = reproduce the reference z-line head read (default) */
   int ortho;         /* LSX_ORTHO_*: FGMRES orthogonalisation                    */
   int coarse_fuse;   /* V-cycle: 1 = levels of <= 4096 cells in one launch (default), 0 = per level */
@@ -93,13 +96,15 @@ int lsx_destroy(lsx_solver* s);
  * (iterative_solver_interface.h:46-55) with its MultiGridSolver::set
  * (multigrid_solver.h:76-101).  The matrix must have the reference pressure-operator
  * pattern: each row's columns are a subset of {num-N*M, num-N, num-1, num, num+1, num+N,
- * num+N*M}.  mem says whether the three arrays are host or device pointers.  N, M, K must
- * halve to 2 along the smallest dimension with every level even (even-even-even transfers). */
+ * num+N*M}, each at most once.  mem says whether the three arrays are host or device pointers.
+ * N, M, K must halve to 2 along the smallest dimension with every level even (even-even-even
+ * transfers). */
 int lsx_set_csr(lsx_solver* s, int N, int M, int K, const int* rowptr, const int* col, const double* val, long nnz,
                 int mem);
 
 /* New values with the same pattern, then rebuild the hierarchy: replaces
- * rebuildPreconditioner (mg_fgmres.h:35-39, multigrid_solver.h:102-106). */
+ * rebuildPreconditioner (mg_fgmres.h:35-39, multigrid_solver.h:102-106).  Between the two calls
+ * the operator is stale: solve, precondition and operator calls return LSX_ERR_STATE. */
 int lsx_update_values(lsx_solver* s, const double* val, int mem);
 int lsx_rebuild(lsx_solver* s);
 

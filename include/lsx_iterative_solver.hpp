@@ -88,7 +88,7 @@ inline lsx_options mg_options(int krylov, int restart, int max_it, double eps, i
   o.eps = eps;
   o.n_post = n_post;
   o.precision = precision;
-  if (precision == LSX_PREC_MIXED) {
+  if (precision != LSX_PREC_FP64) {  // both mixed modes run the V-cycle
     o.mg = LSX_MG_VCYCLE;
     o.n_pre = 2;
     o.n_post = 2;

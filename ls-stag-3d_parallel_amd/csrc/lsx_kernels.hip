@@ -137,6 +137,7 @@ __global__ void k_csr_to_stencil7(const int* __restrict__ rowptr, const int* __r
     int x, y, z;
     decode(i, N, M, &x, &y, &z);
     double a[7] = {0, 0, 0, 0, 0, 0, 0};
+    unsigned seen = 0;
     for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) {
       const int off = col[j] - i;
       int q = -1;
@@ -149,7 +150,10 @@ __global__ void k_csr_to_stencil7(const int* __restrict__ rowptr, const int* __r
       else if (off == NM && z < K - 1) q = 6;
       if (q < 0) {
         atomicOr(err, 1);
+      } else if (seen & (1u << q)) {  // a repeated column would drop a term
+        atomicOr(err, 2);
       } else {
+        seen |= 1u << q;
         a[q] = val[j];
       }
     }

@@ -43,7 +43,8 @@ constexpr int CGS_MAX = 32;
 int reduction_blocks(long n);
 
 // ---- setup --------------------------------------------------------------------------------
-// CSR (device) -> 7 stencil planes; err_flag set to 1 if a column is not a 7-point neighbour.
+// CSR (device) -> 7 stencil planes; err_flag gets bit 1 if a column is not a 7-point neighbour of
+// its row, bit 2 if a row names a column twice.
 void csr_to_stencil7(const int* rowptr, const int* col, const double* val, Dims d, double* coef7, int* err_flag,
                      hipStream_t s);
 // Galerkin coarse operator A_c = R A_f P (even-even-even transfers), 27 planes on the coarse grid.

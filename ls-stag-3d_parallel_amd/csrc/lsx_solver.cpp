@@ -38,6 +38,9 @@ Solver::Solver(const lsx_options& o, hipStream_t s) : opt(o) {
   if (opt.smoother == LSX_SMOOTH_JACOBI && !(opt.omega > 0.0 && opt.omega <= 1.0))
     throw Error(LSX_ERR_ARG, "Jacobi damping omega must be in (0, 1]");
   if (opt.fused_sweep < 0 || opt.fused_sweep > 2) throw Error(LSX_ERR_ARG, "fused_sweep must be 0, 1 or 2");
+  if (opt.levels < 0 || opt.levels == 1) throw Error(LSX_ERR_ARG, "levels must be 0 (automatic) or >= 2");
+  if (opt.levels != 0 && opt.mg != LSX_MG_VCYCLE)
+    throw Error(LSX_ERR_ARG, "the reference cascade uses the automatic level count (levels = 0)");
   if (s) {
     stream = s;
   } else {
@@ -157,10 +160,17 @@ static int level_count(int N, int M, int K) {
   return num;
 }
 
-static void check_dims(int N, int M, int K) {
+// coarsening steps of the hierarchy: the reference's automatic count, or (V-cycle) fewer when the
+// options cap the number of levels (opt.levels counts the finest level too)
+int Solver::coarsenings(int N, int M, int K) const {
+  const int nmg = level_count(N, M, K);
+  return opt.levels > 0 ? std::min(nmg, opt.levels - 1) : nmg;
+}
+
+void Solver::check_dims(int N, int M, int K) const {
   if (N < 3 || M < 3 || K < 3) throw Error(LSX_ERR_ARG, "grid dimensions must be >= 3");
   if ((long)N * M * K >= (1L << 31)) throw Error(LSX_ERR_ARG, "grid too large for 32-bit row indices");
-  const int nmg = level_count(N, M, K);
+  const int nmg = coarsenings(N, M, K);
   int n = N, m = M, k = K;
   for (int l = 0; l < nmg; ++l) {
     if ((n & 1) || (m & 1) || (k & 1))
@@ -169,11 +179,13 @@ static void check_dims(int N, int M, int K) {
     m /= 2;
     k /= 2;
   }
-  if ((long)n * m * k > 64) throw Error(LSX_ERR_ARG, "coarsest level exceeds 64 unknowns");
+  if (opt.levels == 0 && (long)n * m * k > 64) throw Error(LSX_ERR_ARG, "coarsest level exceeds 64 unknowns");
+  if ((long)n * m * k > vc::DENSE_MAX)
+    throw Error(LSX_ERR_ARG, "the level cap leaves more than 4096 unknowns on the coarsest level");
 }
 
 void Solver::setup_levels(int N, int M, int K) {
-  const int nmg = level_count(N, M, K);
+  const int nmg = coarsenings(N, M, K);
   d0 = Dims{N, M, K};
   levels.clear();
   levels.resize(nmg + 1);
@@ -205,16 +217,22 @@ void Solver::setup_levels(int N, int M, int K) {
   }
   if (!vcyc()) cwork.alloc(d0.n());
   const long nc = levels.back().d.n();
-  lu_scale.alloc(nc);
-  lu_perm.alloc(nc);
-  lu_L.alloc(nc * nc);
-  lu_U.alloc(nc * nc);
-  lu_col.alloc(nc);
+  dense_c.release();
+  dense_r.release();
+  dense_part.release();
+  if (nc <= 64) {
+    lu_scale.alloc(nc);
+    lu_perm.alloc(nc);
+    lu_L.alloc(nc * nc);
+    lu_U.alloc(nc * nc);
+    lu_col.alloc(nc);
+  }
 }
 
 void Solver::set_geometry(const Geometry& g, int pin) {
   check_dims(g.N, g.M, g.K);
   ready = false;
+  values_dirty = false;
   setup_levels(g.N, g.M, g.K);
   src_geo = nullptr;
   g.laplace7(pin, levels[0].coef.p, stream);
@@ -233,6 +251,7 @@ void Solver::set_csr(int N, int M, int K, const int* rp, const int* cl, const do
   if (!rp || !cl || !vl || nnz <= 0) throw Error(LSX_ERR_ARG, "null CSR array or empty matrix");
   if (mem != LSX_MEM_HOST && mem != LSX_MEM_DEVICE) throw Error(LSX_ERR_ARG, "mem must be LSX_MEM_HOST or LSX_MEM_DEVICE");
   ready = false;
+  values_dirty = false;
   const long n = (long)N * M * K;
   if (mem == LSX_MEM_HOST) {
     // rows must be well formed before anything touches the device
@@ -271,13 +290,24 @@ void Solver::set_csr(int N, int M, int K, const int* rp, const int* cl, const do
 
 void Solver::update_values(const double* vl, int mem) {
   if (!ready || !from_csr) throw Error(LSX_ERR_STATE, "update_values needs a matrix set with set_csr");
+  if (!vl) throw Error(LSX_ERR_ARG, "null values");
+  if (mem != LSX_MEM_HOST && mem != LSX_MEM_DEVICE) throw Error(LSX_ERR_ARG, "mem must be LSX_MEM_HOST or LSX_MEM_DEVICE");
   const hipMemcpyKind kind = mem == LSX_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
   LSX_HIP(hipMemcpyAsync(val.p, vl, nnz0 * sizeof(double), kind, stream));
+  LSX_HIP(hipStreamSynchronize(stream));  // the caller may reuse its host array on return
+  values_dirty = true;  // the operator and hierarchy still hold the old values until rebuild()
 }
 
 void Solver::rebuild() {
   if (!ready) throw Error(LSX_ERR_STATE, "rebuild before set_csr");
   build_hierarchy();
+  values_dirty = false;
+}
+
+void Solver::require_ready(const char* what) const {
+  if (!ready) throw Error(LSX_ERR_STATE, std::string(what) + " before the matrix is set");
+  if (values_dirty)
+    throw Error(LSX_ERR_STATE, std::string(what) + " after update_values without rebuild: the operator is stale");
 }
 
 void Solver::build_hierarchy() {
@@ -289,7 +319,8 @@ void Solver::build_hierarchy() {
     int herr = 0;
     LSX_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, stream));
     LSX_HIP(hipStreamSynchronize(stream));
-    if (herr) throw Error(LSX_ERR_ARG, "matrix has a column outside the 7-point neighbourhood of its row");
+    if (herr & 1) throw Error(LSX_ERR_ARG, "matrix has a column outside the 7-point neighbourhood of its row");
+    if (herr & 2) throw Error(LSX_ERR_ARG, "matrix has a row that names the same column twice");
   }
   sym_setup();
   if (vcyc()) {
@@ -368,11 +399,22 @@ void Solver::build_vcycle() {
       vc::pin_sink(dg[nxt].p, coarse_of(fp0, l), coarse_of(fp1, l), sigma, stream);
       cur = nxt;
     }
-    if (l == L) {  // coarsest: FP64 LU of the 7-point operator
-      DevBuf<double> c27;
-      c27.alloc(27 * nl);
-      vc::to27(fe[cur].p, fn[cur].p, ft[cur].p, dg[cur].p, d, pin0, pin1, c27.p, stream);
-      lu_factor(c27.p, d, lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, lu_col.p, stream);
+    if (l == L) {  // coarsest: FP64 LU of the 7-point operator, or its dense inverse (capped levels)
+      if (nl <= 64) {
+        DevBuf<double> c27;
+        c27.alloc(27 * nl);
+        vc::to27(fe[cur].p, fn[cur].p, ft[cur].p, dg[cur].p, d, pin0, pin1, c27.p, stream);
+        lu_factor(c27.p, d, lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, lu_col.p, stream);
+        LSX_HIP(hipStreamSynchronize(stream));
+      } else {
+        DevBuf<double> work;
+        work.alloc(2 * nl * nl + 2 * nl + 1);
+        dense_c.alloc(nl * nl);
+        dense_r.alloc(nl * nl);
+        dense_part.alloc(nl * ((nl + 127) / 128));
+        vc::dense_inverse(fe[cur].p, fn[cur].p, ft[cur].p, dg[cur].p, d, dense_c.p, dense_r.p, work.p, stream);
+        LSX_HIP(hipStreamSynchronize(stream));
+      }
       S.db.alloc(nl);
       S.dx.alloc(nl);
       if (m32) {
@@ -517,7 +559,8 @@ void Solver::vcycle_run(const double* v, T* z) {
   }
   // levels from `lc` down are small enough for one workgroup: a single launch does their cycle
   int lc = L + 1;
-  for (int l = 1; l <= L && opt.coarse_fuse; ++l)
+  const bool fuse_ok = opt.coarse_fuse && (!dense_c.p || vlev[L].L.d.n() <= vc::COARSE_DENSE);
+  for (int l = 1; l <= L && fuse_ok; ++l)
     if (rb_level(l) && vlev[l].L.d.n() <= vc::COARSE_CELLS && L - l + 1 <= vc::COARSE_MAX) {
       lc = l;
       break;
@@ -539,17 +582,9 @@ void Solver::vcycle_run(const double* v, T* z) {
       C.x[q] = vx<T>(vlev[lc + q]);
       C.b[q] = vb<T>(vlev[lc + q]);
     }
-    vc::coarse_cycle<T>(C, opt.n_pre, opt.n_post, lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, stream);
+    vc::coarse_cycle<T>(C, opt.n_pre, opt.n_post, lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, dense_c.p, stream);
   } else {
-    VStore& C = vlev[L];
-    const long nc = C.L.d.n();
-    if constexpr (std::is_same<T, float>::value) {
-      vc::convert<float, double>(C.fb.p, C.db.p, nc, stream);
-      lu_solve(lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, (int)nc, C.db.p, C.dx.p, stream);
-      vc::convert<double, float>(C.dx.p, C.fx.p, nc, stream);
-    } else {
-      lu_solve(lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, (int)nc, C.db.p, C.dx.p, stream);
-    }
+    coarse_solve<T>(false);
   }
   for (int l = top - 1; l >= 0; --l) {
     VStore& S = vlev[l];
@@ -671,23 +706,33 @@ void Solver::vcycle_apply_tr(const double* v, T* z) {
     vc::resid_tr<T>(S.L, x, b, vr<T>(S), stream);
     vc::restrict_pt<T>(vr<T>(S), S.L.d, vb<T>(vlev[l + 1]), stream);
   }
-  {
-    VStore& C = vlev[L];
-    const long nc = C.L.d.n();
-    if constexpr (std::is_same<T, float>::value) {
-      vc::convert<float, double>(C.fb.p, C.db.p, nc, stream);
-      lu_solve(lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, (int)nc, C.db.p, C.dx.p, stream);
-      vc::convert<double, float>(C.dx.p, C.fx.p, nc, stream);
-    } else {
-      lu_solve(lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, (int)nc, C.db.p, C.dx.p, stream);
-    }
-  }
+  coarse_solve<T>(true);
   for (int l = L - 1; l >= 0; --l) {
     VStore& S = vlev[l];
     T* x = l == 0 ? z : vx<T>(S);
     const T* b = l == 0 ? b0 : vb<T>(S);
     vc::prolong_inject_add<T>(vx<T>(vlev[l + 1]), vlev[l + 1].L.d, x, S.L.d, stream);
     smooth<T>(l, x, b, opt.n_pre, false, true);  // transposed pre-smoothing (lines z, y, x)
+  }
+}
+
+// x = A_c^{-1} b on the coarsest V-cycle level (the symmetric coarse operator: its LU, or the
+// dense inverse of a capped hierarchy, transposed for the transposed cycle so that it is the
+// exact transpose of the forward one)
+template <class T>
+void Solver::coarse_solve(bool transposed) {
+  VStore& C = vlev.back();
+  const long nc = C.L.d.n();
+  if (dense_c.p) {
+    vc::dense_apply<T>(transposed ? dense_r.p : dense_c.p, (int)nc, vb<T>(C), vx<T>(C), dense_part.p, stream);
+    return;
+  }
+  if constexpr (std::is_same<T, float>::value) {
+    vc::convert<float, double>(C.fb.p, C.db.p, nc, stream);
+    lu_solve(lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, (int)nc, C.db.p, C.dx.p, stream);
+    vc::convert<double, float>(C.dx.p, C.fx.p, nc, stream);
+  } else {
+    lu_solve(lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, (int)nc, C.db.p, C.dx.p, stream);
   }
 }
 
@@ -1023,7 +1068,7 @@ int Solver::bicgstab(const double* b, double* x) {
 // public operations
 // -----------------------------------------------------------------------------------------------
 void Solver::solve(const double* b, double* x, int mem, int* it_out, double* nr_out) {
-  if (!ready) throw Error(LSX_ERR_STATE, "solve before set_csr");
+  require_ready("solve");
   if (!b || !x) throw Error(LSX_ERR_ARG, "null vector");
   const long n = d0.n();
   const double* db = b;
@@ -1053,7 +1098,7 @@ void Solver::solve(const double* b, double* x, int mem, int* it_out, double* nr_
 }
 
 void Solver::precondition(const double* v, double* z, int mem) {
-  if (!ready) throw Error(LSX_ERR_STATE, "precondition before set_csr");
+  require_ready("precondition");
   const long n = d0.n();
   if (mem == LSX_MEM_HOST) {
     hb.alloc(n);
@@ -1069,7 +1114,7 @@ void Solver::precondition(const double* v, double* z, int mem) {
 }
 
 void Solver::apply_operator(int level, const double* x, double* yv, int mem) {
-  if (!ready) throw Error(LSX_ERR_STATE, "apply_operator before set_csr");
+  require_ready("apply_operator");
   if (level < 0 || level >= (int)levels.size()) throw Error(LSX_ERR_ARG, "level out of range");
   if (vcyc() && level > 0) throw Error(LSX_ERR_STATE, "the V-cycle keeps coarse levels as face coefficients only");
   const MGLevel& L = levels[level];
@@ -1093,7 +1138,7 @@ void Solver::apply_operator(int level, const double* x, double* yv, int mem) {
 }
 
 void Solver::require_transpose() const {
-  if (!ready) throw Error(LSX_ERR_STATE, "adjoint operation before the matrix is set");
+  require_ready("adjoint operation");
   if (!vcyc()) throw Error(LSX_ERR_STATE, "adjoint operations need the V-cycle preconditioner (LSX_MG_VCYCLE)");
   if (!sym_ok)
     throw Error(LSX_ERR_STATE, "adjoint operations need an operator symmetric off its pinned rows");
@@ -1112,7 +1157,7 @@ void Solver::precondition_transpose(const double* v, double* z, int mem) {
 }
 
 void Solver::smooth_fine(const double* b, double* x, int sweeps, int zero_guess, int mem) {
-  if (!ready) throw Error(LSX_ERR_STATE, "smooth before the matrix is set");
+  require_ready("smooth");
   if (!vcyc()) throw Error(LSX_ERR_STATE, "the smoother belongs to the V-cycle (LSX_MG_VCYCLE)");
   if (sweeps < 1) throw Error(LSX_ERR_ARG, "sweeps must be >= 1");
   if (!b || !x) throw Error(LSX_ERR_ARG, "null vector");

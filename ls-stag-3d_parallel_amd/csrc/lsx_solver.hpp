@@ -113,6 +113,8 @@ class Solver {
 
  private:
   void setup_levels(int N, int M, int K);
+  int coarsenings(int N, int M, int K) const;
+  void check_dims(int N, int M, int K) const;
   void build_hierarchy();
   bool from_csr = false;
   const Geometry* src_geo = nullptr;
@@ -135,6 +137,10 @@ class Solver {
   std::vector<MGLevel> levels;  // levels.back() is solved by the dense LU
   DevBuf<double> lu_scale, lu_L, lu_U, lu_col;
   DevBuf<int> lu_perm;
+  // dense inverse of the coarsest level of a capped V-cycle hierarchy (column- and row-major)
+  DevBuf<double> dense_c, dense_r, dense_part;
+  template <class T>
+  void coarse_solve(bool transposed);
   DevBuf<double> cwork;   // correction increment c (fine size, reused on every level)
   DevBuf<double> partial, scalars;
   DevBuf<double> mpartial, cgs_h;  // block Gram-Schmidt partial sums and coefficients
@@ -144,6 +150,8 @@ class Solver {
   DevBuf<double> hb, hx;                              // staging for host inputs
   std::vector<double> H, cs, sn, y;
   bool ready = false;
+  bool values_dirty = false;  // update_values() called, rebuild() not yet
+  void require_ready(const char* what) const;
 
   // fine operator in symmetric four-plane storage (when the operator allows it)
   DevBuf<double> s4;

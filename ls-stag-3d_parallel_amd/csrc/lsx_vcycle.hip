@@ -10,6 +10,7 @@
 
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "lsx_vcycle.hpp"
@@ -510,7 +511,8 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse_cycle(CoarseLevels<T>
                                                                 const double* __restrict__ lu_scale,
                                                                 const int* __restrict__ lu_perm,
                                                                 const double* __restrict__ Lf,
-                                                                const double* __restrict__ Uf) {
+                                                                const double* __restrict__ Uf,
+                                                                const double* __restrict__ dinv) {
   const long tid = threadIdx.x, nth = blockDim.x;
   const int last = C.count - 1;
   for (int q = 0; q < last; ++q) {
@@ -531,7 +533,15 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse_cycle(CoarseLevels<T>
                                C.b[q + 1], tid, nth);
     __syncthreads();
   }
-  if (tid == 0) {
+  if (dinv) {  // dense inverse of the last level, column-major: x_i = sum_j inv[j n + i] b_j
+    const int n = (int)C.L[last].d.n();
+    const T* bb = C.b[last];
+    for (int i = (int)tid; i < n; i += (int)nth) {
+      double acc = 0.0;
+      for (int j = 0; j < n; ++j) acc += dinv[(long)j * n + i] * (double)bb[j];
+      C.x[last][i] = (T)acc;
+    }
+  } else if (tid == 0) {
     const int n = (int)C.L[last].d.n();
     const T* bb = C.b[last];
     double bs[64], yv[64];
@@ -564,6 +574,156 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse_cycle(CoarseLevels<T>
       __syncthreads();
     }
   }
+}
+
+// ---- coarsest level as a dense inverse ---------------------------------------------------------
+// A hierarchy capped above its natural depth ends on a level of up to DENSE_MAX unknowns, too many
+// for the single-thread LU.  Its inverse is formed once at setup by Gauss-Jordan elimination with
+// partial pivoting on the symmetrically scaled matrix S A S (s_i = 1 / sqrt|a_ii|, the LU's
+// scaling) augmented with the identity, W = [S A S | I] (n x 2n, row-major); pivots below
+// 1e-13 max|S A S| are clamped to that size as the LU clamps them.  Empty rows (every face 0:
+// coarse cells inside the body, whose right-hand side is 0) become identity rows.  A cycle then
+// applies the inverse as a dense product in FP64.
+__global__ void k_dense_fill(const double* __restrict__ fe, const double* __restrict__ fn,
+                             const double* __restrict__ ft, const double* __restrict__ dg, int N, int M, int K,
+                             double* __restrict__ sc, double* __restrict__ W) {
+  const long n = (long)N * M * K, NM = (long)N * M, w = 2 * n;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n * w; q += (long)gridDim.x * blockDim.x) {
+    const long i = q / w, j = q - i * w;
+    auto scl = [&](long r) {
+      const double d = fabs(dg[r]);
+      return d > 1e-5 ? 1.0 / sqrt(d) : 1.0;
+    };
+    double v = 0.0;
+    if (j >= n) {
+      v = j - n == i ? 1.0 : 0.0;
+    } else if (j == i) {
+      v = dg[i] == 0.0 ? 1.0 : scl(i) * dg[i] * scl(i);
+    } else {
+      int xi, yi, zi;
+      decode(i, N, M, &xi, &yi, &zi);
+      double a = 0.0;
+      if (j == i + 1 && xi < N - 1) a = fe[i];
+      if (j == i - 1 && xi > 0) a = fe[j];
+      if (j == i + N && yi < M - 1) a = fn[i];
+      if (j == i - N && yi > 0) a = fn[j];
+      if (j == i + NM && zi < K - 1) a = ft[i];
+      if (j == i - NM && zi > 0) a = ft[j];
+      v = a == 0.0 ? 0.0 : scl(i) * a * scl(j);
+    }
+    W[q] = v;
+    if (j == 0) sc[i] = scl(i);
+  }
+}
+
+// max |W| over the left half, into out[0] (one workgroup, setup only)
+__global__ void __launch_bounds__(1024) k_dense_absmax(const double* __restrict__ W, int n, double* out) {
+  __shared__ double lds[16];
+  double m = 0.0;
+  const long w = 2L * n;
+  for (long q = threadIdx.x; q < (long)n * n; q += blockDim.x) m = fmax(m, fabs(W[(q / n) * w + q % n]));
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) r = fmax(r, lds[k]);
+    out[0] = r;
+  }
+}
+
+// pivot step k (one workgroup): the row of largest |W[i][k]|, i >= k (ties: the lowest i), swapped
+// into row k, the row scaled by its pivot, and the multipliers f[i] = W[i][k] (f[k] = 0) saved
+// for the elimination kernel
+__global__ void __launch_bounds__(1024) k_gj_pivot(double* __restrict__ W, int n, int k,
+                                                   const double* __restrict__ amax, double* __restrict__ f) {
+  __shared__ double bv[1024];
+  __shared__ int bi[1024];
+  const long w = 2L * n;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  double best = -1.0;
+  int bidx = k;
+  for (int i = k + tid; i < n; i += nth) {
+    const double v = fabs(W[(long)i * w + k]);
+    if (v > best) {
+      best = v;
+      bidx = i;
+    }
+  }
+  bv[tid] = best;
+  bi[tid] = bidx;
+  __syncthreads();
+  for (int st = nth / 2; st > 0; st >>= 1) {
+    if (tid < st) {
+      const double o = bv[tid + st];
+      const int oi = bi[tid + st];
+      if (o > bv[tid] || (o == bv[tid] && oi < bi[tid])) {
+        bv[tid] = o;
+        bi[tid] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  const int p = bi[0];
+  if (p != k)
+    for (long j = tid; j < w; j += nth) {
+      const double a = W[(long)k * w + j];
+      W[(long)k * w + j] = W[(long)p * w + j];
+      W[(long)p * w + j] = a;
+    }
+  __syncthreads();
+  const double tiny = 1e-13 * amax[0];
+  double piv = W[(long)k * w + k];
+  if (fabs(piv) < tiny) piv = piv < 0.0 ? -tiny : tiny;
+  __syncthreads();  // every thread holds the pivot before the row changes
+  for (long j = tid; j < w; j += nth) W[(long)k * w + j] = j == k ? 1.0 : W[(long)k * w + j] / piv;
+  for (int i = tid; i < n; i += nth) f[i] = i == k ? 0.0 : W[(long)i * w + k];
+}
+
+// elimination step k: W[i][j] -= f[i] W[k][j] for every row i != k (row k is only read)
+__global__ void k_gj_elim(double* __restrict__ W, int n, int k, const double* __restrict__ f) {
+  const long w = 2L * n;
+  const int i = blockIdx.y;
+  const double fi = f[i];
+  if (i == k || fi == 0.0) return;
+  for (long j = blockIdx.x * (long)blockDim.x + threadIdx.x; j < w; j += (long)gridDim.x * blockDim.x)
+    W[(long)i * w + j] -= fi * W[(long)k * w + j];
+}
+
+// inv = S (S A S)^-1 S from the right half of W: column-major copy (inv_c[j n + i] = inv(i, j))
+// for the product, row-major copy (inv_r[i n + j]) for the transposed product
+__global__ void k_dense_finish(const double* __restrict__ W, const double* __restrict__ sc, int n,
+                               double* __restrict__ inv_c, double* __restrict__ inv_r) {
+  const long w = 2L * n;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < (long)n * n; q += (long)gridDim.x * blockDim.x) {
+    const long i = q / n, j = q - i * n;
+    const double v = sc[i] * W[i * w + n + j] * sc[j];
+    inv_r[q] = v;
+    inv_c[j * n + i] = v;
+  }
+}
+
+// x = inv b in column chunks: block (r, c) sums columns [c * DCH, (c + 1) * DCH) of rows
+// [256 r, 256 r + 256) into part[c][i]; k_dense_sum adds the chunks in order
+constexpr int DCH = 128;
+template <class T>
+__global__ void __launch_bounds__(256) k_dense_part(const double* __restrict__ inv, int n, const T* __restrict__ b,
+                                                    double* __restrict__ part) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j0 = blockIdx.y * DCH, j1 = j0 + DCH < n ? j0 + DCH : n;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int j = j0; j < j1; ++j) acc += inv[(long)j * n + i] * (double)b[j];
+  part[(long)blockIdx.y * n + i] = acc;
+}
+
+template <class T>
+__global__ void k_dense_sum(const double* __restrict__ part, int n, int nch, T* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int c = 0; c < nch; ++c) acc += part[(long)c * n + i];
+  x[i] = (T)acc;
 }
 
 template <class A, class B>
@@ -1582,15 +1742,42 @@ template void copy_vec<double>(const double*, double*, long, hipStream_t);
 
 template <class T>
 void coarse_cycle(const CoarseLevels<T>& C, int n_pre, int n_post, const double* lu_scale, const int* lu_perm,
-                  const double* L, const double* U, hipStream_t s) {
+                  const double* L, const double* U, const double* dinv, hipStream_t s) {
   if (C.count < 2 || C.count > COARSE_MAX) return;
   if (C.L[0].ftype == FACE_F16)
-    k_coarse_cycle<T, _Float16><<<1, COARSE_THREADS, 0, s>>>(C, n_pre, n_post, lu_scale, lu_perm, L, U);
+    k_coarse_cycle<T, _Float16><<<1, COARSE_THREADS, 0, s>>>(C, n_pre, n_post, lu_scale, lu_perm, L, U, dinv);
   else if (C.L[0].ftype == FACE_F32)
-    k_coarse_cycle<T, float><<<1, COARSE_THREADS, 0, s>>>(C, n_pre, n_post, lu_scale, lu_perm, L, U);
+    k_coarse_cycle<T, float><<<1, COARSE_THREADS, 0, s>>>(C, n_pre, n_post, lu_scale, lu_perm, L, U, dinv);
   else
-    k_coarse_cycle<T, double><<<1, COARSE_THREADS, 0, s>>>(C, n_pre, n_post, lu_scale, lu_perm, L, U);
+    k_coarse_cycle<T, double><<<1, COARSE_THREADS, 0, s>>>(C, n_pre, n_post, lu_scale, lu_perm, L, U, dinv);
 }
+
+void dense_inverse(const double* fe, const double* fn, const double* ft, const double* dg, Dims d, double* inv_c,
+                   double* inv_r, double* work, hipStream_t s) {
+  const int n = (int)d.n();
+  double* W = work;                    // n x 2n
+  double* sc = W + 2L * n * n;         // n
+  double* f = sc + n;                  // n
+  double* amax = f + n;                // 1
+  const long tot = 2L * n * n;
+  k_dense_fill<<<(unsigned)std::min<long>((tot + 255) / 256, 65536), 256, 0, s>>>(fe, fn, ft, dg, d.N, d.M, d.K, sc, W);
+  k_dense_absmax<<<1, 1024, 0, s>>>(W, n, amax);
+  const dim3 eg((unsigned)((2L * n + 255) / 256), (unsigned)n);
+  for (int k = 0; k < n; ++k) {
+    k_gj_pivot<<<1, 1024, 0, s>>>(W, n, k, amax, f);
+    k_gj_elim<<<eg, 256, 0, s>>>(W, n, k, f);
+  }
+  k_dense_finish<<<(unsigned)std::min<long>(((long)n * n + 255) / 256, 65536), 256, 0, s>>>(W, sc, n, inv_c, inv_r);
+}
+
+template <class T>
+void dense_apply(const double* inv, int n, const T* b, T* x, double* part, hipStream_t s) {
+  const int nch = (n + DCH - 1) / DCH;
+  k_dense_part<T><<<dim3((unsigned)((n + 255) / 256), (unsigned)nch), 256, 0, s>>>(inv, n, b, part);
+  k_dense_sum<T><<<(n + 255) / 256, 256, 0, s>>>(part, n, nch, x);
+}
+template void dense_apply<float>(const double*, int, const float*, float*, double*, hipStream_t);
+template void dense_apply<double>(const double*, int, const double*, double*, double*, hipStream_t);
 
 template <class T>
 void prolong_add(const T* xc, Dims dc, T* xf, Dims df, hipStream_t s) {
@@ -1600,9 +1787,9 @@ void prolong_add(const T* xc, Dims dc, T* xf, Dims df, hipStream_t s) {
 template void prolong_add<float>(const float*, Dims, float*, Dims, hipStream_t);
 template void prolong_add<double>(const double*, Dims, double*, Dims, hipStream_t);
 template void coarse_cycle<float>(const CoarseLevels<float>&, int, int, const double*, const int*, const double*,
-                                  const double*, hipStream_t);
+                                  const double*, const double*, hipStream_t);
 template void coarse_cycle<double>(const CoarseLevels<double>&, int, int, const double*, const int*, const double*,
-                                   const double*, hipStream_t);
+                                   const double*, const double*, hipStream_t);
 
 template <class A, class B>
 void convert(const A* a, B* b, long n, hipStream_t s) {

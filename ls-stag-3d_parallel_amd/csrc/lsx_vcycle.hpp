@@ -85,7 +85,20 @@ void convert(const A* a, B* b, long n, hipStream_t s);
 // them, C.x[0] receives its correction (lu_*: the dense LU factors of the last level)
 template <class T>
 void coarse_cycle(const CoarseLevels<T>& C, int n_pre, int n_post, const double* lu_scale, const int* lu_perm,
-                  const double* L, const double* U, hipStream_t s);
+                  const double* L, const double* U, const double* dinv, hipStream_t s);
+
+// Coarsest level of a capped hierarchy (DENSE_MAX unknowns at most) as a dense FP64 inverse,
+// formed at setup by Gauss-Jordan elimination with partial pivoting on the scaled operator:
+// inv_c column-major (inv_c[j n + i] = inv(i, j)), inv_r row-major; work holds 2 n^2 + 2 n + 1
+// doubles.  dense_apply: x = inv b with inv = inv_c (or inv_r for x = inv^T b); part holds
+// n * ceil(n / 128) doubles.  The single-workgroup coarse cycle applies inv_c itself when the
+// last level has at most COARSE_DENSE cells.
+constexpr int DENSE_MAX = 4096;
+constexpr int COARSE_DENSE = 512;
+void dense_inverse(const double* fe, const double* fn, const double* ft, const double* dg, Dims d, double* inv_c,
+                   double* inv_r, double* work, hipStream_t s);
+template <class T>
+void dense_apply(const double* inv, int n, const T* b, T* x, double* part, hipStream_t s);
 
 // transposed cycle: one colour of a half-sweep on A^T (the finest level's pinned rows are full
 // rows of A^T, other rows drop their couplings to them); r = b - A^T x on the whole level;

@@ -346,7 +346,8 @@ class IterativeSolver:
     krylov = KRYLOV_FGMRES
 
     def __init__(self, precision=PREC_FP64, mg=MG_CASCADE, z_head_fix=False, stream=None, ortho=0, coarse_fuse=True,
-                 graphs=False, smoother="rbgs", fused_sweep=True, line_levels=0):
+                 graphs=False, smoother="rbgs", fused_sweep=True, line_levels=0, levels=0):
+        self.levels_cap = levels
         self.ortho = ortho
         self.coarse_fuse = coarse_fuse
         self.graphs = graphs
