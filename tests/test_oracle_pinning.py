@@ -8,7 +8,8 @@ vectors, and its sources do not build unmodified).
 * coarsest solve: with no smoothing, one multigrid application equals P A_c^{-1} R b with
   numpy.linalg.solve;
 * ADLJ smoothing: one multigrid application with smoothing equals the same cascade written
-  with scipy banded solves of each line's tridiagonal system (no tiny pivots here);
+  with scipy banded solves of each line's tridiagonal system, on an empty box and on cut
+  geometries where the reference's zero-pivot rule (tridiagonal_matrix.h:45-67) decouples rows;
 * FGMRES / BiCGStab: the returned residual estimate equals the true residual (Saad's
   identity for FGMRES), and the solution matches a sparse direct solve.
 """
@@ -230,3 +231,124 @@ def test_fgmres_max_it_and_zero_rhs():
     assert it == 4 and nr > 0
     x, it, nr = O.Solver(A, eps=1e-8).solve(np.zeros(A.n))
     assert it == 0 and nr == 0.0
+
+
+def line_systems(Ad, dims, direction, z_head_quirk=True):
+    """Cells of every line of one direction, in line order, and the cell each line's first
+    right-hand side is read from (the reference z sweep's head quirk)."""
+    N, M, K = dims
+    idx = lambda x, y, z: x + N * (y + M * z)
+    if direction == 0:
+        lines = [[idx(x, y, z) for x in range(N)] for z in range(K) for y in range(M)]
+        return lines, [l[0] for l in lines]
+    if direction == 1:
+        lines = [[idx(x, y, z) for y in range(M)] for z in range(K) for x in range(N)]
+        return lines, [l[0] for l in lines]
+    lines, heads = [], []
+    for y in range(M):
+        for x in range(N):
+            lines.append([idx(x, y, z) for z in range(K)])
+            heads.append(y * N * K + x if z_head_quirk else idx(x, y, 0))
+    return lines, heads
+
+
+def decoupled_rows(T, tiny):
+    """Rows of the tridiagonal T that the reference factorisation decouples: eliminating without
+    pivoting, the pivot of row i is det(T[s:i+1, s:i+1]) / det(T[s:i, s:i]), s the first row after
+    the last decoupled one; a pivot of magnitude below `tiny` is set to 0, which makes x_i = 0 and
+    cuts the line there (tridiagonal_matrix.h:55-65)."""
+    out, s = [], 0
+    for i in range(T.shape[0]):
+        den = np.linalg.det(T[s:i, s:i]) if i > s else 1.0
+        piv = np.linalg.det(T[s:i + 1, s:i + 1]) / den
+        if abs(piv) < tiny:
+            out.append(i)
+            s = i + 1
+    return out
+
+
+def banded_line_solve_decoupled(Ad, dims, direction, rhs, tiny, stats=None):
+    """The line solves of one direction with the decoupled rows written out: such a row is an
+    identity row with right-hand side 0 and its couplings dropped, the rest by solve_banded."""
+    out = np.zeros_like(rhs)
+    lines, heads = line_systems(Ad, dims, direction)
+    for line, head in zip(lines, heads):
+        L = len(line)
+        T = Ad[np.ix_(line, line)]
+        T = np.diag(np.diag(T)) + np.diag(np.diag(T, 1), 1) + np.diag(np.diag(T, -1), -1)
+        rr = rhs[line].copy()
+        rr[0] = rhs[head]
+        dec = decoupled_rows(T, tiny)
+        if stats is not None:
+            stats.extend(line[k] for k in dec)
+        for k in dec:
+            T[k, :] = 0.0
+            T[:, k] = 0.0
+            T[k, k] = 1.0
+            rr[k] = 0.0
+        ab = np.zeros((3, L))
+        ab[1] = np.diag(T)
+        ab[0, 1:] = np.diag(T, 1)
+        ab[2, :-1] = np.diag(T, -1)
+        out[line] = sl.solve_banded((1, 1), ab, rr)
+    return out
+
+
+def cut_with_tiny_cells(n, seed):
+    """synthetic_cut with three cut cells whose six faces are closed to 1e-7: rows with a tiny
+    (not zero) diagonal, which the zero-pivot rule decouples like the solid rows."""
+    g, _ = P.synthetic_cut(n, seed=seed)
+    f = g.fields()
+    N, M, K = g.dims
+    t = f["type"]
+    cut = [i for i in np.nonzero(t == P.CUT_CODE)[0]]
+    picked = cut[::max(1, len(cut) // 3)][:3]
+    thx, thy, thz = f["thx"].copy(), f["thy"].copy(), f["thz"].copy()
+    for i in picked:
+        thx[i] = thx[i - 1] = 1e-7
+        thy[i] = thy[i - N] = 1e-7
+        thz[i] = thz[i - N * M] = 1e-7
+    g.set_fields(thx=thx, thy=thy, thz=thz)
+    return g, g.laplace(pin=True), picked
+
+
+@pytest.mark.parametrize("case", ["sphere16", "cut16_tiny"])
+def test_adlj_cascade_zero_pivots_on_cut_geometry(case):
+    if case == "sphere16":
+        g, A = P.sphere(16)
+        planted = []
+    else:
+        g, A, planted = cut_with_tiny_cells(16, seed=7)
+    omega, n_post = 0.805, 2
+    S = O.Solver(A, eps=1.0, n_post=n_post, omega=omega)
+    lv = S.levels()
+    b = P.random_rhs(g, seed=4, A=A) + 0.1 * np.random.default_rng(5).standard_normal(A.n)
+    mats = [kron_transfers(l.dims) for l in lv[:-1]]
+    r = [b.copy()]
+    for l in range(len(lv) - 1):
+        r.append(mats[l][1] @ r[-1])
+    ch = np.linalg.solve(lv[-1].to_scipy().toarray(), r[-1])
+    fine_decoupled = []
+    for l in range(len(lv) - 2, -1, -1):
+        Ad = lv[l].to_scipy().toarray()
+        # one threshold per level: EQ_TINY2 * max|diagonal|, at least EQ_TINY (ls_stag_real_constants.h:30-31)
+        tiny = max(1e-5 * np.abs(np.diag(Ad)).max(), 1e-12)
+        ch = mats[l][0] @ ch
+        c = ch.copy()
+        for _ in range(n_post):
+            for direction in (2, 1, 0):
+                r[l] = r[l] - Ad @ c
+                stats = fine_decoupled if l == 0 else None
+                c = omega * banded_line_solve_decoupled(Ad, lv[l].dims, direction, r[l], tiny, stats)
+                ch = ch + c
+    z = S.mg_apply(b)
+    types = g.fields()["type"]
+    empty = set(int(i) for i in np.nonzero(A.to_scipy().diagonal() == 0.0)[0])  # cut cells with every face closed
+    dec = set(fine_decoupled)
+    n_solid = int(np.sum(types == 0))
+    print(f"{case}: decoupled fine rows {len(dec)} (solid cells {n_solid}, empty rows {len(empty)}, planted tiny "
+          f"cells {planted}); rel diff {np.linalg.norm(z - ch) / np.linalg.norm(ch):.3e}")
+    assert n_solid > 0 and all(types[i] == 0 or i in planted or i in empty for i in dec)
+    assert {int(i) for i in np.nonzero(types == 0)[0]} <= dec
+    assert set(planted) <= dec
+    assert np.linalg.norm(z - ch) <= 1e-12 * np.linalg.norm(ch)
