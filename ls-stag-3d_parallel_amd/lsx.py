@@ -56,6 +56,7 @@ MG_CASCADE, MG_VCYCLE = 0, 1
 PREC_FP64, PREC_MIXED, PREC_MIXED16 = 0, 1, 2
 ORTHO_AUTO, ORTHO_MGS, ORTHO_CGS, ORTHO_CGS2 = 0, 1, 2, 3
 MEM_HOST, MEM_DEVICE = 0, 1
+OK, ERR_ARG, ERR_HIP, ERR_STATE = 0, -1, -2, -3
 
 
 class LsxError(RuntimeError):

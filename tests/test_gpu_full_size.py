@@ -49,6 +49,40 @@ def test_full_size_mixed_and_fp64_solves(lsx, workload, smoother, line_levels):
     assert diff <= 1e-6
 
 
+@pytest.mark.parametrize("kind,levels", [("bicgstab", 0), ("fgmres", 6), ("bicgstab", 6)])
+def test_full_size_c3_variants(lsx, workload, kind, levels):
+    """C3's other legs: BiCGStab, and the 6-level hierarchy the configuration names (256 -> 8, the
+    coarsest 512 unknowns by its dense inverse inside the one-workgroup coarse cycle), mixed and
+    FP64, each to the 1e-8 true relative residual, the two pressures agreeing."""
+    geom = workload
+    out = {}
+    for precision in ("mixed", "fp64-vcycle"):
+        S = lsx.solver(kind, precision, smoother="line", line_levels=2, levels=levels)
+        S.set_geometry(geom, None, None, lsx.mixed_task(1.0, restart=30, max_it=200))
+        diag = S.operator_diagonal()
+        b = lsx.random_divergence_rhs(diag, geom.dims)
+        nb = float(np.linalg.norm(b))
+        S.eps = 1e-8 * nb
+        db, dx = lsx.DeviceVector(geom.n), lsx.DeviceVector(geom.n)
+        db.upload(b)
+        dx.zero()
+        it = S.solve_device(db, dx)
+        x = dx.download()
+        rel = float(np.linalg.norm(b - S.apply_operator(x)) / nb)
+        print(f"kind={kind} levels={S.num_levels()} precision={precision} it={it} true_rel_res={rel:.3e}",
+              flush=True)
+        # BiCGStab stops on its recursively updated residual, which drifts from the true one
+        assert rel <= (1.05e-8 if kind == "fgmres" else 1.2e-8) and it < 200
+        if levels:
+            assert S.num_levels() == levels
+        out[precision] = (x, diag < 0)
+        S.close()
+    (xm, act), (xd, _) = out["mixed"], out["fp64-vcycle"]
+    diff = float(np.linalg.norm((xm - xd)[act]) / np.linalg.norm(xd[act]))
+    print(f"kind={kind} levels={levels} pressure rel diff mixed vs fp64 = {diff:.3e}", flush=True)
+    assert diff <= 1e-6
+
+
 def test_full_size_transposed_cycle(lsx, workload):
     geom = workload
     S = lsx.solver("fgmres", "mixed", smoother="line", line_levels=2)

@@ -8,9 +8,20 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_projection_steps(lsx):
+@pytest.mark.parametrize("dims,steps", [((96, 64, 64), 3), ((384, 256, 256), 20)], ids=["96x64x64", "c5_384x256x256"])
+@pytest.mark.timeout(900)
+def test_projection_steps(lsx, dims, steps):
+    """The second case is C5 at its size: 384x256x256 cells, sphere r = 0.1, dt = 0.002, 20 steps
+    on one operator and hierarchy with graph-replayed V-cycles, then the adjoint of the drag."""
     import lsx_pkg
-    out = lsx_pkg.projection.run(dims=(96, 64, 64), steps=3, graphs=True, compare_fp64=True, adjoint=True)
+
+    def log(r):
+        print(f"dims={dims} step {r['step']}: it {r['iterations']} ({r['solve_s'] * 1e3:.1f} ms) fp64 it "
+              f"{r['fp64_iterations']} ({r['fp64_solve_s'] * 1e3:.1f} ms) divergence {r['divergence_before']:.3e} -> "
+              f"{r['divergence_after']:.3e} dp diff {r['dp_rel_diff_vs_fp64']:.2e}", flush=True)
+
+    out = lsx_pkg.projection.run(dims=dims, steps=steps, graphs=True, compare_fp64=True, adjoint=True, log=log)
+    assert len(out["records"]) == steps
     for r in out["records"]:
         print({k: (round(v, 6) if isinstance(v, float) and k.endswith("_s") else v) for k, v in r.items()}, flush=True)
         assert r["divergence_after"] <= 1.05e-8 * r["rhs_norm"]
