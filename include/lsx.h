@@ -271,6 +271,16 @@ int lsx_set_geometry(lsx_solver* s, const lsx_geometry* g, int pin);
  * term of LoadsCalculator::calc (loads_calculator.h:97-116).  dJdp (may be NULL) gets dJ/dp. */
 int lsx_geometry_drag(lsx_geometry* g, const double* p, int mem, double* J, double* dJdp);
 
+/* Allocation-free variant of lsx_geometry_drag.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, time-stepping loops
+ * that must not call hipMalloc).  p, J and dJdp (may be NULL) are device pointers; J receives
+ * the same value lsx_geometry_drag returns, in device memory, with no host synchronisation
+ * (ordered on the geometry's stream).  The reduction's partial sums live in a fixed device
+ * array of the library (256 doubles, 2 KiB per GPU, held until the process ends), so the call
+ * allocates nothing; grids above 2^31 - 1 cells are rejected.  Calls that overlap on one GPU
+ * share that array and race. */
+int lsx_geometry_drag_noalloc(lsx_geometry* g, const double* p, double* J, double* dJdp);
+
 /* Shape gradient of an objective at the solution p of A p = b, given lambda = A^{-T} dJ/dp:
  * dJ/dphi per level-set corner (dJdphi: N*M*K values in `mem` space, may be NULL) and, for a
  * Bezier body, dJ/dctrl (288 host doubles, may be NULL).  u == NULL: b does not depend on the

@@ -258,6 +258,11 @@ int lsx_geometry_drag(lsx_geometry* g, const double* p, int mem, double* J, doub
   return guarded([&] { *J = g->impl.drag(p, dJdp, mem); });
 }
 
+int lsx_geometry_drag_noalloc(lsx_geometry* g, const double* p, double* J, double* dJdp) {
+  if (!g || !p || !J) return LSX_ERR_ARG;
+  return guarded([&] { g->impl.drag_noalloc(p, J, dJdp); });
+}
+
 int lsx_geometry_shape_gradient(lsx_geometry* g, const double* p, const double* lambda, int objective,
                                 const double* u, const double* v, const double* w, const double* u_west,
                                 const double* u_out, const lsx_flow_bc* bc, int pin, double h_rel, int mem,

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 
 #include "lsx_geometry.hpp"
 
@@ -1142,6 +1143,50 @@ __global__ void k_bz_reduce(const int* __restrict__ hf, const double* __restrict
 }  // namespace
 
 constexpr int DRAG_BLOCKS = 256;
+
+// ---- allocation-free drag -------------------------------------------------------------------
+// The partial sums of the drag reduction (one per workgroup of the fixed DRAG_BLOCKS-workgroup
+// launch, for any grid size) in a device array of the library instead of caller- or
+// library-allocated memory: 2 KiB per GPU, held until the process ends.  Each GPU has its own
+// copy at its own address; host code uses the address hipGetSymbolAddress returns on that
+// device (kept per device below), never the array's name.  Every call on one GPU shares it.
+constexpr int DRAG_SCRATCH = 256;
+static_assert(DRAG_BLOCKS <= DRAG_SCRATCH, "drag partials exceed the scratch array");
+static __device__ __align__(256) double scratch[DRAG_SCRATCH];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+double* scratch_dev[MAX_DEVICES] = {};
+std::mutex scratch_mu;
+
+// J = sum of the partials in block order, the order of the host sum in drag() (bit-identical)
+__global__ void k_drag_sum(const double* __restrict__ partial, double* __restrict__ J) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double acc = 0.0;
+    for (int b = 0; b < DRAG_BLOCKS; ++b) acc += partial[b];
+    J[0] = acc;
+  }
+}
+}  // namespace
+
+int drag_scratch(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return -1;
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return -2;
+    scratch_dev[dev] = (double*)ptr;
+  }
+  *out = scratch_dev[dev];
+  return 0;
+}
+
+void drag_noalloc(const int* type, const double* tx, const double* dy, const double* dz, int N, int M, int K, Box box,
+                  const double* p, double* dJdp, double* partial, double* J, hipStream_t s) {
+  k_drag<<<DRAG_BLOCKS, 256, 0, s>>>(type, tx, dy, dz, N, M, K, box, p, dJdp, partial);
+  k_drag_sum<<<1, 64, 0, s>>>(partial, J);
+}
 
 double drag(const int* type, const double* tx, const double* dy, const double* dz, int N, int M, int K, Box box,
             const double* p, double* dJdp, double* partial, hipStream_t s) {

@@ -85,6 +85,14 @@ struct RhsArgs {
 // coefficients; partial: 256 doubles of device scratch.  Synchronises the stream.
 double drag(const int* type, const double* tx, const double* dy, const double* dz, int N, int M, int K, Box box,
             const double* p, double* dJdp, double* partial, hipStream_t s);
+// The same J and dJdp with no allocation and no host synchronisation: the partial sums go to
+// `partial` (the address drag_scratch returns for the current device), J (device pointer)
+// receives the sum in the order drag() adds them.
+void drag_noalloc(const int* type, const double* tx, const double* dy, const double* dz, int N, int M, int K, Box box,
+                  const double* p, double* dJdp, double* partial, double* J, hipStream_t s);
+// Address of the library's drag scratch array on the current device (looked up once per device
+// with hipGetSymbolAddress); 0, or a negative value when the lookup fails.
+int drag_scratch(double** out);
 // phi_p = phi_raw + h, phi_m = phi_raw - h at the in-box corners of class cls (x%6 + 6 (y%6) + 36 (z%6))
 void perturb_class(const double* phi_raw, int N, int M, int K, Box box, int cls, double h, double* phi_p,
                    double* phi_m, hipStream_t s);

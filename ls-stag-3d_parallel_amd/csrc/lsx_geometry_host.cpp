@@ -215,6 +215,28 @@ double Geometry::drag(const double* p, double* dJdp, int mem) {
   return J;
 }
 
+static bool device_pointer(const void* ptr) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, ptr) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+void Geometry::drag_noalloc(const double* p, double* J, double* dJdp) {
+  if (!built) throw Error(LSX_ERR_STATE, "geometry not built");
+  if (!p || !J) throw Error(LSX_ERR_ARG, "null pressure or J");
+  if (!device_pointer(p) || !device_pointer(J) || (dJdp && !device_pointer(dJdp)))
+    throw Error(LSX_ERR_ARG, "p, J and dJdp must be device pointers (the call stages nothing)");
+  if (n() > DRAG_NOALLOC_MAX_CELLS)
+    throw Error(LSX_ERR_ARG, "grid exceeds the 2^31 - 1 cells of the allocation-free drag");
+  double* part = nullptr;
+  if (geo::drag_scratch(&part) != 0) throw Error(LSX_ERR_HIP, "hipGetSymbolAddress of the drag scratch failed");
+  geo::drag_noalloc(type.p, tx.p, dy.p, dz.p, N, M, K, box_, p, dJdp, part, J, stream);
+  LSX_HIP(hipGetLastError());
+}
+
 void Geometry::shape_gradient(const double* p, const double* lam, int objective, const geo::RhsArgs* rhs, int pin,
                               double h_rel, double* dJdphi, double* dJdctrl, int mem) {
   if (!built) throw Error(LSX_ERR_STATE, "geometry not built");

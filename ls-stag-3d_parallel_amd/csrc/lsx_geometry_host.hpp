@@ -40,6 +40,10 @@ class Geometry {
 
   // pressure drag J and (dJdp non-null) its gradient; p and dJdp in `mem` space
   double drag(const double* p, double* dJdp, int mem);
+  // the same J (written to the device pointer J) and dJdp with no device allocation and no host
+  // synchronisation; p, J, dJdp device pointers; partial sums in the library's scratch array
+  static constexpr long DRAG_NOALLOC_MAX_CELLS = (1L << 31) - 1;
+  void drag_noalloc(const double* p, double* J, double* dJdp);
   // Shape gradient of J at the solution p with the adjoint lambda = A^{-T} dJ/dp: dF/dphi per
   // level-set corner (dJdphi, `mem` space, may be null) and, for a Bezier body, dJ/dctrl
   // (288 host doubles, may be null).  `rhs` null: the right-hand side does not depend on the

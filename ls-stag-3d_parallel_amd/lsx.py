@@ -104,6 +104,7 @@ def lib():
     L.lsx_apply_operator.argtypes = [vp, C.c_int, vp, vp, C.c_int]
     L.lsx_solve_adjoint.argtypes = [vp, vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.lsx_geometry_drag.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_double), vp]
+    L.lsx_geometry_drag_noalloc.argtypes = [vp, vp, vp, vp]
     L.lsx_geometry_shape_gradient.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, C.POINTER(FlowBc), C.c_int,
                                               C.c_double, C.c_int, vp, vp]
     L.lsx_adjoint_forward.argtypes = [vp, vp, C.c_int, vp, vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_int),
