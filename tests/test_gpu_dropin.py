@@ -246,7 +246,7 @@ def test_cpp_mirror_solves_on_gpu(lsx, tmp_path, precision):
     r = subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-L", LIBDIR,
                         "-llsx", f"-Wl,-rpath,{LIBDIR}", "-o", str(exe)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
-    r = subprocess.run([str(exe), "32", str(len(A1.val)), str(prec), repr(eps)], capture_output=True, text=True,
+    r = subprocess.run([str(exe), "32", str(len(A1.val)), str(prec), f"{float(eps):.17g}"], capture_output=True, text=True,
                        timeout=120, cwd=tmp_path)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0

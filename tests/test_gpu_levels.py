@@ -22,7 +22,8 @@ def fine_faces(A, dims):
     S = A.to_scipy().tocsr()
     n = S.shape[0]
     d = S.diagonal().copy()
-    pins = [i for i in range(n) if S[i, i] == 1.0 and S.getrow(i).nnz == 1]
+    # identity rows (the reference keeps their pattern with explicit zeros: count the values)
+    pins = [i for i in range(n) if S[i, i] == 1.0 and np.count_nonzero(S.getrow(i).data) == 1]
     fe, fn, ft = np.zeros(n), np.zeros(n), np.zeros(n)
     idx = np.arange(n)
     x, y, z = idx % N, (idx // N) % M, idx // (N * M)
