@@ -34,6 +34,7 @@ SOURCES = {
     "lsx_vcycle.hip": ["-ffp-contract=off"],
     "lsx_bezier.cpp": [],
     "lsx_capi.cpp": [],
+    "lsx_state.cpp": [],
 }
 
 
