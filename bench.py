@@ -22,6 +22,7 @@ import ctypes as C
 import importlib.util
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -103,31 +104,102 @@ def build_body_geometry(geom, body, center, axes):
     return geom.build("ellipsoid", center, axes)
 
 
-def cpu_baseline(sample_n, gpu_n, restart, krylov, body="bezier"):
-    """The CPU oracle (the reference's FP64 cascade algorithm) timed on this host, solving the same
-    workload at sample_n^3 to 1e-8, scaled to gpu_n^3 by the cell count at the same iteration
-    count.  At 256^3 the reference algorithm does not reach 1e-8 at all, so this scaled rate is an
-    upper bound on what the CPU path achieves there."""
+def cpu_problem(case, restart, krylov, body):
+    """The oracle's (the reference FP64 cascade algorithm's) problem of one workload: C1 is the 64^3
+    sphere projection step (b = D u* + flux of u* = 1 + U(-0.05, 0.05), v*, w* = U(-0.05, 0.05),
+    seed 5), C2 the 128^3 sphere with a unit Gaussian right-hand side, C3@128 the 256^3 workload's
+    stretched Bezier body at 128^3 (its per-iteration cost scales with the cells)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py as O
     lsx = load_lsx()
-    widths, center, axes = workload(sample_n)
-    g = O.Geometry(*widths)
-    build_body_geometry(g, body, center, axes)
-    A = g.laplace()
-    b = lsx.random_divergence_rhs(A.to_scipy().diagonal(), g.dims)
+    if case in ("C1", "C2"):
+        n = 64 if case == "C1" else 128
+        g = O.Geometry(*([np.full(n, 1.0 / n)] * 3)).build_body("sphere", (0.35, 0.5, 0.5), (0.2,))
+        A = g.laplace()
+        if case == "C1":
+            nu, nv, nw = O.face_shapes(g.dims)
+            rng = np.random.default_rng(5)
+            b = O.pressure_rhs(g, 1.0 + rng.uniform(-0.05, 0.05, nu), rng.uniform(-0.05, 0.05, nv),
+                               rng.uniform(-0.05, 0.05, nw), U_W=1.0)
+        else:
+            b = lsx.random_divergence_rhs(A.to_scipy().diagonal(), g.dims)
+    else:
+        n = 128
+        widths, center, axes = workload(n)
+        g = O.Geometry(*widths)
+        build_body_geometry(g, body, center, axes)
+        A = g.laplace()
+        b = lsx.random_divergence_rhs(A.to_scipy().diagonal(), g.dims)
     eps = 1e-8 * float(np.linalg.norm(b))
-    S = O.Solver(A, kind=krylov, eps=eps, restart=restart, max_it=1000)
+    return O.Solver(A, kind=krylov, eps=eps, restart=restart, max_it=1000), b, n
+
+
+def cpu_worker(case, restart, krylov, body):
+    """One timed oracle solve (setup excluded), printed as one JSON line; run in a child process
+    whose OMP_NUM_THREADS / OMP_PROC_BIND set its threads."""
+    S, b, n = cpu_problem(case, restart, krylov, body)
     t0 = time.perf_counter()
     x, it, nr = S.solve(b)
     t = time.perf_counter() - t0
-    scale = (gpu_n / sample_n) ** 3
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    print(json.dumps({"case": case, "n": n, "it": it, "seconds": t, "norm_r": nr,
+                      "threads": int(os.environ.get("OMP_NUM_THREADS", "0"))}), flush=True)
+
+
+def physical_cores():
+    """Physical cores (distinct socket/core pairs from lscpu) among the CPUs this process may use,
+    and the machine's total."""
+    mine = os.sched_getaffinity(0)
+    pairs, total = set(), set()
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, timeout=30).stdout
+        for line in out.splitlines():
+            if line.startswith("#") or not line.strip():
+                continue
+            cpu, core, sock = (int(q) for q in line.split(",")[:3])
+            total.add((sock, core))
+            if cpu in mine:
+                pairs.add((sock, core))
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return (len(pairs) or len(mine)), (len(total) or len(mine))
+
+
+def cpu_baseline(gpu_n, restart, krylov, body="bezier"):
+    """The CPU oracle (the reference's FP64 cascade algorithm, C++/OpenMP) timed on this host, in
+    child processes pinned with OMP_PROC_BIND=close: C1 (64^3) and C2 (128^3) measured directly on
+    1 thread and on every physical core this process may use; the headline value is the 256^3
+    workload extrapolated from its 128^3 solve (same body and stretching) by the cell count at the
+    same iteration count.  The reference algorithm does not reach 1e-8 at 256^3 at all, so that
+    extrapolated rate is an upper bound on the CPU path there."""
+    phys, phys_total = physical_cores()
+    cap = int(os.environ.get("OMP_NUM_THREADS", phys) or phys)
+    nthreads = max(1, min(phys, cap))
+    runs = {}
+    for case, threads in (("C1", 1), ("C1", nthreads), ("C2", 1), ("C2", nthreads), ("C3@128", nthreads)):
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", case, "--restart",
+                            str(restart), "--krylov", krylov, "--body", body], env=env, capture_output=True,
+                           text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"cpu worker {case} failed: {r.stderr[-400:]}")
+        res = json.loads(r.stdout.strip().splitlines()[-1])
+        runs[f"{case}@{threads}t"] = res
+        print(f"[bench] cpu oracle {case} on {threads} thread(s): {res['it']} its in {res['seconds']:.3f} s",
+              file=sys.stderr, flush=True)
+    c3 = runs[f"C3@128@{nthreads}t"]
+    scale = (gpu_n / c3["n"]) ** 3
+    measured = {k: {"grid": f"{v['n']}^3", "threads": v["threads"], "iterations": v["it"], "seconds": v["seconds"],
+                    "solves_per_s": 1.0 / v["seconds"]} for k, v in runs.items()}
     return {
-        "value": 1.0 / (t * scale), "unit": "solves/s", "cores": cores, "kind": "port",
-        "sample": (f"oracle (reference FP64 cascade) {krylov.upper()} solve at {sample_n}^3 to 1e-8: {it} iterations "
-                   f"in {t:.2f} s on {cores} OpenMP threads, scaled to {gpu_n}^3 by cells x{scale:.0f} at the same "
-                   f"iteration count (an upper bound: the reference algorithm does not reach 1e-8 at {gpu_n}^3)"),
+        "value": 1.0 / (c3["seconds"] * scale), "unit": "solves/s", "cores": nthreads, "kind": "port",
+        "physical_cores_available": phys, "physical_cores_machine": phys_total, "omp_proc_bind": "close",
+        "sample": (f"oracle (reference FP64 cascade, C++/OpenMP) {krylov.upper()}({restart}) solve of the {gpu_n}^3 "
+                   f"workload's body and stretching at 128^3 to 1e-8: {c3['it']} iterations in {c3['seconds']:.2f} s "
+                   f"on {nthreads} threads (one per physical core, OMP_PROC_BIND=close), extrapolated to {gpu_n}^3 "
+                   f"by cells x{scale:.0f} at the same iteration count (an upper bound: the reference algorithm "
+                   f"does not reach 1e-8 at {gpu_n}^3); C1 and C2 measured directly on 1 and {nthreads} threads "
+                   f"in 'measured'"),
+        "measured": measured,
     }
 
 
@@ -240,7 +312,7 @@ def main():
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--cascade-max-it", type=int, default=200,
                     help="iteration cap of the reference-cascade comparison solve (0 = skip it)")
-    ap.add_argument("--cpu-sample-grid", type=int, default=128)
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graphs", action="store_true", help="replay the V-cycles from captured HIP graphs")
     ap.add_argument("--smoother", choices=["rbgs", "line", "jacobi"], default="line",
@@ -252,6 +324,9 @@ def main():
                     help="instead of the solve benchmark: this many time steps of the 384x256x256 sphere "
                          "projection run (one JSON line of per-step results)")
     args = ap.parse_args()
+    if args.cpu_worker:
+        cpu_worker(args.cpu_worker, args.restart, args.krylov, args.body)
+        return
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
         return
@@ -375,7 +450,7 @@ def main():
     }
     if not args.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline(args.cpu_sample_grid, n, args.restart, args.krylov, args.body)
+            out["cpu_baseline"] = cpu_baseline(n, args.restart, args.krylov, args.body)
         except Exception as e:  # the GPU measurement stands on its own
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out), flush=True)

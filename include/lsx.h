@@ -262,6 +262,21 @@ int lsx_geometry_correct(lsx_geometry* g, const double* dp, const double* us, co
  * IterativeSolverInterface::set. */
 int lsx_set_geometry(lsx_solver* s, const lsx_geometry* g, int pin);
 
+/* Set the solver's matrix to the cut-cell pressure operator assembled on the device from the
+ * caller's own mesh fields and build the preconditioner: the stencil-native form of
+ * IterativeSolverInterface::set for a caller that holds the reference's mesh functions
+ * (LSSTAGUniformMesh::buildMeshFunctions, ls_stag_uniform_mesh.h:3968-4062) instead of the CSR.
+ * dx[N], dy[M], dz[K]: cell widths of the tensor grid (m_delta); type: cell type codes
+ * (CellTypes, cut_cells.h:26-168); thx, thy, thz: east, north and top face fractions (m_theta);
+ * vof: cut-cell volumes (baseMesh.VOF; the staggered masses and their inverses are formed from
+ * them, ls_stag_uniform_mesh.h:2804-2826, ls_stag_base_mesh_params.h:77-85); N*M*K values each,
+ * all in `mem` space.  pin as in lsx_set_geometry.  The operator is the one LaplaceDiscreteOperator
+ * ::calcLap (laplace_discrete_operator.h:107-686) builds; given the fields a geometry holds it is
+ * bit-identical to lsx_set_geometry's. */
+int lsx_set_fields(lsx_solver* s, int N, int M, int K, const double* dx, const double* dy, const double* dz,
+                   const int* type, const double* thx, const double* thy, const double* thz, const double* vof, int pin,
+                   int mem);
+
 /* ---- objective and shape gradient ----------------------------------------------------------
  * The reference computes loads but has no adjoint; these entries have no counterpart there. */
 #define LSX_OBJ_NONE 0   /* J depends on the geometry only through p                           */
@@ -309,6 +324,55 @@ int lsx_adjoint_backward(lsx_tape* t, const double* dJdp, int objective, const d
                          const double* w, const double* u_west, const double* u_out, const lsx_flow_bc* bc,
                          double h_rel, int mem, double* dJdb, double* dJdctrl, int* it, double* norm_r);
 int lsx_tape_destroy(lsx_tape* t);
+
+/* ---- state formats ---------------------------------------------------------------------------
+ * Checkpoint of a time-stepped run in the reference's binary layout, replacing
+ * SetSolutions::saveState / load (set_solutions.h:317-339 / 247-269): int32 it, double time, per
+ * immersed body its centre and centre velocity (body_pos / body_vel: 3 doubles per body), per
+ * body the loads F_aer_x, F_aer_y, F_aer_z, moment_aer (loads: 4 doubles per body,
+ * BaseTaskDescription::saveState, base_task_description.h:151-160), then u, v, w, p of the
+ * current solution and of the previous one as raw doubles (Solution::saveState, solution.h:
+ * 76-81): u (N-1)*M*K, v N*(M-1)*K, w N*M*(K-1), p N*M*K values, N, M, K the pressure grid.
+ * Solution arrays in `mem` space; body arrays on the host.  lsx_state_load checks the file size
+ * against the grid and the body count before reading. */
+int lsx_state_save(const char* path, int it, double time, int n_bodies, const double* body_pos, const double* body_vel,
+                   const double* loads, int N, int M, int K, const double* u, const double* v, const double* w,
+                   const double* p, const double* u_prev, const double* v_prev, const double* w_prev,
+                   const double* p_prev, int mem);
+int lsx_state_load(const char* path, int* it, double* time, int n_bodies, double* body_pos, double* body_vel,
+                   double* loads, int N, int M, int K, double* u, double* v, double* w, double* p, double* u_prev,
+                   double* v_prev, double* w_prev, double* p_prev, int mem);
+/* Bytes of such a checkpoint, or a negative status. */
+long lsx_state_bytes(int N, int M, int K, int n_bodies);
+
+/* Wall velocities of the task (BaseTaskDescription U_*, V_*, W_*; the west wall's u is the
+ * inflow profile u_west). */
+typedef struct lsx_walls {
+  double V_W, W_W;
+  double U_E, V_E, W_E;
+  double U_S, V_S, W_S;
+  double U_N, V_N, W_N;
+  double U_B, V_B, W_B;
+  double U_T, V_T, W_T;
+} lsx_walls;
+
+/* Tecplot point file of the solution on the geometry's grid, replacing
+ * SolTecplotPrinter::printField (sol_tecplot_printer.h:254-347): "x y z P SPD_x SPD_y SPD_z SPD"
+ * on the (N+2) x (M+2) x (K+2) points of the cell centres and the walls, face velocities
+ * averaged to the centres (cut cells weighted by their face fractions), in the reference's
+ * point order and number format.  u, v, w, p (and u_west, M*K values, may be NULL) in `mem`
+ * space; outflow: the east boundary is an outflow (flow tasks: the first and last cell planes
+ * are repeated at the bottom and top walls). */
+int lsx_geometry_write_tecplot(const lsx_geometry* g, const char* path, const double* u, const double* v,
+                               const double* w, const double* p, const double* u_west, const lsx_walls* walls,
+                               int outflow, int mem);
+/* The same file from host arrays of a tensor grid: cell centres xc[N], yc[M], zc[K] and widths
+ * dx[N], dy[M], dz[K] (the reference's m_nodes / m_delta), cell type codes (CellTypes,
+ * cut_cells.h:26-168) and east / north / top face fractions (m_theta) of the N*M*K cells. */
+int lsx_write_tecplot(const char* path, int N, int M, int K, const double* xc, const double* yc, const double* zc,
+                      const double* dx, const double* dy, const double* dz, const int* type, const double* thx,
+                      const double* thy, const double* thz, const double* u, const double* v, const double* w,
+                      const double* p, const double* u_west, const lsx_walls* walls, int outflow);
 
 /* Thread-local message of the last failure ("" if none). */
 const char* lsx_last_error(void);

@@ -4,6 +4,7 @@
 #include <exception>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/lsx.h"
 #include "lsx_geometry_host.hpp"
@@ -263,6 +264,58 @@ int lsx_geometry_drag_noalloc(lsx_geometry* g, const double* p, double* J, doubl
   return guarded([&] { g->impl.drag_noalloc(p, J, dJdp); });
 }
 
+int lsx_state_save(const char* path, int it, double time, int n_bodies, const double* body_pos, const double* body_vel,
+                   const double* loads, int N, int M, int K, const double* u, const double* v, const double* w,
+                   const double* p, const double* u_prev, const double* v_prev, const double* w_prev,
+                   const double* p_prev, int mem) {
+  if (mem != LSX_MEM_HOST && mem != LSX_MEM_DEVICE) return LSX_ERR_ARG;
+  const double* cur[4] = {u, v, w, p};
+  const double* prev[4] = {u_prev, v_prev, w_prev, p_prev};
+  return guarded([&] { lsx::state_save(path, it, time, n_bodies, body_pos, body_vel, loads, N, M, K, cur, prev, mem); });
+}
+
+int lsx_state_load(const char* path, int* it, double* time, int n_bodies, double* body_pos, double* body_vel,
+                   double* loads, int N, int M, int K, double* u, double* v, double* w, double* p, double* u_prev,
+                   double* v_prev, double* w_prev, double* p_prev, int mem) {
+  if (mem != LSX_MEM_HOST && mem != LSX_MEM_DEVICE) return LSX_ERR_ARG;
+  double* cur[4] = {u, v, w, p};
+  double* prev[4] = {u_prev, v_prev, w_prev, p_prev};
+  return guarded([&] { lsx::state_load(path, it, time, n_bodies, body_pos, body_vel, loads, N, M, K, cur, prev, mem); });
+}
+
+long lsx_state_bytes(int N, int M, int K, int n_bodies) {
+  long r = LSX_ERR_ARG;
+  if (n_bodies < 0) return LSX_ERR_ARG;
+  const int st = guarded([&] { r = lsx::state_bytes(N, M, K, n_bodies); });
+  return st == LSX_OK ? r : st;
+}
+
+int lsx_geometry_write_tecplot(const lsx_geometry* g, const char* path, const double* u, const double* v,
+                               const double* w, const double* p, const double* u_west, const lsx_walls* walls,
+                               int outflow, int mem) {
+  if (!g || !path || !walls) return LSX_ERR_ARG;
+  if (mem != LSX_MEM_HOST && mem != LSX_MEM_DEVICE) return LSX_ERR_ARG;
+  return guarded([&] { g->impl.write_tecplot(path, u, v, w, p, u_west, *walls, outflow != 0, mem); });
+}
+
+int lsx_write_tecplot(const char* path, int N, int M, int K, const double* xc, const double* yc, const double* zc,
+                      const double* dx, const double* dy, const double* dz, const int* type, const double* thx,
+                      const double* thy, const double* thz, const double* u, const double* v, const double* w,
+                      const double* p, const double* u_west, const lsx_walls* walls, int outflow) {
+  if (!path || !xc || !yc || !zc || !dx || !dy || !dz || !type || !thx || !thy || !thz || !u || !v || !w || !p ||
+      !walls)
+    return LSX_ERR_ARG;
+  if (N < 3 || M < 3 || K < 3 || (long)N * M * K >= (1L << 31)) return LSX_ERR_ARG;
+  return guarded([&] {
+    const long n = (long)N * M * K;
+    auto vec = [](const double* a, long len) { return std::vector<double>(a, a + len); };
+    lsx::write_tecplot(path, N, M, K, vec(xc, N), vec(yc, M), vec(zc, K), vec(dx, N), vec(dy, M), vec(dz, K),
+                       std::vector<int>(type, type + n), vec(thx, n), vec(thy, n), vec(thz, n),
+                       vec(u, (long)(N - 1) * M * K), vec(v, (long)N * (M - 1) * K), vec(w, (long)N * M * (K - 1)),
+                       vec(p, n), u_west ? vec(u_west, (long)M * K) : std::vector<double>(), *walls, outflow != 0);
+  });
+}
+
 int lsx_geometry_shape_gradient(lsx_geometry* g, const double* p, const double* lambda, int objective,
                                 const double* u, const double* v, const double* w, const double* u_west,
                                 const double* u_out, const lsx_flow_bc* bc, int pin, double h_rel, int mem,
@@ -336,6 +389,13 @@ int lsx_adjoint_backward(lsx_tape* t, const double* dJdp, int objective, const d
 
 int lsx_tape_destroy(lsx_tape* t) {
   return guarded([&] { delete t; });
+}
+
+int lsx_set_fields(lsx_solver* s, int N, int M, int K, const double* dx, const double* dy, const double* dz,
+                   const int* type, const double* thx, const double* thy, const double* thz, const double* vof, int pin,
+                   int mem) {
+  if (!s) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.set_fields(N, M, K, dx, dy, dz, type, thx, thy, thz, vof, pin, mem); });
 }
 
 int lsx_set_geometry(lsx_solver* s, const lsx_geometry* g, int pin) {

@@ -215,6 +215,28 @@ double Geometry::drag(const double* p, double* dJdp, int mem) {
   return J;
 }
 
+void Geometry::write_tecplot(const char* path, const double* u, const double* v, const double* w, const double* p,
+                            const double* u_west, const lsx_walls& walls, bool outflow, int mem) const {
+  if (!built) throw Error(LSX_ERR_STATE, "geometry not built");
+  if (!path || !u || !v || !w || !p) throw Error(LSX_ERR_ARG, "null argument");
+  const long nn = n(), nu = (long)(N - 1) * M * K, nv = (long)N * (M - 1) * K, nw = (long)N * M * (K - 1);
+  auto get = [&](const double* a, long len) {
+    std::vector<double> h(len);
+    if (mem == LSX_MEM_DEVICE)
+      LSX_HIP(hipMemcpy(h.data(), a, len * sizeof(double), hipMemcpyDeviceToHost));
+    else
+      std::copy(a, a + len, h.begin());
+    return h;
+  };
+  std::vector<int> t(nn);
+  std::vector<double> thx(nn), thy(nn), thz(nn);
+  fields(t.data(), thx.data(), thy.data(), thz.data(), nullptr, nullptr);
+  const std::vector<double> hu = get(u, nu), hv = get(v, nv), hw = get(w, nw), hp = get(p, nn);
+  const std::vector<double> uw = u_west ? get(u_west, (long)M * K) : std::vector<double>();
+  lsx::write_tecplot(path, N, M, K, centres(hdx, x0), centres(hdy, y0), centres(hdz, z0), hdx, hdy, hdz, t, thx, thy,
+                     thz, hu, hv, hw, hp, uw, walls, outflow);
+}
+
 static bool device_pointer(const void* ptr) {
   hipPointerAttribute_t a;
   if (hipPointerGetAttributes(&a, ptr) != hipSuccess) {

@@ -17,6 +17,22 @@ struct FlowStage {
   geo::RhsArgs args{};
 };
 
+// Checkpoint in the reference's binary layout (lsx_state.cpp); cur / prev: u, v, w, p in `mem`
+// space.  state_bytes: the file size for a grid and body count.
+long state_bytes(int N, int M, int K, int n_bodies);
+void state_save(const char* path, int it, double time, int n_bodies, const double* body_pos, const double* body_vel,
+                const double* loads, int N, int M, int K, const double* const cur[4], const double* const prev[4],
+                int mem);
+void state_load(const char* path, int* it, double* time, int n_bodies, double* body_pos, double* body_vel,
+                double* loads, int N, int M, int K, double* const cur[4], double* const prev[4], int mem);
+// Tecplot point file of the solution (lsx_state.cpp), host arrays
+void write_tecplot(const char* path, int N, int M, int K, const std::vector<double>& xc, const std::vector<double>& yc,
+                   const std::vector<double>& zc, const std::vector<double>& wx, const std::vector<double>& wy,
+                   const std::vector<double>& wz, const std::vector<int>& type, const std::vector<double>& thx,
+                   const std::vector<double>& thy, const std::vector<double>& thz, const std::vector<double>& u,
+                   const std::vector<double>& v, const std::vector<double>& w, const std::vector<double>& p,
+                   const std::vector<double>& uw, const lsx_walls& bc, bool outflow);
+
 class Geometry {
  public:
   // stage u*, v*, w*, u_west, u_out (`mem` space) for rhs-dependent gradients
@@ -51,6 +67,10 @@ class Geometry {
   // step as a fraction of the smallest cell width.
   void shape_gradient(const double* p, const double* lam, int objective, const geo::RhsArgs* rhs, int pin, double h_rel,
                       double* dJdphi, double* dJdctrl, int mem);
+  // Tecplot point file of u, v, w, p (`mem` space) on this grid; u_west: M*K inflow velocities
+  // or null
+  void write_tecplot(const char* path, const double* u, const double* v, const double* w, const double* p,
+                     const double* u_west, const lsx_walls& walls, bool outflow, int mem) const;
   hipStream_t get_stream() const { return stream; }
   long version() const { return build_count; }
 

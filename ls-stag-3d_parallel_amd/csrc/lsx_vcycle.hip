@@ -1274,8 +1274,11 @@ __global__ void __launch_bounds__(1024) k_line_seg(const T* __restrict__ x_in, T
                                                    int pin1, int S) {
   #pragma clang fp contract(fast)  // line solves: no bitwise twin to match
   extern __shared__ unsigned char lds_raw[];
-  T* ifc = reinterpret_cast<T*>(lds_raw);  // [8][S][LPW]: a d c r of the reduced rows, G H Q of first cells, x_E
-  auto IF = [&](int q, int j, int l) -> T& { return ifc[((long)q * S + j) * LPW + l]; };
+  // [8][S][LPW + 1]: a d c r of the reduced rows, G H Q of first cells, x_E; rows padded by one
+  // element so that both the (segment, line) and the (line, segment) thread mappings read
+  // consecutive banks
+  T* ifc = reinterpret_cast<T*>(lds_raw);
+  auto IF = [&](int q, int j, int l) -> T& { return ifc[((long)q * S + j) * (LPW + 1) + l]; };
   const long NM = (long)N * M;
   const int lane = threadIdx.x % LPW, seg = threadIdx.x / LPW;
   // (column, plane) pairs in XCD-contiguous order, columns fastest: the halo columns and
@@ -1398,40 +1401,83 @@ __global__ void __launch_bounds__(1024) k_line_seg(const T* __restrict__ x_in, T
     eliminate(std::true_type{});
   else
     eliminate(std::false_type{});
-  IF(4, seg, lane) = Gf;
-  IF(5, seg, lane) = Hf;
-  IF(6, seg, lane) = Qf;
-  __syncthreads();
-  const bool nxt = seg + 1 < S;
-  const T G1 = nxt ? IF(4, seg + 1, lane) : (T)0;
-  const T H1 = nxt ? IF(5, seg + 1, lane) : (T)0;
-  const T Q1 = nxt ? IF(6, seg + 1, lane) : (T)0;
-  // row of the last cell: lo x_before + dg x_E + up x_first(next segment)
-  T a = le * Hb;
-  T d = de + le * Qb + ue * H1;
-  T c = ue * Q1;
-  T r = re - le * Gb - ue * G1;
-  for (int st = 1; st < S; st <<= 1) {  // S is uniform over the workgroup
-    IF(0, seg, lane) = a;
-    IF(1, seg, lane) = d;
-    IF(2, seg, lane) = c;
-    IF(3, seg, lane) = r;
+  if ((S & (S - 1)) == 0) {
+    // S a power of two (<= 32): the reduced systems are solved inside wavefronts.  Each thread
+    // posts its partial last-cell row and its first cell; after one barrier thread t takes
+    // segment t % S of line t / S, so the S rows of a line sit in S adjacent lanes of one
+    // wavefront and the cyclic reduction runs on cross-lane shuffles with no further barrier.
+    IF(0, seg, lane) = le * Hb;         // a
+    IF(1, seg, lane) = de + le * Qb;    // d without the next segment's term
+    IF(2, seg, lane) = re - le * Gb;    // r without the next segment's term
+    IF(3, seg, lane) = ue;
+    IF(4, seg, lane) = Gf;
+    IF(5, seg, lane) = Hf;
+    IF(6, seg, lane) = Qf;
     __syncthreads();
-    const bool okm = seg - st >= 0, okp = seg + st < S;
-    const T am = okm ? IF(0, seg - st, lane) : (T)0, dm = okm ? IF(1, seg - st, lane) : (T)1;
-    const T cm = okm ? IF(2, seg - st, lane) : (T)0, rm = okm ? IF(3, seg - st, lane) : (T)0;
-    const T ap = okp ? IF(0, seg + st, lane) : (T)0, dp = okp ? IF(1, seg + st, lane) : (T)1;
-    const T cq = okp ? IF(2, seg + st, lane) : (T)0, rp = okp ? IF(3, seg + st, lane) : (T)0;
-    const T al = -a * recip0(dm), ga = -c * recip0(dp);
-    a = al * am;
-    c = ga * cq;
-    d = d + al * cm + ga * ap;
-    r = r + al * rm + ga * rp;
+    const int tl = (int)threadIdx.x / S, ts = (int)threadIdx.x % S;  // line, segment
+    const int w64 = (int)threadIdx.x & 63;                         // lane in the wavefront
+    const bool nx = ts + 1 < S;
+    const T u0 = IF(3, ts, tl);
+    const T G1 = nx ? IF(4, ts + 1, tl) : (T)0;
+    const T H1 = nx ? IF(5, ts + 1, tl) : (T)0;
+    const T Q1 = nx ? IF(6, ts + 1, tl) : (T)0;
+    // row of the last cell: lo x_before + dg x_E + up x_first(next segment)
+    T a = IF(0, ts, tl);
+    T d = IF(1, ts, tl) + u0 * H1;
+    T c = u0 * Q1;
+    T r = IF(2, ts, tl) - u0 * G1;
+    for (int st = 1; st < S; st <<= 1) {
+      const bool okm = ts - st >= 0, okp = ts + st < S;
+      const int sm = okm ? w64 - st : w64, sp = okp ? w64 + st : w64;
+      const T am0 = __shfl(a, sm, 64), dm0 = __shfl(d, sm, 64), cm0 = __shfl(c, sm, 64), rm0 = __shfl(r, sm, 64);
+      const T ap0 = __shfl(a, sp, 64), dp0 = __shfl(d, sp, 64), cp0 = __shfl(c, sp, 64), rp0 = __shfl(r, sp, 64);
+      const T am = okm ? am0 : (T)0, dm = okm ? dm0 : (T)1, cm = okm ? cm0 : (T)0, rm = okm ? rm0 : (T)0;
+      const T ap = okp ? ap0 : (T)0, dp = okp ? dp0 : (T)1, cq = okp ? cp0 : (T)0, rp = okp ? rp0 : (T)0;
+      const T al = -a * recip0(dm), ga = -c * recip0(dp);
+      a = al * am;
+      c = ga * cq;
+      d = d + al * cm + ga * ap;
+      r = r + al * rm + ga * rp;
+    }
+    IF(7, ts, tl) = r * recip0(d);
+    __syncthreads();
+  } else {
+    // general S: cyclic reduction across the wavefronts through LDS, one barrier pair per step
+    IF(4, seg, lane) = Gf;
+    IF(5, seg, lane) = Hf;
+    IF(6, seg, lane) = Qf;
+    __syncthreads();
+    const bool nxt = seg + 1 < S;
+    const T G1 = nxt ? IF(4, seg + 1, lane) : (T)0;
+    const T H1 = nxt ? IF(5, seg + 1, lane) : (T)0;
+    const T Q1 = nxt ? IF(6, seg + 1, lane) : (T)0;
+    // row of the last cell: lo x_before + dg x_E + up x_first(next segment)
+    T a = le * Hb;
+    T d = de + le * Qb + ue * H1;
+    T c = ue * Q1;
+    T r = re - le * Gb - ue * G1;
+    for (int st = 1; st < S; st <<= 1) {  // S is uniform over the workgroup
+      IF(0, seg, lane) = a;
+      IF(1, seg, lane) = d;
+      IF(2, seg, lane) = c;
+      IF(3, seg, lane) = r;
+      __syncthreads();
+      const bool okm = seg - st >= 0, okp = seg + st < S;
+      const T am = okm ? IF(0, seg - st, lane) : (T)0, dm = okm ? IF(1, seg - st, lane) : (T)1;
+      const T cm = okm ? IF(2, seg - st, lane) : (T)0, rm = okm ? IF(3, seg - st, lane) : (T)0;
+      const T ap = okp ? IF(0, seg + st, lane) : (T)0, dp = okp ? IF(1, seg + st, lane) : (T)1;
+      const T cq = okp ? IF(2, seg + st, lane) : (T)0, rp = okp ? IF(3, seg + st, lane) : (T)0;
+      const T al = -a * recip0(dm), ga = -c * recip0(dp);
+      a = al * am;
+      c = ga * cq;
+      d = d + al * cm + ga * ap;
+      r = r + al * rm + ga * rp;
+      __syncthreads();
+    }
+    IF(7, seg, lane) = r * recip0(d);
     __syncthreads();
   }
-  const T xe = r * recip0(d);
-  IF(7, seg, lane) = xe;
-  __syncthreads();
+  const T xe = IF(7, seg, lane);
   const T xl = seg > 0 ? IF(7, seg - 1, lane) : (T)0;
   if (live) {
 #pragma unroll
@@ -1662,7 +1708,7 @@ static void line_pass_z(const Level& L, int dir, const T* x_in, T* x_out, const 
     const int len = dir == 1 ? d.M : d.K;
     const int S = (len + SEG - 1) / SEG;
     const dim3 grid(((d.N + LPW - 1) / LPW) * (dir == 1 ? d.K : d.M));
-    const size_t lds = 8 * (size_t)S * LPW * sizeof(T);
+    const size_t lds = 8 * (size_t)S * (LPW + 1) * sizeof(T);
 #define LSX_LINE_SEG(DIR, TR)                                                                                  \
   k_line_seg<T, FT, DIR, ZERO, TR><<<grid, LPW * S, lds, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, \
                                                               L.pin0, L.pin1, S)

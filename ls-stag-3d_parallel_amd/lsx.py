@@ -105,6 +105,14 @@ def lib():
     L.lsx_solve_adjoint.argtypes = [vp, vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.lsx_geometry_drag.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_double), vp]
     L.lsx_geometry_drag_noalloc.argtypes = [vp, vp, vp, vp]
+    L.lsx_state_save.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int,
+                                 vp, vp, vp, vp, vp, vp, vp, vp, C.c_int]
+    L.lsx_state_load.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int, vp, vp, vp,
+                                 C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int]
+    L.lsx_state_bytes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    L.lsx_state_bytes.restype = C.c_long
+    L.lsx_geometry_write_tecplot.argtypes = [vp, C.c_char_p, vp, vp, vp, vp, vp, C.POINTER(Walls), C.c_int, C.c_int]
+    L.lsx_write_tecplot.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int] + [vp] * 15 + [C.POINTER(Walls), C.c_int]
     L.lsx_geometry_shape_gradient.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, C.POINTER(FlowBc), C.c_int,
                                               C.c_double, C.c_int, vp, vp]
     L.lsx_adjoint_forward.argtypes = [vp, vp, C.c_int, vp, vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_int),
@@ -125,6 +133,7 @@ def lib():
     L.lsx_geometry_build.argtypes = [vp, C.POINTER(Body), C.c_int, C.c_int, C.c_int, C.c_int]
     L.lsx_geometry_fields.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.lsx_set_geometry.argtypes = [vp, vp, C.c_int]
+    L.lsx_set_fields.argtypes = [vp, C.c_int, C.c_int, C.c_int] + [vp] * 8 + [C.c_int, C.c_int]
     L.lsx_geometry_rhs.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(FlowBc), vp, C.c_int]
     L.lsx_geometry_correct.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_int]
     L.lsx_operator_diagonal.argtypes = [vp, vp]
@@ -183,6 +192,91 @@ OBJ_DRAG_X = 1
 class FlowBc(C.Structure):
     _fields_ = [("U_E", C.c_double), ("V_S", C.c_double), ("V_N", C.c_double), ("W_B", C.c_double),
                 ("W_T", C.c_double), ("outflow", C.c_int), ("pin", C.c_int)]
+
+
+WALL_FIELDS = ("V_W", "W_W", "U_E", "V_E", "W_E", "U_S", "V_S", "W_S", "U_N", "V_N", "W_N", "U_B", "V_B", "W_B",
+               "U_T", "V_T", "W_T")
+
+
+class Walls(C.Structure):
+    """Wall velocities of the task (reference BaseTaskDescription U_*, V_*, W_*)."""
+    _fields_ = [(n, C.c_double) for n in WALL_FIELDS]
+
+
+def walls(**kw):
+    """Walls with the given velocities, 0 elsewhere (e.g. walls(U_E=1.0))."""
+    bad = set(kw) - set(WALL_FIELDS)
+    if bad:
+        raise LsxError(f"unknown wall velocities {sorted(bad)}")
+    return Walls(*[float(kw.get(n, 0.0)) for n in WALL_FIELDS])
+
+
+def face_sizes(N, M, K):
+    """Sizes of u (x faces), v (y faces), w (z faces) and p on an N x M x K pressure grid."""
+    return (N - 1) * M * K, N * (M - 1) * K, N * M * (K - 1), N * M * K
+
+
+def state_bytes(N, M, K, n_bodies=0):
+    r = lib().lsx_state_bytes(N, M, K, n_bodies)
+    if r < 0:
+        _check(r)
+    return r
+
+
+def state_save(path, it, time, dims, cur, prev, bodies=None):
+    """Checkpoint in the reference's binary layout (SetSolutions::saveState): cur and prev are
+    (u, v, w, p) host arrays; bodies = (pos (nb, 3), vel (nb, 3), loads (nb, 4)) or None."""
+    N, M, K = dims
+    sizes = face_sizes(N, M, K)
+    arr = [np.ascontiguousarray(a, np.float64) for a in list(cur) + list(prev)]
+    for a, n in zip(arr, sizes * 2):
+        if a.size != n:
+            raise LsxError("solution array of the wrong size")
+    nb = 0
+    bp = bv = bl = None
+    if bodies is not None:
+        bp, bv, bl = (np.ascontiguousarray(a, np.float64) for a in bodies)
+        nb = bp.size // 3
+        if bp.size != 3 * nb or bv.size != 3 * nb or bl.size != 4 * nb:
+            raise LsxError("body arrays of the wrong size")
+    _check(lib().lsx_state_save(os.fsencode(path), int(it), float(time), nb,
+                                None if bp is None else _ptr(bp), None if bv is None else _ptr(bv),
+                                None if bl is None else _ptr(bl), N, M, K, *[_ptr(a) for a in arr], MEM_HOST))
+
+
+def state_load(path, dims, n_bodies=0):
+    """Read a checkpoint written by state_save (or the reference): returns a dict with it, time,
+    cur and prev (u, v, w, p) and the body arrays."""
+    N, M, K = dims
+    sizes = face_sizes(N, M, K)
+    arr = [np.zeros(n) for n in sizes * 2]
+    bp, bv, bl = np.zeros(3 * n_bodies), np.zeros(3 * n_bodies), np.zeros(4 * n_bodies)
+    it = C.c_int(0)
+    t = C.c_double(0.0)
+    _check(lib().lsx_state_load(os.fsencode(path), C.byref(it), C.byref(t), int(n_bodies),
+                                _ptr(bp) if n_bodies else None, _ptr(bv) if n_bodies else None,
+                                _ptr(bl) if n_bodies else None, N, M, K, *[_ptr(a) for a in arr], MEM_HOST))
+    return dict(it=it.value, time=t.value, cur=tuple(arr[:4]), prev=tuple(arr[4:]), body_pos=bp.reshape(-1, 3),
+                body_vel=bv.reshape(-1, 3), loads=bl.reshape(-1, 4))
+
+
+def write_tecplot(path, dims, centres, widths, fields, u, v, w, p, u_west=None, walls_=None, outflow=True):
+    """Tecplot point file (reference SolTecplotPrinter) from host arrays: centres / widths are
+    per-axis tuples, fields a dict with type, thx, thy, thz (as Geometry.fields returns)."""
+    N, M, K = dims
+    c = [np.ascontiguousarray(a, np.float64) for a in centres]
+    d = [np.ascontiguousarray(a, np.float64) for a in widths]
+    t = np.ascontiguousarray(fields["type"], np.int32)
+    th = [np.ascontiguousarray(fields[k], np.float64) for k in ("thx", "thy", "thz")]
+    f = [np.ascontiguousarray(a, np.float64) for a in (u, v, w, p)]
+    for a, n in zip(f, face_sizes(N, M, K)):
+        if a.size != n:
+            raise LsxError("field of the wrong size")
+    uw = None if u_west is None else np.ascontiguousarray(np.broadcast_to(np.asarray(u_west, np.float64), (M * K,)))
+    wl = walls_ if walls_ is not None else walls()
+    _check(lib().lsx_write_tecplot(os.fsencode(path), N, M, K, *[_ptr(a) for a in c + d], _ptr(t),
+                                   *[_ptr(a) for a in th + f], None if uw is None else _ptr(uw), C.byref(wl),
+                                   1 if outflow else 0))
 
 
 class Geometry:
@@ -309,6 +403,20 @@ class Geometry:
                                           None if p is None else _ptr(p), coef, MEM_HOST))
         return u, v, w
 
+    def write_tecplot(self, path, u, v, w, p, u_west=None, walls_=None, outflow=True):
+        """Tecplot point file of the solution on this grid (reference SolTecplotPrinter::printField)."""
+        N, M, K = self.dims
+        f = [np.ascontiguousarray(a, np.float64) for a in (u, v, w, p)]
+        for a, n in zip(f, face_sizes(N, M, K)):
+            if a.size != n:
+                raise LsxError("field of the wrong size")
+        uw = None if u_west is None else np.ascontiguousarray(np.broadcast_to(np.asarray(u_west, np.float64),
+                                                                              (M * K,)))
+        wl = walls_ if walls_ is not None else walls()
+        _check(lib().lsx_geometry_write_tecplot(self.h, os.fsencode(path), *[_ptr(a) for a in f],
+                                                None if uw is None else _ptr(uw), C.byref(wl), 1 if outflow else 0,
+                                                MEM_HOST))
+
     def close(self):
         if self.h:
             lib().lsx_geometry_destroy(self.h)
@@ -407,6 +515,28 @@ class IterativeSolver:
 
     def set_geometry(self, geometry, rhs, sol, task, pin=True):
         """Like set(), with the cut-cell operator built on the device from `geometry`."""
+        self._make(task)
+        _check(lib().lsx_set_geometry(self.h, geometry.h, 1 if pin else 0))
+        self._rhs = rhs
+        self._sol = sol
+
+    def set_fields(self, dims, widths, fields, rhs, sol, task, pin=True):
+        """Like set(), with the cut-cell operator assembled on the device from the caller's mesh
+        fields: widths (dx, dy, dz) and a dict with type, thx, thy, thz, vof (N*M*K each)."""
+        N, M, K = dims
+        n = N * M * K
+        w = [np.ascontiguousarray(a, np.float64) for a in widths]
+        t = np.ascontiguousarray(fields["type"], np.int32)
+        f = [np.ascontiguousarray(fields[k], np.float64) for k in ("thx", "thy", "thz", "vof")]
+        if [a.size for a in w] != [N, M, K] or t.size != n or any(a.size != n for a in f):
+            raise LsxError("mesh fields do not match the grid")
+        self._make(task)
+        _check(lib().lsx_set_fields(self.h, N, M, K, *[_ptr(a) for a in w], _ptr(t), *[_ptr(a) for a in f],
+                                    1 if pin else 0, MEM_HOST))
+        self._rhs = rhs
+        self._sol = sol
+
+    def _make(self, task):
         self.close()
         opt = Options()
         lib().lsx_options_default(C.byref(opt))
@@ -431,9 +561,6 @@ class IterativeSolver:
         _check(lib().lsx_create(C.byref(h), C.byref(opt), self.stream))
         self.h = h
         self.eps = float(task.eps)
-        _check(lib().lsx_set_geometry(self.h, geometry.h, 1 if pin else 0))
-        self._rhs = rhs
-        self._sol = sol
 
     def solve(self):
         b = np.ascontiguousarray(self._rhs, dtype=np.float64)

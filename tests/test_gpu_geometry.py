@@ -137,3 +137,40 @@ def test_bezier_geometry_bitwise(lsx, name, dims, ratio):
     S.set_geometry(g_gpu, np.zeros(A.n), np.zeros(A.n), lsx.Task())
     x = np.random.default_rng(3).standard_normal(A.n)
     assert np.array_equal(S.apply_operator(x, 0), A.matvec(x))
+
+
+@pytest.mark.parametrize("name,dims,ratio,kind,center,axes", CASES[::2], ids=[c[0] for c in CASES[::2]])
+def test_operator_from_caller_fields_bitwise(lsx, name, dims, ratio, kind, center, axes):
+    """lsx_set_fields with the oracle's mesh fields (the arrays a reference caller holds): the
+    device-assembled operator equals the oracle's CSR operator and lsx_set_geometry's, and the
+    solve equals the oracle's at an equal iteration count."""
+    w = axes_for(lsx, dims, ratio)
+    g_ref = O.Geometry(*w).build_body(kind, center, axes)
+    fr = g_ref.fields()
+    A = g_ref.laplace(pin=True)
+    n = A.n
+    N, M, K = dims
+    b = P.random_rhs(g_ref, seed=11, A=A)
+    eps = 1e-8 * np.linalg.norm(b)
+    task = lsx.Task(eps=eps, FGMRES_RESTART_NUM=30, MAX_ITER=500)
+    x = np.zeros(n)
+    S = lsx.MGFGMRES()
+    S.set_fields(dims, w, fr, b, x, task)
+    st = S.level_stencil(0)
+    dense = np.zeros((7, n))
+    offs = {-N * M: 0, -N: 1, -1: 2, 0: 3, 1: 4, N: 5, N * M: 6}
+    rows = np.repeat(np.arange(n), np.diff(A.rowptr))
+    q = np.array([offs[c - r] for c, r in zip(A.col, rows)])
+    dense[q, rows] = A.val
+    assert np.array_equal(st, dense)
+    g_gpu = lsx.Geometry(*w).build(kind, center, axes)
+    S2 = lsx.MGFGMRES()
+    S2.set_geometry(g_gpu, np.zeros(n), np.zeros(n), task)
+    assert np.array_equal(S2.level_stencil(0), st)
+    it = S.solve()
+    x_ref, it_ref, _ = O.Solver(A, eps=eps, restart=30, max_it=500).solve(b)
+    rel = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    print(f"case={name} it={it} it_ref={it_ref} rel={rel:.3e}", flush=True)
+    assert it == it_ref and rel <= 1e-10
+    with pytest.raises(lsx.LsxError, match="do not match"):
+        S.set_fields((N, M, K - 1), w, fr, b, x, task)
