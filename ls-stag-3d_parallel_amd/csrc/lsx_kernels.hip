@@ -741,6 +741,10 @@ __global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, lon
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
 }
 
+__global__ void k_fill_zero(double* __restrict__ b, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = 0.0;
+}
+
 __global__ void k_scale_copy(const double* __restrict__ w, const double* __restrict__ normw, double* __restrict__ v,
                              long n) {
   const double f = 1.0 / normw[0];
@@ -1116,6 +1120,8 @@ void lu_solve(const double* scale, const int* perm, const double* L, const doubl
 }
 
 void copy(const double* a, double* b, long n, hipStream_t s) { k_copy<<<grid_1d(n), 256, 0, s>>>(a, b, n); }
+
+void fill_zero(double* b, long n, hipStream_t s) { k_fill_zero<<<grid_1d(n), 256, 0, s>>>(b, n); }
 
 void scale_copy(const double* w, const double* normw, double* v, long n, hipStream_t s) {
   k_scale_copy<<<grid_1d(n), 256, 0, s>>>(w, normw, v, n);

@@ -73,6 +73,7 @@ void lu_solve(const double* scale, const int* perm, const double* L, const doubl
 
 // ---- BLAS-1 (FP64) ------------------------------------------------------------------------
 void copy(const double* a, double* b, long n, hipStream_t s);
+void fill_zero(double* b, long n, hipStream_t s);  // a kernel, not a memset: safe under graph capture
 void scale_copy(const double* w, const double* normw, double* v, long n, hipStream_t s);  // v = (1/normw) w
 void dot_partial(const double* a, const double* b, long n, double* partial, hipStream_t s);
 void finalize(const double* partial, int nb, double* out, int sqrt_flag, hipStream_t s);

@@ -38,9 +38,10 @@ Solver::Solver(const lsx_options& o, hipStream_t s) : opt(o) {
   if (opt.smoother == LSX_SMOOTH_JACOBI && !(opt.omega > 0.0 && opt.omega <= 1.0))
     throw Error(LSX_ERR_ARG, "Jacobi damping omega must be in (0, 1]");
   if (opt.fused_sweep < 0 || opt.fused_sweep > 2) throw Error(LSX_ERR_ARG, "fused_sweep must be 0, 1 or 2");
-  if (opt.levels < 0 || opt.levels == 1) throw Error(LSX_ERR_ARG, "levels must be 0 (automatic) or >= 2");
-  if (opt.levels != 0 && opt.mg != LSX_MG_VCYCLE)
-    throw Error(LSX_ERR_ARG, "the reference cascade uses the automatic level count (levels = 0)");
+  if (opt.levels < 0) throw Error(LSX_ERR_ARG, "levels must be >= 0");
+  if (opt.mg == LSX_MG_VCYCLE && opt.levels == 1) throw Error(LSX_ERR_ARG, "V-cycle levels must be 0 (automatic) or >= 2");
+  if (opt.mg != LSX_MG_VCYCLE && opt.levels > 1)
+    throw Error(LSX_ERR_ARG, "the reference cascade uses the automatic level count (levels = 0) or one level (1)");
   if (s) {
     stream = s;
   } else {
@@ -179,6 +180,7 @@ void Solver::check_dims(int N, int M, int K) const {
     m /= 2;
     k /= 2;
   }
+  if (single_level()) return;  // no coarse levels: any dimensions, no coarsest solve
   if (opt.levels == 0 && (long)n * m * k > 64) throw Error(LSX_ERR_ARG, "coarsest level exceeds 64 unknowns");
   if ((long)n * m * k > vc::DENSE_MAX)
     throw Error(LSX_ERR_ARG, "the level cap leaves more than 4096 unknowns on the coarsest level");
@@ -209,7 +211,7 @@ void Solver::setup_levels(int N, int M, int K) {
     L.coef.alloc((size_t)L.npts * nl);
     L.r.alloc(nl);
     if (l > 0) L.ch.alloc(nl);
-    if (l + 1 < levels.size())
+    if (l + 1 < levels.size() || single_level())
       for (int dir = 0; dir < 3; ++dir) {
         L.dinv[dir].alloc(nl);
         L.eu[dir].alloc(nl);
@@ -220,7 +222,7 @@ void Solver::setup_levels(int N, int M, int K) {
   dense_c.release();
   dense_r.release();
   dense_part.release();
-  if (nc <= 64) {
+  if (nc <= 64 && !single_level()) {
     lu_scale.alloc(nc);
     lu_perm.alloc(nc);
     lu_L.alloc(nc * nc);
@@ -243,6 +245,41 @@ void Solver::set_geometry(const Geometry& g, int pin) {
   src_geo = &g;
   src_version = g.version();
   src_pin = pin;
+}
+
+void Solver::set_fields(int N, int M, int K, const double* dx, const double* dy, const double* dz, const int* type,
+                        const double* thx, const double* thy, const double* thz, const double* vof, int pin, int mem) {
+  check_dims(N, M, K);
+  if (!dx || !dy || !dz || !type || !thx || !thy || !thz || !vof) throw Error(LSX_ERR_ARG, "null mesh field");
+  if (mem != LSX_MEM_HOST && mem != LSX_MEM_DEVICE) throw Error(LSX_ERR_ARG, "mem must be LSX_MEM_HOST or LSX_MEM_DEVICE");
+  src_geo = nullptr;
+  ready = false;
+  values_dirty = false;
+  const long n = (long)N * M * K;
+  // device copies of the fields (one staging allocation; the operator keeps nothing of them)
+  DevBuf<double> w3, f4;
+  DevBuf<int> t;
+  w3.alloc(N + M + K);
+  f4.alloc(4 * n);
+  t.alloc(n);
+  const hipMemcpyKind kind = mem == LSX_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  LSX_HIP(hipMemcpyAsync(w3.p, dx, N * sizeof(double), kind, stream));
+  LSX_HIP(hipMemcpyAsync(w3.p + N, dy, M * sizeof(double), kind, stream));
+  LSX_HIP(hipMemcpyAsync(w3.p + N + M, dz, K * sizeof(double), kind, stream));
+  LSX_HIP(hipMemcpyAsync(f4.p, thx, n * sizeof(double), kind, stream));
+  LSX_HIP(hipMemcpyAsync(f4.p + n, thy, n * sizeof(double), kind, stream));
+  LSX_HIP(hipMemcpyAsync(f4.p + 2 * n, thz, n * sizeof(double), kind, stream));
+  LSX_HIP(hipMemcpyAsync(f4.p + 3 * n, vof, n * sizeof(double), kind, stream));
+  LSX_HIP(hipMemcpyAsync(t.p, type, n * sizeof(int), kind, stream));
+  setup_levels(N, M, K);
+  geo::laplace7(t.p, f4.p, f4.p + n, f4.p + 2 * n, f4.p + 3 * n, w3.p, w3.p + N, w3.p + N + M, N, M, K, pin,
+                levels[0].coef.p, stream);
+  LSX_HIP(hipGetLastError());
+  LSX_HIP(hipStreamSynchronize(stream));
+  from_csr = false;
+  build_hierarchy();
+  alloc_krylov();
+  ready = true;
 }
 
 void Solver::set_csr(int N, int M, int K, const int* rp, const int* cl, const double* vl, long nnz, int mem) {
@@ -335,8 +372,15 @@ void Solver::build_hierarchy() {
     for (int dir = 0; dir < 3; ++dir)
       line_factor(f.coef.p, f.npts, f.d, dir, scalars.p + SC_TINY, f.dinv[dir].p, f.eu[dir].p, stream);
   }
-  MGLevel& C = levels.back();
-  lu_factor(C.coef.p, C.d, lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, lu_col.p, stream);
+  if (single_level()) {  // one level: its line factors, no coarsest LU
+    MGLevel& f = levels[0];
+    diag_absmax(f.coef.p, f.npts, f.d, partial.p, scalars.p + SC_TINY, stream);
+    for (int dir = 0; dir < 3; ++dir)
+      line_factor(f.coef.p, f.npts, f.d, dir, scalars.p + SC_TINY, f.dinv[dir].p, f.eu[dir].p, stream);
+  } else {
+    MGLevel& C = levels.back();
+    lu_factor(C.coef.p, C.d, lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, lu_col.p, stream);
+  }
   LSX_HIP(hipGetLastError());
   LSX_HIP(hipStreamSynchronize(stream));
 }
@@ -819,11 +863,17 @@ void Solver::mg_apply(const double* v, double* z) {
   const int L = (int)levels.size() - 1;
   copy(v, levels[0].r.p, d0.n(), stream);
   for (int k = 0; k < L; ++k) restrict_full(levels[k].r.p, levels[k].d, levels[k + 1].r.p, levels[k + 1].d, stream);
-  lu_solve(lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, (int)levels[L].d.n(), levels[L].r.p, levels[L].ch.p, stream);
-  for (int k = L - 1; k >= 0; --k) {
+  if (L > 0)
+    lu_solve(lu_scale.p, lu_perm.p, lu_L.p, lu_U.p, (int)levels[L].d.n(), levels[L].r.p, levels[L].ch.p, stream);
+  for (int k = L > 0 ? L - 1 : 0; k >= 0; --k) {
     MGLevel& lv = levels[k];
     double* ch = k == 0 ? z : lv.ch.p;
-    prolong_copy(levels[k + 1].ch.p, levels[k + 1].d, ch, cwork.p, lv.d, stream);
+    if (L > 0) {
+      prolong_copy(levels[k + 1].ch.p, levels[k + 1].d, ch, cwork.p, lv.d, stream);
+    } else {  // one level: the line sweeps start from a zero correction
+      fill_zero(ch, lv.d.n(), stream);
+      fill_zero(cwork.p, lv.d.n(), stream);
+    }
     for (int s = 0; s < opt.n_post; ++s)
       for (int dir = 2; dir >= 0; --dir) {  // z, y, x
         hipEvent_t e0 = nullptr;

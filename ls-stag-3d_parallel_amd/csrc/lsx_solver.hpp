@@ -78,6 +78,13 @@ class Solver {
 
   void set_csr(int N, int M, int K, const int* rowptr, const int* col, const double* val, long nnz, int mem);
   void set_geometry(const Geometry& g, int pin);
+  // the cascade with opt.levels == 1: no coarse levels, the preconditioner is the line sweeps of
+  // the fine level from a zero correction (grids of any dimensions)
+  bool single_level() const { return opt.mg != LSX_MG_VCYCLE && opt.levels == 1; }
+  // the pressure operator assembled on the device from the caller's mesh fields (widths, cell
+  // types, face fractions, cut volumes; `mem` space), then the hierarchy
+  void set_fields(int N, int M, int K, const double* dx, const double* dy, const double* dz, const int* type,
+                  const double* thx, const double* thy, const double* thz, const double* vof, int pin, int mem);
   // the geometry (and its build and pin choice) the current operator was assembled from
   bool operator_from(const Geometry* g, long version, int pin) const {
     return ready && src_geo == g && src_version == version && src_pin == pin;

@@ -196,6 +196,18 @@ long lsxo_geom_build_laplace(void* p, int pin) {
   return (long)h->A.rowptr[h->A.rows];
 }
 
+// Helmholtz operator of component dir into the handle's CSR; returns nnz (dims: 3 ints)
+long lsxo_geom_build_helmholtz(void* p, int dir, double nu, double coef, int flow, int* dims) {
+  auto* h = static_cast<GeomH*>(p);
+  try {
+    h->A = build_helmholtz(h->g, dir, nu, coef, flow != 0, dims);
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  return (long)h->A.rowptr[h->A.rows];
+}
+
 int lsxo_geom_get_csr(void* p, int* rowptr, int* col, double* val) {
   auto* h = static_cast<GeomH*>(p);
   std::memcpy(rowptr, h->A.rowptr.data(), (h->A.rows + 1) * sizeof(int));
@@ -220,7 +232,7 @@ void* lsxo_solver_create(int N, int M, int K, const int* rowptr, const int* col,
     h->eps = eps;
     h->precond = precond;
     h->id.n = n;
-    if (precond == 1) h->mg.set(h->A, N, M, K, n_post, omega);
+    if (precond == 1 || precond == 2) h->mg.set(h->A, N, M, K, n_post, omega, precond == 2);
     return h.release();
   } catch (std::exception& e) {
     g_err = e.what();
@@ -234,7 +246,7 @@ void lsxo_solver_set_eps(void* p, double eps) { static_cast<SolverH*>(p)->eps = 
 
 int lsxo_solver_solve(void* p, const double* b, double* x, double* norm_r) {
   auto* h = static_cast<SolverH*>(p);
-  Preconditioner& pr = h->precond == 1 ? (Preconditioner&)h->mg : (Preconditioner&)h->id;
+  Preconditioner& pr = h->precond != 0 ? (Preconditioner&)h->mg : (Preconditioner&)h->id;
   if (h->kind == 0) return fgmres_solve(h->A, pr, b, x, h->eps, h->max_it, h->restart, norm_r);
   return bicgstab_solve(h->A, pr, b, x, h->eps, h->max_it, norm_r);
 }

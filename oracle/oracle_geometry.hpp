@@ -66,6 +66,9 @@ std::vector<double> pressure_flux(const Geometry& g, const double* u_west, doubl
                                   double W_B, double W_T, const double* u_out);
 void pressure_rhs(const Geometry& g, const double* u, const double* v, const double* w, const std::vector<double>& flux,
                   bool pin_outlet, double* b);
+// Momentum (Helmholtz) operator of velocity component dir (0 u, 1 v, 2 w) on its staggered mesh,
+// for a box without cut cells (oracle_helmholtz.cpp); dims_out receives the mesh dimensions.
+Csr build_helmholtz(const Geometry& g, int dir, double nu, double coef, bool flow, int* dims_out);
 // u = (G_x dp) inv_Mx + u* (and v, w), the velocity part of SetSolutions::correction
 void velocity_correction(const Geometry& g, const double* dp, const double* us, const double* vs, const double* ws,
                          double* u, double* v, double* w);

@@ -76,7 +76,9 @@ struct Multigrid : Preconditioner {
   double omega = DEFAULT_OMEGA_3D;
   std::vector<double> c, ch, cHH, q;
   std::vector<std::vector<double>> r;
-  void set(const Csr& A0, int N, int M, int K, int n_post, double omega);
+  // single: one level only (no transfers, no coarsest LU): the apply is n_post ADLJ sweeps of the
+  // fine level from a zero correction, for grids of any dimensions
+  void set(const Csr& A0, int N, int M, int K, int n_post, double omega, bool single = false);
   void apply(const double* b, double* x) override;
 };
 

@@ -494,10 +494,10 @@ int count_levels(int N, int M, int K) {
   return num;
 }
 
-void Multigrid::set(const Csr& A0, int N, int M, int K, int n_post_, double omega_) {
+void Multigrid::set(const Csr& A0, int N, int M, int K, int n_post_, double omega_, bool single) {
   n_post = n_post_;
   omega = omega_;
-  const int numMG = count_levels(N, M, K);
+  const int numMG = single ? 0 : count_levels(N, M, K);
   levels.clear();
   levels.resize(numMG + 1);
   levels[0].A = A0;
@@ -518,7 +518,10 @@ void Multigrid::set(const Csr& A0, int N, int M, int K, int n_post_, double omeg
     cgrid.A = spgemm(V, f.P, f.P.cols);
     f.adlj.build(f.A, f.N, f.M, f.K);
   }
-  lu.factor(levels[numMG].A);
+  if (single)
+    levels[0].adlj.build(levels[0].A, N, M, K);
+  else
+    lu.factor(levels[numMG].A);
   const long n0 = A0.rows;
   c.assign(n0, 0.0);
   ch.assign(n0, 0.0);
@@ -532,11 +535,15 @@ void Multigrid::apply(const double* b, double* x) {
   const int numMG = (int)levels.size() - 1;
   std::copy(b, b + levels[0].A.rows, r[0].begin());
   for (int k = 0; k < numMG; ++k) spmv(levels[k].R, r[k].data(), r[k + 1].data());
-  lu.solve(r[numMG].data(), cHH.data());
-  for (int k = numMG - 1; k >= 0; --k) {
+  if (numMG > 0) lu.solve(r[numMG].data(), cHH.data());
+  for (int k = numMG > 0 ? numMG - 1 : 0; k >= 0; --k) {
     Level& L = levels[k];
     const long n = L.A.rows;
-    spmv(L.P, cHH.data(), ch.data());
+    if (numMG > 0) {
+      spmv(L.P, cHH.data(), ch.data());
+    } else {  // one level: the sweeps start from a zero correction
+      std::fill(ch.begin(), ch.begin() + n, 0.0);
+    }
     std::copy(ch.begin(), ch.begin() + n, c.begin());
     for (int s = 0; s < n_post; ++s) {
       for (int dir = 2; dir >= 0; --dir) {  // z, y, x  (smoother_adlj.h:121-126)

@@ -57,6 +57,8 @@ def lib():
     L.lsxo_geom_correct.argtypes = [C.c_void_p, PD, PD, PD, PD, PD, PD, PD]
     L.lsxo_geom_rhs.argtypes = [C.c_void_p, PD, PD, PD, PD] + [C.c_double] * 5 + [C.c_void_p, C.c_int, PD,
                                                                                   C.c_void_p]
+    L.lsxo_geom_build_helmholtz.restype = C.c_long
+    L.lsxo_geom_build_helmholtz.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, PI]
     _lib = L
     return L
 
@@ -164,6 +166,22 @@ class Geometry:
         return Csr(self.dims, rp, col, val)
 
 
+def helmholtz(g, direction, nu, coef, flow=True):
+    """Momentum (Helmholtz) operator of velocity component `direction` (0 u, 1 v, 2 w) on its
+    staggered mesh of the body-free geometry g: nu = 1/Re, coef = COEF_INTEGR_TIME (1.5/dt);
+    flow: a flow task (outflow east boundary, no bottom/top wall terms)."""
+    dims = np.zeros(3, np.int32)
+    nnz = lib().lsxo_geom_build_helmholtz(g.h, int(direction), float(nu), float(coef), 1 if flow else 0, dims)
+    if nnz < 0:
+        raise RuntimeError(lib().lsxo_last_error().decode())
+    n = int(np.prod(dims))
+    rp = np.zeros(n + 1, np.int32)
+    col = np.zeros(nnz, np.int32)
+    val = np.zeros(nnz)
+    lib().lsxo_geom_get_csr(g.h, rp, col, val)
+    return Csr(tuple(int(d) for d in dims), rp, col, val)
+
+
 def bezier_ellipsoid(center, axes):
     """Control points (6, 4, 4, 3) of the six-patch bicubic Bezier fit of an ellipsoid."""
     ctrl = np.zeros(288)
@@ -219,13 +237,14 @@ def velocity_correction(g, dp, us, vs, ws):
 
 
 class Solver:
-    """Oracle FGMRES / BiCGStab with the reference multigrid cascade (or no preconditioner)."""
+    """Oracle FGMRES / BiCGStab with the reference multigrid cascade ("mg"), its fine level alone
+    ("mg1": ADLJ sweeps from a zero correction, any grid dimensions) or no preconditioner."""
 
     def __init__(self, A, kind="fgmres", restart=30, max_it=1000, eps=1e-8, precond="mg", n_post=3, omega=0.805):
         self.A = A
         N, M, K = A.dims
         self.h = lib().lsxo_solver_create(N, M, K, A.rowptr, A.col, A.val, 0 if kind == "fgmres" else 1, restart,
-                                          max_it, eps, 1 if precond == "mg" else 0, n_post, omega)
+                                          max_it, eps, {"mg": 1, "mg1": 2}.get(precond, 0), n_post, omega)
         if not self.h:
             raise RuntimeError(lib().lsxo_last_error().decode())
 

@@ -1,0 +1,99 @@
+"""Momentum (Helmholtz) solves on the GPU through the drop-in boundary, against the oracle.
+
+The reference solves the three momentum equations with the same Krylov and multigrid classes as
+the pressure (HelmholtzDiscreteEquation with LS_STAG_FULL_MG, helmholtz_discrete_equation.h:79-345,
+selecting MGFGMRES or MGBiCGStab at :285-333).  Their meshes are the staggered ones, (N-1) x M x K
+for u, N x (M-1) x K for v, N x M x (K-1) for w, so one dimension is odd on every even base grid:
+the library solves them with the one-level cascade (lsx_options.levels = 1, ADLJ sweeps of the
+fine level from a zero correction) — the reference's odd-parity transfer operators are not rebuilt.
+
+Each test assembles the operator with the oracle's restatement of the reference's
+HelmholtzDiscreteOperatorX/Y/Z (oracle/oracle_helmholtz.cpp, pinned in test_oracle_helmholtz.py),
+takes the time-derivative part of the reference's right-hand side, (-2 u^n + 0.5 u^(n-1)) M / dt
+(HelmholtzDiscreteEquation::initRM, helmholtz_discrete_equation.h:70-73) with seeded velocities,
+hands the CSR to lsx_set_csr exactly as the equation's solver->set(A, rhs, sol) would, and
+compares the GPU solve with the oracle's solve of the same system: equal iteration counts and a
+relative L2 difference of at most 1e-10."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+RE, DT = 300.0, 0.002
+NU, COEF = 1.0 / RE, 1.5 / DT
+
+
+def axis(m, ratio):
+    w = ratio ** np.abs(np.arange(m) + 0.5 - m / 2)
+    return w / w.sum()
+
+
+def masses(g, d):
+    """Staggered control-volume masses M = (V_i + V_nb) / 2 of component d (row order of its mesh)."""
+    N, M, K = g.dims
+    v = g.fields()["vof"].reshape(K, M, N)
+    if d == 0:
+        m = 0.5 * (v[:, :, :-1] + v[:, :, 1:])
+    elif d == 1:
+        m = 0.5 * (v[:, :-1, :] + v[:, 1:, :])
+    else:
+        m = 0.5 * (v[:-1, :, :] + v[1:, :, :])
+    return m.ravel()
+
+
+def system(dims, ratios, d, seed, flow=True, re=RE, dt=DT):
+    g = O.Geometry(*[axis(m, r) for m, r in zip(dims, ratios)])
+    A = O.helmholtz(g, d, 1.0 / re, 1.5 / dt, flow=flow)
+    rng = np.random.default_rng(seed)
+    un = 1.0 + 0.1 * rng.standard_normal(A.n)
+    uprev = un + 0.01 * rng.standard_normal(A.n)
+    b = (-2.0 * un + 0.5 * uprev) * masses(g, d) / dt
+    return A, b
+
+
+# (grid, stretching, Re, dt): the time-stepped run's Re = 300, dt = 0.002 (a mass-dominated system,
+# 2 iterations) and a viscous one (Re = 10, dt = 0.02: 8 to 34 iterations) that exercises the
+# Krylov loop
+CASES = [((32, 24, 20), (1.04, 1.06, 1.02), RE, DT), ((64, 64, 64), (1.0, 1.0, 1.0), RE, DT),
+         ((48, 32, 40), (1.02, 1.02, 1.02), 10.0, 0.02), ((64, 64, 64), (1.0, 1.0, 1.0), 10.0, 0.02)]
+
+
+@pytest.mark.parametrize("kind", ["fgmres", "bicgstab"])
+@pytest.mark.parametrize("d", [0, 1, 2])
+@pytest.mark.parametrize("dims,ratios,re,dt", CASES, ids=["32x24x20s", "64cube", "48x32x40s-re10", "64cube-re10"])
+def test_helmholtz_solve_matches_oracle(lsx, dims, ratios, re, dt, d, kind):
+    A, b = system(dims, ratios, d, seed=7 + d, re=re, dt=dt)
+    eps = 1e-8 * np.linalg.norm(b)
+    x_ref, it_ref, nr_ref = O.Solver(A, kind=kind, eps=eps, restart=30, max_it=200, precond="mg1").solve(b)
+    S = (lsx.MGFGMRES if kind == "fgmres" else lsx.MGBiCGStab)(levels=1)
+    x = np.zeros(A.n)
+    S.set(A, b, x, lsx.Task(eps=eps, MAX_ITER=200, FGMRES_RESTART_NUM=30))
+    it = S.solve()
+    rel = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    res = np.linalg.norm(b - A.matvec(x))
+    print(f"mesh {A.dims} component {'uvw'[d]} {kind}: it={it} it_ref={it_ref} rel={rel:.3e} "
+          f"|b-Ax|={res:.3e} eps={eps:.3e}", flush=True)
+    assert it == it_ref and rel <= 1e-10
+    assert res <= 1.05 * eps
+
+
+def test_helmholtz_one_level_preconditioner_matches_oracle(lsx):
+    """One application of the one-level cascade (z, y, x line sweeps from zero) equals the oracle's."""
+    A, b = system((24, 20, 16), (1.05, 1.03, 1.07), 0, seed=3)
+    zr = O.Solver(A, precond="mg1").mg_apply(b)
+    S = lsx.MGFGMRES(levels=1)
+    S.set(A, b, np.zeros(A.n), lsx.Task(eps=1.0, FGMRES_RESTART_NUM=30))
+    z = S.precondition(b)
+    rel = np.linalg.norm(z - zr) / np.linalg.norm(zr)
+    print(f"one-level cascade vs oracle: rel={rel:.3e}", flush=True)
+    assert rel <= 1e-12
+
+
+def test_one_level_rejected_for_vcycle_and_odd_grid_needs_it(lsx):
+    A, b = system((16, 12, 10), (1.0, 1.0, 1.0), 0, seed=1)
+    with pytest.raises(lsx.LsxError, match="even"):
+        lsx.MGFGMRES().set(A, b, np.zeros(A.n), lsx.Task(eps=1e-8))
+    with pytest.raises(lsx.LsxError, match="levels"):
+        lsx.solver("fgmres", "fp64-vcycle", levels=1).set(A, b, np.zeros(A.n), lsx.mixed_task(1e-8))
