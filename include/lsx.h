@@ -277,6 +277,19 @@ int lsx_set_fields(lsx_solver* s, int N, int M, int K, const double* dx, const d
                    const int* type, const double* thx, const double* thy, const double* thz, const double* vof, int pin,
                    int mem);
 
+/* Set the solver's matrix to the momentum (Helmholtz) operator of velocity component `component`
+ * (0 u, 1 v, 2 w) of `g` on its staggered mesh ((N-1) x M x K, N x (M-1) x K or N x M x (K-1)),
+ * assembled on the device, and build the preconditioner: the stencil-native form of the set
+ * call of HelmholtzDiscreteEquation (helmholtz_discrete_equation.h:99-129) with the operators of
+ * HelmholtzDiscreteOperatorX / Y / Z::calcHelm* (helmholtz_discrete_operator_x.h:98-689 and the _y,
+ * _z files).  nu = 1/Re (the task's nu), coef_time = COEF_INTEGR_TIME (1.5/dt,
+ * base_task_description.h:337-339), flow != 0 for a flow task (outflow east boundary, no bottom
+ * and top wall terms).  The geometry must be a box without cut cells (LSX_ERR_ARG otherwise: the
+ * immersed-boundary terms of cut cells are not assembled); the staggered mesh has an odd
+ * dimension on an even grid, so the solver needs options that accept it (the one-level cascade,
+ * levels = 1).  Solve with lsx_solve as usual. */
+int lsx_set_helmholtz(lsx_solver* s, const lsx_geometry* g, int component, double nu, double coef_time, int flow);
+
 /* ---- objective and shape gradient ----------------------------------------------------------
  * The reference computes loads but has no adjoint; these entries have no counterpart there. */
 #define LSX_OBJ_NONE 0   /* J depends on the geometry only through p                           */

@@ -391,6 +391,11 @@ int lsx_tape_destroy(lsx_tape* t) {
   return guarded([&] { delete t; });
 }
 
+int lsx_set_helmholtz(lsx_solver* s, const lsx_geometry* g, int component, double nu, double coef_time, int flow) {
+  if (!s || !g) return LSX_ERR_ARG;
+  return guarded([&] { s->impl.set_helmholtz(g->impl, component, nu, coef_time, flow); });
+}
+
 int lsx_set_fields(lsx_solver* s, int N, int M, int K, const double* dx, const double* dy, const double* dz,
                    const int* type, const double* thx, const double* thy, const double* thz, const double* vof, int pin,
                    int mem) {

@@ -684,6 +684,215 @@ __global__ void k_laplace7(const int* __restrict__ type, const double* __restric
   }
 }
 
+// ---- momentum (Helmholtz) operators on the staggered meshes --------------------------------------
+// Rows of HelmholtzDiscreteOperatorX / Y / Z::calcHelm* (helmholtz_discrete_operator_x.h:98-689 and
+// the _y, _z files) for a box without cut cells, in stencil form [b, s, w, p, e, n, t]: viscous
+// couplings across the cross faces ((area nu) (sum of the two widths) / (sum of the two distances)),
+// nu (area)^2 / V along the component's own axis, no-slip wall terms (bottom and top ones dropped
+// for flow tasks), diagonal -(coef M) - sum in each boundary type's own order, and the outflow term
+// added to the last cell of every x row of a flow task.  Face and edge fractions are 1 in such a
+// box and the reference's products by them are exact, so they are left out.  One thread per
+// staggered cell; the type of a staggered cell is the smaller of its two base cells' types
+// (ls_stag_uniform_mesh.h:108-130), the mass half the sum of their volumes (:2804-2826).
+__device__ inline bool hb_(int t) { return t == B_Border || t == BN_Edge || t == BS_Edge || t == BE_Edge || t == BW_Edge || t == BNE_Corner || t == BNW_Corner || t == BSE_Corner || t == BSW_Corner; }
+__device__ inline bool ht_(int t) { return t == T_Border || t == TN_Edge || t == TS_Edge || t == TE_Edge || t == TW_Edge || t == TNE_Corner || t == TNW_Corner || t == TSE_Corner || t == TSW_Corner; }
+__device__ inline bool hs_(int t) { return t == S_Border || t == TS_Edge || t == BS_Edge || t == SE_Edge || t == SW_Edge || t == TSE_Corner || t == TSW_Corner || t == BSE_Corner || t == BSW_Corner; }
+__device__ inline bool hn_(int t) { return t == N_Border || t == TN_Edge || t == BN_Edge || t == NE_Edge || t == NW_Edge || t == TNE_Corner || t == TNW_Corner || t == BNE_Corner || t == BNW_Corner; }
+__device__ inline bool hw_(int t) { return t == W_Border || t == TW_Edge || t == BW_Edge || t == NW_Edge || t == SW_Edge || t == TNW_Corner || t == TSW_Corner || t == BNW_Corner || t == BSW_Corner; }
+__device__ inline bool he_(int t) { return t == E_Border || t == TE_Edge || t == BE_Edge || t == NE_Edge || t == SE_Edge || t == TNE_Corner || t == TSE_Corner || t == BNE_Corner || t == BSE_Corner; }
+
+__global__ void k_helmholtz7(const int* __restrict__ type, const double* __restrict__ thx,
+                             const double* __restrict__ thy, const double* __restrict__ thz,
+                             const double* __restrict__ vof, const double* __restrict__ dxs,
+                             const double* __restrict__ dys, const double* __restrict__ dzs, int N, int M, int K,
+                             int dir, double nu, double cf, int flow, double* __restrict__ out) {
+  const int Nd = N - (dir == 0), Md = M - (dir == 1), Kd = K - (dir == 2);
+  const long nd = (long)Nd * Md * Kd, NM = (long)N * M;
+  const long step = dir == 0 ? 1 : (dir == 1 ? N : NM);
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < nd; q += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(q % Nd), j = (int)((q / Nd) % Md), k = (int)(q / ((long)Nd * Md));
+    const long c = i + N * (j + (long)M * k);
+    const int tc = type[c], td = type[c + step];
+    const double th = dir == 0 ? thx[c] : (dir == 1 ? thy[c] : thz[c]);
+    const int t = th < EQ_DOUBLE ? SolidCell : (td < tc ? td : tc);
+    auto iv = [&](long cc) { return vof[cc] < EQ_TINY_DOUBLE ? 0.0 : 1.0 / vof[cc]; };
+    const double dx = dxs[i], dy = dys[j], dz = dzs[k];
+    const double mass = 0.5 * (vof[c] + vof[c + step]);
+    double rb = 0, rs = 0, rw = 0, re = 0, rn = 0, rt = 0, rp = 0;
+    if (t == FluidCell) {
+      if (dir == 0) {
+        const double dxx = nu * (dxs[i + 1] + dx);
+        double aux = dz * dxx;
+        rs = aux / (dy + dys[j - 1]);
+        rn = aux / (dy + dys[j + 1]);
+        aux = dy * dxx;
+        rb = aux / (dz + dzs[k - 1]);
+        rt = aux / (dz + dzs[k + 1]);
+        aux = dy * dz;
+        const double d2 = nu * aux * aux;
+        re = d2 * iv(c + 1);
+        rw = d2 * iv(c);
+      } else if (dir == 1) {
+        const double dyy = nu * (dys[j + 1] + dy);
+        double aux = dz * dyy;
+        rw = aux / (dx + dxs[i - 1]);
+        re = aux / (dx + dxs[i + 1]);
+        aux = dx * dyy;
+        rb = aux / (dz + dzs[k - 1]);
+        rt = aux / (dz + dzs[k + 1]);
+        aux = dx * dz;
+        const double d2 = nu * aux * aux;
+        rn = d2 * iv(c + N);
+        rs = d2 * iv(c);
+      } else {
+        const double dzz = nu * (dzs[k + 1] + dz);
+        double aux = dzz * dy;
+        rw = aux / (dx + dxs[i - 1]);
+        re = aux / (dx + dxs[i + 1]);
+        aux = dx * dzz;
+        rs = aux / (dy + dys[j - 1]);
+        rn = aux / (dy + dys[j + 1]);
+        aux = dx * dy;
+        const double d2 = nu * aux * aux;
+        rt = d2 * iv(c + NM);
+        rb = d2 * iv(c);
+      }
+      rp = -(cf * mass) - rb - rs - rw - re - rn - rt;
+    } else if (t != SolidCell) {
+      const bool hb = !hb_(t), ht = !ht_(t), hs = !hs_(t), hn = !hn_(t), hw = !hw_(t);
+      const bool zw = hb_(t) || ht_(t), yw = hn_(t) || hs_(t);
+      double p = -(cf * mass);
+      if (dir == 0) {
+        const double dxx = dxs[i + 1], sx = dx + dxx;
+        const double az = dz * nu, ay = dy * nu;
+        if (hs) rs = az * sx / (dy + dys[j - 1]);
+        if (hn) rn = az * sx / (dy + dys[j + 1]);
+        if (hb) rb = ay * sx / (dz + dzs[k - 1]);
+        if (ht) rt = ay * sx / (dz + dzs[k + 1]);
+        const double a2 = dy * dz, d2 = nu * a2 * a2;
+        const double ke = d2 * iv(c + 1), kw = hw ? d2 * iv(c) : 0.0;
+        re = ke;
+        rw = kw;
+        double wz = 0.0, wy = 0.0;
+        if (t == TN_Edge) {  // the reference's TN edge: (dy dz) / dz in place of dy
+          const double q2 = a2 / dz;
+          wz = flow ? 0.0 : q2 * nu * sx / dz;
+          wy = nu * dz * sx / q2;
+        } else {
+          if (zw) wz = flow ? 0.0 : dy * nu * sx / dz;
+          if (yw) wy = nu * dz * sx / dy;
+        }
+        if (hb) p -= rb;
+        if (hs) p -= rs;
+        p -= hw ? (kw + ke) : ke;
+        if (hn) p -= rn;
+        if (ht) p -= rt;
+        if (zw) p -= wz;
+        if (yw) p -= wy;
+      } else if (dir == 1) {
+        const double dyy = dys[j + 1], sy = dy + dyy;
+        const double az = dz * nu, ax = dx * nu;
+        if (hw) rw = az * sy / (dx + dxs[i - 1]);
+        const double ke = az * sy / (dx + dxs[(c + 1) % N]);  // east boundary: the next row's first cell
+        re = ke;
+        if (hb) rb = ax * sy / (dz + dzs[k - 1]);
+        if (ht) rt = ax * sy / (dz + dzs[k + 1]);
+        const double a2 = dx * dz, d2 = nu * a2 * a2;
+        const double ks = d2 * iv(c), kn = d2 * iv(c + N);
+        rs = ks;
+        rn = kn;
+        double wz = 0.0;
+        if (zw) {
+          const double dxw = (t == B_Border || t == T_Border) ? dx : a2;  // edges and corners: dx dz
+          wz = flow ? 0.0 : dxw * nu * sy / dz;
+        }
+        const double wy = hn_(t) ? kn : (hs_(t) ? ks : 0.0);
+        if (hb) p -= rb;
+        if (hn_(t)) {
+          if (t == N_Border) {
+            p -= rw;
+            p -= ks;
+          } else {
+            p -= ks;
+            if (hw) p -= rw;
+          }
+          p -= ke;
+        } else if (hs_(t)) {
+          if (hw) p -= rw;
+          p -= ke;
+          p -= kn;
+        } else {
+          p -= ks + kn;
+          if (hw) p -= rw;
+          p -= ke;
+        }
+        if (ht) p -= rt;
+        if (zw) p -= wz;
+        if (yw) p -= wy;
+      } else {
+        const double dzz = dzs[k + 1], sz = dz + dzz;
+        const double ay = dy * nu, ax = dx * nu;
+        if (hw) rw = ay * sz / (dx + dxs[i - 1]);
+        const double ke = ay * sz / (dx + dxs[(c + 1) % N]);
+        re = ke;
+        if (hs) rs = ax * sz / (dy + dys[j - 1]);
+        if (hn) rn = ax * sz / (dy + dys[j + 1]);
+        const double wy = yw ? ax * sz / dy : 0.0;
+        const double a2 = dx * dy, d2 = nu * a2 * a2;
+        const double kb = d2 * iv(c), kt = d2 * iv(c + NM);
+        rb = kb;
+        rt = kt;
+        if (hb_(t)) {
+          const double wb = flow ? 0.0 : kb;
+          if (hs) p -= rs;
+          if (hw) p -= rw;
+          p -= ke;
+          if (hn) p -= rn;
+          p -= kt + wb;
+        } else if (ht_(t)) {
+          const double wt = flow ? 0.0 : kt;
+          p -= kb + wt;
+          if (hs) p -= rs;
+          if (hw) p -= rw;
+          p -= ke;
+          if (hn) p -= rn;
+        } else if (t == N_Border) {
+          p -= rs;
+          p -= rw;
+          p -= kb + kt;
+          p -= ke;
+        } else {
+          p -= kb + kt;
+          if (hs) p -= rs;
+          if (hw) p -= rw;
+          p -= ke;
+          if (hn) p -= rn;
+        }
+        if (yw) p -= wy;
+      }
+      rp = p;
+    } else {
+      rp = PLUS_TINY;
+    }
+    if (flow && t != SolidCell && i == Nd - 1) {  // outflow term (addPlusOutFlow)
+      if (dir == 0)
+        rp += nu * dys[j] * dzs[k] / dxs[N - 1];
+      else if (dir == 1)
+        rp += nu * (dys[j] + dys[j + 1]) * dzs[k] / dxs[N - 1];
+      else
+        rp += nu * (dzs[k] + dzs[k + 1]) * dys[j] / dxs[N - 1];
+    }
+    const bool fl = t == FluidCell, so = t == SolidCell;
+    out[q] = (fl || (!so && !hb_(t))) ? rb : 0.0;
+    out[nd + q] = (fl || (!so && !hs_(t))) ? rs : 0.0;
+    out[2 * nd + q] = (fl || (!so && !hw_(t))) ? rw : 0.0;
+    out[3 * nd + q] = rp;
+    out[4 * nd + q] = (fl || (!so && !he_(t))) ? re : 0.0;
+    out[5 * nd + q] = (fl || (!so && !hn_(t))) ? rn : 0.0;
+    out[6 * nd + q] = (fl || (!so && !ht_(t))) ? rt : 0.0;
+  }
+}
+
 int grid_1d(long n) {
   long g = (n + 255) / 256;
   if (g > 16384) g = 16384;
@@ -746,6 +955,13 @@ void vof(const int* type, const double* ax, const double* ay, const double* az, 
   F.N = N;
   F.NM = N * M;
   k_vof<<<grid_1d((long)N * M * K), 256, 0, s>>>(type, F, dx, dy, dz, N, M, K, v);
+}
+
+void helmholtz7(const int* type, const double* tx, const double* ty, const double* tz, const double* v,
+                const double* dx, const double* dy, const double* dz, int N, int M, int K, int dir, double nu,
+                double coef_time, int flow, double* coef, hipStream_t s) {
+  const long nd = (long)(N - (dir == 0)) * (M - (dir == 1)) * (K - (dir == 2));
+  k_helmholtz7<<<grid_1d(nd), 256, 0, s>>>(type, tx, ty, tz, v, dx, dy, dz, N, M, K, dir, nu, coef_time, flow, coef);
 }
 
 void laplace7(const int* type, const double* tx, const double* ty, const double* tz, const double* v, const double* dx,

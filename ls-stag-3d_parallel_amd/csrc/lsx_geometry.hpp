@@ -47,6 +47,12 @@ void theta(const double* ax, const double* ay, const double* az, const int* sx, 
 void vof(const int* type, const double* ax, const double* ay, const double* az, const double* tx, const double* ty,
          const double* tz, const double* dx, const double* dy, const double* dz, int N, int M, int K, double* v,
          hipStream_t s);
+// momentum operator of velocity component dir (0 u, 1 v, 2 w) on its staggered mesh
+// ((N-1) x M x K, N x (M-1) x K or N x M x (K-1)) as 7 planes [b,s,w,p,e,n,t] of that mesh's size;
+// a box without cut cells (the caller checks)
+void helmholtz7(const int* type, const double* tx, const double* ty, const double* tz, const double* v,
+                const double* dx, const double* dy, const double* dz, int N, int M, int K, int dir, double nu,
+                double coef_time, int flow, double* coef, hipStream_t s);
 void laplace7(const int* type, const double* tx, const double* ty, const double* tz, const double* v, const double* dx,
               const double* dy, const double* dz, int N, int M, int K, int pin, double* coef, hipStream_t s);
 

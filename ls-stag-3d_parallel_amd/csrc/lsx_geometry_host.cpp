@@ -326,6 +326,20 @@ void Geometry::fields(int* htype, double* hthx, double* hthy, double* hthz, doub
   LSX_HIP(hipStreamSynchronize(stream));
 }
 
+void Geometry::helmholtz7(int dir, double nu, double coef_time, int flow, double* coef_dev, hipStream_t s) const {
+  if (!built) throw Error(LSX_ERR_STATE, "geometry not built");
+  if (dir < 0 || dir > 2) throw Error(LSX_ERR_ARG, "velocity component must be 0 (u), 1 (v) or 2 (w)");
+  if (!(nu > 0.0) || !(coef_time >= 0.0)) throw Error(LSX_ERR_ARG, "nu must be positive and coef_time >= 0");
+  std::vector<int> t(n());
+  LSX_HIP(hipMemcpyAsync(t.data(), type.p, n() * sizeof(int), hipMemcpyDeviceToHost, stream));
+  LSX_HIP(hipStreamSynchronize(stream));
+  for (int v : t)
+    if (v > 0 && v != geo::FluidCell)
+      throw Error(LSX_ERR_ARG, "the momentum operators are assembled for boxes without cut cells");
+  geo::helmholtz7(type.p, tx.p, ty.p, tz.p, vof.p, dx.p, dy.p, dz.p, N, M, K, dir, nu, coef_time, flow, coef_dev, s);
+  LSX_HIP(hipGetLastError());
+}
+
 void Geometry::laplace7(int pin, double* coef_dev, hipStream_t s) const {
   if (!built) throw Error(LSX_ERR_STATE, "geometry not built");
   // the geometry stream must have finished before another stream reads the fields

@@ -47,6 +47,9 @@ class Geometry {
   void fields(int* type, double* thx, double* thy, double* thz, double* vof, double* phi) const;
   // 7 coefficient planes [b,s,w,p,e,n,t] of the pressure operator (device pointer, 7*n doubles).
   void laplace7(int pin, double* coef_dev, hipStream_t s) const;
+  // the same for the momentum operator of component dir on its staggered mesh (7 * that mesh's
+  // cell count doubles); the geometry must have no cut cells
+  void helmholtz7(int dir, double nu, double coef_time, int flow, double* coef_dev, hipStream_t s) const;
   // b = D u + flux (pressure right-hand side); arrays in `mem` space (LSX_MEM_HOST / DEVICE)
   void pressure_rhs(const double* u, const double* v, const double* w, const double* u_west, const double* u_out,
                     const lsx_flow_bc& bc, double* b, int mem);

@@ -751,6 +751,16 @@ __global__ void k_scale_copy(const double* __restrict__ w, const double* __restr
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) v[i] = f * w[i];
 }
 
+__global__ void k_scale_copy_f32(const double* __restrict__ w, const double* __restrict__ normw, double* __restrict__ v,
+                                 float* __restrict__ vf, long n) {
+  const double f = 1.0 / normw[0];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double a = f * w[i];
+    v[i] = a;
+    vf[i] = (float)a;
+  }
+}
+
 template <int BS>
 __global__ void k_dot_partial(const double* __restrict__ a, const double* __restrict__ b, long n, double* partial) {
   long lo, hi;
@@ -1120,6 +1130,10 @@ void lu_solve(const double* scale, const int* perm, const double* L, const doubl
 }
 
 void copy(const double* a, double* b, long n, hipStream_t s) { k_copy<<<grid_1d(n), 256, 0, s>>>(a, b, n); }
+
+void scale_copy_f32(const double* w, const double* normw, double* v, float* vf, long n, hipStream_t s) {
+  k_scale_copy_f32<<<grid_1d(n), 256, 0, s>>>(w, normw, v, vf, n);
+}
 
 void fill_zero(double* b, long n, hipStream_t s) { k_fill_zero<<<grid_1d(n), 256, 0, s>>>(b, n); }
 

@@ -75,6 +75,8 @@ void lu_solve(const double* scale, const int* perm, const double* L, const doubl
 void copy(const double* a, double* b, long n, hipStream_t s);
 void fill_zero(double* b, long n, hipStream_t s);  // a kernel, not a memset: safe under graph capture
 void scale_copy(const double* w, const double* normw, double* v, long n, hipStream_t s);  // v = (1/normw) w
+// v = (1/normw) w and vf = (float)v in one pass
+void scale_copy_f32(const double* w, const double* normw, double* v, float* vf, long n, hipStream_t s);
 void dot_partial(const double* a, const double* b, long n, double* partial, hipStream_t s);
 void finalize(const double* partial, int nb, double* out, int sqrt_flag, hipStream_t s);
 // w -= h*v ; then partial dot(w, vnext) (vnext may be null => partial sum of w^2)

@@ -247,6 +247,22 @@ void Solver::set_geometry(const Geometry& g, int pin) {
   src_pin = pin;
 }
 
+void Solver::set_helmholtz(const Geometry& g, int dir, double nu, double coef_time, int flow) {
+  if (dir < 0 || dir > 2) throw Error(LSX_ERR_ARG, "velocity component must be 0 (u), 1 (v) or 2 (w)");
+  const int N = g.N - (dir == 0), M = g.M - (dir == 1), K = g.K - (dir == 2);
+  check_dims(N, M, K);
+  src_geo = nullptr;
+  ready = false;
+  values_dirty = false;
+  setup_levels(N, M, K);
+  g.helmholtz7(dir, nu, coef_time, flow, levels[0].coef.p, stream);
+  LSX_HIP(hipStreamSynchronize(stream));
+  from_csr = false;
+  build_hierarchy();
+  alloc_krylov();
+  ready = true;
+}
+
 void Solver::set_fields(int N, int M, int K, const double* dx, const double* dy, const double* dz, const int* type,
                         const double* thx, const double* thy, const double* thz, const double* vof, int pin, int mem) {
   check_dims(N, M, K);
@@ -553,19 +569,19 @@ template <>
 double* Solver::vr<double>(VStore& S) { return S.dr.p; }
 
 template <class T>
-void Solver::vcycle_apply(const double* v, T* z) {
+void Solver::vcycle_apply(const double* v, T* z, bool pre) {
   if (trans) {
     vcycle_apply_tr<T>(v, z);
     return;
   }
   if (!opt.graphs || capturing) {
-    vcycle_run<T>(v, z);
+    vcycle_run<T>(v, z, pre);
     return;
   }
   // one captured graph per (v, z) pair: the Krylov loops apply the cycle to a fixed set of
   // basis slots, so a graph replays every launch of a cycle with one submission
   for (const CycleGraph& g : cgraphs)
-    if (g.v == v && g.z == (const void*)z) {
+    if (g.v == v && g.z == (const void*)z && g.pre == pre) {
       LSX_HIP(hipGraphLaunch(g.exec, stream));
       return;
     }
@@ -573,7 +589,7 @@ void Solver::vcycle_apply(const double* v, T* z) {
   LSX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   capturing = true;
   try {
-    vcycle_run<T>(v, z);
+    vcycle_run<T>(v, z, pre);
   } catch (...) {
     capturing = false;
     (void)hipStreamEndCapture(stream, &graph);
@@ -586,17 +602,17 @@ void Solver::vcycle_apply(const double* v, T* z) {
   const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   LSX_HIP(ie);
-  cgraphs.push_back({v, (const void*)z, exec});
+  cgraphs.push_back({v, (const void*)z, pre, exec});
   LSX_HIP(hipGraphLaunch(exec, stream));
 }
 
 template <class T>
-void Solver::vcycle_run(const double* v, T* z) {
+void Solver::vcycle_run(const double* v, T* z, bool pre) {
   const int L = (int)vlev.size() - 1;
   // level 0 right-hand side: the input itself in FP64, an FP32 copy in mixed precision
   const T* b0;
   if constexpr (std::is_same<T, float>::value) {
-    vc::convert<double, float>(v, vlev[0].fb.p, d0.n(), stream);
+    if (!pre) vc::convert<double, float>(v, vlev[0].fb.p, d0.n(), stream);
     b0 = vlev[0].fb.p;
   } else {
     b0 = v;
@@ -959,13 +975,17 @@ int Solver::fgmres(const double* b, double* x) {
       double* vj = V.p + (size_t)j * n;
       double* zj = m32 ? nullptr : Z.p + (size_t)j * n;
       float* zj32 = m32 ? Z32.p + (size_t)j * n : nullptr;
-      scale_copy(w.p, dnorm, vj, n, stream);
-      if (m32)
-        vcycle_apply<float>(vj, zj32);
-      else if (vcyc())
-        vcycle_apply<double>(vj, zj);
-      else
-        mg_apply(vj, zj);
+      if (m32) {
+        // v_j = w / ||w|| and its FP32 copy (the cycle's finest right-hand side) in one pass
+        scale_copy_f32(w.p, dnorm, vj, vlev[0].fb.p, n, stream);
+        vcycle_apply<float>(vj, zj32, true);
+      } else {
+        scale_copy(w.p, dnorm, vj, n, stream);
+        if (vcyc())
+          vcycle_apply<double>(vj, zj);
+        else
+          mg_apply(vj, zj);
+      }
       hipEvent_t e0 = nullptr;
       tbeg(LSX_K_OPERATOR, &e0);
       if (m32) {

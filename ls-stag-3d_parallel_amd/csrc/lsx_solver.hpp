@@ -83,6 +83,10 @@ class Solver {
   bool single_level() const { return opt.mg != LSX_MG_VCYCLE && opt.levels == 1; }
   // the pressure operator assembled on the device from the caller's mesh fields (widths, cell
   // types, face fractions, cut volumes; `mem` space), then the hierarchy
+  // the momentum operator of velocity component dir of `g` (a box without cut cells) on its
+  // staggered mesh, assembled on the device, then the hierarchy (the one-level cascade for its odd
+  // dimension)
+  void set_helmholtz(const Geometry& g, int dir, double nu, double coef_time, int flow);
   void set_fields(int N, int M, int K, const double* dx, const double* dy, const double* dz, const int* type,
                   const double* thx, const double* thy, const double* thz, const double* vof, int pin, int mem);
   // the geometry (and its build and pin choice) the current operator was assembled from
@@ -207,15 +211,17 @@ class Solver {
   bool vcyc() const { return opt.mg == LSX_MG_VCYCLE; }
   void build_vcycle();
   template <class T>
-  void vcycle_apply(const double* v, T* z);
+  // pre: (mixed) the FP32 copy of v is already in level 0's right-hand side buffer
+  void vcycle_apply(const double* v, T* z, bool pre = false);
   template <class T>
   void vcycle_apply_tr(const double* v, T* z);
   template <class T>
-  void vcycle_run(const double* v, T* z);
+  void vcycle_run(const double* v, T* z, bool pre = false);
   // captured V-cycle applications, one per (input, output) pair (opt.graphs)
   struct CycleGraph {
     const double* v;
     const void* z;
+    bool pre;
     hipGraphExec_t exec;
   };
   std::vector<CycleGraph> cgraphs;

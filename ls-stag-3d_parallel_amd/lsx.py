@@ -134,6 +134,7 @@ def lib():
     L.lsx_geometry_fields.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.lsx_set_geometry.argtypes = [vp, vp, C.c_int]
     L.lsx_set_fields.argtypes = [vp, C.c_int, C.c_int, C.c_int] + [vp] * 8 + [C.c_int, C.c_int]
+    L.lsx_set_helmholtz.argtypes = [vp, vp, C.c_int, C.c_double, C.c_double, C.c_int]
     L.lsx_geometry_rhs.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(FlowBc), vp, C.c_int]
     L.lsx_geometry_correct.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_int]
     L.lsx_operator_diagonal.argtypes = [vp, vp]
@@ -533,6 +534,15 @@ class IterativeSolver:
         self._make(task)
         _check(lib().lsx_set_fields(self.h, N, M, K, *[_ptr(a) for a in w], _ptr(t), *[_ptr(a) for a in f],
                                     1 if pin else 0, MEM_HOST))
+        self._rhs = rhs
+        self._sol = sol
+
+    def set_helmholtz(self, geometry, component, nu, coef_time, rhs, sol, task, flow=True):
+        """Like set(), with the momentum operator of velocity component 0/1/2 of `geometry` (a
+        box without cut cells) assembled on the device on its staggered mesh."""
+        self._make(task)
+        _check(lib().lsx_set_helmholtz(self.h, geometry.h, int(component), float(nu), float(coef_time),
+                                       1 if flow else 0))
         self._rhs = rhs
         self._sol = sol
 

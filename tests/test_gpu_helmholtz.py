@@ -54,15 +54,15 @@ def system(dims, ratios, d, seed, flow=True, re=RE, dt=DT):
 
 
 # (grid, stretching, Re, dt): the time-stepped run's Re = 300, dt = 0.002 (a mass-dominated system,
-# 2 iterations) and a viscous one (Re = 10, dt = 0.02: 8 to 34 iterations) that exercises the
-# Krylov loop
+# 2 iterations) and viscous ones that exercise the Krylov loop (Re = 10 at 48x32x40: 8 to 34
+# iterations; Re = 30 at 64^3: 6 to 24)
 CASES = [((32, 24, 20), (1.04, 1.06, 1.02), RE, DT), ((64, 64, 64), (1.0, 1.0, 1.0), RE, DT),
-         ((48, 32, 40), (1.02, 1.02, 1.02), 10.0, 0.02), ((64, 64, 64), (1.0, 1.0, 1.0), 10.0, 0.02)]
+         ((48, 32, 40), (1.02, 1.02, 1.02), 10.0, 0.02), ((64, 64, 64), (1.0, 1.0, 1.0), 30.0, 0.02)]
 
 
 @pytest.mark.parametrize("kind", ["fgmres", "bicgstab"])
 @pytest.mark.parametrize("d", [0, 1, 2])
-@pytest.mark.parametrize("dims,ratios,re,dt", CASES, ids=["32x24x20s", "64cube", "48x32x40s-re10", "64cube-re10"])
+@pytest.mark.parametrize("dims,ratios,re,dt", CASES, ids=["32x24x20s", "64cube", "48x32x40s-re10", "64cube-re30"])
 def test_helmholtz_solve_matches_oracle(lsx, dims, ratios, re, dt, d, kind):
     A, b = system(dims, ratios, d, seed=7 + d, re=re, dt=dt)
     eps = 1e-8 * np.linalg.norm(b)
@@ -97,3 +97,52 @@ def test_one_level_rejected_for_vcycle_and_odd_grid_needs_it(lsx):
         lsx.MGFGMRES().set(A, b, np.zeros(A.n), lsx.Task(eps=1e-8))
     with pytest.raises(lsx.LsxError, match="levels"):
         lsx.solver("fgmres", "fp64-vcycle", levels=1).set(A, b, np.zeros(A.n), lsx.mixed_task(1e-8))
+
+
+def dense7(A):
+    """The CSR operator as 7 stencil planes [b, s, w, p, e, n, t] of its mesh (level_stencil layout)."""
+    N, M, K = A.dims
+    n = A.n
+    out = np.zeros((7, n))
+    offs = {-N * M: 0, -N: 1, -1: 2, 0: 3, 1: 4, N: 5, N * M: 6}
+    rows = np.repeat(np.arange(n), np.diff(A.rowptr))
+    q = np.array([offs[c - r] for c, r in zip(A.col, rows)])
+    out[q, rows] = A.val
+    return out
+
+
+@pytest.mark.parametrize("flow", [True, False])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_device_helmholtz_operator_bitwise(lsx, d, flow):
+    """lsx_set_helmholtz: the momentum operator assembled on the device from a GPU geometry equals
+    the oracle's restatement bit for bit, and its solve equals the oracle's."""
+    dims, ratios = (24, 20, 16), (1.05, 1.03, 1.07)
+    w = [axis(m, r) for m, r in zip(dims, ratios)]
+    g_ref = O.Geometry(*w)  # no body
+    g = lsx.Geometry(*w).build("sphere", (-5.0, -5.0, -5.0), (0.01,))  # a body outside the box
+    fr, fg = g_ref.fields(), g.fields()
+    for key in ("type", "thx", "thy", "thz", "vof"):
+        assert np.array_equal(fr[key], fg[key]), key
+    A = O.helmholtz(g_ref, d, NU, COEF, flow=flow)
+    rng = np.random.default_rng(11 + d)
+    b = rng.standard_normal(A.n) * 1e-3
+    eps = 1e-8 * np.linalg.norm(b)
+    x = np.zeros(A.n)
+    S = lsx.MGFGMRES(levels=1)
+    S.set_helmholtz(g, d, NU, COEF, b, x, lsx.Task(eps=eps, MAX_ITER=200, FGMRES_RESTART_NUM=30), flow=flow)
+    st = S.level_stencil(0)
+    ref = dense7(A)
+    print(f"component {'uvw'[d]} flow={flow}: mesh {A.dims}, max |stencil - oracle| = {np.abs(st - ref).max():.3e}",
+          flush=True)
+    assert np.array_equal(st, ref)
+    it = S.solve()
+    x_ref, it_ref, _ = O.Solver(A, eps=eps, restart=30, max_it=200, precond="mg1").solve(b)
+    assert it == it_ref and np.linalg.norm(x - x_ref) <= 1e-10 * np.linalg.norm(x_ref)
+
+
+def test_device_helmholtz_rejects_cut_cells(lsx):
+    w = [np.full(16, 1.0 / 16)] * 3
+    g = lsx.Geometry(*w).build("sphere", (0.5, 0.5, 0.5), (0.2,))
+    S = lsx.MGFGMRES(levels=1)
+    with pytest.raises(lsx.LsxError, match="without cut cells"):
+        S.set_helmholtz(g, 0, NU, COEF, np.zeros(15 * 16 * 16), np.zeros(15 * 16 * 16), lsx.Task(eps=1e-8))
