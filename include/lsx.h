@@ -70,6 +70,10 @@ typedef struct lsx_options {
                         >= 2^23 cells (default), 2 = on every level, 0 = never (half-sweeps)    */
   int line_levels;   /* V-cycle with LSX_SMOOTH_LINE: the finest line_levels levels are smoothed
                         with lines, coarser ones with red-black sweeps; 0 = every level (default) */
+  int diag_faces;    /* V-cycle: 1 = on levels whose rows sum to zero (checked at set-up: every
+                        pressure operator; pinned cells and their coarse sinks excepted), the line
+                        passes and the residual restriction rebuild a_p from the faces instead of
+                        reading 1/a_p (default); 0 = always read 1/a_p                            */
 } lsx_options;
 
 #define LSX_SMOOTH_RBGS 0 /* red-black Gauss-Seidel                                             */

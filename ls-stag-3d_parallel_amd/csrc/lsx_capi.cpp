@@ -73,6 +73,7 @@ void lsx_options_default(lsx_options* o) {
   o->smoother = LSX_SMOOTH_RBGS;
   o->fused_sweep = 1;
   o->line_levels = 0;
+  o->diag_faces = 1;
 }
 
 int lsx_create(lsx_solver** out, const lsx_options* opt, void* stream) {

@@ -439,6 +439,9 @@ void Solver::build_vcycle() {
     return (x >> l) + d.N * ((y >> l) + d.M * (z >> l));
   };
   const int fp0 = pins[0], fp1 = pins.size() > 1 ? pins[1] : -1;
+  DevBuf<double> dev;  // per-row deviation from the diagonal-from-faces rule
+  if (opt.diag_faces) dev.alloc(d0.n());
+  double sig_h[2] = {0.0, 0.0};
   for (int l = 0; l <= L; ++l) {
     const Dims d = levels[l].d;
     const long nl = d.n();
@@ -451,6 +454,8 @@ void Solver::build_vcycle() {
     if (l == 0) {
       vc::faces_from7(levels[0].coef.p, d, pin0, pin1, fe[cur].p, fn[cur].p, ft[cur].p, dg[cur].p, stream);
       vc::pin_sigma(fe[cur].p, fn[cur].p, ft[cur].p, d, pin0, pin1, sigma, stream);
+      sig_h[0] = reduce_to_host(sigma);
+      sig_h[1] = reduce_to_host(sigma + 1);
     } else {
       const Dims df = levels[l - 1].d;
       const int nxt = cur ^ 1;
@@ -458,6 +463,41 @@ void Solver::build_vcycle() {
       vc::diag_from_faces(fe[nxt].p, fn[nxt].p, ft[nxt].p, d, dg[nxt].p, stream);
       vc::pin_sink(dg[nxt].p, coarse_of(fp0, l), coarse_of(fp1, l), sigma, stream);
       cur = nxt;
+    }
+    // FP16 faces (mixed: coarse levels; mixed16: every level) are stored divided by a per-level
+    // scale that puts the largest at 16384
+    const bool f16 = m32 && !(l == 0 && opt.precision != LSX_PREC_MIXED16);
+    double fscale = 1.0;
+    if (l < L && f16) {
+      vc::face_absmax(fe[cur].p, fn[cur].p, ft[cur].p, nl, partial.p, scalars.p + SC_TMP, stream);
+      const double amax = reduce_to_host(scalars.p + SC_TMP);
+      fscale = amax > 0 ? amax / 16384.0 : 1.0;
+    }
+    if (l < L && opt.diag_faces && nl > vc::COARSE_CELLS) {
+      // a_p = -(sum of the faces) - sigma on every row but the pinned ones, rows without faces
+      // holding 0 or one common value: true of every pressure operator (a Neumann Laplacian whose
+      // solid cells carry a tiny diagonal) and of its aggregated levels; an operator with a mass
+      // term (a momentum operator) fails the check and its levels read 1/a_p.  The small levels
+      // keep 1/a_p, so that their per-level launches stay bit-identical to the one-workgroup
+      // coarse cycle.
+      const int c0 = l == 0 ? -1 : coarse_of(fp0, l), c1 = l == 0 ? -1 : coarse_of(fp1, l);
+      double* out = scalars.p + SC_TMP;
+      auto rule = [&](double dz, int mode) {
+        vc::diag_rule(fe[cur].p, fn[cur].p, ft[cur].p, dg[cur].p, d, pin0, pin1, c0, c1, sig_h[0], sig_h[1], dz,
+                      f16 ? 1.0 / fscale : 0.0, mode, dev.p, partial.p, out, stream);
+      };
+      rule(0.0, 0);
+      const double dz = reduce_to_host(out);
+      rule(dz, 1);
+      if (reduce_to_host(out) <= 1e-12) {
+        rule(dz, 2);  // -0.0 own east faces of the rows without faces whose a_p is dz
+        S.L.dff = 1;
+        S.L.sink0 = c0;
+        S.L.sink1 = c1;
+        S.L.sig0 = sig_h[0];
+        S.L.sig1 = sig_h[1];
+        S.L.dz = dz;
+      }
     }
     if (l == L) {  // coarsest: FP64 LU of the 7-point operator, or its dense inverse (capped levels)
       if (nl <= 64) {
@@ -531,9 +571,7 @@ void Solver::build_vcycle() {
       S.L.f_t = S.ft.p;
       S.L.fscale = 1.0;
     } else {
-      vc::face_absmax(fe[cur].p, fn[cur].p, ft[cur].p, nl, partial.p, scalars.p + SC_TMP, stream);
-      const double amax = reduce_to_host(scalars.p + SC_TMP);
-      const double scale = amax > 0 ? amax / 16384.0 : 1.0;
+      const double scale = fscale;
       S.he.alloc(nl);
       S.hn.alloc(nl);
       S.ht.alloc(nl);
@@ -663,9 +701,14 @@ template <class T>
 void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess, bool rev_lines) {
   VStore& S = vlev[l];
   // finest-level launches are timed one by one: compulsory bytes per cell of a full red-black
-  // sweep or of one line pass (x in and out, b, 1/a_p, three faces)
+  // sweep or of one line pass (x in and out, b, 1/a_p, three faces; a line pass from a zero
+  // start reads no x, one that rebuilds a_p from the faces no 1/a_p)
   const double vbytes = sizeof(T), fbytes = S.L.ftype == vc::FACE_F16 ? 2.0 : (S.L.ftype == vc::FACE_F32 ? 4.0 : 8.0);
   const double pass_bytes = (4.0 * vbytes + 3.0 * fbytes) * (double)S.L.d.n();
+  auto line_bytes = [&](bool zero) {
+    const double v = 4.0 - (S.L.dff && !trans ? 1.0 : 0.0) - (zero ? 1.0 : 0.0);
+    return (v * vbytes + 3.0 * fbytes) * (double)S.L.d.n();
+  };
   hipEvent_t e0 = nullptr;
   auto t0 = [&](int k) {
     if (l == 0) tbeg(k, &e0);
@@ -731,7 +774,7 @@ void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess, bool r
     const int dir = rev_lines ? 2 - (p - 1) % 3 : (p - 1) % 3;
     t0(LSX_K_LINE_X + dir);
     vc::line_pass<T>(S.L, dir, src, dst, b, zero_guess && p == 1, trans ? 1 : 0, stream);
-    t1(LSX_K_LINE_X + dir, pass_bytes);
+    t1(LSX_K_LINE_X + dir, line_bytes(zero_guess && p == 1));
     src = dst;
   }
 }

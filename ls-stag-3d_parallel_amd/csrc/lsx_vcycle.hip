@@ -50,6 +50,25 @@ __device__ inline int xcd_order(int b, int nb) {
   return x * q + (x < r ? x : r) + k;
 }
 
+// a_p of a non-pinned row of a Level::dff level from the sum of its (scaled, present) faces
+// (own_e: the row's own east face as stored, -0.0 marking a decoupled row whose a_p is dz)
+template <class T>
+struct DiagRule {
+  int c0, c1;  // sink cells (-1: none)
+  T s0, s1, dz;
+  __device__ inline T of(T fsum, long i, T own_e) const { return of_marked(fsum, i, __builtin_signbit(own_e)); }
+  __device__ inline T of_marked(T fsum, long i, bool marked) const {
+    T d = -fsum;
+    d = i == c0 ? d - s0 : d;
+    d = i == c1 ? d - s1 : d;
+    return d == (T)0 && marked ? dz : d;
+  }
+};
+template <class T>
+inline DiagRule<T> diag_rule(const Level& L) {
+  return DiagRule<T>{L.sink0, L.sink1, (T)L.sig0, (T)L.sig1, (T)L.dz};
+}
+
 // ---- setup -----------------------------------------------------------------------------------
 __global__ void k_faces_from7(const double* __restrict__ c7, int N, int M, int K, int pin0, int pin1,
                               double* __restrict__ fe, double* __restrict__ fn, double* __restrict__ ft,
@@ -123,6 +142,67 @@ __global__ void k_pin_sigma(const double* __restrict__ fe, const double* __restr
     if (z > 0) sgm += ft[i - NM];
   }
   sigma[q] = sgm;
+}
+
+// Level::dff set-up.  Row i's rebuilt diagonal: d = -(sum of its present faces) - sigma (sink
+// cells); a decoupled row (d = 0) is an empty row (a_p = 0) or, when its own east face is stored
+// as -0.0, a row whose a_p is the level's common value dz (the reference's solid cells carry a
+// tiny positive diagonal).  mode 0: v_i = |a_p| of the decoupled non-pinned rows (absmax: dz);
+// mode 1: v_i = deviation of a_p from the rule (0 on pinned rows; decoupled rows 0 when a_p is 0
+// or dz, else 1; with hscale > 0, 1 on a coupled row whose faces all round to 0 as FP16 faces
+// stored times hscale); mode 2: fe_i = -0.0 on the decoupled rows with a_p = dz != 0.
+__device__ inline double face_sum(const double* __restrict__ fe, const double* __restrict__ fn,
+                                  const double* __restrict__ ft, long i, int x, int y, int z, int N, long NM,
+                                  double hscale, bool* under) {
+  double f = fe[i] + fn[i] + ft[i], h = 0.0;
+  auto q = [&](double v) { return (double)(_Float16)(v * hscale); };
+  if (hscale > 0.0) h = q(fe[i]) + q(fn[i]) + q(ft[i]);
+  if (x > 0) {
+    f += fe[i - 1];
+    if (hscale > 0.0) h += q(fe[i - 1]);
+  }
+  if (y > 0) {
+    f += fn[i - N];
+    if (hscale > 0.0) h += q(fn[i - N]);
+  }
+  if (z > 0) {
+    f += ft[i - NM];
+    if (hscale > 0.0) h += q(ft[i - NM]);
+  }
+  *under = hscale > 0.0 && f != 0.0 && h == 0.0;
+  return f;
+}
+
+__global__ void k_diag_rule(double* __restrict__ fe, const double* __restrict__ fn, const double* __restrict__ ft,
+                            const double* __restrict__ dg, int N, int M, int K, int pin0, int pin1, int c0, int c1,
+                            double s0, double s1, double dz, double hscale, int mode, double* __restrict__ v) {
+  const long n = (long)N * M * K, NM = (long)N * M;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    bool under;
+    double d = -face_sum(fe, fn, ft, i, x, y, z, N, NM, hscale, &under);
+    if (i == c0) d -= s0;
+    if (i == c1) d -= s1;
+    const double a = dg[i];
+    const bool pin = i == pin0 || i == pin1;
+    if (mode == 0) {
+      v[i] = !pin && d == 0.0 ? fabs(a) : 0.0;
+    } else if (mode == 1) {
+      double e;
+      if (pin)
+        e = 0.0;
+      else if (d == 0.0)
+        e = a == 0.0 || a == dz ? 0.0 : 1.0;
+      else if (a == 0.0 || under)
+        e = 1.0;
+      else
+        e = fabs(a - d) / fabs(a);
+      v[i] = e;
+    } else if (!pin && d == 0.0 && dz != 0.0 && a == dz) {
+      fe[i] = -0.0;
+    }
+  }
 }
 
 __global__ void k_pin_sink(double* __restrict__ dg, int c0, int c1, const double* __restrict__ sigma) {
@@ -231,6 +311,26 @@ __device__ inline T offdiag_u(const T* __restrict__ x, const FT* __restrict__ ae
   return acc;
 }
 
+// offdiag_u with the sum of the present faces and the row's own east face as stored (the inputs
+// of DiagRule::of)
+template <class T, class FT>
+__device__ inline T offdiag_fsum(const T* __restrict__ x, const FT* __restrict__ ae, const FT* __restrict__ an,
+                                 const FT* __restrict__ at, long i, int xx, int yy, int zz, int N, int M, int K,
+                                 T& fsum, T& own_e) {
+  const long NM = (long)N * M;
+  const bool hw = xx > 0, he = xx < N - 1, hs = yy > 0, hn = yy < M - 1, hb = zz > 0, ht = zz < K - 1;
+  const long iw = hw ? i - 1 : i, ie = he ? i + 1 : i, is = hs ? i - N : i, in = hn ? i + N : i;
+  const long ib = hb ? i - NM : i, it = ht ? i + NM : i;
+  const T fw0 = ld<T>(ae, iw), fe0 = ld<T>(ae, i), fs0 = ld<T>(an, is), fn0 = ld<T>(an, i);
+  const T fb0 = ld<T>(at, ib), ft0 = ld<T>(at, i);
+  const T xw = x[iw], xe = x[ie], xs = x[is], xn = x[in], xb = x[ib], xt = x[it];
+  const T fw = hw ? fw0 : (T)0, fe = he ? fe0 : (T)0, fs_ = hs ? fs0 : (T)0, fn = hn ? fn0 : (T)0;
+  const T fb = hb ? fb0 : (T)0, ft = ht ? ft0 : (T)0;
+  fsum = fw + fe + fs_ + fn + fb + ft;
+  own_e = fe0;
+  return fw * xw + fe * xe + fs_ * xs + fn * xn + fb * xb + ft * xt;
+}
+
 // one colour of a red-black sweep; work item t (from start, by stride) owns the t-th cell of it.
 // zero_guess: 0 normal; 1 x is zero before this half-sweep; 2 first update of this colour after
 // a zero start (its values are not read).  An empty row (1/a_p stored as 0) keeps its value, so
@@ -299,12 +399,13 @@ __device__ inline void resid_restrict_body(const T* __restrict__ x, const T* __r
 // 2 z per workgroup, so loads coalesce and each row's loads are in flight together), residuals
 // through LDS, and the 8 children of each of the 32 coarse cells summed by one thread in the
 // order of resid_restrict_body (bit-identical results).
-template <class T, class FT>
+// DFF: a_p rebuilt from the faces (Level::dff), s not read; no bitwise twin then.
+template <class T, class FT, bool DFF>
 __global__ void __launch_bounds__(256) k_resid_restrict_cells(const T* __restrict__ x, const T* __restrict__ b,
                                                               const FT* __restrict__ ae, const FT* __restrict__ an,
                                                               const FT* __restrict__ at, const T* __restrict__ s,
                                                               T fs, int N, int M, int K, int pin0, int pin1,
-                                                              T* __restrict__ bc) {
+                                                              T* __restrict__ bc, DiagRule<T> dr) {
   __shared__ T rr[2][2][64];
   const int tx = threadIdx.x & 63, ty = (threadIdx.x >> 6) & 1, tz = threadIdx.x >> 7;
   const int Nc = N / 2, Mc = M / 2;
@@ -317,11 +418,19 @@ __global__ void __launch_bounds__(256) k_resid_restrict_cells(const T* __restric
   const bool live = xx < N;
   const int xc = live ? xx : N - 1;
   const long i = xc + (long)N * yy + (long)N * M * zz;
-  const T si = s[i];
-  const T ap = si != (T)0 ? (T)1 / si : (T)0;
+  const bool pin = i == pin0 || i == pin1;
+  T ap, od;
+  if constexpr (DFF) {
+    T fsum, own_e;
+    od = offdiag_fsum<T, FT>(x, ae, an, at, i, xc, yy, zz, N, M, K, fsum, own_e);
+    ap = pin ? (T)1 : dr.of(fs * fsum, i, own_e);
+  } else {
+    const T si = s[i];
+    ap = si != (T)0 ? (T)1 / si : (T)0;
+    od = offdiag_u<T, FT>(x, ae, an, at, i, xc, yy, zz, N, M, K);
+  }
   const T r0 = b[i] - ap * x[i];
-  const T od = offdiag_u<T, FT>(x, ae, an, at, i, xc, yy, zz, N, M, K);
-  rr[tz][ty][tx] = (i != pin0 && i != pin1) ? r0 - fs * od : r0;
+  rr[tz][ty][tx] = pin ? r0 : r0 - fs * od;
   __syncthreads();
   if (threadIdx.x < 32) {
     const int X = bx * 32 + threadIdx.x;
@@ -969,12 +1078,13 @@ __device__ inline double recip0(double v) { return v != 0.0 ? 1.0 / v : 0.0; }
 
 // Row of cell i on its line of direction DIR: lower (coupling to the previous cell of the line),
 // diag, upper, rhs.  Pinned rows are identity rows; an empty row (1/a_p stored as 0) keeps its
-// value, as in the point smoother.  ZERO: x taken as 0 (not read).
-template <class T, class FT, int DIR, bool ZERO>
+// value, as in the point smoother.  ZERO: x taken as 0 (not read).  DFF (forward rows only,
+// trans false): a_p rebuilt from the faces by dr, s not read.
+template <class T, class FT, int DIR, bool ZERO, bool DFF = false>
 __device__ inline void line_row(const T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
                                 const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s, T fs,
                                 long i, int xx, int yy, int zz, int N, int M, int K, int pin0, int pin1, bool trans,
-                                T& lo, T& dg, T& up, T& rhs) {
+                                T& lo, T& dg, T& up, T& rhs, DiagRule<T> dr = DiagRule<T>{-1, -1, (T)0, (T)0}) {
   #pragma clang fp contract(fast)  // line solves: no bitwise twin to match
   // Straight-line code: every load is issued unconditionally (absent neighbours read the cell
   // itself and are masked out), so that a caller evaluating many rows can have all their loads
@@ -988,7 +1098,9 @@ __device__ inline void line_row(const T* __restrict__ x, const T* __restrict__ b
   const long iw = xx > 0 ? i - 1 : i, ie = xx < N - 1 ? i + 1 : i;
   const long is = yy > 0 ? i - N : i, in = yy < M - 1 ? i + N : i;
   const long ib = zz > 0 ? i - NM : i, it = zz < K - 1 ? i + NM : i;
-  const T si = s[i], bi = b[i];
+  T si = (T)0;
+  if constexpr (!DFF) si = s[i];
+  const T bi = b[i];
   const T fw = fs * ld<T>(ae, iw), fe = fs * ld<T>(ae, i);
   const T fsf = fs * ld<T>(an, is), fn = fs * ld<T>(an, i);
   const T fb = fs * ld<T>(at, ib), ft = fs * ld<T>(at, i);
@@ -1001,13 +1113,18 @@ __device__ inline void line_row(const T* __restrict__ x, const T* __restrict__ b
     else if constexpr (DIR == 1) off = cw * x[iw] + ce * x[ie] + cb * x[ib] + ct * x[it];
     else off = cw * x[iw] + ce * x[ie] + cs * x[is] + cn * x[in];
   }
-  const bool ident = own && !trans, empty = si == (T)0;
+  T ap;
+  if constexpr (DFF)
+    ap = dr.of(cw + ce + cs + cn + cb + ct, i, fe);
+  else
+    ap = recip0(si);
+  const bool ident = own && !trans, empty = DFF ? ap == (T)0 : si == (T)0;
   const bool cut = ident || empty;  // identity row (pinned) or kept value (empty)
   const T l0 = DIR == 0 ? cw : (DIR == 1 ? cs : cb);
   const T u0 = DIR == 0 ? ce : (DIR == 1 ? cn : ct);
   lo = cut ? (T)0 : l0;
   up = cut ? (T)0 : u0;
-  dg = cut ? (T)1 : recip0(si);
+  dg = cut ? (T)1 : ap;
   rhs = ident ? bi : (empty ? xi : bi - off);
 }
 
@@ -1036,18 +1153,19 @@ __device__ inline void ld_pack(const E* __restrict__ p, long i, E (&out)[C]) {
 // The rows of cells i0 .. i0 + CPL - 1 of one x line (i0 = row start + CPL * lane, N a multiple
 // of CPL): the same rows as line_row, each array read with one wide load per lane.  The lines'
 // neighbours in y and z exist or not for the whole row; pinned cells are checked per cell.
-template <class T, class FT, int CPL, bool ZERO>
+template <class T, class FT, int CPL, bool ZERO, bool DFF>
 __device__ inline void line_rows_x(const T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
                                    const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s,
                                    T fs, long i0, int k0, int yy, int zz, int N, int M, int K, int pin0, int pin1,
-                                   bool trans, T (&lo)[CPL], T (&dg)[CPL], T (&up)[CPL], T (&rh)[CPL]) {
+                                   bool trans, T (&lo)[CPL], T (&dg)[CPL], T (&up)[CPL], T (&rh)[CPL],
+                                   DiagRule<T> dr) {
   #pragma clang fp contract(fast)  // line solves: no bitwise twin to match
   const long NM = (long)N * M;
   const bool ys = yy > 0, yn = yy < M - 1, zb = zz > 0, zt = zz < K - 1;
   const long is0 = ys ? i0 - N : i0, in0 = yn ? i0 + N : i0, ib0 = zb ? i0 - NM : i0, it0 = zt ? i0 + NM : i0;
   T sv[CPL], bv[CPL];
   FT fev[CPL], fnv[CPL], fsv[CPL], ftv[CPL], fbv[CPL];
-  ld_pack<T, CPL>(s, i0, sv);
+  if constexpr (!DFF) ld_pack<T, CPL>(s, i0, sv);
   ld_pack<T, CPL>(b, i0, bv);
   ld_pack<FT, CPL>(ae, i0, fev);
   ld_pack<FT, CPL>(an, i0, fnv);
@@ -1083,37 +1201,47 @@ __device__ inline void line_rows_x(const T* __restrict__ x, const T* __restrict_
       xi = xv[j];
       off = cs * xs[j] + cn * xn[j] + cb * xb[j] + ct * xt[j];
     }
-    const T si = sv[j], bi = bv[j];
-    const bool ident = own && !trans, empty = si == (T)0, cut = ident || empty;
+    const T bi = bv[j];
+    T ap;
+    bool empty;
+    if constexpr (DFF) {
+      ap = dr.of(cw + ce + cs + cn + cb + ct, i, fe);
+      empty = ap == (T)0;
+    } else {
+      ap = recip0(sv[j]);
+      empty = sv[j] == (T)0;
+    }
+    const bool ident = own && !trans, cut = ident || empty;
     lo[j] = cut ? (T)0 : cw;
     up[j] = cut ? (T)0 : ce;
-    dg[j] = cut ? (T)1 : recip0(si);
+    dg[j] = cut ? (T)1 : ap;
     rh[j] = ident ? bi : (empty ? xi : bi - off);
   }
 }
 
-template <class T, class FT, int CPL, bool ZERO, bool TRANS>
+template <class T, class FT, int CPL, bool ZERO, bool TRANS, bool DFF>
 __global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* __restrict__ x_out,
                                                 const T* __restrict__ b, const FT* __restrict__ ae,
                                                 const FT* __restrict__ an, const FT* __restrict__ at,
                                                 const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
-                                                int pin1) {
+                                                int pin1, DiagRule<T> dr) {
   #pragma clang fp contract(fast)  // line solves: no bitwise twin to match
+  static_assert(!(TRANS && DFF), "a_p from the faces is a forward-operator rule");
   const int lane = threadIdx.x & 63;
   const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= (long)M * K) return;  // whole wavefronts leave together
   const int yy = (int)(row % M), zz = (int)(row / M);
   T lo[CPL], dg[CPL], up[CPL], rh[CPL];
   if (N % CPL == 0 && lane * CPL < N) {
-    line_rows_x<T, FT, CPL, ZERO>(x_in, b, ae, an, at, s, fs, row * N + lane * CPL, lane * CPL, yy, zz, N, M, K,
-                                  pin0, pin1, TRANS, lo, dg, up, rh);
+    line_rows_x<T, FT, CPL, ZERO, DFF>(x_in, b, ae, an, at, s, fs, row * N + lane * CPL, lane * CPL, yy, zz, N, M,
+                                       K, pin0, pin1, TRANS, lo, dg, up, rh, dr);
   } else {
 #pragma unroll
     for (int j = 0; j < CPL; ++j) {
       const int k = lane * CPL + j;
       if (k < N) {
-        line_row<T, FT, 0, ZERO>(x_in, b, ae, an, at, s, fs, row * N + k, k, yy, zz, N, M, K, pin0, pin1,
-                                 TRANS, lo[j], dg[j], up[j], rh[j]);
+        line_row<T, FT, 0, ZERO, DFF>(x_in, b, ae, an, at, s, fs, row * N + k, k, yy, zz, N, M, K, pin0, pin1,
+                                      TRANS, lo[j], dg[j], up[j], rh[j], dr);
       } else {  // padding: decoupled identity rows
         lo[j] = up[j] = rh[j] = (T)0;
         dg[j] = (T)1;
@@ -1214,8 +1342,8 @@ __device__ inline E ldo(const E* __restrict__ p, unsigned i) {
 // Row of cell i of a y (DIR 1) or z (DIR 2) line of the forward operator, with the line's
 // per-thread constants: the x neighbours (A) and the other cross direction (B) exist or not for
 // the whole line (offsets of absent ones are 0: the cell itself, masked).  Same values and
-// operand order as line_row.
-template <class T, class FT, int DIR, bool ZERO>
+// operand order as line_row.  DFF: a_p rebuilt from the faces by dr, s not read.
+template <class T, class FT, int DIR, bool ZERO, bool DFF>
 struct SegLine {
   const T* x;
   const T* b;
@@ -1230,6 +1358,7 @@ struct SegLine {
   // oW0 / oE0 from the cell (the same address for the whole group)
   int l32, oW0, oE0;
   bool hAl, hAu, hBl, hBu;
+  DiagRule<T> dr;
   // a face in operator units: FP16 faces are stored divided by fs; FP32 and FP64 faces are
   // stored as they are (fs = 1)
   __device__ inline T scl(T f) const {
@@ -1241,38 +1370,57 @@ struct SegLine {
   __device__ inline void row(unsigned i, int kk, bool first, T& fl_carry, T& lo, T& dg, T& up, T& rh) const {
   #pragma clang fp contract(fast)  // line solves: no bitwise twin to match
     const bool hl = kk > 0, hu = kk < len - 1;
-    const T sv = ldo(s, i), bv = ldo(b, i);
+    T sv = (T)0;
+    if constexpr (!DFF) sv = ldo(s, i);
+    const T bv = ldo(b, i);
     const T fau = (T)ldo(ae, i);
+    const bool marked = DFF && __builtin_signbit(fau);
     const T fbl = (T)ldo(fB, i + oBl), fbu = (T)ldo(fB, i);
     const T flu = (T)ldo(fL, i);
     const T fll = first ? (T)ldo(fL, hl ? i - strideL : i) : fl_carry;  // upper face of the row before
     fl_carry = flu;
-    T off = (T)0, xi = (T)0;
+    T off = (T)0, xi = (T)0, fal = (T)0;
     if constexpr (!ZERO) {
       xi = ldo(x, i);
       const T xbl = ldo(x, i + oBl), xbu = ldo(x, i + oBu);
       const T xw0 = ldo(x, i + oW0), xe0 = ldo(x, i + oE0), fw0 = (T)ldo(ae, i + oW0);
       const T xws = __shfl_up(xi, 1, 32), xes = __shfl_down(xi, 1, 32), fws = __shfl_up(fau, 1, 32);
-      const T xal = l32 == 0 ? xw0 : xws, xau = l32 == 31 ? xe0 : xes, fal = l32 == 0 ? fw0 : fws;
+      const T xal = l32 == 0 ? xw0 : xws, xau = l32 == 31 ? xe0 : xes;
+      fal = l32 == 0 ? fw0 : fws;
       const T cAl = hAl ? scl(fal) : (T)0, cAu = hAu ? scl(fau) : (T)0;
       const T cBl = hBl ? scl(fbl) : (T)0, cBu = hBu ? scl(fbu) : (T)0;
       off = cAl * xal + cAu * xau + cBl * xbl + cBu * xbu;
+    } else if constexpr (DFF) {
+      const T fw0 = (T)ldo(ae, i + oW0), fws = __shfl_up(fau, 1, 32);
+      fal = l32 == 0 ? fw0 : fws;
     }
-    const bool own = (int)i == pin0 || (int)i == pin1, empty = sv == (T)0, cut = own || empty;
+    T ap;
+    bool empty;
+    if constexpr (DFF) {
+      const T fsum = (hAl ? fal : (T)0) + (hAu ? fau : (T)0) + (hBl ? fbl : (T)0) + (hBu ? fbu : (T)0) +
+                     (hl ? fll : (T)0) + (hu ? flu : (T)0);
+      ap = dr.of_marked(scl(fsum), (long)i, marked);
+      empty = ap == (T)0;
+    } else {
+      ap = recip0(sv);
+      empty = sv == (T)0;
+    }
+    const bool own = (int)i == pin0 || (int)i == pin1, cut = own || empty;
     lo = cut || !hl ? (T)0 : scl(fll);
     up = cut || !hu ? (T)0 : scl(flu);
-    dg = cut ? (T)1 : recip0(sv);
+    dg = cut ? (T)1 : ap;
     rh = own ? bv : (empty ? xi : bv - off);
   }
 };
 
-template <class T, class FT, int DIR, bool ZERO, bool TRANS>
+template <class T, class FT, int DIR, bool ZERO, bool TRANS, bool DFF>
 __global__ void __launch_bounds__(1024) k_line_seg(const T* __restrict__ x_in, T* __restrict__ x_out,
                                                    const T* __restrict__ b, const FT* __restrict__ ae,
                                                    const FT* __restrict__ an, const FT* __restrict__ at,
                                                    const T* __restrict__ s, T fs, int N, int M, int K, int pin0,
-                                                   int pin1, int S) {
+                                                   int pin1, int S, DiagRule<T> dr) {
   #pragma clang fp contract(fast)  // line solves: no bitwise twin to match
+  static_assert(!(TRANS && DFF), "a_p from the faces is a forward-operator rule");
   extern __shared__ unsigned char lds_raw[];
   // [8][S][LPW + 1]: a d c r of the reduced rows, G H Q of first cells, x_E; rows padded by one
   // element so that both the (segment, line) and the (line, segment) thread mappings read
@@ -1325,7 +1473,8 @@ __global__ void __launch_bounds__(1024) k_line_seg(const T* __restrict__ x_in, T
         re = last ? rh : re;
       };
       if constexpr (!TRANS) {
-        SegLine<T, FT, DIR, ZERO> Ln;
+        SegLine<T, FT, DIR, ZERO, DFF> Ln;
+        Ln.dr = dr;
         const int nB = DIR == 1 ? K : M, dB = DIR == 1 ? N * M : N;
         Ln.x = x_in;
         Ln.b = b;
@@ -1516,6 +1665,14 @@ void pin_sink(double* dg, int c0, int c1, const double* sigma, hipStream_t s) {
   k_pin_sink<<<1, 64, 0, s>>>(dg, c0, c1, sigma);
 }
 
+void diag_rule(double* fe, const double* fn, const double* ft, const double* dg, Dims d, int pin0, int pin1,
+               int sink0, int sink1, double sig0, double sig1, double dz, double hscale, int mode, double* v,
+               double* partial, double* out, hipStream_t s) {
+  k_diag_rule<<<grid_1d(d.n()), 256, 0, s>>>(fe, fn, ft, dg, d.N, d.M, d.K, pin0, pin1, sink0, sink1, sig0, sig1, dz,
+                                             hscale, mode, v);
+  if (mode != 2) face_absmax(v, v, v, d.n(), partial, out, s);
+}
+
 void face_absmax(const double* fe, const double* fn, const double* ft, long n, double* partial, double* out,
                  hipStream_t s) {
   int nb = (int)((n + 1023) / 1024);
@@ -1608,9 +1765,15 @@ void rbgs<float>(const Level& L, float* x, const float* b, int color, int zero_g
 template <class T, class FT>
 static void resid_restrict_t(const Level& L, const T* x, const T* b, T* bc, hipStream_t s) {
   const long blocks = (long)((L.d.N + 63) / 64) * (L.d.M / 2) * (L.d.K / 2);
-  k_resid_restrict_cells<T, FT><<<(unsigned)blocks, 256, 0, s>>>(x, b, (const FT*)L.f_e, (const FT*)L.f_n, (const FT*)L.f_t,
-                                                     (const T*)L.s, (T)L.fscale, L.d.N, L.d.M, L.d.K, L.pin0,
-                                                     L.pin1, bc);
+#define LSX_RESID(DFF)                                                                                          \
+  k_resid_restrict_cells<T, FT, DFF><<<(unsigned)blocks, 256, 0, s>>>(                                          \
+      x, b, (const FT*)L.f_e, (const FT*)L.f_n, (const FT*)L.f_t, (const T*)L.s, (T)L.fscale, L.d.N, L.d.M, L.d.K, \
+      L.pin0, L.pin1, bc, diag_rule<T>(L))
+  if (L.dff)
+    LSX_RESID(true);
+  else
+    LSX_RESID(false);
+#undef LSX_RESID
 }
 
 template <>
@@ -1684,16 +1847,20 @@ static void line_pass_z(const Level& L, int dir, const T* x_in, T* x_out, const 
   const FT* at = (const FT*)L.f_t;
   const T* sv = (const T*)L.s;
   const T fs = (T)L.fscale;
+  const DiagRule<T> dr = diag_rule<T>(L);
   if (dir == 0) {
     const int blocks = (int)(((long)d.M * d.K + 3) / 4);
 #define LSX_LINE_X(CPL)                                                                                        \
   do {                                                                                                         \
     if (trans)                                                                                                 \
-      k_line_x<T, FT, CPL, ZERO, true><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, \
-                                                              L.pin0, L.pin1);                                 \
+      k_line_x<T, FT, CPL, ZERO, true, false><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N,   \
+                                                                     d.M, d.K, L.pin0, L.pin1, dr);            \
+    else if (L.dff)                                                                                            \
+      k_line_x<T, FT, CPL, ZERO, false, true><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N,   \
+                                                                     d.M, d.K, L.pin0, L.pin1, dr);            \
     else                                                                                                       \
-      k_line_x<T, FT, CPL, ZERO, false><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, \
-                                                               L.pin0, L.pin1);                                \
+      k_line_x<T, FT, CPL, ZERO, false, false><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N,  \
+                                                                      d.M, d.K, L.pin0, L.pin1, dr);           \
   } while (0)
     if (d.N <= 64)
       LSX_LINE_X(1);
@@ -1709,19 +1876,23 @@ static void line_pass_z(const Level& L, int dir, const T* x_in, T* x_out, const 
     const int S = (len + SEG - 1) / SEG;
     const dim3 grid(((d.N + LPW - 1) / LPW) * (dir == 1 ? d.K : d.M));
     const size_t lds = 8 * (size_t)S * (LPW + 1) * sizeof(T);
-#define LSX_LINE_SEG(DIR, TR)                                                                                  \
-  k_line_seg<T, FT, DIR, ZERO, TR><<<grid, LPW * S, lds, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M, d.K, \
-                                                              L.pin0, L.pin1, S)
+#define LSX_LINE_SEG(DIR, TR, DFF)                                                                             \
+  k_line_seg<T, FT, DIR, ZERO, TR, DFF><<<grid, LPW * S, lds, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M,  \
+                                                                   d.K, L.pin0, L.pin1, S, dr)
     if (dir == 1) {
       if (trans)
-        LSX_LINE_SEG(1, true);
+        LSX_LINE_SEG(1, true, false);
+      else if (L.dff)
+        LSX_LINE_SEG(1, false, true);
       else
-        LSX_LINE_SEG(1, false);
+        LSX_LINE_SEG(1, false, false);
     } else {
       if (trans)
-        LSX_LINE_SEG(2, true);
+        LSX_LINE_SEG(2, true, false);
+      else if (L.dff)
+        LSX_LINE_SEG(2, false, true);
       else
-        LSX_LINE_SEG(2, false);
+        LSX_LINE_SEG(2, false, false);
     }
 #undef LSX_LINE_SEG
   }

@@ -19,6 +19,14 @@ enum FaceType { FACE_F64 = 0, FACE_F32 = 1, FACE_F16 = 2 };
 // One level.  Faces: fe[i] couples cell i with i+1 (x), fn[i] with i+N, ft[i] with i+N*M; a
 // face on the domain boundary is 0.  s[i] = 1 / a_p (0 for an empty row).  pin0/pin1 are
 // identity rows (the finest level's pinned cells), -1 on coarse levels.
+//
+// dff: every row other than the pinned ones has a_p = -(sum of its six faces) - sigma, sigma the
+// pinned cells' couplings on the sink cells sink0/sink1 of a coarse level (sig0/sig1) and 0
+// elsewhere; a row without faces has a_p = 0 (empty) or a_p = dz, the level's one common value
+// (the reference's solid cells), and then its own east face is stored as -0.0 (checked and
+// marked at set-up in FP64).  The forward line passes and the residual restriction then rebuild
+// a_p from the faces they load anyway instead of reading s (4 of 28 bytes per cell of an FP32
+// line pass); the other kernels read s.
 struct Level {
   Dims d{0, 0, 0};
   int ftype = FACE_F64;
@@ -26,7 +34,21 @@ struct Level {
   const void* s = nullptr;  // value type of the hierarchy
   double fscale = 1.0;      // faces are stored divided by fscale
   int pin0 = -1, pin1 = -1;
+  int dff = 0;
+  int sink0 = -1, sink1 = -1;
+  double sig0 = 0.0, sig1 = 0.0, dz = 0.0;
 };
+
+// Set-up of Level::dff on the FP64 faces and diagonal of one level (v: a scratch vector of n
+// doubles, partial as for face_absmax, out: one device double):
+//   mode 0: out = max |a_p| over the non-pinned rows without faces (the candidate dz);
+//   mode 1: out = max over the rows of the relative deviation of a_p from the rule with dz (1 for a
+//           row without faces whose a_p is neither 0 nor dz, and with hscale > 0 for a row whose
+//           faces all round to 0 when stored as FP16 times hscale);
+//   mode 2: marks the rows without faces whose a_p is dz != 0 (fe = -0.0).
+void diag_rule(double* fe, const double* fn, const double* ft, const double* dg, Dims d, int pin0, int pin1,
+               int sink0, int sink1, double sig0, double sig1, double dz, double hscale, int mode, double* v,
+               double* partial, double* out, hipStream_t s);
 
 // The small levels handled by one workgroup in a single launch (k_coarse_cycle): levels with at
 // most COARSE_CELLS cells (16^3; a 32^3 level already runs faster on the whole device), down to

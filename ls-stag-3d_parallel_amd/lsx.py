@@ -69,7 +69,7 @@ class Options(C.Structure):
         ("mg", C.c_int), ("n_post", C.c_int), ("n_pre", C.c_int), ("omega", C.c_double),
         ("precision", C.c_int), ("levels", C.c_int), ("z_head_fix", C.c_int), ("ortho", C.c_int),
         ("coarse_fuse", C.c_int), ("graphs", C.c_int), ("smoother", C.c_int), ("fused_sweep", C.c_int),
-        ("line_levels", C.c_int),
+        ("line_levels", C.c_int), ("diag_faces", C.c_int),
     ]
 
 
@@ -457,8 +457,9 @@ class IterativeSolver:
     krylov = KRYLOV_FGMRES
 
     def __init__(self, precision=PREC_FP64, mg=MG_CASCADE, z_head_fix=False, stream=None, ortho=0, coarse_fuse=True,
-                 graphs=False, smoother="rbgs", fused_sweep=True, line_levels=0, levels=0):
+                 graphs=False, smoother="rbgs", fused_sweep=True, line_levels=0, levels=0, diag_faces=True):
         self.levels_cap = levels
+        self.diag_faces = diag_faces
         self.ortho = ortho
         self.coarse_fuse = coarse_fuse
         self.graphs = graphs
@@ -481,30 +482,7 @@ class IterativeSolver:
     # -- reference interface --------------------------------------------------------------
     def set(self, A, rhs, sol, task):
         """A: object with dims (N, M, K) and CSR arrays rowptr/col/val (reference SparseMatrix)."""
-        self.close()
-        opt = Options()
-        lib().lsx_options_default(C.byref(opt))
-        opt.krylov = self.krylov
-        opt.restart = int(task.FGMRES_RESTART_NUM)
-        opt.max_it = int(task.MAX_ITER)
-        opt.eps = float(task.eps)
-        opt.n_post = int(task.n_post)
-        opt.n_pre = int(getattr(task, "n_pre", 0))
-        opt.omega = float(task.omega)
-        opt.precision = self.precision
-        opt.mg = self.mg
-        opt.z_head_fix = 1 if self.z_head_fix else 0
-        opt.ortho = int(self.ortho)
-        opt.coarse_fuse = 1 if self.coarse_fuse else 0
-        opt.graphs = 1 if self.graphs else 0
-        opt.levels = int(getattr(self, "levels_cap", 0))
-        opt.smoother = SMOOTHERS[self.smoother]
-        opt.fused_sweep = int(self.fused_sweep)
-        opt.line_levels = int(self.line_levels)
-        h = C.c_void_p()
-        _check(lib().lsx_create(C.byref(h), C.byref(opt), self.stream))
-        self.h = h
-        self.eps = float(task.eps)
+        self._make(task)
         N, M, K = A.dims
         self._A = A
         rp = np.ascontiguousarray(A.rowptr, dtype=np.int32)
@@ -567,6 +545,7 @@ class IterativeSolver:
         opt.smoother = SMOOTHERS[self.smoother]
         opt.fused_sweep = int(self.fused_sweep)
         opt.line_levels = int(self.line_levels)
+        opt.diag_faces = 1 if getattr(self, "diag_faces", True) else 0
         h = C.c_void_p()
         _check(lib().lsx_create(C.byref(h), C.byref(opt), self.stream))
         self.h = h
