@@ -916,6 +916,54 @@ __global__ void __launch_bounds__(BS) k_cgs_update(const double* __restrict__ V,
   }
 }
 
+// out = post (w - sum_k (h_k isq_k) v_k) (out may alias w), out32 = (float)out when given;
+// with_norm: partial ||out||^2 per block
+template <int BS>
+__global__ void __launch_bounds__(BS) k_cgs_update_scaled(const double* __restrict__ V, long ldv, int cnt,
+                                                          const double* __restrict__ h, CgsScales sc, double post,
+                                                          const double* w, double* out, float* __restrict__ out32,
+                                                          long n, int with_norm, double* __restrict__ partial) {
+  __shared__ double hk[CGS_MAX];
+  if ((int)threadIdx.x < cnt) hk[threadIdx.x] = h[threadIdx.x] * sc.isq[threadIdx.x];
+  __syncthreads();
+  long lo, hi;
+  chunk_of(n, blockIdx.x, gridDim.x, &lo, &hi);
+  double ss = 0.0;
+  constexpr int R = CGS_ROWS;
+  long i = lo + threadIdx.x;
+  for (; i + (R - 1) * BS < hi; i += R * BS) {
+    double wr[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) wr[r] = w[i + r * BS];
+    const double* vp = V + i;
+#pragma unroll 8
+    for (int k = 0; k < cnt; ++k)
+#pragma unroll
+      for (int r = 0; r < R; ++r) wr[r] -= hk[k] * vp[k * ldv + r * BS];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double o = wr[r] * post;
+      out[i + r * BS] = o;
+      if (out32) out32[i + r * BS] = (float)o;
+      ss += o * o;
+    }
+  }
+  for (; i < hi; i += BS) {
+    double wn = w[i];
+    const double* vp = V + i;
+#pragma unroll 8
+    for (int k = 0; k < cnt; ++k) wn -= hk[k] * vp[k * ldv];
+    const double o = wn * post;
+    out[i] = o;
+    if (out32) out32[i] = (float)o;
+    ss += o * o;
+  }
+  if (with_norm) {
+    ss = block_sum<BS>(ss);
+    if (threadIdx.x == 0) partial[blockIdx.x] = ss;
+  }
+}
+
 template <int BS>
 __global__ void k_finalize_multi(const double* __restrict__ partial, int nb, double* __restrict__ out,
                                  int sqrt_flag) {
@@ -1185,6 +1233,15 @@ void cgs_update(const double* V, long ldv, int cnt, const double* h, double* w, 
                 double* norm_out, hipStream_t s) {
   const int nb = cgs_blocks(n);
   k_cgs_update<RED_THREADS><<<nb, RED_THREADS, 0, s>>>(V, ldv, cnt, h, w, n, norm_out != nullptr, partial);
+  if (norm_out) k_finalize_multi<RED_THREADS><<<1, RED_THREADS, 0, s>>>(partial, nb, norm_out, 1);
+}
+
+void cgs_update_scaled(const double* V, long ldv, int cnt, const double* h, const CgsScales& sc, double post,
+                       const double* w, double* out, float* out32, long n, double* partial, double* norm_out,
+                       hipStream_t s) {
+  const int nb = cgs_blocks(n);
+  k_cgs_update_scaled<RED_THREADS><<<nb, RED_THREADS, 0, s>>>(V, ldv, cnt, h, sc, post, w, out, out32, n,
+                                                              norm_out != nullptr, partial);
   if (norm_out) k_finalize_multi<RED_THREADS><<<1, RED_THREADS, 0, s>>>(partial, nb, norm_out, 1);
 }
 

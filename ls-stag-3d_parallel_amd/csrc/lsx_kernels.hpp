@@ -94,6 +94,15 @@ void cgs_dot(const double* V, long ldv, int cnt, const double* w, long n, double
              hipStream_t s);
 void cgs_update(const double* V, long ldv, int cnt, const double* h, double* w, long n, double* partial,
                 double* norm_out, hipStream_t s);
+// the update for a basis stored unnormalised (vector k = s_k times the orthonormal one):
+// out = post (w - sum_k (h_k isq_k) v_k) with isq_k = 1 / s_k^2 (out may be w), its FP32 copy in
+// out32 when given, and norm_out[0] = ||out|| when given
+struct CgsScales {
+  double isq[CGS_MAX];
+};
+void cgs_update_scaled(const double* V, long ldv, int cnt, const double* h, const CgsScales& sc, double post,
+                       const double* w, double* out, float* out32, long n, double* partial, double* norm_out,
+                       hipStream_t s);
 // mixed precision: y = A x with FP32 x (fine 7-point operator); x += sum_k y_k z_k with FP32 z
 void stencil7_prod_f32x(const double* coef, Dims d, const float* x, double* y, hipStream_t s);
 void stencil_prod_sym_f32x(const double* s4, const SymExc& ex, Dims d, const float* x, double* y, hipStream_t s);

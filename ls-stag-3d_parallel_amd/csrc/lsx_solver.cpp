@@ -979,6 +979,21 @@ int Solver::fgmres(const double* b, double* x) {
   const MGLevel& F = levels[0];
 
   const bool m32 = vcyc() && mixed();
+  // orthogonalisation: the reference's MGS for the cascade; classical Gram-Schmidt (one pass, or
+  // two for CGS2) for the V-cycle loops, whose windows must fit CGS_MAX
+  int ortho = opt.ortho;
+  if (ortho == LSX_ORTHO_AUTO) ortho = vcyc() ? LSX_ORTHO_CGS : LSX_ORTHO_MGS;
+  if (ortho != LSX_ORTHO_MGS && m > CGS_MAX) ortho = LSX_ORTHO_MGS;
+  const int cgs_passes = ortho == LSX_ORTHO_CGS ? 1 : (ortho == LSX_ORTHO_CGS2 ? 2 : 0);
+  // The V-cycle loops with classical Gram-Schmidt keep basis vector k unnormalised, u_k = s_k v_k
+  // (s_0 = 1): the update pass writes u_{k+1} = w - V h, whose norm is the Hessenberg entry
+  // s_{k+1} = h_{k+1,k}, and its FP32 copy straight into the next basis slot, with no scaling
+  // pass.  The cycle and the operator are linear, so z_k and A z_k carry the factor s_k, which the
+  // update pass and the host remove (from w - V h, the Hessenberg column and the solution
+  // coefficients).
+  const bool unnorm = vcyc() && cgs_passes > 0;
+  std::vector<double> sc(m + 1, 1.0);
+  CgsScales isq{};
   auto sol_upd = [&](int last, int lastH) {
     std::vector<double> ycoef(last + 1);
     int hNum = lastH;
@@ -988,7 +1003,7 @@ int Solver::fgmres(const double* b, double* x) {
         --hNum;
         y[i] -= curY * H[hNum];
       }
-      ycoef[k] = curY;
+      ycoef[k] = unnorm ? curY / sc[k] : curY;
     }
     if (m32)
       sol_update_f32(x, Z32.p, n, ycoef.data(), last + 1, n, stream);
@@ -1004,12 +1019,6 @@ int Solver::fgmres(const double* b, double* x) {
   double normW = norm_r;
   double curH = norm_r;
   std::vector<double> hcol(m + 2), hcgs(2 * CGS_MAX);
-  // orthogonalisation: the reference's MGS for the cascade; classical Gram-Schmidt (one pass, or
-  // two for CGS2) for the V-cycle loops, whose windows must fit CGS_MAX
-  int ortho = opt.ortho;
-  if (ortho == LSX_ORTHO_AUTO) ortho = vcyc() ? LSX_ORTHO_CGS : LSX_ORTHO_MGS;
-  if (ortho != LSX_ORTHO_MGS && m > CGS_MAX) ortho = LSX_ORTHO_MGS;
-  const int cgs_passes = ortho == LSX_ORTHO_CGS ? 1 : (ortho == LSX_ORTHO_CGS2 ? 2 : 0);
   bool resume = false;  // a V-cycle loop whose estimate converged but whose true residual did not
   for (;;) {
     // normW lives on the device for the scaling kernel
@@ -1018,12 +1027,14 @@ int Solver::fgmres(const double* b, double* x) {
       double* vj = V.p + (size_t)j * n;
       double* zj = m32 ? nullptr : Z.p + (size_t)j * n;
       float* zj32 = m32 ? Z32.p + (size_t)j * n : nullptr;
+      const bool scaled = !unnorm || j == 0;  // else u_j (and its FP32 copy) came from the update
+      if (scaled) sc[j] = 1.0;
       if (m32) {
         // v_j = w / ||w|| and its FP32 copy (the cycle's finest right-hand side) in one pass
-        scale_copy_f32(w.p, dnorm, vj, vlev[0].fb.p, n, stream);
+        if (scaled) scale_copy_f32(w.p, dnorm, vj, vlev[0].fb.p, n, stream);
         vcycle_apply<float>(vj, zj32, true);
       } else {
-        scale_copy(w.p, dnorm, vj, n, stream);
+        if (scaled) scale_copy(w.p, dnorm, vj, n, stream);
         if (vcyc())
           vcycle_apply<double>(vj, zj);
         else
@@ -1054,14 +1065,30 @@ int Solver::fgmres(const double* b, double* x) {
           cgs_dot(V.p, n, cnt, w.p, n, mpartial.p, hp, stream);
           tend(LSX_K_CGS_DOT, e0, vb + 8.0 * n);
           tbeg(LSX_K_CGS_UPDATE, &e0);
-          cgs_update(V.p, n, cnt, hp, w.p, n, mpartial.p, last ? dnorm : nullptr, stream);
-          tend(LSX_K_CGS_UPDATE, e0, vb + 16.0 * n);
+          if (unnorm) {
+            // the last pass writes u_{j+1} into the next slot (and the cycle's FP32 right-hand
+            // side); the window's last step only needs the norm
+            for (int k = 0; k < cnt; ++k) isq.isq[k] = 1.0 / (sc[k] * sc[k]);
+            const bool to_next = last && j + 1 < m;
+            double* out = to_next ? V.p + (size_t)(j + 1) * n : w.p;
+            float* out32 = to_next && m32 ? vlev[0].fb.p : nullptr;
+            cgs_update_scaled(V.p, n, cnt, hp, isq, last ? 1.0 / sc[j] : 1.0, w.p, out, out32, n, mpartial.p,
+                              last ? dnorm : nullptr, stream);
+            tend(LSX_K_CGS_UPDATE, e0, vb + (out32 ? 20.0 : 16.0) * n);
+          } else {
+            cgs_update(V.p, n, cnt, hp, w.p, n, mpartial.p, last ? dnorm : nullptr, stream);
+            tend(LSX_K_CGS_UPDATE, e0, vb + 16.0 * n);
+          }
         }
         hNum = hColStart + j + 1;
         LSX_HIP(hipMemcpyAsync(hcgs.data(), cgs_h.p, 2 * CGS_MAX * sizeof(double), hipMemcpyDeviceToHost, stream));
         LSX_HIP(hipMemcpyAsync(&normW, dnorm, sizeof(double), hipMemcpyDeviceToHost, stream));
         LSX_HIP(hipStreamSynchronize(stream));
         for (int i = 0; i <= j; ++i) hcol[i] = cgs_passes == 2 ? hcgs[i] + hcgs[CGS_MAX + i] : hcgs[i];
+        if (unnorm) {  // back to the orthonormal basis: h_i / (s_i s_j); u_{j+1} = ||w|| v_{j+1}
+          for (int i = 0; i <= j; ++i) hcol[i] /= sc[i] * sc[j];
+          sc[j + 1] = normW;
+        }
       } else {
         // modified Gram-Schmidt: h_i = w . v_i ; w -= h_i v_i  (the dot of the next basis vector
         // is fused into the update pass; the last pass yields ||w||^2)
