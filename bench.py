@@ -424,12 +424,25 @@ def main():
                                      "GBs": v[2] / (v[1] / v[0] / 1e3) / 1e9,
                                      "share_of_step": v[1] / 1e3 / elapsed}
                for k, v in stats.items() if v[0]}
-    traffic = None
+    # HBM traffic per launch of the dominant kernel: FETCH_SIZE + WRITE_SIZE counter runs recorded in
+    # profiles/traffic.json with the revision they measured; used only when that record's algorithmic
+    # bytes per launch equal this run's (same kernel variant and workload), else reported as null
+    traffic, traffic_note = None, "no counter record for this kernel, precision and grid"
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
         try:
-            traffic = json.load(open(tfile)).get(f"{lsx.KERNEL_NAMES[dom]}@{args.precision}@{n}")
-        except (ValueError, OSError):
+            rec = json.load(open(tfile))
+            ent = rec.get("kernels", {}).get(f"{lsx.KERNEL_NAMES[dom]}@{args.precision}@{n}")
+            if ent is not None:
+                if abs(ent["algorithmic_bytes"] - bytes_per) <= 1e-3 * bytes_per:
+                    traffic = ent["traffic_bytes"]
+                    traffic_note = (f"rocprofv3 FETCH_SIZE + WRITE_SIZE of {ent['kernel']} at revision "
+                                    f"{rec.get('revision')}, scaled by a copy of known size")
+                else:
+                    traffic_note = (f"the counter record (revision {rec.get('revision')}) was taken on a kernel "
+                                    f"moving {ent['algorithmic_bytes']:.4g} algorithmic bytes per launch, this run "
+                                    f"{bytes_per:.4g}: stale, not used")
+        except (ValueError, OSError, KeyError):
             traffic = None
     value = args.steps / elapsed
     sm_name = "red-black GS" if args.smoother == "rbgs" else "damped Jacobi" if args.smoother == "jacobi" else (
@@ -488,7 +501,7 @@ def main():
         "data": "synthetic",
         "config": cfg,
         "roofline": {"bound": "hbm", "kernel": lsx.KERNEL_NAMES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": traffic_note,
                      "algorithmic_bytes_per_launch": bytes_per, "avg_launch_us": avg_s * 1e6},
         "kernels": kernels,
         "cpu_baseline": None,

@@ -717,23 +717,40 @@ __global__ void k_line_solve(const double* __restrict__ coef, int npts, int N, i
   }
 }
 
-// lu_decomposition_sparse_matrix.h:216-259 + decomposition_sparse_matrix_interface.h:43-71
-__global__ void k_lu_solve(const double* scale, const int* perm, const double* L, const double* U, int n,
-                           const double* __restrict__ b, double* __restrict__ x) {
+// lu_decomposition_sparse_matrix.h:216-259 + decomposition_sparse_matrix_interface.h:43-71.
+// The factors (n <= 64) and the right-hand side are staged in LDS by the whole workgroup; thread 0
+// then runs the triangular solves on LDS operands (the same operations in the same order).
+constexpr int LU_SOLVE_THREADS = 256;
+__global__ void __launch_bounds__(LU_SOLVE_THREADS) k_lu_solve(const double* scale, const int* perm,
+                                                               const double* L, const double* U, int n,
+                                                               const double* __restrict__ b, double* __restrict__ x) {
+  __shared__ double sL[64 * 64], sU[64 * 64], ssc[64], sb[64], slu_b[64], slu_y[64];
+  __shared__ int sp[64];
+  for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
+    sL[k] = L[k];
+    sU[k] = U[k];
+  }
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    ssc[k] = scale[k];
+    sp[k] = perm[k];
+    sb[k] = b[k];
+  }
+  __syncthreads();
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  double bs[64], yv[64];
-  for (int i = 0; i < n; ++i) bs[i] = scale[perm[i]] * b[perm[i]];
+  double* bs = slu_b;  // LDS: as local arrays they would live in scratch memory
+  double* yv = slu_y;
+  for (int i = 0; i < n; ++i) bs[i] = ssc[sp[i]] * sb[sp[i]];
   for (int i = 0; i < n; ++i) {
     double aux = bs[i];
-    for (int t = 0; t < i; ++t) aux -= L[i * n + t] * yv[t];
+    for (int t = 0; t < i; ++t) aux -= sL[i * n + t] * yv[t];
     yv[i] = aux;
   }
   for (int i = n - 1; i >= 0; --i) {
     double aux = yv[i];
-    for (int j = i + 1; j < n; ++j) aux -= U[i * n + j] * bs[j];
-    bs[i] = aux / U[i * n + i];
+    for (int j = i + 1; j < n; ++j) aux -= sU[i * n + j] * bs[j];
+    bs[i] = aux / sU[i * n + i];
   }
-  for (int i = 0; i < n; ++i) x[i] = scale[i] * bs[i];
+  for (int i = 0; i < n; ++i) x[i] = ssc[i] * bs[i];
 }
 
 // ---- kernels: BLAS-1 ---------------------------------------------------------------------------
@@ -1174,7 +1191,7 @@ void line_solve(const double* coef, int npts, Dims d, int dir, const double* din
 
 void lu_solve(const double* scale, const int* perm, const double* L, const double* U, int n, const double* b,
               double* x, hipStream_t s) {
-  k_lu_solve<<<1, 1, 0, s>>>(scale, perm, L, U, n, b, x);
+  k_lu_solve<<<1, LU_SOLVE_THREADS, 0, s>>>(scale, perm, L, U, n, b, x);
 }
 
 void copy(const double* a, double* b, long n, hipStream_t s) { k_copy<<<grid_1d(n), 256, 0, s>>>(a, b, n); }

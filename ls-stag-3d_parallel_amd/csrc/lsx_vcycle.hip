@@ -20,12 +20,14 @@ namespace vc {
 
 namespace {
 
+// cell index -> (x, y, z) in 32-bit arithmetic (every level has fewer than 2^31 cells: the
+// CSR boundary indexes rows with 32-bit integers); 64-bit division is a long instruction sequence
 __device__ inline void decode(long i, int N, int M, int* x, int* y, int* z) {
-  const long NM = (long)N * M;
-  *z = (int)(i / NM);
-  const long r = i - (long)*z * NM;
-  *y = (int)(r / N);
-  *x = (int)(r - (long)*y * N);
+  const unsigned ii = (unsigned)i, NM = (unsigned)N * (unsigned)M;
+  const unsigned zz = ii / NM, r = ii - zz * NM, yy = r / (unsigned)N;
+  *z = (int)zz;
+  *y = (int)yy;
+  *x = (int)(r - yy * (unsigned)N);
 }
 
 inline int grid_1d(long n, int bs = 256) {
@@ -275,22 +277,9 @@ __global__ void k_to27(const double* __restrict__ fe, const double* __restrict__
 }
 
 // ---- cycle -----------------------------------------------------------------------------------
-template <class T, class FT>
-__device__ inline T offdiag(const T* __restrict__ x, const FT* __restrict__ ae, const FT* __restrict__ an,
-                            const FT* __restrict__ at, long i, int xx, int yy, int zz, int N, int M, int K) {
-  const long NM = (long)N * M;
-  T acc = 0;
-  if (xx > 0) acc += ld<T>(ae, i - 1) * x[i - 1];
-  if (xx < N - 1) acc += ld<T>(ae, i) * x[i + 1];
-  if (yy > 0) acc += ld<T>(an, i - N) * x[i - N];
-  if (yy < M - 1) acc += ld<T>(an, i) * x[i + N];
-  if (zz > 0) acc += ld<T>(at, i - NM) * x[i - NM];
-  if (zz < K - 1) acc += ld<T>(at, i) * x[i + NM];
-  return acc;
-}
-
-// offdiag with every load issued unconditionally (absent neighbours read the cell itself) and
-// absent terms leaving the sum unchanged: the same value, with the row's loads in flight together
+// sum over the present neighbours j of a_ij x_j in the order w, e, s, n, b, t, with every load
+// issued unconditionally (absent neighbours read the cell itself) and absent terms leaving the sum
+// unchanged, so that the row's loads are in flight together
 template <class T, class FT>
 __device__ inline T offdiag_u(const T* __restrict__ x, const FT* __restrict__ ae, const FT* __restrict__ an,
                               const FT* __restrict__ at, long i, int xx, int yy, int zz, int N, int M, int K) {
@@ -340,24 +329,26 @@ __device__ inline void rbgs_body(T* __restrict__ x, const T* __restrict__ b, con
                                  const FT* __restrict__ an, const FT* __restrict__ at, const T* __restrict__ s, T fs,
                                  int N, int M, int K, int color, int pin0, int pin1, int zero_guess, long start,
                                  long stride) {
-  const long half_row = (N + 1) / 2;
-  const long rows = (long)M * K;
-  const long total = half_row * rows;
-  for (long t = start; t < total; t += stride) {
-    const long row = t / half_row;
-    const int yy = (int)(row % M), zz = (int)(row / M);
-    const int xx = 2 * (int)(t - row * half_row) + ((yy + zz + color) & 1);
+  // 32-bit index arithmetic (a level has fewer than 2^31 cells); the neighbours' loads are issued
+  // unconditionally (offdiag_u: the same sum in the same order)
+  const unsigned half_row = (unsigned)(N + 1) / 2;
+  const unsigned total = half_row * (unsigned)M * (unsigned)K;
+  for (long t = start; t < (long)total; t += stride) {
+    const unsigned tt = (unsigned)t, row = tt / half_row;
+    const int yy = (int)(row % (unsigned)M), zz = (int)(row / (unsigned)M);
+    const int xx = 2 * (int)(tt - row * half_row) + ((yy + zz + color) & 1);
     if (xx >= N) continue;
     const long i = (long)xx + (long)N * row;
+    const T si = s[i], bi = b[i];
     T v;
     if (i == pin0 || i == pin1) {
-      v = b[i] * s[i];
-    } else if (s[i] == (T)0) {
+      v = bi * si;
+    } else if (si == (T)0) {
       v = zero_guess ? (T)0 : x[i];  // empty row: keeps its value (0 from a zero start)
     } else if (zero_guess == 1) {
-      v = b[i] * s[i];
+      v = bi * si;
     } else {
-      v = (b[i] - fs * offdiag<T>(x, ae, an, at, i, xx, yy, zz, N, M, K)) * s[i];
+      v = (bi - fs * offdiag_u<T, FT>(x, ae, an, at, i, xx, yy, zz, N, M, K)) * si;
     }
     x[i] = v;
   }
@@ -386,9 +377,11 @@ __device__ inline void resid_restrict_body(const T* __restrict__ x, const T* __r
     for (int c = 0; c < 8; ++c) {
       const int xx = 2 * X + (c & 1), yy = 2 * Y + ((c >> 1) & 1), zz = 2 * Z + (c >> 2);
       const long i = xx + (long)N * yy + NM * zz;
-      const T ap = s[i] != (T)0 ? (T)1 / s[i] : (T)0;
+      const T si = s[i];
+      const T ap = si != (T)0 ? (T)1 / si : (T)0;
       T r = b[i] - ap * x[i];
-      if (i != pin0 && i != pin1) r -= fs * offdiag<T>(x, ae, an, at, i, xx, yy, zz, N, M, K);
+      const T od = offdiag_u<T, FT>(x, ae, an, at, i, xx, yy, zz, N, M, K);
+      if (i != pin0 && i != pin1) r -= fs * od;
       acc += r;
     }
     bc[I] = acc;
@@ -624,63 +617,127 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse_cycle(CoarseLevels<T>
                                                                 const double* __restrict__ dinv) {
   const long tid = threadIdx.x, nth = blockDim.x;
   const int last = C.count - 1;
-  for (int q = 0; q < last; ++q) {
-    const Level& L = C.L[q];
-    const FT* e = (const FT*)L.f_e;
-    const FT* nn = (const FT*)L.f_n;
-    const FT* t = (const FT*)L.f_t;
-    const T* s = (const T*)L.s;
-    for (int sw = 0; sw < n_pre; ++sw) {
-      rbgs_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, 0, L.pin0, L.pin1,
-                       sw == 0 ? 1 : 0, tid, nth);
-      __syncthreads();
-      rbgs_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, 1, L.pin0, L.pin1,
-                       sw == 0 ? 2 : 0, tid, nth);
-      __syncthreads();
+  int n_last = 0;
+  T* x_last = nullptr;
+  const T* b_last = nullptr;
+#pragma unroll
+  for (int q = 0; q < COARSE_MAX; ++q)
+    if (q == last) {
+      n_last = (int)C.L[q].d.n();
+      x_last = C.x[q];
+      b_last = C.b[q];
     }
-    resid_restrict_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, L.pin0, L.pin1,
-                               C.b[q + 1], tid, nth);
-    __syncthreads();
+  // the LU factors of the last level (at most 64 unknowns) are staged in LDS by the whole
+  // workgroup at the start: thread 0's triangular solves then wait on LDS, not on one
+  // (by then evicted) global load after another
+  __shared__ double sL[64 * 64], sU[64 * 64], ssc[64], slu_b[64], slu_y[64];
+  __shared__ int sperm[64];
+  __shared__ T sres[COARSE_CELLS];  // residuals of the level being restricted
+  if (!dinv) {
+    const int n = n_last;
+    for (int k = (int)tid; k < n * n; k += (int)nth) {
+      sL[k] = Lf[k];
+      sU[k] = Uf[k];
+    }
+    for (int k = (int)tid; k < n; k += (int)nth) {
+      ssc[k] = lu_scale[k];
+      sperm[k] = lu_perm[k];
+    }
+  }
+  __syncthreads();
+  for (int q = 0; q < COARSE_MAX - 1; ++q) {
+    if (q < last) {
+      const Level& L = C.L[q];
+      const FT* e = (const FT*)L.f_e;
+      const FT* nn = (const FT*)L.f_n;
+      const FT* t = (const FT*)L.f_t;
+      const T* s = (const T*)L.s;
+      for (int sw = 0; sw < n_pre; ++sw) {
+        rbgs_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, 0, L.pin0, L.pin1,
+                         sw == 0 ? 1 : 0, tid, nth);
+        __syncthreads();
+        rbgs_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, 1, L.pin0, L.pin1,
+                         sw == 0 ? 2 : 0, tid, nth);
+        __syncthreads();
+      }
+      // residual restriction: every cell's residual into LDS (one thread per cell), then each coarse
+      // cell sums its 8 children in the order of resid_restrict_body (the same values; one cell's
+      // loads per thread instead of eight keeps the kernel clear of register spills)
+      {
+        const T* x = C.x[q];
+        const T* b = C.b[q];
+        const T fs = (T)L.fscale;
+        const int N = L.d.N, M = L.d.M, K = L.d.K, n = N * M * K;
+        for (int i = (int)tid; i < n; i += (int)nth) {
+          int xx, yy, zz;
+          decode(i, N, M, &xx, &yy, &zz);
+          const T si = s[i];
+          const T ap = si != (T)0 ? (T)1 / si : (T)0;
+          T r = b[i] - ap * x[i];
+          const T od = offdiag_u<T, FT>(x, e, nn, t, i, xx, yy, zz, N, M, K);
+          if (i != L.pin0 && i != L.pin1) r -= fs * od;
+          sres[i] = r;
+        }
+        __syncthreads();
+        const int Nc = N / 2, Mc = M / 2, nc = Nc * Mc * (K / 2);
+        T* bc = C.b[q + 1];
+        for (int I = (int)tid; I < nc; I += (int)nth) {
+          int X, Y, Z;
+          decode(I, Nc, Mc, &X, &Y, &Z);
+          T acc = 0;
+#pragma unroll
+          for (int c = 0; c < 8; ++c)
+            acc += sres[(2 * X + (c & 1)) + N * ((2 * Y + ((c >> 1) & 1)) + M * (2 * Z + (c >> 2)))];
+          bc[I] = acc;
+        }
+        __syncthreads();
+      }
+    }
   }
   if (dinv) {  // dense inverse of the last level, column-major: x_i = sum_j inv[j n + i] b_j
-    const int n = (int)C.L[last].d.n();
-    const T* bb = C.b[last];
+    const int n = n_last;
+    const T* bb = b_last;
     for (int i = (int)tid; i < n; i += (int)nth) {
       double acc = 0.0;
       for (int j = 0; j < n; ++j) acc += dinv[(long)j * n + i] * (double)bb[j];
-      C.x[last][i] = (T)acc;
+      x_last[i] = (T)acc;
     }
   } else if (tid == 0) {
-    const int n = (int)C.L[last].d.n();
-    const T* bb = C.b[last];
-    double bs[64], yv[64];
-    for (int i = 0; i < n; ++i) bs[i] = lu_scale[lu_perm[i]] * (double)bb[lu_perm[i]];
+    const int n = n_last;
+    const T* bb = b_last;
+    double* bs = slu_b;  // LDS: as local arrays they would live in scratch memory
+    double* yv = slu_y;
+    for (int i = 0; i < n; ++i) bs[i] = ssc[sperm[i]] * (double)bb[sperm[i]];
     for (int i = 0; i < n; ++i) {
       double aux = bs[i];
-      for (int k = 0; k < i; ++k) aux -= Lf[i * n + k] * yv[k];
+      for (int k = 0; k < i; ++k) aux -= sL[i * n + k] * yv[k];
       yv[i] = aux;
     }
     for (int i = n - 1; i >= 0; --i) {
       double aux = yv[i];
-      for (int j = i + 1; j < n; ++j) aux -= Uf[i * n + j] * bs[j];
-      bs[i] = aux / Uf[i * n + i];
+      for (int j = i + 1; j < n; ++j) aux -= sU[i * n + j] * bs[j];
+      bs[i] = aux / sU[i * n + i];
     }
-    for (int i = 0; i < n; ++i) C.x[last][i] = (T)(lu_scale[i] * bs[i]);
+    for (int i = 0; i < n; ++i) x_last[i] = (T)(ssc[i] * bs[i]);
   }
   __syncthreads();
-  for (int q = last - 1; q >= 0; --q) {
-    const Level& L = C.L[q];
-    const FT* e = (const FT*)L.f_e;
-    const FT* nn = (const FT*)L.f_n;
-    const FT* t = (const FT*)L.f_t;
-    const T* s = (const T*)L.s;
-    prolong_add_body<T>(C.x[q + 1], C.L[q + 1].d.N, C.L[q + 1].d.M, C.x[q], L.d.N, L.d.M, L.d.K, tid, nth);
-    __syncthreads();
-    for (int sw = 0; sw < n_post; ++sw) {
-      rbgs_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, 0, L.pin0, L.pin1, 0, tid, nth);
+  for (int q = COARSE_MAX - 2; q >= 0; --q) {
+    if (q < last) {
+      const Level& L = C.L[q];
+      const FT* e = (const FT*)L.f_e;
+      const FT* nn = (const FT*)L.f_n;
+      const FT* t = (const FT*)L.f_t;
+      const T* s = (const T*)L.s;
+      prolong_add_body<T>(C.x[q + 1], C.L[q + 1].d.N, C.L[q + 1].d.M, C.x[q], L.d.N, L.d.M, L.d.K, tid, nth);
       __syncthreads();
-      rbgs_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, 1, L.pin0, L.pin1, 0, tid, nth);
-      __syncthreads();
+      for (int sw = 0; sw < n_post; ++sw) {
+        rbgs_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, 0, L.pin0, L.pin1, 0, tid,
+                         nth);
+        __syncthreads();
+        rbgs_body<T, FT>(C.x[q], C.b[q], e, nn, t, s, (T)L.fscale, L.d.N, L.d.M, L.d.K, 1, L.pin0, L.pin1, 0, tid,
+                         nth);
+        __syncthreads();
+      }
     }
   }
 }
