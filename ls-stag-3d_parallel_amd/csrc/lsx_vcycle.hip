@@ -593,6 +593,63 @@ __device__ inline void prolong_add_body(const T* __restrict__ xc, int Nc, int Mc
   }
 }
 
+// One thread per coarse cell (64 x by 4 y coarse cells per block, blocks x fastest, then y, then
+// z) adds the prolongation to its 8 children.  The 27 coarse values around it are loaded once
+// (indices clamped to the level) and every child is computed as prolong_cell computes it: the
+// same two taps per axis (at the level's boundary the first tap with weight 1 and a second,
+// weight 0, that reads the same value), weights multiplied in FP32 and the 8 terms added in the
+// same order, so the result is bit-identical with 27 instead of 64 coarse loads per 8 cells.
+template <class T>
+__global__ void __launch_bounds__(256) k_prolong_add8(const T* __restrict__ xc, int Nc, int Mc, int Kc,
+                                                      T* __restrict__ xf) {
+  const int gx = (Nc + 63) / 64, gy = (Mc + 3) / 4;
+  const int r = (int)(blockIdx.x / gx);
+  const int X = (int)(blockIdx.x % gx) * 64 + (int)(threadIdx.x & 63);
+  const int Y = (r % gy) * 4 + (int)(threadIdx.x >> 6), Z = r / gy;
+  if (X >= Nc || Y >= Mc) return;
+  const int Nf = 2 * Nc, Mf = 2 * Mc;
+  const long NMc = (long)Nc * Mc, NMf = (long)Nf * Mf;
+  T nb[3][3][3];  // [dz + 1][dy + 1][dx + 1]
+#pragma unroll
+  for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx) {
+        const int cx = min(max(X + dx - 1, 0), Nc - 1), cy = min(max(Y + dy - 1, 0), Mc - 1);
+        const int cz = min(max(Z + dz - 1, 0), Kc - 1);
+        nb[dz][dy][dx] = xc[cx + Nc * (long)cy + NMc * cz];
+      }
+  // per axis and child parity i: taps at offsets (i - 1, i) around the coarse cell, weights
+  // (0.25, 0.75) / (0.75, 0.25), or (1, 0) on the level's first / last fine cell
+  float wx[2][2], wy[2][2], wz[2][2];
+  auto wts = [](int P, int n, float (&w)[2][2]) {
+    w[0][0] = P == 0 ? 1.0f : 0.25f;
+    w[0][1] = P == 0 ? 0.0f : 0.75f;
+    w[1][0] = P == n - 1 ? 1.0f : 0.75f;
+    w[1][1] = P == n - 1 ? 0.0f : 0.25f;
+  };
+  wts(X, Nc, wx);
+  wts(Y, Mc, wy);
+  wts(Z, Kc, wz);
+#pragma unroll
+  for (int iz = 0; iz < 2; ++iz)
+#pragma unroll
+    for (int iy = 0; iy < 2; ++iy) {
+      const long row = (long)(2 * X) + Nf * (long)(2 * Y + iy) + NMf * (2 * Z + iz);
+#pragma unroll
+      for (int ix = 0; ix < 2; ++ix) {
+        T acc = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const int tx = c & 1, ty = (c >> 1) & 1, tz = c >> 2;
+          acc += (T)(wx[ix][tx] * wy[iy][ty] * wz[iz][tz]) * nb[iz + tz][iy + ty][ix + tx];
+        }
+        xf[row + ix] += acc;
+      }
+    }
+}
+
 // one thread per fine cell: 64 x by 4 y cells per block, blocks x fastest, then y, then z
 template <class T>
 __global__ void __launch_bounds__(256) k_prolong_add(const T* __restrict__ xc, int Nc, int Mc, T* __restrict__ xf,
@@ -2055,6 +2112,11 @@ template void dense_apply<double>(const double*, int, const double*, double*, do
 
 template <class T>
 void prolong_add(const T* xc, Dims dc, T* xf, Dims df, hipStream_t s) {
+  if (df.N == 2 * dc.N && df.M == 2 * dc.M && df.K == 2 * dc.K) {
+    k_prolong_add8<T><<<(unsigned)((long)((dc.N + 63) / 64) * ((dc.M + 3) / 4) * dc.K), 256, 0, s>>>(xc, dc.N, dc.M,
+                                                                                                dc.K, xf);
+    return;
+  }
   k_prolong_add<T><<<(unsigned)((long)((df.N + 63) / 64) * ((df.M + 3) / 4) * df.K), 256, 0, s>>>(xc, dc.N, dc.M, xf,
                                                                                              df.N, df.M, df.K);
 }
