@@ -705,8 +705,9 @@ void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess, bool r
   // start reads no x, one that rebuilds a_p from the faces no 1/a_p)
   const double vbytes = sizeof(T), fbytes = S.L.ftype == vc::FACE_F16 ? 2.0 : (S.L.ftype == vc::FACE_F32 ? 4.0 : 8.0);
   const double pass_bytes = (4.0 * vbytes + 3.0 * fbytes) * (double)S.L.d.n();
-  auto line_bytes = [&](bool zero) {
-    const double v = 4.0 - (S.L.dff && !trans ? 1.0 : 0.0) - (zero ? 1.0 : 0.0);
+  auto line_bytes = [&](int dir, bool zero) {
+    const bool dff = S.L.dff && !trans && (dir != 0 || sizeof(T) == 4);  // as vc::line_pass
+    const double v = 4.0 - (dff ? 1.0 : 0.0) - (zero ? 1.0 : 0.0);
     return (v * vbytes + 3.0 * fbytes) * (double)S.L.d.n();
   };
   hipEvent_t e0 = nullptr;
@@ -774,7 +775,7 @@ void Solver::smooth(int l, T* x, const T* b, int sweeps, bool zero_guess, bool r
     const int dir = rev_lines ? 2 - (p - 1) % 3 : (p - 1) % 3;
     t0(LSX_K_LINE_X + dir);
     vc::line_pass<T>(S.L, dir, src, dst, b, zero_guess && p == 1, trans ? 1 : 0, stream);
-    t1(LSX_K_LINE_X + dir, line_bytes(zero_guess && p == 1));
+    t1(LSX_K_LINE_X + dir, line_bytes(dir, zero_guess && p == 1));
     src = dst;
   }
 }

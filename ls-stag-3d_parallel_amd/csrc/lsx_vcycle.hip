@@ -1435,16 +1435,21 @@ __global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* _
   }
 }
 
-// y and z lines: a workgroup holds LPW = 32 lines (adjacent x: every load is a full 128-byte
-// line for FP32) split into S segments of SEG = 8 rows; thread t takes line t % 32 and segment
-// t / 32 (S = ceil(len / SEG) <= 32: lines of at most 256 cells).  The partition method of the
-// x lines with segments in place of lanes: each thread eliminates its segment's first cnt - 1
-// rows keeping x_L (last cell of the segment before) and x_E (its own last cell) as unknowns,
-// x_k = G_k + H_k x_L + Q_k x_E; the S last-cell rows of a line form a tridiagonal system,
-// solved by cyclic reduction across the wavefronts through LDS (log2 S steps); every segment
-// then writes x_k.  Inputs are read once and x written once.
+// y and z lines: a workgroup holds LPW lines (adjacent x, LPW = 128 bytes / value size: 32 FP32
+// or 16 FP64 lines, so that every row load is one full 128-byte line) split into S segments of
+// SEG = 8 rows; thread t takes line t % LPW and segment t / LPW (S = ceil(len / SEG) <= 32: lines
+// of at most 256 cells).  The partition method of the x lines with segments in place of lanes:
+// each thread eliminates its segment's first cnt - 1 rows keeping x_L (last cell of the segment
+// before) and x_E (its own last cell) as unknowns, x_k = G_k + H_k x_L + Q_k x_E; the S last-cell
+// rows of a line form a tridiagonal system, solved by cyclic reduction (log2 S steps); every
+// segment then writes x_k.  Inputs are read once and x written once.  The FP64 workgroups are half
+// the size of the FP32 ones (512 threads), so two of them share a CU at the FP64 kernel's register
+// count.
 constexpr int SEG = 8;
-constexpr int LPW = 32;
+template <class T>
+constexpr int lpw() {
+  return 128 / (int)sizeof(T);
+}
 
 // element i of p addressed by a 32-bit byte offset (i * sizeof(E) < 2^32), so that loads take a
 // uniform base and one 32-bit offset register
@@ -1467,9 +1472,10 @@ struct SegLine {
   const FT* fL;  // faces along the line
   T fs;
   int strideL, oAl, oAu, oBl, oBu, len, pin0, pin1;
-  // x neighbours come from the neighbouring lanes of the 32-lane group (lane l32 holds cell
+  // x neighbours come from the neighbouring lanes of the LPW-lane group (lane l32 holds cell
   // x0 + l32 of the row); only the group's west and east halo cells are loaded, at offsets
   // oW0 / oE0 from the cell (the same address for the whole group)
+  static constexpr int LPW = lpw<T>();
   int l32, oW0, oE0;
   bool hAl, hAu, hBl, hBu;
   DiagRule<T> dr;
@@ -1498,14 +1504,14 @@ struct SegLine {
       xi = ldo(x, i);
       const T xbl = ldo(x, i + oBl), xbu = ldo(x, i + oBu);
       const T xw0 = ldo(x, i + oW0), xe0 = ldo(x, i + oE0), fw0 = (T)ldo(ae, i + oW0);
-      const T xws = __shfl_up(xi, 1, 32), xes = __shfl_down(xi, 1, 32), fws = __shfl_up(fau, 1, 32);
-      const T xal = l32 == 0 ? xw0 : xws, xau = l32 == 31 ? xe0 : xes;
+      const T xws = __shfl_up(xi, 1, LPW), xes = __shfl_down(xi, 1, LPW), fws = __shfl_up(fau, 1, LPW);
+      const T xal = l32 == 0 ? xw0 : xws, xau = l32 == LPW - 1 ? xe0 : xes;
       fal = l32 == 0 ? fw0 : fws;
       const T cAl = hAl ? scl(fal) : (T)0, cAu = hAu ? scl(fau) : (T)0;
       const T cBl = hBl ? scl(fbl) : (T)0, cBu = hBu ? scl(fbu) : (T)0;
       off = cAl * xal + cAu * xau + cBl * xbl + cBu * xbu;
     } else if constexpr (DFF) {
-      const T fw0 = (T)ldo(ae, i + oW0), fws = __shfl_up(fau, 1, 32);
+      const T fw0 = (T)ldo(ae, i + oW0), fws = __shfl_up(fau, 1, LPW);
       fal = l32 == 0 ? fw0 : fws;
     }
     T ap;
@@ -1535,6 +1541,7 @@ __global__ void __launch_bounds__(1024) k_line_seg(const T* __restrict__ x_in, T
                                                    int pin1, int S, DiagRule<T> dr) {
   #pragma clang fp contract(fast)  // line solves: no bitwise twin to match
   static_assert(!(TRANS && DFF), "a_p from the faces is a forward-operator rule");
+  constexpr int LPW = lpw<T>();
   extern __shared__ unsigned char lds_raw[];
   // [8][S][LPW + 1]: a d c r of the reduced rows, G H Q of first cells, x_E; rows padded by one
   // element so that both the (segment, line) and the (line, segment) thread mappings read
@@ -1963,13 +1970,15 @@ static void line_pass_z(const Level& L, int dir, const T* x_in, T* x_out, const 
   const T fs = (T)L.fscale;
   const DiagRule<T> dr = diag_rule<T>(L);
   if (dir == 0) {
+    // the FP64 x-line pass reads 1/a_p even on a Level::dff level: rebuilding a_p there measured
+    // slower at 256^3 (226 against 195 us), the FP32 one faster (80 against 99 us)
     const int blocks = (int)(((long)d.M * d.K + 3) / 4);
 #define LSX_LINE_X(CPL)                                                                                        \
   do {                                                                                                         \
     if (trans)                                                                                                 \
       k_line_x<T, FT, CPL, ZERO, true, false><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N,   \
                                                                      d.M, d.K, L.pin0, L.pin1, dr);            \
-    else if (L.dff)                                                                                            \
+    else if (L.dff && sizeof(T) == 4)                                                                          \
       k_line_x<T, FT, CPL, ZERO, false, true><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N,   \
                                                                      d.M, d.K, L.pin0, L.pin1, dr);            \
     else                                                                                                       \
@@ -1988,6 +1997,7 @@ static void line_pass_z(const Level& L, int dir, const T* x_in, T* x_out, const 
   } else {
     const int len = dir == 1 ? d.M : d.K;
     const int S = (len + SEG - 1) / SEG;
+    constexpr int LPW = lpw<T>();
     const dim3 grid(((d.N + LPW - 1) / LPW) * (dir == 1 ? d.K : d.M));
     const size_t lds = 8 * (size_t)S * (LPW + 1) * sizeof(T);
 #define LSX_LINE_SEG(DIR, TR, DFF)                                                                             \
