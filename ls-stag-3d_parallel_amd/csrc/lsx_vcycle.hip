@@ -1247,9 +1247,10 @@ __device__ inline void line_row(const T* __restrict__ x, const T* __restrict__ b
 // the lane before (x_L) and its own last cell (x_E): x_k = G_k + H_k x_L + Q_k x_E.  The rows of
 // the 64 last cells then form a tridiagonal system in x_E, one unknown per lane, solved by
 // parallel cyclic reduction with cross-lane shuffles (6 steps); every lane substitutes back.
-template <class T>
+// value of v in lane src of this lane's group of W lanes (src relative to the group)
+template <int W = 64, class T>
 __device__ inline T wave_get(T v, int src, bool ok) {
-  const T r = __shfl(v, ok ? src : 0, 64);
+  const T r = __shfl(v, ok ? src : 0, W);
   return r;
 }
 
@@ -1333,7 +1334,9 @@ __device__ inline void line_rows_x(const T* __restrict__ x, const T* __restrict_
   }
 }
 
-template <class T, class FT, int CPL, bool ZERO, bool TRANS, bool DFF>
+// LANES = 64: one wavefront per line; LANES = 32: two lines per wavefront (lines of at most
+// 32 CPL cells: shorter cyclic reduction, wider loads per lane)
+template <class T, class FT, int CPL, bool ZERO, bool TRANS, bool DFF, int LANES = 64>
 __global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* __restrict__ x_out,
                                                 const T* __restrict__ b, const FT* __restrict__ ae,
                                                 const FT* __restrict__ an, const FT* __restrict__ at,
@@ -1341,9 +1344,12 @@ __global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* _
                                                 int pin1, DiagRule<T> dr) {
   #pragma clang fp contract(fast)  // line solves: no bitwise twin to match
   static_assert(!(TRANS && DFF), "a_p from the faces is a forward-operator rule");
-  const int lane = threadIdx.x & 63;
-  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= (long)M * K) return;  // whole wavefronts leave together
+  const int lane = threadIdx.x & (LANES - 1);
+  const long row_t = (long)blockIdx.x * (256 / LANES) + (threadIdx.x / LANES);
+  if constexpr (LANES == 64)
+    if (row_t >= (long)M * K) return;  // whole wavefronts leave together
+  const bool live = row_t < (long)M * K;  // two-line wavefronts: a line past the end repeats the last
+  const long row = live ? row_t : (long)M * K - 1;
   const int yy = (int)(row % M), zz = (int)(row / M);
   T lo[CPL], dg[CPL], up[CPL], rh[CPL];
   if (N % CPL == 0 && lane * CPL < N) {
@@ -1393,9 +1399,10 @@ __global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* _
   const T Hb = CPL > 1 ? H[CPL > 1 ? CPL - 2 : 0] : (T)1;
   const T Qb = CPL > 1 ? Q[CPL > 1 ? CPL - 2 : 0] : (T)0;
   const T Gf = CPL > 1 ? G[0] : (T)0, Hf = CPL > 1 ? H[0] : (T)0, Qf = CPL > 1 ? Q[0] : (T)1;
-  const bool nxt = lane < 63;
+  const bool nxt = lane < LANES - 1;
   // every lane takes part in the shuffles (a lane outside a branch supplies no value)
-  const T G1v = wave_get(Gf, lane + 1, nxt), H1v = wave_get(Hf, lane + 1, nxt), Q1v = wave_get(Qf, lane + 1, nxt);
+  const T G1v = wave_get<LANES>(Gf, lane + 1, nxt), H1v = wave_get<LANES>(Hf, lane + 1, nxt);
+  const T Q1v = wave_get<LANES>(Qf, lane + 1, nxt);
   const T G1 = nxt ? G1v : (T)0, H1 = nxt ? H1v : (T)0, Q1 = nxt ? Q1v : (T)0;
   // row of the last cell: lo x_{CPL-2} + dg x_E + up x_0(next lane)
   const T le = lo[CPL - 1], ue = up[CPL - 1];
@@ -1404,12 +1411,12 @@ __global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* _
   T c = ue * Q1;
   T r = rh[CPL - 1] - le * Gb - ue * G1;
 #pragma unroll
-  for (int st = 1; st < 64; st <<= 1) {
-    const bool okm = lane - st >= 0, okp = lane + st < 64;
-    T am = wave_get(a, lane - st, okm), dm = wave_get(d, lane - st, okm);
-    T cm = wave_get(c, lane - st, okm), rm = wave_get(r, lane - st, okm);
-    T ap = wave_get(a, lane + st, okp), dp = wave_get(d, lane + st, okp);
-    T cp = wave_get(c, lane + st, okp), rp = wave_get(r, lane + st, okp);
+  for (int st = 1; st < LANES; st <<= 1) {
+    const bool okm = lane - st >= 0, okp = lane + st < LANES;
+    T am = wave_get<LANES>(a, lane - st, okm), dm = wave_get<LANES>(d, lane - st, okm);
+    T cm = wave_get<LANES>(c, lane - st, okm), rm = wave_get<LANES>(r, lane - st, okm);
+    T ap = wave_get<LANES>(a, lane + st, okp), dp = wave_get<LANES>(d, lane + st, okp);
+    T cp = wave_get<LANES>(c, lane + st, okp), rp = wave_get<LANES>(r, lane + st, okp);
     if (!okm) {
       am = cm = rm = (T)0;
       dm = (T)1;
@@ -1425,13 +1432,13 @@ __global__ void __launch_bounds__(256) k_line_x(const T* __restrict__ x_in, T* _
     r = r + al * rm + ga * rp;
   }
   const T xe = r * recip0(d);
-  const T xl0 = wave_get(xe, lane - 1, lane > 0);
+  const T xl0 = wave_get<LANES>(xe, lane - 1, lane > 0);
   const T xl = lane > 0 ? xl0 : (T)0;
 #pragma unroll
   for (int j = 0; j < CPL; ++j) {
     const int k = lane * CPL + j;
     const T v = j == CPL - 1 ? xe : G[j] + H[j] * xl + Q[j] * xe;
-    if (k < N) x_out[row * N + k] = v;
+    if (live && k < N) x_out[row * N + k] = v;
   }
 }
 
@@ -1972,27 +1979,29 @@ static void line_pass_z(const Level& L, int dir, const T* x_in, T* x_out, const 
   if (dir == 0) {
     // the FP64 x-line pass reads 1/a_p even on a Level::dff level: rebuilding a_p there measured
     // slower at 256^3 (226 against 195 us), the FP32 one faster (80 against 99 us)
-    const int blocks = (int)(((long)d.M * d.K + 3) / 4);
-#define LSX_LINE_X(CPL)                                                                                        \
+#define LSX_LINE_X(CPL, LANES)                                                                                 \
   do {                                                                                                         \
+    const int blocks = (int)(((long)d.M * d.K + 256 / LANES - 1) / (256 / LANES));                             \
     if (trans)                                                                                                 \
-      k_line_x<T, FT, CPL, ZERO, true, false><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N,   \
-                                                                     d.M, d.K, L.pin0, L.pin1, dr);            \
+      k_line_x<T, FT, CPL, ZERO, true, false, LANES><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, \
+                                                                            d.N, d.M, d.K, L.pin0, L.pin1, dr); \
     else if (L.dff && sizeof(T) == 4)                                                                          \
-      k_line_x<T, FT, CPL, ZERO, false, true><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N,   \
-                                                                     d.M, d.K, L.pin0, L.pin1, dr);            \
+      k_line_x<T, FT, CPL, ZERO, false, true, LANES><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, \
+                                                                            d.N, d.M, d.K, L.pin0, L.pin1, dr); \
     else                                                                                                       \
-      k_line_x<T, FT, CPL, ZERO, false, false><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N,  \
-                                                                      d.M, d.K, L.pin0, L.pin1, dr);           \
+      k_line_x<T, FT, CPL, ZERO, false, false, LANES><<<blocks, 256, 0, s>>>(x_in, x_out, b, ae, an, at, sv,    \
+                                                                             fs, d.N, d.M, d.K, L.pin0, L.pin1, \
+                                                                             dr);                               \
   } while (0)
+    // lines of up to 128 cells: two per wavefront, four cells per lane
     if (d.N <= 64)
-      LSX_LINE_X(1);
+      LSX_LINE_X(1, 64);
     else if (d.N <= 128)
-      LSX_LINE_X(2);
+      LSX_LINE_X(4, 32);
     else if (d.N <= 256)
-      LSX_LINE_X(4);
+      LSX_LINE_X(4, 64);
     else
-      LSX_LINE_X(8);
+      LSX_LINE_X(8, 64);
 #undef LSX_LINE_X
   } else {
     const int len = dir == 1 ? d.M : d.K;
