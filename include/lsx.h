@@ -101,8 +101,9 @@ int lsx_destroy(lsx_solver* s);
  * (multigrid_solver.h:76-101).  The matrix must have the reference pressure-operator
  * pattern: each row's columns are a subset of {num-N*M, num-N, num-1, num, num+1, num+N,
  * num+N*M}, each at most once.  mem says whether the three arrays are host or device pointers.
- * N, M, K must halve to 2 along the smallest dimension with every level even (even-even-even
- * transfers). */
+ * Every level of the reference cascade that is coarsened must have all three dimensions even
+ * (even-even-even transfers, coarse size n / 2) or all three odd (odd-odd-odd transfers, coarse
+ * size (n + 1) / 2); the V-cycle preconditioners need even dimensions on every coarsened level. */
 int lsx_set_csr(lsx_solver* s, int N, int M, int K, const int* rowptr, const int* col, const double* val, long nnz,
                 int mem);
 

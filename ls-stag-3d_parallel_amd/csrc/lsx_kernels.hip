@@ -234,6 +234,139 @@ __global__ void k_galerkin(const double* __restrict__ cf, int Nf, int Mf, int Kf
   }
 }
 
+// ---- odd-odd-odd transfers (transfer_operators_odd_odd_odd.h:44-100, R = P^T by
+// transfer_operators_interface.h:572-578 and transposed_sparse_matrix.h:47-68) --------------------
+// An odd axis of 2n - 1 fine cells coarsens to n cells centred on the even fine cells: P weights
+// 1 on a coincident fine cell, 1/2 on each coarse neighbour of an odd fine cell; a row's entries
+// run z, then y, then x, coarse index ascending.  R row C lists fine cells 2C - 1, 2C, 2C + 1 per
+// axis (weights 1/2, 1, 1/2, those inside the level) in ascending fine index.
+__device__ inline void axis_w_odd(int f, int* c, double* w, int* cnt) {
+  if ((f & 1) == 0) {
+    c[0] = f >> 1; w[0] = 1.0; *cnt = 1;
+  } else {
+    c[0] = (f - 1) >> 1; w[0] = 0.5; c[1] = (f + 1) >> 1; w[1] = 0.5; *cnt = 2;
+  }
+}
+__device__ inline int axis_r_odd(int C, int nf, int* f, double* w) {
+  int n = 0;
+  if (2 * C - 1 >= 0) { f[n] = 2 * C - 1; w[n] = 0.5; ++n; }
+  f[n] = 2 * C; w[n] = 1.0; ++n;
+  if (2 * C + 1 < nf) { f[n] = 2 * C + 1; w[n] = 0.5; ++n; }
+  return n;
+}
+
+// Galerkin product A_c = (R A_f) P for odd-odd-odd levels, terms in the order of
+// product_sparse_matrix.h:72-104: V = R A_f accumulates over R's fine cells in ascending order,
+// each with its A_f row in ascending column order; A_c then accumulates V's fine columns in
+// ascending order (V's window: fine cells 2C - 2 .. 2C + 2 per axis).
+template <int NPTS>
+__global__ void k_galerkin_odd(const double* __restrict__ cf, int Nf, int Mf, int Kf, double* __restrict__ cc,
+                               int Nc, int Mc, int Kc) {
+  const long nc = (long)Nc * Mc * Kc, nf = (long)Nf * Mf * Kf;
+  const int NMf = Nf * Mf;
+  for (long li = blockIdx.x * (long)blockDim.x + threadIdx.x; li < nc; li += (long)gridDim.x * blockDim.x) {
+    const int I = (int)li;
+    int X, Y, Z;
+    decode(I, Nc, Mc, &X, &Y, &Z);
+    double V[125];
+    for (int q = 0; q < 125; ++q) V[q] = 0.0;
+    int rx[3], ry[3], rz[3];
+    double wx[3], wy[3], wz[3];
+    const int nx = axis_r_odd(X, Nf, rx, wx), ny = axis_r_odd(Y, Mf, ry, wy), nz = axis_r_odd(Z, Kf, rz, wz);
+    for (int c3 = 0; c3 < nz; ++c3)
+      for (int c2 = 0; c2 < ny; ++c2)
+        for (int c1 = 0; c1 < nx; ++c1) {
+          const double w = wx[c1] * wy[c2] * wz[c3];
+          const int hx = rx[c1], hy = ry[c2], hz = rz[c3];
+          const int h = hx + Nf * hy + NMf * hz;
+          for (int q = 0; q < NPTS; ++q) {
+            int dx, dy, dz;
+            if (NPTS == 7) {
+              const int dxs[7] = {0, 0, -1, 0, 1, 0, 0}, dys[7] = {0, -1, 0, 0, 0, 1, 0}, dzs[7] = {-1, 0, 0, 0, 0, 0, 1};
+              dx = dxs[q];
+              dy = dys[q];
+              dz = dzs[q];
+            } else {
+              dx = q % 3 - 1;
+              dy = (q / 3) % 3 - 1;
+              dz = q / 9 - 1;
+            }
+            const int fx = hx + dx, fy = hy + dy, fz = hz + dz;
+            if (fx < 0 || fx >= Nf || fy < 0 || fy >= Mf || fz < 0 || fz >= Kf) continue;
+            const int b = (fx - 2 * X + 2) + 5 * (fy - 2 * Y + 2) + 25 * (fz - 2 * Z + 2);
+            V[b] = V[b] + w * cf[q * nf + h];
+          }
+        }
+    double A[27];
+    for (int q = 0; q < 27; ++q) A[q] = 0.0;
+    for (int b = 0; b < 125; ++b) {
+      const int fx = 2 * X - 2 + b % 5, fy = 2 * Y - 2 + (b / 5) % 5, fz = 2 * Z - 2 + b / 25;
+      if (fx < 0 || fx >= Nf || fy < 0 || fy >= Mf || fz < 0 || fz >= Kf) continue;
+      int cx[2], cy[2], cz[2], mx, my, mz;
+      double px[2], py[2], pz[2];
+      axis_w_odd(fx, cx, px, &mx);
+      axis_w_odd(fy, cy, py, &my);
+      axis_w_odd(fz, cz, pz, &mz);
+      const double v = V[b];
+      for (int c3 = 0; c3 < mz; ++c3)
+        for (int c2 = 0; c2 < my; ++c2)
+          for (int c1 = 0; c1 < mx; ++c1) {
+            const double w = px[c1] * py[c2] * pz[c3];
+            const int q = (cx[c1] - X + 1) + 3 * (cy[c2] - Y + 1) + 9 * (cz[c3] - Z + 1);
+            A[q] = A[q] + v * w;
+          }
+    }
+    for (int q = 0; q < 27; ++q) cc[q * nc + I] = A[q];
+  }
+}
+
+// r_c = R r_f on odd-odd-odd levels (R = P^T, fine cells in ascending order)
+__global__ void k_restrict_odd(const double* __restrict__ rf, int Nf, int Mf, int Kf, double* __restrict__ rc, int Nc,
+                               int Mc, int Kc) {
+  const long nc = (long)Nc * Mc * Kc;
+  const int NMf = Nf * Mf;
+  for (long li = blockIdx.x * (long)blockDim.x + threadIdx.x; li < nc; li += (long)gridDim.x * blockDim.x) {
+    int X, Y, Z;
+    decode((int)li, Nc, Mc, &X, &Y, &Z);
+    int rx[3], ry[3], rz[3];
+    double wx[3], wy[3], wz[3];
+    const int nx = axis_r_odd(X, Nf, rx, wx), ny = axis_r_odd(Y, Mf, ry, wy), nz = axis_r_odd(Z, Kf, rz, wz);
+    double acc = 0.0;
+    for (int c3 = 0; c3 < nz; ++c3)
+      for (int c2 = 0; c2 < ny; ++c2)
+        for (int c1 = 0; c1 < nx; ++c1) {
+          const double w = wx[c1] * wy[c2] * wz[c3];
+          acc = acc + w * rf[rx[c1] + Nf * ry[c2] + NMf * rz[c3]];
+        }
+    rc[li] = acc;
+  }
+}
+
+// ch_f = P ch_c, c_f = ch_f on odd-odd-odd levels
+__global__ void k_prolong_odd(const double* __restrict__ chc, int Nc, int Mc, double* __restrict__ chf,
+                              double* __restrict__ cf, int Nf, int Mf, int Kf) {
+  const long nf = (long)Nf * Mf * Kf;
+  const int NMc = Nc * Mc;
+  for (long li = blockIdx.x * (long)blockDim.x + threadIdx.x; li < nf; li += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode((int)li, Nf, Mf, &x, &y, &z);
+    int cx[2], cy[2], cz[2], nx, ny, nz;
+    double wx[2], wy[2], wz[2];
+    axis_w_odd(x, cx, wx, &nx);
+    axis_w_odd(y, cy, wy, &ny);
+    axis_w_odd(z, cz, wz, &nz);
+    double acc = 0.0;
+    for (int c3 = 0; c3 < nz; ++c3)
+      for (int c2 = 0; c2 < ny; ++c2)
+        for (int c1 = 0; c1 < nx; ++c1) {
+          const double w = wx[c1] * wy[c2] * wz[c3];
+          acc = acc + w * chc[cx[c1] + Nc * cy[c2] + NMc * cz[c3]];
+        }
+    chf[li] = acc;
+    cf[li] = acc;
+  }
+}
+
 template <int BS>
 __global__ void k_diag_absmax_partial(const double* __restrict__ diag, long n, double* partial) {
   long lo, hi;
@@ -1109,6 +1242,13 @@ void csr_to_stencil7(const int* rowptr, const int* col, const double* val, Dims 
 }
 
 void galerkin(const double* coef_f, int npts_f, Dims df, double* coef27_c, Dims dc, hipStream_t s) {
+  if (df.N & 1) {  // odd-odd-odd level (the only odd class the hierarchy admits)
+    if (npts_f == 7)
+      k_galerkin_odd<7><<<grid_1d(dc.n(), 64), 64, 0, s>>>(coef_f, df.N, df.M, df.K, coef27_c, dc.N, dc.M, dc.K);
+    else
+      k_galerkin_odd<27><<<grid_1d(dc.n(), 64), 64, 0, s>>>(coef_f, df.N, df.M, df.K, coef27_c, dc.N, dc.M, dc.K);
+    return;
+  }
   if (npts_f == 7)
     k_galerkin<7><<<grid_1d(dc.n(), 64), 64, 0, s>>>(coef_f, df.N, df.M, df.K, coef27_c, dc.N, dc.M, dc.K);
   else
@@ -1175,10 +1315,18 @@ void stencil_apply_sym(const double* s4, const SymExc& ex, Dims d, const double*
 }
 
 void restrict_full(const double* rf, Dims df, double* rc, Dims dc, hipStream_t s) {
+  if (df.N & 1) {
+    k_restrict_odd<<<grid_1d(dc.n()), 256, 0, s>>>(rf, df.N, df.M, df.K, rc, dc.N, dc.M, dc.K);
+    return;
+  }
   k_restrict<<<grid_1d(dc.n()), 256, 0, s>>>(rf, df.N, df.M, rc, dc.N, dc.M, dc.K);
 }
 
 void prolong_copy(const double* chc, Dims dc, double* chf, double* cf, Dims df, hipStream_t s) {
+  if (df.N & 1) {
+    k_prolong_odd<<<grid_1d(df.n()), 256, 0, s>>>(chc, dc.N, dc.M, chf, cf, df.N, df.M, df.K);
+    return;
+  }
   k_prolong<<<grid_1d(df.n()), 256, 0, s>>>(chc, dc.N, dc.M, chf, cf, df.N, df.M, df.K);
 }
 

@@ -150,6 +150,10 @@ void Solver::synchronize() { LSX_HIP(hipStreamSynchronize(stream)); }
 // -----------------------------------------------------------------------------------------------
 // setup
 // -----------------------------------------------------------------------------------------------
+// cells of the next coarser level along an axis: n / 2, or (n + 1) / 2 for odd n
+// (coarse_grid.h:60-67)
+static int coarse_count(int n) { return (n & 1) ? (n + 1) / 2 : n / 2; }
+
 static int level_count(int N, int M, int K) {
   // coarse_grids_hierarchy.h:73-81: halve the smallest dimension while it exceeds 2
   double aux = (double)std::min(std::min(N, M), K);
@@ -174,11 +178,18 @@ void Solver::check_dims(int N, int M, int K) const {
   const int nmg = coarsenings(N, M, K);
   int n = N, m = M, k = K;
   for (int l = 0; l < nmg; ++l) {
-    if ((n & 1) || (m & 1) || (k & 1))
-      throw Error(LSX_ERR_ARG, "every multigrid level must have even dimensions (even-even-even transfers)");
-    n /= 2;
-    m /= 2;
-    k /= 2;
+    const int odd = (n & 1) + (m & 1) + (k & 1);
+    if (vcyc() && odd)
+      throw Error(LSX_ERR_ARG, "every V-cycle level must have even dimensions");
+    // the reference's mixed-parity transfer classes (R = P^T with the cell-centred prolongation on
+    // their even axes) widen the coarse operator beyond the 27-point stencils stored here
+    if (odd == 1 || odd == 2)
+      throw Error(LSX_ERR_ARG,
+                  "a multigrid level mixes even and odd dimensions: only even-even-even and odd-odd-odd transfers "
+                  "are supported");
+    n = coarse_count(n);
+    m = coarse_count(m);
+    k = coarse_count(k);
   }
   if (single_level()) return;  // no coarse levels: any dimensions, no coarsest solve
   if (opt.levels == 0 && (long)n * m * k > 64) throw Error(LSX_ERR_ARG, "coarsest level exceeds 64 unknowns");
@@ -196,9 +207,9 @@ void Solver::setup_levels(int N, int M, int K) {
     for (int l = 0; l <= nmg; ++l) {
       levels[l].d = Dims{nn, mm, kk};
       levels[l].npts = l == 0 ? 7 : 27;
-      nn /= 2;
-      mm /= 2;
-      kk /= 2;
+      nn = coarse_count(nn);
+      mm = coarse_count(mm);
+      kk = coarse_count(kk);
     }
   }
   for (size_t l = 0; l < levels.size(); ++l) {

@@ -204,6 +204,78 @@ static void axis_weights(int f, int nf, int* c, double* w, int* cnt) {
   *cnt = 2;
 }
 
+// Odd-odd-odd transfer operators (transfer_operators_odd_odd_odd.h:44-100 with
+// transfer_operators_interface.h:48-57, 131-154, 181-198, 203-235): an axis with an odd number
+// 2n - 1 of fine cells coarsens to n cells whose centres sit on the even fine cells; P weights 1
+// on a coincident fine cell and 1/2 on each of the two coarse neighbours of an odd fine cell, each
+// row's entries ordered z, then y, then x, coarse index ascending.  R = P^T
+// (transfer_operators_interface.h:572-578, the transposition of transposed_sparse_matrix.h:47-68:
+// each R row lists its fine cells in ascending order).  The mixed-parity classes build R the same
+// way but keep the cell-centred 3/4, 1/4 prolongation on their even axes, which widens the coarse
+// operator to five cells along those axes; they are not restated.
+static void axis_weights_odd(int f, int* c, double* w, int* cnt) {
+  if ((f & 1) == 0) {
+    c[0] = f >> 1;
+    w[0] = 1.0;
+    *cnt = 1;
+  } else {
+    c[0] = (f - 1) >> 1;
+    w[0] = 0.5;
+    c[1] = (f + 1) >> 1;
+    w[1] = 0.5;
+    *cnt = 2;
+  }
+}
+
+int coarse_count(int n) { return (n & 1) ? (n + 1) / 2 : n / 2; }  // coarse_grid.h:60-67
+
+Csr build_P_odd(int NN, int MM, int KK) {
+  const int N = coarse_count(NN), M = coarse_count(MM), K = coarse_count(KK);
+  Csr P;
+  P.rows = (long)NN * MM * KK;
+  P.cols = (long)N * M * K;
+  P.rowptr.assign(P.rows + 1, 0);
+  P.col.reserve(P.rows * 8);
+  P.val.reserve(P.rows * 8);
+  long r = 0;
+  for (int z = 0; z < KK; ++z)
+    for (int y = 0; y < MM; ++y)
+      for (int x = 0; x < NN; ++x, ++r) {
+        int cx[2], cy[2], cz[2], nx, ny, nz;
+        double wx[2], wy[2], wz[2];
+        axis_weights_odd(x, cx, wx, &nx);
+        axis_weights_odd(y, cy, wy, &ny);
+        axis_weights_odd(z, cz, wz, &nz);
+        for (int c = 0; c < nz; ++c)
+          for (int b = 0; b < ny; ++b)
+            for (int a = 0; a < nx; ++a) {
+              P.col.push_back(cx[a] + N * cy[b] + N * M * cz[c]);
+              P.val.push_back(wx[a] * wy[b] * wz[c]);
+            }
+        P.rowptr[r + 1] = (int)P.col.size();
+      }
+  return P;
+}
+
+Csr transpose(const Csr& P) {
+  Csr R;
+  R.rows = P.cols;
+  R.cols = P.rows;
+  R.rowptr.assign(R.rows + 1, 0);
+  for (int c : P.col) ++R.rowptr[c + 1];
+  for (long i = 0; i < R.rows; ++i) R.rowptr[i + 1] += R.rowptr[i];
+  R.col.assign(P.col.size(), 0);
+  R.val.assign(P.col.size(), 0.0);
+  std::vector<int> fill(R.rowptr.begin(), R.rowptr.end() - 1);
+  for (long i = 0; i < P.rows; ++i)
+    for (int j = P.rowptr[i]; j < P.rowptr[i + 1]; ++j) {
+      const int k = fill[P.col[j]]++;
+      R.col[k] = (int)i;
+      R.val[k] = P.val[j];
+    }
+  return R;
+}
+
 Csr build_P(int NN, int MM, int KK) {
   const int N = NN / 2, M = MM / 2, K = KK / 2;
   Csr P;
@@ -506,14 +578,21 @@ void Multigrid::set(const Csr& A0, int N, int M, int K, int n_post_, double omeg
   levels[0].K = K;
   for (int l = 0; l < numMG; ++l) {
     Level& f = levels[l];
-    if ((f.N & 1) || (f.M & 1) || (f.K & 1))
-      throw std::runtime_error("odd level dimensions are not supported (even-even-even transfers only)");
-    f.P = build_P(f.N, f.M, f.K);
-    f.R = build_R(f.N, f.M, f.K);
+    const int odd = (f.N & 1) + (f.M & 1) + (f.K & 1);
+    if (odd == 0) {
+      f.P = build_P(f.N, f.M, f.K);
+      f.R = build_R(f.N, f.M, f.K);
+    } else if (odd == 3) {
+      f.P = build_P_odd(f.N, f.M, f.K);
+      f.R = transpose(f.P);
+    } else {
+      throw std::runtime_error("mixed-parity level dimensions are not supported (even-even-even and odd-odd-odd "
+                               "transfers only)");
+    }
     Level& cgrid = levels[l + 1];
-    cgrid.N = f.N / 2;
-    cgrid.M = f.M / 2;
-    cgrid.K = f.K / 2;
+    cgrid.N = coarse_count(f.N);
+    cgrid.M = coarse_count(f.M);
+    cgrid.K = coarse_count(f.K);
     Csr V = spgemm(f.R, f.A, f.A.cols);
     cgrid.A = spgemm(V, f.P, f.P.cols);
     f.adlj.build(f.A, f.N, f.M, f.K);

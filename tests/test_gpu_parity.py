@@ -44,6 +44,10 @@ CASES = [
     ("cut32x16x24", lambda: P.synthetic_cut(32, 16, 24, seed=3)),
     ("sphere32", lambda: P.sphere(32)),
     ("ellipsoid32_stretched", lambda: P.ellipsoid_stretched(32)),
+    # odd-odd-odd transfers on every coarsened level (coarse size (n + 1) / 2 per axis)
+    ("free17", lambda: P.body_free(17)),
+    ("sphere33", lambda: P.sphere(33)),
+    ("cut17x33x9", lambda: P.synthetic_cut(17, 33, 9, seed=6)),
 ]
 
 

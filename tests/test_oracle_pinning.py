@@ -132,6 +132,60 @@ def test_galerkin_levels_equal_RAP(mk):
         assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref)), f"level {l + 1}"
 
 
+def kron_transfers_odd(dims):
+    """The odd-odd-odd prolongation as a Kronecker product of 1-D node-centred interpolations
+    (2n - 1 fine cells onto n coarse cells: weight 1 on the coincident fine cell, 1/2 on each
+    coarse neighbour of an odd fine cell) and R = P^T."""
+    def p1(nf):
+        nc = (nf + 1) // 2
+        Pm = np.zeros((nf, nc))
+        for f in range(nf):
+            if f % 2 == 0:
+                Pm[f, f // 2] = 1.0
+            else:
+                Pm[f, (f - 1) // 2] = Pm[f, (f + 1) // 2] = 0.5
+        return Pm
+
+    N, M, K = dims
+    Pk = sp.kron(p1(K), sp.kron(p1(M), p1(N))).tocsr()
+    return Pk, Pk.T.tocsr()
+
+
+@pytest.mark.parametrize("mk", [lambda: P.body_free(17), lambda: P.sphere(33),
+                                lambda: P.synthetic_cut(17, 33, 9, seed=6)])
+def test_galerkin_levels_equal_PtAP_on_odd_grids(mk):
+    g, A = mk()
+    lv = O.Solver(A, eps=1.0).levels()
+    assert len(lv) >= 3
+    for l in range(len(lv) - 1):
+        dims = tuple(int(v) for v in lv[l].dims)
+        assert all(v % 2 for v in dims)
+        Pk, Rk = kron_transfers_odd(dims)
+        assert tuple(lv[l + 1].dims) == tuple((v + 1) // 2 for v in dims)
+        ref = (Rk @ lv[l].to_scipy() @ Pk).tocsr()
+        got = lv[l + 1].to_scipy().tocsr()
+        assert abs(got - ref).max() <= 1e-13 * abs(ref).max(), f"level {l + 1}"
+        # the coarse operator keeps a 27-point stencil
+        assert np.all(np.diff(lv[l + 1].rowptr) <= 27)
+
+
+@pytest.mark.parametrize("kind", ["fgmres", "bicgstab"])
+@pytest.mark.parametrize("mk", [lambda: P.sphere(33), lambda: P.synthetic_cut(17, 33, 9, seed=6)])
+def test_odd_grid_solves_converge(mk, kind):
+    g, A = mk()
+    b = P.random_rhs(g, seed=3, A=A)
+    S = O.Solver(A, kind=kind, eps=1e-8 * np.linalg.norm(b), max_it=200)
+    x, it, nr = S.solve(b)
+    assert it < 40
+    assert np.linalg.norm(b - A.matvec(x)) <= 1.05e-8 * np.linalg.norm(b)
+
+
+def test_mixed_parity_levels_are_rejected():
+    g, A = P.body_free(17, 16, 17)
+    with pytest.raises(RuntimeError, match="mixed-parity"):
+        O.Solver(A, eps=1.0)
+
+
 def test_coarsest_lu_with_no_smoothing():
     g, A = P.synthetic_cut(8, seed=2)
     S = O.Solver(A, eps=1.0, n_post=0)
