@@ -51,10 +51,6 @@ __constant__ double BZ_S[6] = {1.0, -1.0, 1.0, -1.0, 1.0, -1.0};
 __constant__ int BZ_U[6] = {1, 2, 2, 0, 0, 1};
 __constant__ int BZ_V[6] = {2, 1, 0, 2, 1, 0};
 
-struct BzEval {
-  double S[3], Su[3], Sv[3], Suu[3], Suv[3], Svv[3];
-};
-
 __device__ inline void bz_basis(double t, double* b, double* db, double* ddb) {
   const double s = 1.0 - t;
   b[0] = s * s * s;
@@ -71,29 +67,50 @@ __device__ inline void bz_basis(double t, double* b, double* db, double* ddb) {
   ddb[3] = 6.0 * t;
 }
 
-__device__ inline void bz_eval(const double* __restrict__ P, double u, double v, BzEval& e) {
+// S, S_u and S_v of patch P at (u, v): each row i of the control net is first reduced along v
+// (q_i = sum_j B_j(v) P_ij and its v derivative), then the rows along u
+__device__ inline void bz_eval1(const double* __restrict__ P, double u, double v, double* S, double* Su, double* Sv) {
   double bu[4], dbu[4], ddbu[4], bv[4], dbv[4], ddbv[4];
   bz_basis(u, bu, dbu, ddbu);
   bz_basis(v, bv, dbv, ddbv);
-  for (int c = 0; c < 3; ++c) {
-    double s = 0.0, su = 0.0, sv = 0.0, suu = 0.0, suv = 0.0, svv = 0.0;
-    for (int i = 0; i < 4; ++i)
+  for (int c = 0; c < 3; ++c) S[c] = Su[c] = Sv[c] = 0.0;
+  for (int i = 0; i < 4; ++i)
+    for (int c = 0; c < 3; ++c) {
+      double q = 0.0, dq = 0.0;
       for (int j = 0; j < 4; ++j) {
         const double p = P[(i * 4 + j) * 3 + c];
-        s += bu[i] * bv[j] * p;
-        su += dbu[i] * bv[j] * p;
-        sv += bu[i] * dbv[j] * p;
-        suu += ddbu[i] * bv[j] * p;
-        suv += dbu[i] * dbv[j] * p;
-        svv += bu[i] * ddbv[j] * p;
+        q += bv[j] * p;
+        dq += dbv[j] * p;
       }
-    e.S[c] = s;
-    e.Su[c] = su;
-    e.Sv[c] = sv;
-    e.Suu[c] = suu;
-    e.Suv[c] = suv;
-    e.Svv[c] = svv;
+      S[c] += bu[i] * q;
+      Su[c] += dbu[i] * q;
+      Sv[c] += bu[i] * dq;
+    }
+}
+
+// r . S_uu, r . S_uv and r . S_vv of patch P at (u, v), through w_ij = P_ij . r
+__device__ inline void bz_eval2(const double* __restrict__ P, double u, double v, const double* r, double* ruu,
+                                double* ruv, double* rvv) {
+  double bu[4], dbu[4], ddbu[4], bv[4], dbv[4], ddbv[4];
+  bz_basis(u, bu, dbu, ddbu);
+  bz_basis(v, bv, dbv, ddbv);
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int i = 0; i < 4; ++i) {
+    double q = 0.0, dq = 0.0, ddq = 0.0;
+    for (int j = 0; j < 4; ++j) {
+      const double* pij = P + (i * 4 + j) * 3;
+      const double w = pij[0] * r[0] + pij[1] * r[1] + pij[2] * r[2];
+      q += bv[j] * w;
+      dq += dbv[j] * w;
+      ddq += ddbv[j] * w;
+    }
+    a += ddbu[i] * q;
+    b += dbu[i] * dq;
+    c += bu[i] * ddq;
   }
+  *ruu = a;
+  *ruv = b;
+  *rvv = c;
 }
 
 __device__ inline double bz_dot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
@@ -105,60 +122,108 @@ struct BzHit {
   double u, v, r[3], dist, sgn;
 };
 
+// Newton search for the closest point of patch f (control values P) to X (d: X in the body's
+// normalised coordinates, for the start); returns |X - S|^2 with (u, v), r = X - S and
+// side = r . (S_u x S_v) there.  A step that leaves (u, v) unchanged ends the search: the steps
+// after it would repeat it exactly.
+__device__ inline double bz_patch(const double* __restrict__ P, int f, const double* X, const double* d, double& u,
+                                  double& v, double* r, double& side) {
+  const double da = fabs(d[BZ_A[f]]);
+  u = 0.5;
+  v = 0.5;
+  if (da > 0.0) {
+    u = bz_clamp(0.5 * (d[BZ_U[f]] / da + 1.0));
+    v = bz_clamp(0.5 * (d[BZ_V[f]] / da + 1.0));
+  }
+  double S[3], Su[3], Sv[3];
+  for (int it = 0; it < 12; ++it) {
+    bz_eval1(P, u, v, S, Su, Sv);
+    double q[3];
+    for (int c = 0; c < 3; ++c) q[c] = S[c] - X[c];
+    const double gu = bz_dot(Su, q), gv = bz_dot(Sv, q);
+    const double juu = bz_dot(Su, Su), juv = bz_dot(Su, Sv), jvv = bz_dot(Sv, Sv);
+    double ruu, ruv, rvv;
+    bz_eval2(P, u, v, q, &ruu, &ruv, &rvv);
+    double huu = juu + ruu, huv = juv + ruv, hvv = jvv + rvv;
+    double det = huu * hvv - huv * huv;
+    if (!(det > 0.0 && huu > 0.0)) {
+      huu = juu;
+      huv = juv;
+      hvv = jvv;
+      det = huu * hvv - huv * huv;
+    }
+    const bool hold_u = (u >= 1.0 && gu < 0.0) || (u <= 0.0 && gu > 0.0);
+    const bool hold_v = (v >= 1.0 && gv < 0.0) || (v <= 0.0 && gv > 0.0);
+    double du = 0.0, dv = 0.0;
+    if (hold_u && hold_v) break;
+    if (hold_u) {
+      dv = -gv / (hvv > 0.0 ? hvv : jvv);
+    } else if (hold_v) {
+      du = -gu / (huu > 0.0 ? huu : juu);
+    } else {
+      if (!(det > 0.0)) break;
+      du = -(hvv * gu - huv * gv) / det;
+      dv = -(huu * gv - huv * gu) / det;
+    }
+    const double un = bz_clamp(u + du), vn = bz_clamp(v + dv);
+    if (un == u && vn == v) break;
+    u = un;
+    v = vn;
+  }
+  bz_eval1(P, u, v, S, Su, Sv);
+  for (int c = 0; c < 3; ++c) r[c] = X[c] - S[c];
+  const double nrm[3] = {Su[1] * Sv[2] - Su[2] * Sv[1], Su[2] * Sv[0] - Su[0] * Sv[2], Su[0] * Sv[1] - Su[1] * Sv[0]};
+  side = bz_dot(r, nrm);
+  return bz_dot(r, r);
+}
+
+// squared distance from X to the bounding box of patch P's control points (the patch lies in
+// their convex hull)
+__device__ inline double bz_box_dist2(const double* __restrict__ P, const double* X) {
+  double acc = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    double lo = P[c], hi = P[c];
+    for (int k = 1; k < 16; ++k) {
+      lo = fmin(lo, P[3 * k + c]);
+      hi = fmax(hi, P[3 * k + c]);
+    }
+    const double e = X[c] < lo ? lo - X[c] : (X[c] > hi ? X[c] - hi : 0.0);
+    acc += e * e;
+  }
+  return acc;
+}
+
+// The patch with the smallest distance wins, the lowest index among equal distances.  Each point
+// first searches its home patch (the cube face its normalised direction points to); another
+// patch is searched only if its control-point box comes within the best distance so far plus a
+// margin of 1e-12, far above rounding (a patch further away cannot win or tie).  The loops run
+// over the patches in order so that the control values stay uniform over the wavefront.
 __device__ double bezier_closest(const double* __restrict__ ctrl, const double* c0, const double* a0, double x,
                                  double y, double z, BzHit& hit) {
   const double X[3] = {x, y, z};
   double d[3];
   for (int c = 0; c < 3; ++c) d[c] = (X[c] - c0[c]) / a0[c];
-  double best = -1.0, side = 0.0;
-  for (int f = 0; f < 6; ++f) {
+  int ax = 0;
+  for (int c = 1; c < 3; ++c)
+    if (fabs(d[c]) > fabs(d[ax])) ax = c;
+  const int home = 2 * ax + (d[ax] < 0.0 ? 1 : 0);  // faces 2a, 2a + 1: normal axis a, sign + / -
+  double best = 0.0, side = 0.0;
+  // pass 0 searches the home patch, pass 1 the others (one call site: one copy of the search)
+  for (int k = 0; k < 12; ++k) {
+    const int f = k < 6 ? k : k - 6;
     const double* P = ctrl + f * 48;
-    const double da = fabs(d[BZ_A[f]]);
-    double u = 0.5, v = 0.5;
-    if (da > 0.0) {
-      u = bz_clamp(0.5 * (d[BZ_U[f]] / da + 1.0));
-      v = bz_clamp(0.5 * (d[BZ_V[f]] / da + 1.0));
+    if (k < 6) {
+      if (f != home) continue;
+    } else {
+      if (f == home) continue;
+      const double m = sqrt(bz_box_dist2(P, X)) - 1e-12;
+      if (m > 0.0 && m * m > best) continue;
     }
-    BzEval e;
-    for (int it = 0; it < 12; ++it) {
-      bz_eval(P, u, v, e);
-      double r[3];
-      for (int c = 0; c < 3; ++c) r[c] = e.S[c] - X[c];
-      const double gu = bz_dot(e.Su, r), gv = bz_dot(e.Sv, r);
-      const double juu = bz_dot(e.Su, e.Su), juv = bz_dot(e.Su, e.Sv), jvv = bz_dot(e.Sv, e.Sv);
-      double huu = juu + bz_dot(e.Suu, r), huv = juv + bz_dot(e.Suv, r), hvv = jvv + bz_dot(e.Svv, r);
-      double det = huu * hvv - huv * huv;
-      if (!(det > 0.0 && huu > 0.0)) {
-        huu = juu;
-        huv = juv;
-        hvv = jvv;
-        det = huu * hvv - huv * huv;
-      }
-      const bool hold_u = (u >= 1.0 && gu < 0.0) || (u <= 0.0 && gu > 0.0);
-      const bool hold_v = (v >= 1.0 && gv < 0.0) || (v <= 0.0 && gv > 0.0);
-      double du = 0.0, dv = 0.0;
-      if (hold_u && hold_v) break;
-      if (hold_u) {
-        dv = -gv / (hvv > 0.0 ? hvv : jvv);
-      } else if (hold_v) {
-        du = -gu / (huu > 0.0 ? huu : juu);
-      } else {
-        if (!(det > 0.0)) break;
-        du = -(hvv * gu - huv * gv) / det;
-        dv = -(huu * gv - huv * gu) / det;
-      }
-      u = bz_clamp(u + du);
-      v = bz_clamp(v + dv);
-    }
-    bz_eval(P, u, v, e);
-    double r[3];
-    for (int c = 0; c < 3; ++c) r[c] = X[c] - e.S[c];
-    const double dist2 = bz_dot(r, r);
-    if (best < 0.0 || dist2 < best) {
+    double u, v, r[3], sd;
+    const double dist2 = bz_patch(P, f, X, d, u, v, r, sd);
+    if (k < 6 || dist2 < best || (dist2 == best && f < hit.f)) {
       best = dist2;
-      const double nrm[3] = {e.Su[1] * e.Sv[2] - e.Su[2] * e.Sv[1], e.Su[2] * e.Sv[0] - e.Su[0] * e.Sv[2],
-                             e.Su[0] * e.Sv[1] - e.Su[1] * e.Sv[0]};
-      side = bz_dot(r, nrm);
+      side = sd;
       hit.f = f;
       hit.u = u;
       hit.v = v;

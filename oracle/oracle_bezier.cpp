@@ -51,33 +51,49 @@ void bern(double t, double* b, double* db, double* ddb) {
   ddb[3] = 6.0 * t;
 }
 
-struct PatchEval {
-  double S[3], Su[3], Sv[3], Suu[3], Suv[3], Svv[3];
-};
-
-void eval_patch(const double* P, double u, double v, PatchEval& e) {
+// S, S_u and S_v of patch P at (u, v): each row i of the control net is first reduced along v
+// (q_i = sum_j B_j(v) P_ij and its v derivative), then the rows along u
+void eval_patch(const double* P, double u, double v, double* S, double* Su, double* Sv) {
   double bu[4], dbu[4], ddbu[4], bv[4], dbv[4], ddbv[4];
   bern(u, bu, dbu, ddbu);
   bern(v, bv, dbv, ddbv);
-  for (int c = 0; c < 3; ++c) {
-    double s = 0.0, su = 0.0, sv = 0.0, suu = 0.0, suv = 0.0, svv = 0.0;
-    for (int i = 0; i < 4; ++i)
+  for (int c = 0; c < 3; ++c) S[c] = Su[c] = Sv[c] = 0.0;
+  for (int i = 0; i < 4; ++i)
+    for (int c = 0; c < 3; ++c) {
+      double q = 0.0, dq = 0.0;
       for (int j = 0; j < 4; ++j) {
         const double p = P[(i * 4 + j) * 3 + c];
-        s += bu[i] * bv[j] * p;
-        su += dbu[i] * bv[j] * p;
-        sv += bu[i] * dbv[j] * p;
-        suu += ddbu[i] * bv[j] * p;
-        suv += dbu[i] * dbv[j] * p;
-        svv += bu[i] * ddbv[j] * p;
+        q += bv[j] * p;
+        dq += dbv[j] * p;
       }
-    e.S[c] = s;
-    e.Su[c] = su;
-    e.Sv[c] = sv;
-    e.Suu[c] = suu;
-    e.Suv[c] = suv;
-    e.Svv[c] = svv;
+      S[c] += bu[i] * q;
+      Su[c] += dbu[i] * q;
+      Sv[c] += bu[i] * dq;
+    }
+}
+
+// r . S_uu, r . S_uv and r . S_vv of patch P at (u, v), through w_ij = P_ij . r
+void eval_patch_second(const double* P, double u, double v, const double* r, double* ruu, double* ruv, double* rvv) {
+  double bu[4], dbu[4], ddbu[4], bv[4], dbv[4], ddbv[4];
+  bern(u, bu, dbu, ddbu);
+  bern(v, bv, dbv, ddbv);
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int i = 0; i < 4; ++i) {
+    double q = 0.0, dq = 0.0, ddq = 0.0;
+    for (int j = 0; j < 4; ++j) {
+      const double* pij = P + (i * 4 + j) * 3;
+      const double w = pij[0] * r[0] + pij[1] * r[1] + pij[2] * r[2];
+      q += bv[j] * w;
+      dq += dbv[j] * w;
+      ddq += ddbv[j] * w;
+    }
+    a += ddbu[i] * q;
+    b += dbu[i] * dq;
+    c += bu[i] * ddq;
   }
+  *ruu = a;
+  *ruv = b;
+  *rvv = c;
 }
 
 double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
@@ -147,14 +163,16 @@ double bezier_level_set(const double* ctrl, const double* center, const double* 
       u = clamp01(0.5 * (d[bu] / da + 1.0));
       v = clamp01(0.5 * (d[bv] / da + 1.0));
     }
-    PatchEval e;
+    double S[3], Su[3], Sv[3];
     for (int it = 0; it < 12; ++it) {
-      eval_patch(P, u, v, e);
+      eval_patch(P, u, v, S, Su, Sv);
       double r[3];
-      for (int c = 0; c < 3; ++c) r[c] = e.S[c] - X[c];
-      const double gu = dot3(e.Su, r), gv = dot3(e.Sv, r);
-      const double juu = dot3(e.Su, e.Su), juv = dot3(e.Su, e.Sv), jvv = dot3(e.Sv, e.Sv);
-      double huu = juu + dot3(e.Suu, r), huv = juv + dot3(e.Suv, r), hvv = jvv + dot3(e.Svv, r);
+      for (int c = 0; c < 3; ++c) r[c] = S[c] - X[c];
+      const double gu = dot3(Su, r), gv = dot3(Sv, r);
+      const double juu = dot3(Su, Su), juv = dot3(Su, Sv), jvv = dot3(Sv, Sv);
+      double ruu, ruv, rvv;
+      eval_patch_second(P, u, v, r, &ruu, &ruv, &rvv);
+      double huu = juu + ruu, huv = juv + ruv, hvv = jvv + rvv;
       double det = huu * hvv - huv * huv;
       if (!(det > 0.0 && huu > 0.0)) {  // Gauss-Newton where the Hessian is not positive definite
         huu = juu;
@@ -180,14 +198,14 @@ double bezier_level_set(const double* ctrl, const double* center, const double* 
       u = clamp01(u + du);
       v = clamp01(v + dv);
     }
-    eval_patch(P, u, v, e);
+    eval_patch(P, u, v, S, Su, Sv);
     double r[3];
-    for (int c = 0; c < 3; ++c) r[c] = X[c] - e.S[c];
+    for (int c = 0; c < 3; ++c) r[c] = X[c] - S[c];
     const double dist2 = dot3(r, r);
     if (best < 0.0 || dist2 < best) {
       best = dist2;
-      const double n[3] = {e.Su[1] * e.Sv[2] - e.Su[2] * e.Sv[1], e.Su[2] * e.Sv[0] - e.Su[0] * e.Sv[2],
-                           e.Su[0] * e.Sv[1] - e.Su[1] * e.Sv[0]};
+      const double n[3] = {Su[1] * Sv[2] - Su[2] * Sv[1], Su[2] * Sv[0] - Su[0] * Sv[2],
+                           Su[0] * Sv[1] - Su[1] * Sv[0]};
       side = dot3(r, n);
     }
   }
