@@ -238,11 +238,12 @@ def cpu_baseline(gpu_n, restart, krylov, body="bezier"):
 
 
 def run_solver(lsx, hip, geom, precision, krylov, restart, steps, warmup, max_it=1000, smooth=(2, 2), graphs=False,
-               smoother="rbgs", line_levels=0, levels=0, profile_steps=0, fused_sweep=1, diag_faces=True):
+               smoother="rbgs", line_levels=0, levels=0, profile_steps=0, fused_sweep=1, diag_faces=True,
+               dot32=True):
     """Set up one solver on the device geometry and time `steps` solves (x0 = 0 each); then
     `profile_steps` more solves with per-kernel event timing."""
     S = lsx.solver(krylov, precision, graphs=graphs, smoother=smoother, line_levels=line_levels, levels=levels,
-                   fused_sweep=fused_sweep, diag_faces=diag_faces) \
+                   fused_sweep=fused_sweep, diag_faces=diag_faces, dot32=dot32) \
         if precision != "fp64" else lsx.solver(krylov, precision)
     task = lsx.Task(eps=1.0, MAX_ITER=max_it, FGMRES_RESTART_NUM=restart) if precision == "fp64" else \
         lsx.mixed_task(1.0, restart=restart, max_it=max_it, n_pre=smooth[0], n_post=smooth[1])
@@ -361,6 +362,8 @@ def main():
     ap.add_argument("--no-diag-faces", action="store_true",
                     help="V-cycle line passes and residual restriction read 1/a_p instead of rebuilding a_p "
                          "from the faces")
+    ap.add_argument("--no-dot32", action="store_true",
+                    help="mixed FGMRES: Gram-Schmidt projections over the FP64 basis instead of its FP32 copies")
     ap.add_argument("--projection-steps", type=int, default=0,
                     help="instead of the solve benchmark: this many time steps of the 384x256x256 sphere "
                          "projection run (one JSON line of per-step results)")
@@ -394,7 +397,8 @@ def main():
     main_run = run_solver(lsx, hip, geom, args.precision, args.krylov, args.restart, args.steps, args.warmup,
                           smooth=smooth, graphs=args.graphs, smoother=args.smoother, line_levels=args.line_levels,
                           levels=args.levels, profile_steps=max(1, min(3, args.steps)),
-                          fused_sweep=args.fused_sweep, diag_faces=not args.no_diag_faces)
+                          fused_sweep=args.fused_sweep, diag_faces=not args.no_diag_faces,
+                          dot32=not args.no_dot32)
     print(f"[bench] {args.precision}: {args.steps} solves in {main_run['elapsed']:.3f} s, its {main_run['its']}",
           file=sys.stderr, flush=True)
     fp64 = None
@@ -456,6 +460,7 @@ def main():
         "precision": args.precision, "krylov": args.krylov, "smoother": args.smoother, "graphs": args.graphs,
         "line_levels": args.line_levels if args.smoother == "line" else None,
         "diag_from_faces": not args.no_diag_faces,
+        "projections_over_fp32_basis_copies": args.precision in ("mixed", "mixed16") and not args.no_dot32,
         "cells": geom.n,
         "iterations": main_run["its"], "true_rel_residual": main_run["rel_res"],
         # ||D u + flux|| / ||b|| of the velocity the solve's pressure would correct: with D inv_M G = -A

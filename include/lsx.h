@@ -74,6 +74,11 @@ typedef struct lsx_options {
                         pressure operator; pinned cells and their coarse sinks excepted), the line
                         passes and the residual restriction rebuild a_p from the faces instead of
                         reading 1/a_p (default); 0 = always read 1/a_p                            */
+  int dot32;         /* mixed-precision FGMRES with classical Gram-Schmidt: 1 = the projections
+                        h = V^T w are summed in FP64 over FP32 copies of the basis vectors (the
+                        copies the V-cycle takes as its right-hand side), the update w - V h and
+                        everything else stays on the FP64 basis (default); 0 = projections over
+                        the FP64 basis                                                          */
 } lsx_options;
 
 #define LSX_SMOOTH_RBGS 0 /* red-black Gauss-Seidel                                             */

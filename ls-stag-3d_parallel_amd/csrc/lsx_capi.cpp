@@ -74,6 +74,7 @@ void lsx_options_default(lsx_options* o) {
   o->fused_sweep = 1;
   o->line_levels = 0;
   o->diag_faces = 1;
+  o->dot32 = 1;
 }
 
 int lsx_create(lsx_solver** out, const lsx_options* opt, void* stream) {

@@ -961,8 +961,8 @@ constexpr int CGS_ROWS = 4;
 
 // acc[k] += sum over the thread's rows i of V_k[i] w[i], rows in increasing order, CGS_ROWS
 // rows (i, i + BS, ...) per step so that CGS_ROWS (KC + 1) loads are in flight
-template <int BS, int KC>
-__device__ inline void cgs_dot_rows(const double* __restrict__ Vg, long ldv, const double* __restrict__ w, long lo,
+template <int BS, int KC, class VT>
+__device__ inline void cgs_dot_rows(const VT* __restrict__ Vg, long ldv, const double* __restrict__ w, long lo,
                                     long hi, double (&acc)[CGS_GROUP]) {
   constexpr int R = CGS_ROWS;
   long i = lo + threadIdx.x;
@@ -972,7 +972,7 @@ __device__ inline void cgs_dot_rows(const double* __restrict__ Vg, long ldv, con
     for (int r = 0; r < R; ++r) {
       wr[r] = w[i + r * BS];
 #pragma unroll
-      for (int k = 0; k < KC; ++k) v[r][k] = Vg[k * ldv + i + r * BS];
+      for (int k = 0; k < KC; ++k) v[r][k] = (double)Vg[k * ldv + i + r * BS];
     }
 #pragma unroll
     for (int k = 0; k < KC; ++k)
@@ -982,32 +982,39 @@ __device__ inline void cgs_dot_rows(const double* __restrict__ Vg, long ldv, con
   for (; i < hi; i += BS) {
     const double w0 = w[i];
 #pragma unroll
-    for (int k = 0; k < KC; ++k) acc[k] += Vg[k * ldv + i] * w0;
+    for (int k = 0; k < KC; ++k) acc[k] += (double)Vg[k * ldv + i] * w0;
   }
 }
 
-template <int BS>
-__global__ void __launch_bounds__(BS) k_cgs_dot(const double* __restrict__ V, long ldv, int cnt,
-                                                const double* __restrict__ w, long n, double* __restrict__ partial) {
+// Block (chunk c, group g) is block c % 8 + 8 (g + ng (c / 8)) of a 1-D grid of 8 ceil(nb / 8) ng
+// blocks (those of chunks past nb exit at once): workgroups go to the 8 XCDs round-robin, so the
+// ng blocks of one chunk run on one XCD one after the other and read its slice of w from that
+// XCD's L2 after the first.
+template <int BS, class VT>
+__global__ void __launch_bounds__(BS) k_cgs_dot(const VT* __restrict__ V, long ldv, int cnt,
+                                                const double* __restrict__ w, long n, int nb, int ng,
+                                                double* __restrict__ partial) {
   __shared__ double lds[BS / 64][CGS_GROUP];
-  const int nb = gridDim.x;
-  const int k0 = blockIdx.y * CGS_GROUP;
+  const int b = blockIdx.x, r = b / 8;
+  const int chunk = b % 8 + 8 * (r / ng), grp = r % ng;
+  if (chunk >= nb) return;  // uniform over the block
+  const int k0 = grp * CGS_GROUP;
   const int kc = cnt - k0 < CGS_GROUP ? cnt - k0 : CGS_GROUP;
   long lo, hi;
-  chunk_of(n, blockIdx.x, nb, &lo, &hi);
-  const double* Vg = V + (long)k0 * ldv;
+  chunk_of(n, chunk, nb, &lo, &hi);
+  const VT* Vg = V + (long)k0 * ldv;
   double acc[CGS_GROUP];
 #pragma unroll
   for (int k = 0; k < CGS_GROUP; ++k) acc[k] = 0.0;
   switch (kc) {  // uniform over the block: every load of a step is issued unconditionally
-    case 1: cgs_dot_rows<BS, 1>(Vg, ldv, w, lo, hi, acc); break;
-    case 2: cgs_dot_rows<BS, 2>(Vg, ldv, w, lo, hi, acc); break;
-    case 3: cgs_dot_rows<BS, 3>(Vg, ldv, w, lo, hi, acc); break;
-    case 4: cgs_dot_rows<BS, 4>(Vg, ldv, w, lo, hi, acc); break;
-    case 5: cgs_dot_rows<BS, 5>(Vg, ldv, w, lo, hi, acc); break;
-    case 6: cgs_dot_rows<BS, 6>(Vg, ldv, w, lo, hi, acc); break;
-    case 7: cgs_dot_rows<BS, 7>(Vg, ldv, w, lo, hi, acc); break;
-    default: cgs_dot_rows<BS, CGS_GROUP>(Vg, ldv, w, lo, hi, acc); break;
+    case 1: cgs_dot_rows<BS, 1, VT>(Vg, ldv, w, lo, hi, acc); break;
+    case 2: cgs_dot_rows<BS, 2, VT>(Vg, ldv, w, lo, hi, acc); break;
+    case 3: cgs_dot_rows<BS, 3, VT>(Vg, ldv, w, lo, hi, acc); break;
+    case 4: cgs_dot_rows<BS, 4, VT>(Vg, ldv, w, lo, hi, acc); break;
+    case 5: cgs_dot_rows<BS, 5, VT>(Vg, ldv, w, lo, hi, acc); break;
+    case 6: cgs_dot_rows<BS, 6, VT>(Vg, ldv, w, lo, hi, acc); break;
+    case 7: cgs_dot_rows<BS, 7, VT>(Vg, ldv, w, lo, hi, acc); break;
+    default: cgs_dot_rows<BS, CGS_GROUP, VT>(Vg, ldv, w, lo, hi, acc); break;
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -1020,7 +1027,7 @@ __global__ void __launch_bounds__(BS) k_cgs_dot(const double* __restrict__ V, lo
   if ((int)threadIdx.x < kc) {
     double s = 0.0;
     for (int q = 0; q < BS / 64; ++q) s += lds[q][threadIdx.x];
-    partial[(long)(k0 + threadIdx.x) * nb + blockIdx.x] = s;
+    partial[(long)(k0 + threadIdx.x) * nb + chunk] = s;
   }
 }
 
@@ -1388,9 +1395,15 @@ int cgs_blocks(long n) {
 
 void cgs_dot(const double* V, long ldv, int cnt, const double* w, long n, double* partial, double* out,
              hipStream_t s) {
-  const int nb = cgs_blocks(n);
-  const dim3 grid(nb, (cnt + CGS_GROUP - 1) / CGS_GROUP);
-  k_cgs_dot<RED_THREADS><<<grid, RED_THREADS, 0, s>>>(V, ldv, cnt, w, n, partial);
+  const int nb = cgs_blocks(n), ng = (cnt + CGS_GROUP - 1) / CGS_GROUP;
+  k_cgs_dot<RED_THREADS, double><<<(nb + 7) / 8 * 8 * ng, RED_THREADS, 0, s>>>(V, ldv, cnt, w, n, nb, ng, partial);
+  k_finalize_multi<RED_THREADS><<<cnt, RED_THREADS, 0, s>>>(partial, nb, out, 0);
+}
+
+void cgs_dot_f32(const float* V, long ldv, int cnt, const double* w, long n, double* partial, double* out,
+                 hipStream_t s) {
+  const int nb = cgs_blocks(n), ng = (cnt + CGS_GROUP - 1) / CGS_GROUP;
+  k_cgs_dot<RED_THREADS, float><<<(nb + 7) / 8 * 8 * ng, RED_THREADS, 0, s>>>(V, ldv, cnt, w, n, nb, ng, partial);
   k_finalize_multi<RED_THREADS><<<cnt, RED_THREADS, 0, s>>>(partial, nb, out, 0);
 }
 

@@ -92,6 +92,9 @@ constexpr int CGS_BLOCKS_MAX = 4096;
 int cgs_blocks(long n);
 void cgs_dot(const double* V, long ldv, int cnt, const double* w, long n, double* partial, double* out,
              hipStream_t s);
+// cgs_dot over FP32 vectors (each widened to FP64 before its product, sums in FP64)
+void cgs_dot_f32(const float* V, long ldv, int cnt, const double* w, long n, double* partial, double* out,
+                 hipStream_t s);
 void cgs_update(const double* V, long ldv, int cnt, const double* h, double* w, long n, double* partial,
                 double* norm_out, hipStream_t s);
 // the update for a basis stored unnormalised (vector k = s_k times the orthonormal one):

@@ -618,19 +618,19 @@ template <>
 double* Solver::vr<double>(VStore& S) { return S.dr.p; }
 
 template <class T>
-void Solver::vcycle_apply(const double* v, T* z, bool pre) {
+void Solver::vcycle_apply(const double* v, T* z, const float* b32) {
   if (trans) {
     vcycle_apply_tr<T>(v, z);
     return;
   }
   if (!opt.graphs || capturing) {
-    vcycle_run<T>(v, z, pre);
+    vcycle_run<T>(v, z, b32);
     return;
   }
   // one captured graph per (v, z) pair: the Krylov loops apply the cycle to a fixed set of
   // basis slots, so a graph replays every launch of a cycle with one submission
   for (const CycleGraph& g : cgraphs)
-    if (g.v == v && g.z == (const void*)z && g.pre == pre) {
+    if (g.v == v && g.z == (const void*)z && g.b32 == b32) {
       LSX_HIP(hipGraphLaunch(g.exec, stream));
       return;
     }
@@ -638,7 +638,7 @@ void Solver::vcycle_apply(const double* v, T* z, bool pre) {
   LSX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   capturing = true;
   try {
-    vcycle_run<T>(v, z, pre);
+    vcycle_run<T>(v, z, b32);
   } catch (...) {
     capturing = false;
     (void)hipStreamEndCapture(stream, &graph);
@@ -651,18 +651,23 @@ void Solver::vcycle_apply(const double* v, T* z, bool pre) {
   const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   LSX_HIP(ie);
-  cgraphs.push_back({v, (const void*)z, pre, exec});
+  cgraphs.push_back({v, (const void*)z, b32, exec});
   LSX_HIP(hipGraphLaunch(exec, stream));
 }
 
 template <class T>
-void Solver::vcycle_run(const double* v, T* z, bool pre) {
+void Solver::vcycle_run(const double* v, T* z, const float* b32) {
   const int L = (int)vlev.size() - 1;
-  // level 0 right-hand side: the input itself in FP64, an FP32 copy in mixed precision
+  // level 0 right-hand side: the input itself in FP64, an FP32 copy in mixed precision (b32 when
+  // the caller already holds one)
   const T* b0;
   if constexpr (std::is_same<T, float>::value) {
-    if (!pre) vc::convert<double, float>(v, vlev[0].fb.p, d0.n(), stream);
-    b0 = vlev[0].fb.p;
+    if (b32) {
+      b0 = b32;
+    } else {
+      vc::convert<double, float>(v, vlev[0].fb.p, d0.n(), stream);
+      b0 = vlev[0].fb.p;
+    }
   } else {
     b0 = v;
   }
@@ -871,6 +876,10 @@ void Solver::alloc_krylov() {
       Z32.alloc((size_t)opt.restart * n);
     else
       Z.alloc((size_t)opt.restart * n);
+    if (mixed() && opt.dot32)
+      V32.alloc((size_t)opt.restart * n);
+    else
+      V32.release();
     w.alloc(n);
     H.assign(opt.restart * (opt.restart + 1) / 2, 0.0);
     cs.assign(opt.restart, 0.0);
@@ -1004,6 +1013,10 @@ int Solver::fgmres(const double* b, double* x) {
   // update pass and the host remove (from w - V h, the Hessenberg column and the solution
   // coefficients).
   const bool unnorm = vcyc() && cgs_passes > 0;
+  // FP32 copy of basis vector k: its own slot of V32 when the projections run over the copies,
+  // else the cycle's right-hand side buffer (only the newest copy is needed)
+  const bool d32 = m32 && cgs_passes > 0 && opt.dot32 && V32.p;
+  auto slot32 = [&](int k) -> float* { return d32 ? V32.p + (size_t)k * n : vlev[0].fb.p; };
   std::vector<double> sc(m + 1, 1.0);
   CgsScales isq{};
   auto sol_upd = [&](int last, int lastH) {
@@ -1043,8 +1056,8 @@ int Solver::fgmres(const double* b, double* x) {
       if (scaled) sc[j] = 1.0;
       if (m32) {
         // v_j = w / ||w|| and its FP32 copy (the cycle's finest right-hand side) in one pass
-        if (scaled) scale_copy_f32(w.p, dnorm, vj, vlev[0].fb.p, n, stream);
-        vcycle_apply<float>(vj, zj32, true);
+        if (scaled) scale_copy_f32(w.p, dnorm, vj, slot32(j), n, stream);
+        vcycle_apply<float>(vj, zj32, slot32(j));
       } else {
         if (scaled) scale_copy(w.p, dnorm, vj, n, stream);
         if (vcyc())
@@ -1074,8 +1087,11 @@ int Solver::fgmres(const double* b, double* x) {
           const bool last = pass + 1 == cgs_passes;
           double* hp = cgs_h.p + pass * CGS_MAX;
           tbeg(LSX_K_CGS_DOT, &e0);
-          cgs_dot(V.p, n, cnt, w.p, n, mpartial.p, hp, stream);
-          tend(LSX_K_CGS_DOT, e0, vb + 8.0 * n);
+          if (d32)
+            cgs_dot_f32(V32.p, n, cnt, w.p, n, mpartial.p, hp, stream);
+          else
+            cgs_dot(V.p, n, cnt, w.p, n, mpartial.p, hp, stream);
+          tend(LSX_K_CGS_DOT, e0, (d32 ? 0.5 * vb : vb) + 8.0 * n);
           tbeg(LSX_K_CGS_UPDATE, &e0);
           if (unnorm) {
             // the last pass writes u_{j+1} into the next slot (and the cycle's FP32 right-hand
@@ -1083,7 +1099,7 @@ int Solver::fgmres(const double* b, double* x) {
             for (int k = 0; k < cnt; ++k) isq.isq[k] = 1.0 / (sc[k] * sc[k]);
             const bool to_next = last && j + 1 < m;
             double* out = to_next ? V.p + (size_t)(j + 1) * n : w.p;
-            float* out32 = to_next && m32 ? vlev[0].fb.p : nullptr;
+            float* out32 = to_next && m32 ? slot32(j + 1) : nullptr;
             cgs_update_scaled(V.p, n, cnt, hp, isq, last ? 1.0 / sc[j] : 1.0, w.p, out, out32, n, mpartial.p,
                               last ? dnorm : nullptr, stream);
             tend(LSX_K_CGS_UPDATE, e0, vb + (out32 ? 20.0 : 16.0) * n);

@@ -207,21 +207,22 @@ class Solver {
   bool rb_level(int l) const { return opt.smoother != LSX_SMOOTH_JACOBI && !line_level(l); }
   std::vector<VStore> vlev;
   DevBuf<float> Z32, ztmp32;
+  DevBuf<float> V32;  // FP32 copies of the FGMRES basis (mixed precision with lsx_options.dot32)
   bool mixed() const { return opt.precision == LSX_PREC_MIXED || opt.precision == LSX_PREC_MIXED16; }
   bool vcyc() const { return opt.mg == LSX_MG_VCYCLE; }
   void build_vcycle();
   template <class T>
   // pre: (mixed) the FP32 copy of v is already in level 0's right-hand side buffer
-  void vcycle_apply(const double* v, T* z, bool pre = false);
+  void vcycle_apply(const double* v, T* z, const float* b32 = nullptr);
   template <class T>
   void vcycle_apply_tr(const double* v, T* z);
   template <class T>
-  void vcycle_run(const double* v, T* z, bool pre = false);
+  void vcycle_run(const double* v, T* z, const float* b32 = nullptr);
   // captured V-cycle applications, one per (input, output) pair (opt.graphs)
   struct CycleGraph {
     const double* v;
     const void* z;
-    bool pre;
+    const float* b32;
     hipGraphExec_t exec;
   };
   std::vector<CycleGraph> cgraphs;

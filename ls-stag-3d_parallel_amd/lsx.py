@@ -70,6 +70,7 @@ class Options(C.Structure):
         ("precision", C.c_int), ("levels", C.c_int), ("z_head_fix", C.c_int), ("ortho", C.c_int),
         ("coarse_fuse", C.c_int), ("graphs", C.c_int), ("smoother", C.c_int), ("fused_sweep", C.c_int),
         ("line_levels", C.c_int), ("diag_faces", C.c_int),
+        ("dot32", C.c_int),
     ]
 
 
@@ -457,9 +458,11 @@ class IterativeSolver:
     krylov = KRYLOV_FGMRES
 
     def __init__(self, precision=PREC_FP64, mg=MG_CASCADE, z_head_fix=False, stream=None, ortho=0, coarse_fuse=True,
-                 graphs=False, smoother="rbgs", fused_sweep=True, line_levels=0, levels=0, diag_faces=True):
+                 graphs=False, smoother="rbgs", fused_sweep=True, line_levels=0, levels=0, diag_faces=True,
+                 dot32=True):
         self.levels_cap = levels
         self.diag_faces = diag_faces
+        self.dot32 = dot32
         self.ortho = ortho
         self.coarse_fuse = coarse_fuse
         self.graphs = graphs
@@ -546,6 +549,7 @@ class IterativeSolver:
         opt.fused_sweep = int(self.fused_sweep)
         opt.line_levels = int(self.line_levels)
         opt.diag_faces = 1 if getattr(self, "diag_faces", True) else 0
+        opt.dot32 = 1 if getattr(self, "dot32", True) else 0
         h = C.c_void_p()
         _check(lib().lsx_create(C.byref(h), C.byref(opt), self.stream))
         self.h = h
