@@ -108,6 +108,39 @@ def test_vcycle_orthogonalisations(lsx, ortho):
     assert abs(S.norm_r - true_res) <= 1e-3 * true_res
 
 
+@pytest.mark.parametrize("restart", [30, 6])
+@pytest.mark.parametrize("ortho", ["cgs", "cgs2"])
+def test_projections_over_fp32_basis_copies(lsx, ortho, restart):
+    """lsx_options.dot32: the Gram-Schmidt projections of the mixed FGMRES summed over the FP32
+    copies of the basis.  The update and the basis stay FP64, so the Arnoldi relation holds to
+    FP64 rounding and only the orthogonality of the basis degrades (to about the FP32 unit
+    roundoff): the solve needs no more iterations, the reported norm is the true residual, and the
+    solution differs from the FP64-projection one by no more than two converged solutions may
+    (restart 6 crosses several windows, each starting from FP32 copies of a fresh basis)."""
+    g, A = P.ellipsoid_stretched(64, ratio=1.08)
+    b = P.random_rhs(g, seed=11, A=A)
+    eps = 1e-8 * np.linalg.norm(b)
+    code = {"cgs": lsx.ORTHO_CGS, "cgs2": lsx.ORTHO_CGS2}[ortho]
+    out = {}
+    for d32 in (True, False):
+        S = lsx.solver("fgmres", "mixed", ortho=code, smoother="line", line_levels=2, dot32=d32)
+        x = np.zeros(A.n)
+        S.set(A, b, x, lsx.mixed_task(eps, restart=restart, max_it=400))
+        it = S.solve()
+        true_res = np.linalg.norm(b - A.matvec(x))
+        print(f"ortho={ortho} restart={restart} dot32={d32}: it={it} true_rel={true_res / np.linalg.norm(b):.3e} "
+              f"norm_r={S.norm_r:.3e}", flush=True)
+        assert true_res <= 1.05 * eps
+        assert abs(S.norm_r - true_res) <= 1e-3 * true_res
+        out[d32] = (x, it)
+    (x1, it1), (x0, it0) = out[True], out[False]
+    active = A.to_scipy().diagonal() < 0
+    diff = np.linalg.norm((x1 - x0)[active]) / np.linalg.norm(x0[active])
+    print(f"|x_dot32 - x| / |x| = {diff:.3e}", flush=True)
+    assert it1 <= it0 + 1
+    assert not np.array_equal(x1, x0) and diff <= 1e-4
+
+
 @pytest.mark.parametrize("precision", ["mixed", "fp64-vcycle"])
 def test_coarse_levels_in_one_launch_match_per_level_launches(lsx, precision):
     """The one-workgroup cycle over the small levels performs the same per-cell operations in the
