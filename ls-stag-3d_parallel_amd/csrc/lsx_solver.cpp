@@ -876,7 +876,10 @@ void Solver::alloc_krylov() {
       Z32.alloc((size_t)opt.restart * n);
     else
       Z.alloc((size_t)opt.restart * n);
-    if (mixed() && opt.dot32)
+    // FP32 basis copies for the projections: mixed precision with classical Gram-Schmidt (the
+    // orthogonalisation fgmres() picks)
+    const bool cgs = (opt.ortho == LSX_ORTHO_AUTO ? vcyc() : opt.ortho != LSX_ORTHO_MGS) && opt.restart <= CGS_MAX;
+    if (mixed() && opt.dot32 && cgs)
       V32.alloc((size_t)opt.restart * n);
     else
       V32.release();
