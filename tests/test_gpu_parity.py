@@ -48,6 +48,7 @@ CASES = [
     ("free17", lambda: P.body_free(17)),
     ("sphere33", lambda: P.sphere(33)),
     ("cut17x33x9", lambda: P.synthetic_cut(17, 33, 9, seed=6)),
+    ("sphere65", lambda: P.sphere(65)),
 ]
 
 
