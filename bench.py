@@ -453,14 +453,19 @@ def main():
                         "+ U(-0.01, 0.01) (seed 0)" if args.body == "bezier" else "ellipsoid body 0.25/0.15/0.15")
                      + f", {args.krylov.upper()}"
                      f"({args.restart}), rel tol 1e-8, "
-                     + {"mixed": f"mixed: FP32 V{smooth} {sm_name} multigrid, FP16 coarse faces, FP64 Krylov",
-                        "mixed16": f"mixed: FP32 V{smooth} {sm_name} multigrid, FP16 faces, FP64 Krylov",
+                     + {"mixed": f"mixed: FP32 V{smooth} {sm_name} multigrid, FP16 coarse faces, FP64 Krylov"
+                                  + ("" if args.no_dot32 or args.krylov != "fgmres"
+                                     else " (Gram-Schmidt projections over FP32 basis copies)"),
+                        "mixed16": f"mixed: FP32 V{smooth} {sm_name} multigrid, FP16 faces, FP64 Krylov"
+                                   + ("" if args.no_dot32 or args.krylov != "fgmres"
+                                      else " (Gram-Schmidt projections over FP32 basis copies)"),
                         "fp64-vcycle": f"FP64 V{smooth} {sm_name} multigrid",
                         "fp64": "FP64 reference multigrid cascade (ADLJ)"}[args.precision]),
         "precision": args.precision, "krylov": args.krylov, "smoother": args.smoother, "graphs": args.graphs,
         "line_levels": args.line_levels if args.smoother == "line" else None,
         "diag_from_faces": not args.no_diag_faces,
-        "projections_over_fp32_basis_copies": args.precision in ("mixed", "mixed16") and not args.no_dot32,
+        "projections_over_fp32_basis_copies": args.precision in ("mixed", "mixed16") and args.krylov == "fgmres"
+                                              and not args.no_dot32,
         "cells": geom.n,
         "iterations": main_run["its"], "true_rel_residual": main_run["rel_res"],
         # ||D u + flux|| / ||b|| of the velocity the solve's pressure would correct: with D inv_M G = -A
