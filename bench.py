@@ -350,7 +350,9 @@ def main():
                     help="iteration cap of the reference-cascade comparison solve (0 = skip it)")
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--graphs", action="store_true", help="replay the V-cycles from captured HIP graphs")
+    ap.add_argument("--no-graphs", dest="graphs", action="store_false",
+                    help="launch every V-cycle kernel from the host instead of replaying the cycles from captured "
+                         "HIP graphs (one graph per Krylov basis slot, captured on first use)")
     ap.add_argument("--smoother", choices=["rbgs", "line", "jacobi"], default="line",
                     help="V-cycle smoother: red-black Gauss-Seidel, x/y/z line relaxation or damped Jacobi")
     ap.add_argument("--line-levels", type=int, default=2,

@@ -623,7 +623,8 @@ void Solver::vcycle_apply(const double* v, T* z, const float* b32) {
     vcycle_apply_tr<T>(v, z);
     return;
   }
-  if (!opt.graphs || capturing) {
+  // per-kernel timing records events around the launches: such cycles run eagerly
+  if (!opt.graphs || capturing || timing) {
     vcycle_run<T>(v, z, b32);
     return;
   }
