@@ -1464,6 +1464,11 @@ template <class E>
 __device__ inline E ldo(const E* __restrict__ p, unsigned i) {
   return *reinterpret_cast<const E*>(reinterpret_cast<const char*>(p) + i * (unsigned)sizeof(E));
 }
+// the same load marked non-temporal (read once by this workgroup only)
+template <class E>
+__device__ inline E ldo_nt(const E* __restrict__ p, unsigned i) {
+  return __builtin_nontemporal_load(reinterpret_cast<const E*>(reinterpret_cast<const char*>(p) + i * (unsigned)sizeof(E)));
+}
 
 // Row of cell i of a y (DIR 1) or z (DIR 2) line of the forward operator, with the line's
 // per-thread constants: the x neighbours (A) and the other cross direction (B) exist or not for
@@ -1499,11 +1504,11 @@ struct SegLine {
     const bool hl = kk > 0, hu = kk < len - 1;
     T sv = (T)0;
     if constexpr (!DFF) sv = ldo(s, i);
-    const T bv = ldo(b, i);
-    const T fau = (T)ldo(ae, i);
+    const T bv = ldo_nt(b, i);
+    const T fau = (T)ldo_nt(ae, i);
     const bool marked = DFF && __builtin_signbit(fau);
     const T fbl = (T)ldo(fB, i + oBl), fbu = (T)ldo(fB, i);
-    const T flu = (T)ldo(fL, i);
+    const T flu = (T)ldo_nt(fL, i);
     const T fll = first ? (T)ldo(fL, hl ? i - strideL : i) : fl_carry;  // upper face of the row before
     fl_carry = flu;
     T off = (T)0, xi = (T)0, fal = (T)0;
@@ -1761,8 +1766,8 @@ __global__ void __launch_bounds__(1024) k_line_seg(const T* __restrict__ x_in, T
     for (int k = 0; k < SEG; ++k)
       if (k < cnt) {
         const unsigned i = (unsigned)(base + stride * (k0 + k));
-        *reinterpret_cast<T*>(reinterpret_cast<char*>(x_out) + i * (unsigned)sizeof(T)) =
-            k == cnt - 1 ? xe : G[k] + H[k] * xl + Q[k] * xe;
+        __builtin_nontemporal_store(k == cnt - 1 ? xe : G[k] + H[k] * xl + Q[k] * xe,
+                                    reinterpret_cast<T*>(reinterpret_cast<char*>(x_out) + i * (unsigned)sizeof(T)));
       }
   }
 }
