@@ -108,7 +108,8 @@ int lsx_destroy(lsx_solver* s);
  * num+N*M}, each at most once.  mem says whether the three arrays are host or device pointers.
  * Every level of the reference cascade that is coarsened must have all three dimensions even
  * (even-even-even transfers, coarse size n / 2) or all three odd (odd-odd-odd transfers, coarse
- * size (n + 1) / 2); the V-cycle preconditioners need even dimensions on every coarsened level. */
+ * size (n + 1) / 2).  The V-cycle preconditioners coarsen any dimensions (n / 2 or (n + 1) / 2 cells,
+ * the last coarse cell of an odd axis with one child). */
 int lsx_set_csr(lsx_solver* s, int N, int M, int K, const int* rowptr, const int* col, const double* val, long nnz,
                 int mem);
 

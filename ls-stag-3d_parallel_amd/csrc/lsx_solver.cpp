@@ -179,11 +179,10 @@ void Solver::check_dims(int N, int M, int K) const {
   int n = N, m = M, k = K;
   for (int l = 0; l < nmg; ++l) {
     const int odd = (n & 1) + (m & 1) + (k & 1);
-    if (vcyc() && odd)
-      throw Error(LSX_ERR_ARG, "every V-cycle level must have even dimensions");
     // the reference's mixed-parity transfer classes (R = P^T with the cell-centred prolongation on
-    // their even axes) widen the coarse operator beyond the 27-point stencils stored here
-    if (odd == 1 || odd == 2)
+    // their even axes) widen the coarse operator beyond the 27-point stencils stored here; the
+    // V-cycle coarsens any level (an odd axis ends on a coarse cell with one child)
+    if (!vcyc() && (odd == 1 || odd == 2))
       throw Error(LSX_ERR_ARG,
                   "a multigrid level mixes even and odd dimensions: only even-even-even and odd-odd-odd transfers "
                   "are supported");
@@ -434,10 +433,13 @@ void Solver::build_vcycle() {
     fn[0].alloc(n);
     ft[0].alloc(n);
     dg[0].alloc(n);
-    fe[1].alloc(std::max<long>(1, n / 8));
-    fn[1].alloc(std::max<long>(1, n / 8));
-    ft[1].alloc(std::max<long>(1, n / 8));
-    dg[1].alloc(std::max<long>(1, n / 8));
+    // levels 1, 3, ... alternate into the second set (level 1 is the largest of them; a level
+    // of odd size holds more than an eighth of the one above it)
+    const long n1 = L >= 1 ? levels[1].d.n() : 1;
+    fe[1].alloc(n1);
+    fn[1].alloc(n1);
+    ft[1].alloc(n1);
+    dg[1].alloc(n1);
   }
   const bool m32 = mixed();
   double* sigma = scalars.p + SC_SIGMA;

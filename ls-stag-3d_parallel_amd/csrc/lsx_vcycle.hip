@@ -87,26 +87,39 @@ __global__ void k_faces_from7(const double* __restrict__ c7, int N, int M, int K
   }
 }
 
+// Coarse cell (X, Y, Z) covers fine cells 2X, 2X + 1 (and so on per axis); on an axis of odd
+// length the last coarse cell has the single child 2X, and the coarse faces sum the fine faces
+// that exist (the same terms in the same order as on even levels).
 __global__ void k_aggregate(const double* __restrict__ fe, const double* __restrict__ fn, const double* __restrict__ ft,
-                            int Nf, int Mf, double* __restrict__ ce, double* __restrict__ cn, double* __restrict__ ct,
-                            int Nc, int Mc, int Kc) {
+                            int Nf, int Mf, int Kf, double* __restrict__ ce, double* __restrict__ cn,
+                            double* __restrict__ ct, int Nc, int Mc, int Kc) {
   const long nc = (long)Nc * Mc * Kc, NMf = (long)Nf * Mf;
   for (long I = blockIdx.x * (long)blockDim.x + threadIdx.x; I < nc; I += (long)gridDim.x * blockDim.x) {
     int X, Y, Z;
     decode(I, Nc, Mc, &X, &Y, &Z);
     const long a = 2L * X + (long)Nf * 2 * Y + NMf * 2 * Z;
+    const bool hx = 2 * X + 1 < Nf, hy = 2 * Y + 1 < Mf, hz = 2 * Z + 1 < Kf;  // second child present
     double se = 0, sn = 0, st = 0;
     if (X < Nc - 1) {
       const long f = a + 1;
-      se = fe[f] + fe[f + Nf] + fe[f + NMf] + fe[f + Nf + NMf];
+      se = fe[f];
+      if (hy) se = se + fe[f + Nf];
+      if (hz) se = se + fe[f + NMf];
+      if (hy && hz) se = se + fe[f + Nf + NMf];
     }
     if (Y < Mc - 1) {
       const long f = a + Nf;
-      sn = fn[f] + fn[f + 1] + fn[f + NMf] + fn[f + 1 + NMf];
+      sn = fn[f];
+      if (hx) sn = sn + fn[f + 1];
+      if (hz) sn = sn + fn[f + NMf];
+      if (hx && hz) sn = sn + fn[f + 1 + NMf];
     }
     if (Z < Kc - 1) {
       const long f = a + NMf;
-      st = ft[f] + ft[f + 1] + ft[f + Nf] + ft[f + 1 + Nf];
+      st = ft[f];
+      if (hx) st = st + ft[f + 1];
+      if (hy) st = st + ft[f + Nf];
+      if (hx && hy) st = st + ft[f + 1 + Nf];
     }
     ce[I] = 0.5 * se;
     cn[I] = 0.5 * sn;
@@ -362,37 +375,12 @@ __global__ void k_rbgs(T* __restrict__ x, const T* __restrict__ b, const FT* __r
                    blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x);
 }
 
-template <class T, class FT>
-__device__ inline void resid_restrict_body(const T* __restrict__ x, const T* __restrict__ b, const FT* __restrict__ ae,
-                                           const FT* __restrict__ an, const FT* __restrict__ at,
-                                           const T* __restrict__ s, T fs, int N, int M, int K, int pin0, int pin1,
-                                           T* __restrict__ bc, long start, long stride) {
-  const int Nc = N / 2, Mc = M / 2, Kc = K / 2;
-  const long nc = (long)Nc * Mc * Kc, NM = (long)N * M;
-  for (long I = start; I < nc; I += stride) {
-    int X, Y, Z;
-    decode(I, Nc, Mc, &X, &Y, &Z);
-    T acc = 0;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int xx = 2 * X + (c & 1), yy = 2 * Y + ((c >> 1) & 1), zz = 2 * Z + (c >> 2);
-      const long i = xx + (long)N * yy + NM * zz;
-      const T si = s[i];
-      const T ap = si != (T)0 ? (T)1 / si : (T)0;
-      T r = b[i] - ap * x[i];
-      const T od = offdiag_u<T, FT>(x, ae, an, at, i, xx, yy, zz, N, M, K);
-      if (i != pin0 && i != pin1) r -= fs * od;
-      acc += r;
-    }
-    bc[I] = acc;
-  }
-}
-
-// The same restriction over the whole device: one thread per fine cell (64 adjacent x by 2 y by
-// 2 z per workgroup, so loads coalesce and each row's loads are in flight together), residuals
+// bc = sum over the children of (b - A x): one thread per fine cell (64 adjacent x by 2 y by 2 z
+// per workgroup, so loads coalesce and each row's loads are in flight together), residuals
 // through LDS, and the 8 children of each of the 32 coarse cells summed by one thread in the
-// order of resid_restrict_body (bit-identical results).
-// DFF: a_p rebuilt from the faces (Level::dff), s not read; no bitwise twin then.
+// order x, then y, then z fastest-first (the order of the one-workgroup coarse cycle, so the two
+// are bit-identical).  On an axis of odd length the last coarse cell has one child: the absent
+// one adds 0.  DFF: a_p rebuilt from the faces (Level::dff), s not read; no bitwise twin then.
 template <class T, class FT, bool DFF>
 __global__ void __launch_bounds__(256) k_resid_restrict_cells(const T* __restrict__ x, const T* __restrict__ b,
                                                               const FT* __restrict__ ae, const FT* __restrict__ an,
@@ -401,29 +389,31 @@ __global__ void __launch_bounds__(256) k_resid_restrict_cells(const T* __restric
                                                               T* __restrict__ bc, DiagRule<T> dr) {
   __shared__ T rr[2][2][64];
   const int tx = threadIdx.x & 63, ty = (threadIdx.x >> 6) & 1, tz = threadIdx.x >> 7;
-  const int Nc = N / 2, Mc = M / 2;
+  const int Nc = (N + 1) / 2, Mc = (M + 1) / 2;
   // block (bx, by, bz) of a 1-D grid, bx fastest, in XCD-contiguous order (neighbouring rows and
   // planes read by the same XCD)
   const int gx = (N + 63) / 64;
   const int lb = xcd_order(blockIdx.x, gridDim.x);
   const int bx = lb % gx, by = (lb / gx) % Mc, bz = (lb / gx) / Mc;
   const int xx = bx * 64 + tx, yy = 2 * by + ty, zz = 2 * bz + tz;
-  const bool live = xx < N;
-  const int xc = live ? xx : N - 1;
-  const long i = xc + (long)N * yy + (long)N * M * zz;
+  // a thread past the level (x past N, or the absent second child on an odd axis) reads its
+  // clamped neighbour and posts 0
+  const bool live = xx < N && yy < M && zz < K;
+  const int xc = xx < N ? xx : N - 1, yc = yy < M ? yy : M - 1, zc = zz < K ? zz : K - 1;
+  const long i = xc + (long)N * yc + (long)N * M * zc;
   const bool pin = i == pin0 || i == pin1;
   T ap, od;
   if constexpr (DFF) {
     T fsum, own_e;
-    od = offdiag_fsum<T, FT>(x, ae, an, at, i, xc, yy, zz, N, M, K, fsum, own_e);
+    od = offdiag_fsum<T, FT>(x, ae, an, at, i, xc, yc, zc, N, M, K, fsum, own_e);
     ap = pin ? (T)1 : dr.of(fs * fsum, i, own_e);
   } else {
     const T si = s[i];
     ap = si != (T)0 ? (T)1 / si : (T)0;
-    od = offdiag_u<T, FT>(x, ae, an, at, i, xc, yy, zz, N, M, K);
+    od = offdiag_u<T, FT>(x, ae, an, at, i, xc, yc, zc, N, M, K);
   }
   const T r0 = b[i] - ap * x[i];
-  rr[tz][ty][tx] = pin ? r0 : r0 - fs * od;
+  rr[tz][ty][tx] = !live ? (T)0 : (pin ? r0 : r0 - fs * od);
   __syncthreads();
   if (threadIdx.x < 32) {
     const int X = bx * 32 + threadIdx.x;
@@ -520,7 +510,7 @@ __device__ inline float axis_wt(int f, int nf, int C) {
 
 template <class T>
 __global__ void k_restrict_pt(const T* __restrict__ r, int Nf, int Mf, int Kf, T* __restrict__ bc) {
-  const int Nc = Nf / 2, Mc = Mf / 2, Kc = Kf / 2;
+  const int Nc = (Nf + 1) / 2, Mc = (Mf + 1) / 2, Kc = (Kf + 1) / 2;
   const long nc = (long)Nc * Mc * Kc, NMf = (long)Nf * Mf;
   for (long I = blockIdx.x * (long)blockDim.x + threadIdx.x; I < nc; I += (long)gridDim.x * blockDim.x) {
     int X, Y, Z;
@@ -718,7 +708,7 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse_cycle(CoarseLevels<T>
         __syncthreads();
       }
       // residual restriction: every cell's residual into LDS (one thread per cell), then each coarse
-      // cell sums its 8 children in the order of resid_restrict_body (the same values; one cell's
+      // cell sums its children in the order of k_resid_restrict_cells (the same values; one cell's
       // loads per thread instead of eight keeps the kernel clear of register spills)
       {
         const T* x = C.x[q];
@@ -736,15 +726,17 @@ __global__ void __launch_bounds__(COARSE_THREADS) k_coarse_cycle(CoarseLevels<T>
           sres[i] = r;
         }
         __syncthreads();
-        const int Nc = N / 2, Mc = M / 2, nc = Nc * Mc * (K / 2);
+        const int Nc = (N + 1) / 2, Mc = (M + 1) / 2, nc = Nc * Mc * ((K + 1) / 2);
         T* bc = C.b[q + 1];
         for (int I = (int)tid; I < nc; I += (int)nth) {
           int X, Y, Z;
           decode(I, Nc, Mc, &X, &Y, &Z);
           T acc = 0;
 #pragma unroll
-          for (int c = 0; c < 8; ++c)
-            acc += sres[(2 * X + (c & 1)) + N * ((2 * Y + ((c >> 1) & 1)) + M * (2 * Z + (c >> 2)))];
+          for (int c = 0; c < 8; ++c) {
+            const int cx = 2 * X + (c & 1), cy = 2 * Y + ((c >> 1) & 1), cz = 2 * Z + (c >> 2);
+            acc += cx < N && cy < M && cz < K ? sres[cx + N * (cy + M * cz)] : (T)0;  // odd axes: one child
+          }
           bc[I] = acc;
         }
         __syncthreads();
@@ -1782,7 +1774,7 @@ void faces_from7(const double* c7, Dims d, int pin0, int pin1, double* fe, doubl
 
 void aggregate(const double* fe, const double* fn, const double* ft, Dims df, double* ce, double* cn, double* ct,
                Dims dc, hipStream_t s) {
-  k_aggregate<<<grid_1d(dc.n()), 256, 0, s>>>(fe, fn, ft, df.N, df.M, ce, cn, ct, dc.N, dc.M, dc.K);
+  k_aggregate<<<grid_1d(dc.n()), 256, 0, s>>>(fe, fn, ft, df.N, df.M, df.K, ce, cn, ct, dc.N, dc.M, dc.K);
 }
 
 void diag_from_faces(const double* fe, const double* fn, const double* ft, Dims d, double* dg, hipStream_t s) {
@@ -1897,7 +1889,7 @@ void rbgs<float>(const Level& L, float* x, const float* b, int color, int zero_g
 
 template <class T, class FT>
 static void resid_restrict_t(const Level& L, const T* x, const T* b, T* bc, hipStream_t s) {
-  const long blocks = (long)((L.d.N + 63) / 64) * (L.d.M / 2) * (L.d.K / 2);
+  const long blocks = (long)((L.d.N + 63) / 64) * ((L.d.M + 1) / 2) * ((L.d.K + 1) / 2);
 #define LSX_RESID(DFF)                                                                                          \
   k_resid_restrict_cells<T, FT, DFF><<<(unsigned)blocks, 256, 0, s>>>(                                          \
       x, b, (const FT*)L.f_e, (const FT*)L.f_n, (const FT*)L.f_t, (const T*)L.s, (T)L.fscale, L.d.N, L.d.M, L.d.K, \
@@ -1960,7 +1952,7 @@ void resid_tr<float>(const Level& L, const float* x, const float* b, float* r, h
 
 template <class T>
 void restrict_pt(const T* r, Dims df, T* bc, hipStream_t s) {
-  const long nc = (long)(df.N / 2) * (df.M / 2) * (df.K / 2);
+  const long nc = (long)((df.N + 1) / 2) * ((df.M + 1) / 2) * ((df.K + 1) / 2);
   k_restrict_pt<T><<<grid_1d(nc), 256, 0, s>>>(r, df.N, df.M, df.K, bc);
 }
 template <class T>
