@@ -1,5 +1,5 @@
-"""The drop-in boundary used the way INTEGRATION.md documents it for the reference's
-IterativeSolverInterface (iterative_solver_interface.h:46-55, mg_fgmres.h:35-39):
+"""The drop-in boundary used the way a reference-side wrapper of the C ABI uses it for the
+reference's IterativeSolverInterface (iterative_solver_interface.h:46-55, mg_fgmres.h:35-39):
 
 * lsx_set_csr with device pointers (the reference's CUDA build hands over device arrays) gives
   the same solve, bit for bit, as the host-pointer call;

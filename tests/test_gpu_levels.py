@@ -1,5 +1,6 @@
 """Capped V-cycle hierarchies (lsx_options.levels): the cycle against a plain numpy restatement
-of the multigrid it describes, and the BASELINE level counts at their sizes.
+of the multigrid it describes, and the benchmark workloads' level counts at their sizes (3 levels
+at 64^3, 5 at 128^3, 6 at 256^3).
 
 Restatement (FP64, on the oracle's CSR operator A): red-black Gauss-Seidel sweeps (red cells
 x+y+z even first; identity rows give x = b; empty rows keep their value), residual restricted by
