@@ -1,6 +1,6 @@
 """Capped V-cycle hierarchies (lsx_options.levels): the cycle against a plain numpy restatement
 of the multigrid it describes, and the benchmark workloads' level counts at their sizes (3 levels
-at 64^3, 5 at 128^3, 6 at 256^3).
+at 64^3, 5 at 128^3; the 6-level hierarchy at 256^3 is in test_gpu_full_size.py).
 
 Restatement (FP64, on the oracle's CSR operator A): red-black Gauss-Seidel sweeps (red cells
 x+y+z even first; identity rows give x = b; empty rows keep their value), residual restricted by
