@@ -80,6 +80,24 @@ def test_odd_levels_coarse_cycle_in_one_launch(lsx, name, mk, mk_even, precision
     assert np.linalg.norm(z[0]) > 0 and np.array_equal(z[0], z[1])
 
 
+@pytest.mark.parametrize("dims", [(5, 7, 9), (9, 5, 17), (3, 3, 3), (17, 16, 15)])
+@pytest.mark.parametrize("precision", ["fp64-vcycle", "mixed"])
+def test_small_odd_grids(lsx, dims, precision):
+    """Small boxes whose hierarchies end on levels of 1-3 cells per axis converge and transpose."""
+    g, A = P.body_free(*dims)
+    b = P.random_rhs(g, seed=1, A=A)
+    it, rel, S = solve(lsx, A, b, precision, "rbgs")
+    rng = np.random.default_rng(2)
+    act = A.to_scipy().diagonal() < 0
+    x, y = rng.standard_normal(A.n) * act, rng.standard_normal(A.n) * act
+    Bx, BTy = S.precondition(x), S.precondition_transpose(y)
+    trel = abs(Bx @ y - x @ BTy) / (np.linalg.norm(Bx) * np.linalg.norm(y))
+    print(f"dims={dims} precision={precision}: it={it} rel={rel:.3e} levels={S.num_levels()} transpose={trel:.3e}",
+          flush=True)
+    assert rel <= 1.05e-8 and it <= 20
+    assert trel <= (1e-12 if precision == "fp64-vcycle" else 2e-5)
+
+
 def test_cascade_still_rejects_mixed_parity(lsx):
     g, A = P.sphere(50, 31, 24)
     with pytest.raises(lsx.LsxError, match="mixes even and odd"):
