@@ -191,9 +191,11 @@ void Solver::check_dims(int N, int M, int K) const {
     k = coarse_count(k);
   }
   if (single_level()) return;  // no coarse levels: any dimensions, no coarsest solve
-  if (opt.levels == 0 && (long)n * m * k > 64) throw Error(LSX_ERR_ARG, "coarsest level exceeds 64 unknowns");
+  // the cascade's coarsest LU holds 64 unknowns; the V-cycle solves a larger coarsest level (a
+  // capped hierarchy, or the automatic one of a grid with one short axis) by its dense inverse
+  if (!vcyc() && (long)n * m * k > 64) throw Error(LSX_ERR_ARG, "coarsest level exceeds 64 unknowns");
   if ((long)n * m * k > vc::DENSE_MAX)
-    throw Error(LSX_ERR_ARG, "the level cap leaves more than 4096 unknowns on the coarsest level");
+    throw Error(LSX_ERR_ARG, "the coarsest level has more than 4096 unknowns (too few levels for the grid)");
 }
 
 void Solver::setup_levels(int N, int M, int K) {
