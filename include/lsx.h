@@ -59,8 +59,8 @@ typedef struct lsx_options {
   int precision;     /* LSX_PREC_*                                            */
   int levels;        /* 0 = the reference's automatic level count; >= 2 = at most this many
                         levels, the finest included (V-cycle only: the cascade rejects a cap);
-                        a capped coarsest level of 65..4096 unknowns is solved by its dense
-                        FP64 inverse                                            */
+                        a V-cycle coarsest level of 65..4096 unknowns (capped, or automatic on
+                        a grid with a short axis) is solved by its dense FP64 inverse */
   int z_head_fix;    /* 0 = reproduce the reference z-line head read (default) */
   int ortho;         /* LSX_ORTHO_*: FGMRES orthogonalisation                    */
   int coarse_fuse;   /* V-cycle: 1 = levels of <= 4096 cells in one launch (default), 0 = per level */
