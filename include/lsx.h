@@ -35,7 +35,9 @@ extern "C" {
 #define LSX_MG_CASCADE 0 /* reference cascade: restrict RHS, coarsest LU, prolong + ADLJ
                             post-smoothing (multigrid_solver.h:48-75) */
 #define LSX_MG_VCYCLE 1  /* V-cycle: red-black Gauss-Seidel, rediscretised face levels, pinned
-                            cells as Dirichlet sinks on coarse levels (no reference counterpart) */
+                            cells as Dirichlet sinks on coarse levels; an operator without pinned
+                            rows (a momentum operator) needs a diagonal beyond its couplings,
+                            summed over the children on coarse levels (no reference counterpart) */
 
 #define LSX_PREC_FP64 0  /* everything in FP64 (cascade or V-cycle) */
 #define LSX_PREC_MIXED 1 /* V-cycle only: FP32 multigrid, FP16 coarse face coefficients, FP64

@@ -445,15 +445,22 @@ void Solver::build_vcycle() {
   }
   const bool m32 = mixed();
   double* sigma = scalars.p + SC_SIGMA;
-  if (pins.empty() || pins.size() > 2)
-    throw Error(LSX_ERR_ARG, "the V-cycle needs one or two pinned (identity) rows to fix the pressure level");
+  if (pins.size() > 2) throw Error(LSX_ERR_ARG, "the V-cycle takes at most two pinned (identity) rows");
+  // without pinned rows the operator must carry a diagonal beyond its couplings (the mass term of
+  // a momentum operator); coarse levels then take the sum of their children's excess
+  const bool massive = pins.empty();
+  std::vector<DevBuf<double>> ex(massive ? 2 : 0);
+  if (massive) {
+    ex[0].alloc(d0.n());
+    ex[1].alloc(L >= 1 ? levels[1].d.n() : 1);
+  }
   auto coarse_of = [&](int p, int l) {  // the level-l cell that contains fine cell p
     if (p < 0) return -1;
     const int x = p % d0.N, y = (p / d0.N) % d0.M, z = p / (d0.N * d0.M);
     const Dims& d = levels[l].d;
     return (x >> l) + d.N * ((y >> l) + d.M * (z >> l));
   };
-  const int fp0 = pins[0], fp1 = pins.size() > 1 ? pins[1] : -1;
+  const int fp0 = pins.size() > 0 ? pins[0] : -1, fp1 = pins.size() > 1 ? pins[1] : -1;
   DevBuf<double> dev;  // per-row deviation from the diagonal-from-faces rule
   if (opt.diag_faces) dev.alloc(d0.n());
   double sig_h[2] = {0.0, 0.0};
@@ -471,11 +478,23 @@ void Solver::build_vcycle() {
       vc::pin_sigma(fe[cur].p, fn[cur].p, ft[cur].p, d, pin0, pin1, sigma, stream);
       sig_h[0] = reduce_to_host(sigma);
       sig_h[1] = reduce_to_host(sigma + 1);
+      if (massive) {
+        vc::excess(fe[cur].p, fn[cur].p, ft[cur].p, dg[cur].p, d, ex[cur].p, stream);
+        vc::face_absmax(ex[cur].p, ex[cur].p, ex[cur].p, nl, partial.p, scalars.p + SC_TMP, stream);
+        const double emax = reduce_to_host(scalars.p + SC_TMP);
+        vc::face_absmax(fe[cur].p, fn[cur].p, ft[cur].p, nl, partial.p, scalars.p + SC_TMP, stream);
+        const double fmax = reduce_to_host(scalars.p + SC_TMP);
+        if (!(emax > 1e-8 * fmax))
+          throw Error(LSX_ERR_ARG,
+                      "the V-cycle needs one or two pinned (identity) rows, or a diagonal beyond the couplings "
+                      "(a mass term): the operator is singular");
+      }
     } else {
       const Dims df = levels[l - 1].d;
       const int nxt = cur ^ 1;
       vc::aggregate(fe[cur].p, fn[cur].p, ft[cur].p, df, fe[nxt].p, fn[nxt].p, ft[nxt].p, d, stream);
-      vc::diag_from_faces(fe[nxt].p, fn[nxt].p, ft[nxt].p, d, dg[nxt].p, stream);
+      if (massive) vc::aggregate_cells(ex[cur].p, df, ex[nxt].p, d, stream);
+      vc::diag_from_faces(fe[nxt].p, fn[nxt].p, ft[nxt].p, d, massive ? ex[nxt].p : nullptr, dg[nxt].p, stream);
       vc::pin_sink(dg[nxt].p, coarse_of(fp0, l), coarse_of(fp1, l), sigma, stream);
       cur = nxt;
     }

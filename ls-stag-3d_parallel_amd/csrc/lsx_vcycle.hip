@@ -127,8 +127,11 @@ __global__ void k_aggregate(const double* __restrict__ fe, const double* __restr
   }
 }
 
+// dg = -(sum of the six faces), plus ex[i] when ex is given (the aggregated excess of an operator
+// with a mass term); ex = null leaves the sum as it is (-0.0 included)
 __global__ void k_diag_from_faces(const double* __restrict__ fe, const double* __restrict__ fn,
-                                  const double* __restrict__ ft, int N, int M, int K, double* __restrict__ dg) {
+                                  const double* __restrict__ ft, int N, int M, int K, const double* __restrict__ ex,
+                                  double* __restrict__ dg) {
   const long n = (long)N * M * K, NM = (long)N * M;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     int x, y, z;
@@ -137,7 +140,42 @@ __global__ void k_diag_from_faces(const double* __restrict__ fe, const double* _
     if (x > 0) d -= fe[i - 1];
     if (y > 0) d -= fn[i - N];
     if (z > 0) d -= ft[i - NM];
+    if (ex) d += ex[i];
     dg[i] = d;
+  }
+}
+
+// excess of the finest level: ex = a_p + (sum of the six faces), the part of the diagonal that
+// is not a coupling (the mass and wall terms of a momentum operator)
+__global__ void k_excess(const double* __restrict__ fe, const double* __restrict__ fn, const double* __restrict__ ft,
+                         const double* __restrict__ dg, int N, int M, int K, double* __restrict__ ex) {
+  const long n = (long)N * M * K, NM = (long)N * M;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    decode(i, N, M, &x, &y, &z);
+    double f = fe[i] + fn[i] + ft[i];
+    if (x > 0) f += fe[i - 1];
+    if (y > 0) f += fn[i - N];
+    if (z > 0) f += ft[i - NM];
+    ex[i] = dg[i] + f;
+  }
+}
+
+// coarse excess: the sum over the children present (the diagonal block of P^T A P with the
+// piecewise-constant prolongation, less its couplings)
+__global__ void k_aggregate_cells(const double* __restrict__ ef, int Nf, int Mf, int Kf, double* __restrict__ ec,
+                                  int Nc, int Mc, int Kc) {
+  const long nc = (long)Nc * Mc * Kc, NMf = (long)Nf * Mf;
+  for (long I = blockIdx.x * (long)blockDim.x + threadIdx.x; I < nc; I += (long)gridDim.x * blockDim.x) {
+    int X, Y, Z;
+    decode(I, Nc, Mc, &X, &Y, &Z);
+    const long a = 2L * X + (long)Nf * 2 * Y + NMf * 2 * Z;
+    const bool hx = 2 * X + 1 < Nf, hy = 2 * Y + 1 < Mf, hz = 2 * Z + 1 < Kf;
+    double s = 0.0;
+    for (int dz = 0; dz <= (int)hz; ++dz)
+      for (int dy = 0; dy <= (int)hy; ++dy)
+        for (int dx = 0; dx <= (int)hx; ++dx) s += ef[a + dx + (long)Nf * dy + NMf * dz];
+    ec[I] = s;
   }
 }
 
@@ -1777,8 +1815,18 @@ void aggregate(const double* fe, const double* fn, const double* ft, Dims df, do
   k_aggregate<<<grid_1d(dc.n()), 256, 0, s>>>(fe, fn, ft, df.N, df.M, df.K, ce, cn, ct, dc.N, dc.M, dc.K);
 }
 
-void diag_from_faces(const double* fe, const double* fn, const double* ft, Dims d, double* dg, hipStream_t s) {
-  k_diag_from_faces<<<grid_1d(d.n()), 256, 0, s>>>(fe, fn, ft, d.N, d.M, d.K, dg);
+void diag_from_faces(const double* fe, const double* fn, const double* ft, Dims d, const double* ex, double* dg,
+                     hipStream_t s) {
+  k_diag_from_faces<<<grid_1d(d.n()), 256, 0, s>>>(fe, fn, ft, d.N, d.M, d.K, ex, dg);
+}
+
+void excess(const double* fe, const double* fn, const double* ft, const double* dg, Dims d, double* ex,
+            hipStream_t s) {
+  k_excess<<<grid_1d(d.n()), 256, 0, s>>>(fe, fn, ft, dg, d.N, d.M, d.K, ex);
+}
+
+void aggregate_cells(const double* ef, Dims df, double* ec, Dims dc, hipStream_t s) {
+  k_aggregate_cells<<<grid_1d(dc.n()), 256, 0, s>>>(ef, df.N, df.M, df.K, ec, dc.N, dc.M, dc.K);
 }
 
 void pin_sigma(const double* fe, const double* fn, const double* ft, Dims d, int pin0, int pin1, double* sigma,

@@ -72,8 +72,13 @@ void faces_from7(const double* c7, Dims d, int pin0, int pin1, double* fe, doubl
 // coarse faces: half the sum of the four fine faces covering each coarse face
 void aggregate(const double* fe, const double* fn, const double* ft, Dims df, double* ce, double* cn, double* ct,
                Dims dc, hipStream_t s);
-// Neumann diagonal (-sum of the six faces)
-void diag_from_faces(const double* fe, const double* fn, const double* ft, Dims d, double* dg, hipStream_t s);
+// Neumann diagonal (-sum of the six faces), plus ex[i] when ex is not null
+void diag_from_faces(const double* fe, const double* fn, const double* ft, Dims d, const double* ex, double* dg,
+                     hipStream_t s);
+// ex = dg + (sum of the six faces): the diagonal beyond the couplings (a mass term)
+void excess(const double* fe, const double* fn, const double* ft, const double* dg, Dims d, double* ex, hipStream_t s);
+// coarse excess: sum of the fine excess over the children of each coarse cell
+void aggregate_cells(const double* ef, Dims df, double* ec, Dims dc, hipStream_t s);
 // sigma[q] = sum of the faces of pinned cell q (q = 0, 1) on the finest level
 void pin_sigma(const double* fe, const double* fn, const double* ft, Dims d, int pin0, int pin1, double* sigma,
                hipStream_t s);
