@@ -189,3 +189,28 @@ def test_vcycle_rejects_singular_operator_without_pins(lsx):
     with pytest.raises(lsx.LsxError, match="singular"):
         S.set(A, np.zeros(A.n), np.zeros(A.n), lsx.mixed_task(1e-8))
         S.solve()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64-vcycle", 1e-12), ("mixed", 2e-5)])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_helmholtz_vcycle_transposed(lsx, d, precision, tol):
+    """The transposed V-cycle on an operator without pinned rows (aggregated mass term) is the
+    transpose of the cycle: <B x, y> = <x, B^T y> (exact in FP64, FP32 rounding in mixed).  The
+    momentum operators are symmetric to rounding only (their mirrored couplings differ in the
+    last bit), and adjoint operations take bitwise-symmetric operators, so the test uses
+    (A + A^T) / 2, which is (each pair is summed in either order to the same value)."""
+    A0, _ = system((48, 32, 40), (1.02, 1.02, 1.02), d, seed=5, re=10.0, dt=0.02)
+    M = A0.to_scipy()
+    M = ((M + M.T) * 0.5).tocsr()
+    M.sort_indices()
+    A = O.Csr(A0.dims, M.indptr, M.indices, M.data)
+    S = lsx.solver("fgmres", precision)
+    S.set(A, np.zeros(A.n), np.zeros(A.n), lsx.mixed_task(1.0))
+    rng = np.random.default_rng(9 + d)
+    x, y = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    Bx, BTy = S.precondition(x), S.precondition_transpose(y)
+    scale = np.linalg.norm(Bx) * np.linalg.norm(y)
+    rel = abs(Bx @ y - x @ BTy) / scale
+    print(f"mesh {A.dims} component {'uvw'[d]} {precision}: levels={S.num_levels()} transpose rel={rel:.3e}",
+          flush=True)
+    assert np.linalg.norm(Bx) > 0 and rel <= tol
