@@ -807,10 +807,13 @@ def mixed_task(eps, restart=30, max_it=1000, n_pre=2, n_post=2):
 
 
 def solver(kind="fgmres", precision="fp64", **kw):
-    """MGFGMRES / MGBiCGStab with one of three preconditioners:
+    """MGFGMRES / MGBiCGStab with one of four preconditioners:
     "fp64"        the reference FP64 multigrid cascade (ADLJ line smoothing, Galerkin levels);
     "fp64-vcycle" the FP64 V-cycle (red-black Gauss-Seidel, rediscretised face levels);
-    "mixed"       the same V-cycle in FP32 with FP16 coarse faces."""
+    "mixed"       the same V-cycle in FP32 with FP16 coarse faces;
+    "mixed16"     as "mixed" with FP16 faces on the finest level too.
+    The V-cycles take pressure operators (one or two pinned rows) and operators with a diagonal
+    beyond their couplings (the momentum operators, whose mass term the coarse levels sum)."""
     cls = MGFGMRES if kind == "fgmres" else MGBiCGStab
     if precision == "mixed":
         return cls(precision=PREC_MIXED, mg=MG_VCYCLE, **kw)
