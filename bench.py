@@ -330,6 +330,66 @@ def projection_run(steps):
     print(json.dumps(out), flush=True)
 
 
+def momentum_run(n, steps):
+    """Momentum (Helmholtz) solves u, v, w on the n^3 stretched grid of the headline workload
+    without its body (the device assembly takes boxes without cut cells), assembled on the device
+    (lsx_set_helmholtz), FGMRES(30) to 1e-8 |b| from x = 0 with b a seeded Gaussian, at the
+    time-stepped run's Re = 300, dt = 0.002; the one-level cascade (the reference's cascade has no
+    transfers for the staggered meshes' one odd dimension) against the mixed V-cycle.  (At
+    Re = 30, dt = 0.02 on this stretched grid the v and w operators are indefinite from 128^3 up:
+    their outflow term outweighs the mass term on the last x plane.)  One JSON line."""
+    lsx = load_lsx()
+    hip = Hip()
+    widths, _, _ = workload(n)
+    geom = lsx.Geometry(*widths).build("sphere", (-5.0, -5.0, -5.0), (0.01,))  # the body lies outside the box
+    hip.sync()
+    L = lsx.lib()
+    out = []
+    for re_, dt in ((300.0, 0.002),):
+        for d in range(3):
+            dims = [n, n, n]
+            dims[d] -= 1
+            m = dims[0] * dims[1] * dims[2]
+            b = np.random.default_rng(17 + d).standard_normal(m) * 1e-3
+            nb = float(np.linalg.norm(b))
+            db, dx = hip.malloc(m * 8), hip.malloc(m * 8)
+            hip.h2d(db, b)
+            for name, make, task in (
+                    ("one-level cascade", lambda: lsx.MGFGMRES(levels=1), lsx.Task(eps=1.0, MAX_ITER=500)),
+                    ("mixed V-cycle", lambda: lsx.solver("fgmres", "mixed", graphs=True, smoother="line",
+                                                         line_levels=2), lsx.mixed_task(1.0, max_it=500))):
+                S = make()
+                S.set_helmholtz(geom, d, 1.0 / re_, 1.5 / dt, None, None, task)
+                lsx._check(L.lsx_set_eps(S.h, 1e-8 * nb))
+                it, nr = C.c_int(), C.c_double()
+
+                def step():
+                    hip.zero(dx, m * 8)
+                    st = L.lsx_solve(S.h, db, dx, lsx.MEM_DEVICE, C.byref(it), C.byref(nr))
+                    if st != 0:
+                        raise RuntimeError(L.lsx_last_error().decode())
+
+                step()
+                hip.sync()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    step()
+                hip.sync()
+                sec = (time.perf_counter() - t0) / steps
+                x = np.zeros(m)
+                hip.d2h(x, dx)
+                r = b - S.apply_operator(x, 0)
+                rec = {"component": "uvw"[d], "mesh": dims, "re": re_, "dt": dt, "preconditioner": name,
+                       "iterations": it.value, "ms_per_solve": sec * 1e3, "solves_per_s": 1.0 / sec,
+                       "true_rel_residual": float(np.linalg.norm(r) / nb)}
+                print(f"[momentum] {rec}", file=sys.stderr, flush=True)
+                out.append(rec)
+                S.close()
+            hip.free(db)
+            hip.free(dx)
+    print(json.dumps({"momentum": out, "grid": n, "steps": steps, "data": "synthetic"}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--copy-calibration", type=float, default=0.0,
@@ -371,6 +431,10 @@ def main():
     ap.add_argument("--projection-steps", type=int, default=0,
                     help="instead of the solve benchmark: this many time steps of the 384x256x256 sphere "
                          "projection run (one JSON line of per-step results)")
+    ap.add_argument("--momentum", type=int, default=0,
+                    help="instead of the solve benchmark: this many timed solves of each momentum (u, v, w) "
+                         "system on the grid without the body, one-level cascade against the mixed V-cycle "
+                         "(one JSON line)")
     args = ap.parse_args()
     if args.cpu_worker:
         cpu_worker(args.cpu_worker, args.restart, args.krylov, args.body)
@@ -380,6 +444,9 @@ def main():
         return
     if args.projection_steps > 0:
         projection_run(args.projection_steps)
+        return
+    if args.momentum > 0:
+        momentum_run(args.grid, args.momentum)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the solve runs on a single "
