@@ -1592,14 +1592,31 @@ __global__ void __launch_bounds__(1024) k_line_seg(const T* __restrict__ x_in, T
   auto IF = [&](int q, int j, int l) -> T& { return ifc[((long)q * S + j) * (LPW + 1) + l]; };
   const long NM = (long)N * M;
   const int lane = threadIdx.x % LPW, seg = threadIdx.x / LPW;
-  // (column, plane) pairs in XCD-contiguous order, columns fastest: the halo columns and
-  // neighbouring planes a workgroup reads are read by workgroups of the same XCD at about the
-  // same time (measured fetch 26 instead of 35 B/cell, 24 B/cell compulsory reads; 9 % less time)
+  // Line groups in XCD-contiguous order, columns fastest: the halo columns and neighbouring
+  // planes a workgroup reads are read by workgroups of the same XCD at about the same time
+  // (measured fetch 26 instead of 35 B/cell, 24 B/cell compulsory reads; 9 % less time).  y lines:
+  // LPW adjacent x of one (y = 0, z) row; z lines: LPW consecutive cells p = x + N y of the (x, y)
+  // plane, so that every row load is one aligned 128-byte line whatever N is (the same groups as
+  // x runs of one row when LPW divides N)
   const int gxb = (N + LPW - 1) / LPW;
   const int lb = xcd_order(blockIdx.x, gridDim.x);
-  const int xr = (lb % gxb) * LPW + lane, other = lb / gxb;
-  const bool live = xr < N;
-  const int xx = live ? xr : N - 1;  // lanes past the last line repeat it and store nothing
+  bool live;
+  int xx, other;
+  long p0 = 0, pl = 0;
+  if constexpr (DIR == 1) {
+    const int xr = (lb % gxb) * LPW + lane;
+    other = lb / gxb;
+    live = xr < N;
+    xx = live ? xr : N - 1;  // lanes past the last line repeat it and store nothing
+  } else {
+    p0 = (long)lb * LPW;
+    pl = p0 + LPW - 1 < NM - 1 ? p0 + LPW - 1 : NM - 1;
+    const long pr = p0 + lane;
+    live = pr < NM;
+    const long pc = live ? pr : NM - 1;
+    xx = (int)(pc % N);
+    other = (int)(pc / N);
+  }
   const int len = DIR == 1 ? M : K;
   const long stride = DIR == 1 ? (long)N : NM;
   const long base = DIR == 1 ? xx + NM * other : xx + (long)N * other;
@@ -1658,11 +1675,16 @@ __global__ void __launch_bounds__(1024) k_line_seg(const T* __restrict__ x_in, T
         Ln.oAu = Ln.hAu ? 1 : 0;
         Ln.oBl = Ln.hBl ? -dB : 0;
         Ln.oBu = Ln.hBu ? dB : 0;
-        {
+        Ln.l32 = lane;
+        if constexpr (DIR == 1) {
           const int x0 = (lb % gxb) * LPW, xl = x0 + LPW - 1 < N - 1 ? x0 + LPW - 1 : N - 1;
-          Ln.l32 = lane;
           Ln.oW0 = (x0 - xx) + (x0 > 0 ? -1 : 0);
           Ln.oE0 = (xl - xx) + (xl < N - 1 ? 1 : 0);
+        } else {
+          // the group's west and east halo cells p0 - 1 and pl + 1 (read only by a first or last
+          // lane that has that x neighbour, which then lies in its row)
+          Ln.oW0 = (int)(p0 - base) + (p0 > 0 ? -1 : 0);
+          Ln.oE0 = (int)(pl - base) + (pl < NM - 1 ? 1 : 0);
         }
         T carry = (T)0;
   #pragma unroll
@@ -2052,7 +2074,7 @@ static void line_pass_z(const Level& L, int dir, const T* x_in, T* x_out, const 
     const int len = dir == 1 ? d.M : d.K;
     const int S = (len + SEG - 1) / SEG;
     constexpr int LPW = lpw<T>();
-    const dim3 grid(((d.N + LPW - 1) / LPW) * (dir == 1 ? d.K : d.M));
+    const dim3 grid(dir == 1 ? ((d.N + LPW - 1) / LPW) * d.K : (unsigned)((d.N * (long)d.M + LPW - 1) / LPW));
     const size_t lds = 8 * (size_t)S * (LPW + 1) * sizeof(T);
 #define LSX_LINE_SEG(DIR, TR, DFF)                                                                             \
   k_line_seg<T, FT, DIR, ZERO, TR, DFF><<<grid, LPW * S, lds, s>>>(x_in, x_out, b, ae, an, at, sv, fs, d.N, d.M,  \
